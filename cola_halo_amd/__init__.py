@@ -1,0 +1,337 @@
+"""cola_halo_amd: MI355X-native 2LPT / particle-mesh / COLA time stepping.
+
+Python binding of the C ABI in include/cola_halo_amd.h (libcola_halo_amd.so,
+built in-tree by ``python -c "import __graft_entry__; __graft_entry__.build()"``
+or ``make -C cola_halo_amd/csrc``).  The method names follow the reference's
+module functions (power_init, lpt_set_displacement, move_particles2,
+pm_calculate_forces, cola_kick, cola_drift, cola_set_snapshot, ...).  Every
+call runs on the GPU through the HIP library; there is no CPU fallback, and a
+missing library raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcola_halo_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cola_halo_amd.h")
+
+# particle.h:6-12 and particle.h:24-28
+PARTICLE_DTYPE = np.dtype([("x", "<f4", 3), ("dx1", "<f4", 3), ("dx2", "<f4", 3),
+                           ("v", "<f4", 3), ("id", "<i8")])
+PARTICLE_MIN_DTYPE = np.dtype([("x", "<f4", 3), ("v", "<f4", 3), ("id", "<i8")])
+
+# status codes (msg_abort codes of the reference)
+OK = 0
+ERR_ALLOC = 4001
+ERR_POWER_FILE = 3000
+ERR_SIGMA8 = 3010
+ERR_MASS = 6040
+ERR_ARG = 9001
+ERR_UNSUPPORTED = 9002
+ERR_HIP = 9003
+ERR_STATE = 9004
+
+
+class ColaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cola_halo_amd error {code}: {msg}")
+        self.code = code
+
+
+_lib_handle = None
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "cola_last_error": (C.c_char_p, []),
+    "cola_context_create": (C.c_int, [C.c_int, C.c_int, C.c_double, C.POINTER(C.c_void_p)]),
+    "cola_context_destroy": (C.c_int, [C.c_void_p]),
+    "cola_synchronize": (C.c_int, [C.c_void_p]),
+    "cola_power_init": (C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "cola_power_init_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_double]),
+    "cola_power_spec": (C.c_double, [C.c_void_p, C.c_double]),
+    "cola_growth_factor": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
+    "cola_power_sigma8": (C.c_double, [C.c_void_p]),
+    "cola_lpt_set_displacement": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double]),
+    "cola_lpt_get_local_nx": (C.c_int, [C.c_void_p]),
+    "cola_move_particles": (C.c_int, [C.c_void_p]),
+    "cola_pm_calculate_forces": (C.c_int, [C.c_void_p]),
+    "cola_pm_assign": (C.c_int, [C.c_void_p]),
+    "cola_kick": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    "cola_drift": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    "cola_step": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+    "cola_set_snapshot": (C.c_int, [C.c_void_p, C.c_double, C.c_float, C.c_void_p, C.c_size_t]),
+    "cola_set_noncola_initial": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_size_t]),
+    "cola_np": (C.c_longlong, [C.c_void_p]),
+    "cola_get_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "cola_set_times": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    "cola_get_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "cola_set_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_float, C.c_float]),
+    "cola_get_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "cola_get_density": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "cola_get_force_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "cola_last_mass_sum": (C.c_double, [C.c_void_p]),
+    "cola_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cola_phase_count": (C.c_int, []),
+    "cola_phase_name": (C.c_char_p, [C.c_int]),
+    "cola_get_phase_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    "cola_reset_phase_stats": (C.c_int, [C.c_void_p]),
+    "cola_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cola_eh98_nowiggle_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
+    "cola_fft_r2c_3d": (C.c_int, [C.c_int, C.c_void_p]),
+    "cola_fft_c2r_3d": (C.c_int, [C.c_int, C.c_void_p]),
+}
+
+
+def lib():
+    """Load libcola_halo_amd.so (raises if it has not been built)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with make -C cola_halo_amd/csrc "
+                              "(there is no CPU fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib_handle = h
+    return _lib_handle
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _check(code):
+    if code != 0:
+        raise ColaError(code, lib().cola_last_error().decode(errors="replace"))
+
+
+# Cosmology of the benchmark configurations
+EH_COSMOLOGY = dict(omega_m=0.273, omega_b=0.046, h=0.705, ns=0.96, sigma8=0.8)
+
+
+def eh98_table(omega_m=0.273, omega_b=0.046, h=0.705, ns=0.96, sigma8=0.8, kmin=1e-4, kmax=1e2, n=500):
+    """Eisenstein & Hu (1998) no-wiggle linear P(k) table [k h/Mpc, P (Mpc/h)^3],
+    normalised to sigma8 by power.c's integral."""
+    k = np.zeros(n)
+    p = np.zeros(n)
+    _check(lib().cola_eh98_nowiggle_table(omega_m, omega_b, h, ns, sigma8, kmin, kmax, n, k.ctypes.data,
+                                          p.ctypes.data))
+    return k, p
+
+
+def padded_real_shape(n):
+    return (n, n, 2 * (n // 2 + 1))
+
+
+def fft_r2c_3d(grid):
+    """FFTW-convention r2c of a real (n, n, n) array on the GPU; returns the
+    (n, n, n//2+1) complex64 half spectrum."""
+    g = np.asarray(grid, dtype=np.float32)
+    n = g.shape[0]
+    buf = np.zeros(padded_real_shape(n), np.float32)
+    buf[:, :, :n] = g
+    _check(lib().cola_fft_r2c_3d(n, buf.ctypes.data))
+    return buf.view(np.complex64)
+
+
+def fft_c2r_3d(spec):
+    """FFTW-convention c2r (unnormalised) of an (n, n, n//2+1) half spectrum."""
+    s = np.ascontiguousarray(spec, dtype=np.complex64)
+    n = s.shape[0]
+    # a private copy: the transform runs in place and must not touch spec
+    buf = np.array(s.view(np.float32).reshape(padded_real_shape(n)), copy=True)
+    _check(lib().cola_fft_c2r_3d(n, buf.ctypes.data))
+    return buf[:, :, :n].copy()
+
+
+class Context:
+    """One realization's device state: the reference's Particles, PM grids,
+    FFT plans and the a_x / a_v clocks (main.c:52-111 set-up)."""
+
+    def __init__(self, nc, ngrid, boxsize):
+        self.nc, self.ngrid, self.boxsize = int(nc), int(ngrid), float(boxsize)
+        h = C.c_void_p()
+        _check(lib().cola_context_create(self.nc, self.ngrid, self.boxsize, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cola_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def np(self):
+        return lib().cola_np(self._h)
+
+    # power.c
+    def power_init(self, filename, a_init, sigma8, omega_m, omega_lambda):
+        _check(lib().cola_power_init(self._h, os.fsencode(filename), a_init, sigma8, omega_m, omega_lambda))
+
+    def power_init_table(self, k, pk, a_init, sigma8, omega_m, omega_lambda):
+        k = np.ascontiguousarray(k, np.float64)
+        pk = np.ascontiguousarray(pk, np.float64)
+        _check(lib().cola_power_init_table(self._h, k.ctypes.data, pk.ctypes.data, len(k), a_init, sigma8,
+                                           omega_m, omega_lambda))
+
+    def power_spec(self, k):
+        return lib().cola_power_spec(self._h, k)
+
+    def growth_factor(self, astart, aend):
+        return lib().cola_growth_factor(self._h, astart, aend)
+
+    def sigma8(self):
+        return lib().cola_power_sigma8(self._h)
+
+    # lpt.c / move.c / pm.c / cola.c
+    def lpt_set_displacement(self, a_init, omega_m, seed, boxsize=None):
+        _check(lib().cola_lpt_set_displacement(self._h, a_init, omega_m, int(seed),
+                                               self.boxsize if boxsize is None else boxsize))
+
+    def move_particles(self):
+        _check(lib().cola_move_particles(self._h))
+
+    def pm_calculate_forces(self):
+        _check(lib().cola_pm_calculate_forces(self._h))
+
+    def pm_assign(self):
+        _check(lib().cola_pm_assign(self._h))
+
+    def kick(self, omega_m, avel1):
+        _check(lib().cola_kick(self._h, omega_m, avel1))
+
+    def drift(self, omega_m, apos1):
+        _check(lib().cola_drift(self._h, omega_m, apos1))
+
+    def step(self, omega_m, avel1, apos1):
+        _check(lib().cola_step(self._h, omega_m, avel1, apos1))
+
+    def synchronize(self):
+        _check(lib().cola_synchronize(self._h))
+
+    def set_snapshot(self, aout, omega_m):
+        out = np.zeros(self.np, PARTICLE_MIN_DTYPE)
+        _check(lib().cola_set_snapshot(self._h, aout, omega_m, out.ctypes.data, len(out)))
+        return out
+
+    def set_noncola_initial(self, omega_m):
+        out = np.zeros(self.np, PARTICLE_MIN_DTYPE)
+        _check(lib().cola_set_noncola_initial(self._h, omega_m, out.ctypes.data, len(out)))
+        return out
+
+    # state
+    def set_times(self, a_x, a_v):
+        _check(lib().cola_set_times(self._h, a_x, a_v))
+
+    def times(self):
+        ax, av = C.c_float(), C.c_float()
+        _check(lib().cola_get_times(self._h, C.byref(ax), C.byref(av)))
+        return ax.value, av.value
+
+    def get_particles(self):
+        out = np.zeros(self.np, PARTICLE_DTYPE)
+        _check(lib().cola_get_particles(self._h, out.ctypes.data, len(out)))
+        return out
+
+    def set_particles(self, parts, a_x, a_v):
+        parts = np.ascontiguousarray(parts, PARTICLE_DTYPE)
+        _check(lib().cola_set_particles(self._h, parts.ctypes.data, len(parts), a_x, a_v))
+
+    def get_forces(self):
+        out = np.zeros((self.np, 3), np.float32)
+        _check(lib().cola_get_forces(self._h, out.ctypes.data, out.size))
+        return out
+
+    def get_density(self):
+        out = np.zeros(padded_real_shape(self.ngrid), np.float32)
+        _check(lib().cola_get_density(self._h, out.ctypes.data, out.size))
+        return out
+
+    def get_force_grid(self, axis):
+        out = np.zeros(padded_real_shape(self.ngrid), np.float32)
+        _check(lib().cola_get_force_grid(self._h, axis, out.ctypes.data, out.size))
+        return out
+
+    def last_mass_sum(self):
+        return lib().cola_last_mass_sum(self._h)
+
+    # profiling / tuning
+    def set_profiling(self, on=True):
+        _check(lib().cola_set_profiling(self._h, 1 if on else 0))
+
+    def phase_stats(self):
+        n = lib().cola_phase_count()
+        ms = np.zeros(n, np.float64)
+        cnt = np.zeros(n, np.int64)
+        _check(lib().cola_get_phase_stats(self._h, ms.ctypes.data, cnt.ctypes.data, n))
+        return {lib().cola_phase_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    def reset_phase_stats(self):
+        _check(lib().cola_reset_phase_stats(self._h))
+
+    def set_tiling(self, tx, ty, tz, fft_log2_tile=-1):
+        _check(lib().cola_set_tiling(self._h, tx, ty, tz, fft_log2_tile))
+
+
+def run_cola(ctx, omega_m, a_final, nsteps, seed, zout, fused=True, on_snapshot=None):
+    """The reference's realization loop for one seed (main.c:117-207),
+    linear-in-a schedule.  Returns the list of (aout, snapshot) pairs.  With
+    fused=True the steps without a snapshot run as cola_step (wrap, force,
+    fused readout+kick+drift); results equal the unfused calls bit for bit."""
+    da = a_final / nsteps
+    a_init = da
+    aout = [float(np.float32(1.0 / (1.0 + z))) for z in zout]
+    ctx.lpt_set_displacement(a_init, omega_m, seed)
+    Om = float(np.float32(omega_m))
+    ctx.set_times(ctx.times()[0], da)  # main.c:145
+    out = []
+    iout = 0
+    nout = len(aout)
+    if nout > 0 and nsteps > 1 and a_final > a_init:
+        for istep in range(1, nsteps + 1):
+            avel0 = (istep - 0.5) * da
+            apos0 = istep * da
+            avel1 = (istep + 0.5) * da
+            apos1 = (istep + 1.0) * da
+            snap_now = iout < nout and avel0 <= aout[iout] <= apos0
+            snap_mid = (not snap_now) and iout < nout and apos0 < aout[iout] <= avel1
+            if fused and not snap_now and not snap_mid and not (iout >= nout):
+                ctx.step(Om, avel1, apos1)
+                continue
+            ctx.move_particles()
+            ctx.pm_calculate_forces()
+            while iout < nout and avel0 <= aout[iout] <= apos0:
+                s = ctx.set_snapshot(aout[iout], Om)
+                out.append((aout[iout], s))
+                if on_snapshot:
+                    on_snapshot(aout[iout], s)
+                iout += 1
+            if iout >= nout:
+                break
+            ctx.kick(Om, avel1)
+            while iout < nout and apos0 < aout[iout] <= avel1:
+                s = ctx.set_snapshot(aout[iout], Om)
+                out.append((aout[iout], s))
+                if on_snapshot:
+                    on_snapshot(aout[iout], s)
+                iout += 1
+            if iout >= nout:
+                break
+            ctx.drift(Om, apos1)
+    ctx.synchronize()
+    return out
+

@@ -1,0 +1,865 @@
+// C ABI of libcola_halo_amd.so: context lifetime, the reference's module
+// entry points (power_init, lpt_set_displacement, move_particles2,
+// pm_calculate_forces, cola_kick, cola_drift, cola_set_snapshot,
+// set_noncola_initial) and state access.  All device work of a context runs
+// in order on its one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cola_halo_amd.h"
+#include "common.h"
+#include "cosmo.h"
+#include "fft.h"
+#include "lpt.h"
+#include "pm.h"
+
+namespace cola {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+// Profiled phases, one per kernel family.
+enum Phase : int {
+  kPhWrap = 0,
+  kPhBinCount,
+  kPhBinScan,
+  kPhBinScatter,
+  kPhDeposit,
+  kPhFftZR2C,
+  kPhFftYFwd,
+  kPhFftXFwd,
+  kPhFftXInv,
+  kPhFftYInv,
+  kPhFftZC2R,
+  kPhReadout,
+  kPhReadoutKickDrift,
+  kPhKick,
+  kPhDrift,
+  kPhSnapshot,
+  kPhLptModes,
+  kPhLptFft,
+  kPhLptOther,
+  kNumPhases
+};
+const char* const kPhaseNames[kNumPhases] = {
+    "wrap",      "bin_count",  "bin_scan",   "bin_scatter", "deposit",   "fft_z_r2c",     "fft_y_fwd",
+    "fft_x_fwd", "fft_x_inv",  "fft_y_inv",  "fft_z_c2r",   "readout",   "readout_kick_drift", "kick",
+    "drift",     "snapshot",   "lpt_modes",  "lpt_fft",     "lpt_other"};
+
+struct EventPair {
+  int phase;
+  hipEvent_t a, b;
+};
+
+}  // namespace cola
+
+using namespace cola;
+
+struct cola_context {
+  int nc = 0, ng = 0;
+  double boxsize = 0;
+  float boxf = 0;
+  long long np = 0;
+  float wpar = 0;
+  hipStream_t stream = nullptr;
+
+  DevParticles p;
+  float* pbuf = nullptr;
+  float* density = nullptr;
+  float* fgrid[3] = {nullptr, nullptr, nullptr};
+  long long grid_floats = 0;
+
+  FftPlan pm_fft, lpt_fft;
+  int fft_log2_tile = -1;
+  CicGeom cic;
+  TileGeom tiles;
+  BinBuffers bins;
+
+  static constexpr int kMassSlots = 64;
+  double* d_mass = nullptr;
+  int mass_slot = 0;     // next slot
+  int mass_pending = 0;  // slots written since the last check
+  double last_mass = 0;
+
+  PowerTable power;
+  double pow_omega_m = 0.3, pow_omega_lambda = 0.7, sigma8_input = 0;
+  double* d_logk = nullptr;
+  double* d_logD = nullptr;
+
+  float a_x = 0, a_v = 0;
+  bool have_particles = false;
+  bool have_force = false;
+  bool have_grids = false;
+  bool density_is_real = false;
+
+  DevParticleMin* d_snap = nullptr;
+
+  bool profiling = false;
+  std::vector<EventPair> events;
+  std::vector<hipEvent_t> free_events;
+  double phase_ms[kNumPhases] = {0};
+  long long phase_n[kNumPhases] = {0};
+  int open_phase = -1;
+  hipEvent_t open_event = nullptr;
+};
+
+namespace {
+
+int get_event(cola_context* c, hipEvent_t* e)
+{
+  if (!c->free_events.empty()) {
+    *e = c->free_events.back();
+    c->free_events.pop_back();
+    return kOk;
+  }
+  COLA_HIP(hipEventCreate(e));
+  return kOk;
+}
+
+void phase_begin(cola_context* c, int ph)
+{
+  if (!c->profiling) return;
+  hipEvent_t e;
+  if (get_event(c, &e) != kOk) return;
+  (void)hipEventRecord(e, c->stream);
+  c->open_phase = ph;
+  c->open_event = e;
+}
+
+void phase_end(cola_context* c, int ph)
+{
+  if (!c->profiling || c->open_phase != ph) return;
+  hipEvent_t e;
+  if (get_event(c, &e) != kOk) return;
+  (void)hipEventRecord(e, c->stream);
+  c->events.push_back({ph, c->open_event, e});
+  c->open_phase = -1;
+}
+
+int collect_events(cola_context* c)
+{
+  if (c->events.empty()) return kOk;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  for (const EventPair& ep : c->events) {
+    float ms = 0;
+    COLA_HIP(hipEventElapsedTime(&ms, ep.a, ep.b));
+    c->phase_ms[ep.phase] += ms;
+    c->phase_n[ep.phase] += 1;
+    c->free_events.push_back(ep.a);
+    c->free_events.push_back(ep.b);
+  }
+  c->events.clear();
+  return kOk;
+}
+
+void fft_hook(void* user, int pass, int begin)
+{
+  cola_context* c = static_cast<cola_context*>(user);
+  static const int map[6] = {kPhFftZR2C, kPhFftYFwd, kPhFftXFwd, kPhFftXInv, kPhFftYInv, kPhFftZC2R};
+  if (begin)
+    phase_begin(c, map[pass]);
+  else
+    phase_end(c, map[pass]);
+}
+
+void lpt_fft_hook(void* user, int pass, int begin)
+{
+  cola_context* c = static_cast<cola_context*>(user);
+  if (begin)
+    phase_begin(c, kPhLptFft);
+  else
+    phase_end(c, kPhLptFft);
+}
+
+// Check the deposit's mass sums written since the last check (pm.c:535-562:
+// |sum| <= 1e-7 * WPAR * Np) and the binning overflow flag.
+int check_pending(cola_context* c)
+{
+  if (c->mass_pending == 0) return kOk;
+  const int n = std::min(c->mass_pending, cola_context::kMassSlots);
+  std::vector<double> sums(cola_context::kMassSlots);
+  int overflow = 0;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  COLA_HIP(hipMemcpy(sums.data(), c->d_mass, sizeof(double) * cola_context::kMassSlots, hipMemcpyDeviceToHost));
+  COLA_HIP(hipMemcpy(&overflow, c->bins.overflow, sizeof(int), hipMemcpyDeviceToHost));
+  c->mass_pending = 0;
+  const double tol = 1.0e-7 * (double)c->wpar * (double)c->np;
+  int status = kOk;
+  for (int k = 0; k < n; ++k) {
+    const int slot = (c->mass_slot - 1 - k + cola_context::kMassSlots) % cola_context::kMassSlots;
+    const double s = sums[slot];
+    if (k == 0) c->last_mass = s;
+    if (std::fabs(s) > tol) {
+      set_error("total CIC density error is large: " + std::to_string(s) + " > " + std::to_string(tol));
+      status = kErrMass;
+    }
+  }
+  if (overflow) {
+    const int zero = 0;
+    COLA_HIP(hipMemcpy(c->bins.overflow, &zero, sizeof(int), hipMemcpyHostToDevice));
+    set_error(overflow & 2 ? "deposit bin capacity exceeded" : "particle position outside the PM grid");
+    status = kErrMass;
+  }
+  return status;
+}
+
+int alloc_grids(cola_context* c)
+{
+  if (c->have_grids) return kOk;
+  const size_t bytes = sizeof(float) * c->grid_floats;
+  COLA_HIP(hipMalloc(&c->density, bytes));
+  for (int a = 0; a < 3; ++a) COLA_HIP(hipMalloc(&c->fgrid[a], bytes));
+  c->have_grids = true;
+  return kOk;
+}
+
+// pm.c:209-279: binning and the tiled deposit into c->density.  wrap_first
+// folds move_particles2 into the binning pass.
+int pm_assign(cola_context* c, bool wrap_first)
+{
+  COLA_TRY(alloc_grids(c));
+  const int slot = c->mass_slot;
+  c->mass_slot = (c->mass_slot + 1) % cola_context::kMassSlots;
+  c->mass_pending += 1;
+  COLA_TRY(launch_zero_doubles(c->d_mass + slot, 1, c->stream));
+  phase_begin(c, kPhBinCount);
+  COLA_TRY(launch_bin_count(c->p, c->cic, c->tiles, c->bins, wrap_first, c->boxf, c->stream));
+  phase_end(c, kPhBinCount);
+  phase_begin(c, kPhBinScan);
+  COLA_TRY(launch_bin_scan(c->tiles, c->bins, c->stream));
+  phase_end(c, kPhBinScan);
+  phase_begin(c, kPhBinScatter);
+  COLA_TRY(launch_bin_scatter(c->p, c->cic, c->tiles, c->bins, c->stream));
+  phase_end(c, kPhBinScatter);
+  phase_begin(c, kPhDeposit);
+  COLA_TRY(launch_deposit_tiles(c->cic, c->tiles, c->bins, c->density, c->d_mass + slot, c->stream));
+  phase_end(c, kPhDeposit);
+  c->density_is_real = true;
+  return kOk;
+}
+
+// pm.c:282-355: r2c of the density, then per axis the Poisson gradient and
+// the c2r into the three force grids.
+int pm_fields(cola_context* c)
+{
+  COLA_TRY(c->pm_fft.r2c(c->density, c->stream));
+  c->density_is_real = false;
+  const float scale = 2.0 * kPiD / c->boxf;  // pm.c:315
+  KOpParams kp;
+  kp.f1 = -1.0 / std::pow((double)c->ng, 3.0) / scale;  // pm.c:317
+  for (int a = 0; a < 3; ++a) {
+    kp.axis = a;
+    COLA_TRY(c->pm_fft.c2r(reinterpret_cast<const float2*>(c->density), c->fgrid[a], KOp::PmPoisson, kp, c->stream));
+  }
+  return kOk;
+}
+
+bool valid(const cola_context* c) { return c != nullptr; }
+
+int require_particles(cola_context* c)
+{
+  if (!c->have_particles) {
+    set_error("no particles: call cola_lpt_set_displacement or cola_set_particles first");
+    return kErrState;
+  }
+  return kOk;
+}
+
+int upload_power(cola_context* c)
+{
+  if (c->d_logk) (void)hipFree(c->d_logk);
+  if (c->d_logD) (void)hipFree(c->d_logD);
+  c->d_logk = c->d_logD = nullptr;
+  const size_t n = c->power.logk.size();
+  COLA_HIP(hipMalloc(&c->d_logk, sizeof(double) * n));
+  COLA_HIP(hipMalloc(&c->d_logD, sizeof(double) * n));
+  COLA_HIP(hipMemcpy(c->d_logk, c->power.logk.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  COLA_HIP(hipMemcpy(c->d_logD, c->power.logD.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  return kOk;
+}
+
+int finish_power_init(cola_context* c, double sigma8, double omega_m, double omega_lambda)
+{
+  c->pow_omega_m = omega_m;
+  c->pow_omega_lambda = omega_lambda;
+  if (c->power.logk.size() < 2) {
+    set_error("power spectrum table needs at least two rows");
+    return kErrPowerFile;
+  }
+  c->sigma8_input = c->power.sigma8();
+  if (std::fabs(c->sigma8_input - sigma8) / sigma8 > 0.05) {
+    set_error("Input sigma8 " + std::to_string(c->sigma8_input) + " is far from target sigma8 " +
+              std::to_string(sigma8));
+    return kErrSigma8;
+  }
+  return upload_power(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cola_last_error(void) { return last_error(); }
+
+int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
+{
+  if (!out || nc <= 0 || ngrid <= 0 || !(boxsize > 0) || nc % 2 || ngrid % 2) {
+    set_error("cola_context_create: nc and ngrid must be positive and even, boxsize > 0");
+    return kErrArg;
+  }
+  *out = nullptr;
+  cola_context* c = new cola_context();
+  c->nc = nc;
+  c->ng = ngrid;
+  c->boxsize = boxsize;
+  c->boxf = (float)boxsize;
+  c->np = (long long)nc * nc * nc;
+  c->wpar = (float)std::pow((double)ngrid / (double)nc, 3.0);
+  c->grid_floats = (long long)ngrid * ngrid * 2 * (ngrid / 2 + 1);
+  int st = kOk;
+  auto fail = [&](int s) {
+    cola_context_destroy(c);
+    return s;
+  };
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    return fail(kErrHip);
+  }
+  st = c->pm_fft.init(ngrid);
+  if (st) {
+    set_error("unsupported PM grid size " + std::to_string(ngrid) + " (needs factors 2, 3, 5)");
+    return fail(st);
+  }
+  st = c->lpt_fft.init(nc);
+  if (st) {
+    set_error("unsupported particle grid size " + std::to_string(nc) + " (needs factors 2, 3, 5)");
+    return fail(st);
+  }
+  c->pm_fft.hook = fft_hook;
+  c->pm_fft.hook_user = c;
+  c->lpt_fft.hook = lpt_fft_hook;
+  c->lpt_fft.hook_user = c;
+  c->cic = make_cic_geom(ngrid, c->boxf, c->wpar);
+  // default deposit tile 8 x 8 x 32 cells, shrunk to divisors of ngrid
+  auto pick = [&](int want) {
+    for (int t = want; t > 1; --t)
+      if (ngrid % t == 0) return t;
+    return 1;
+  };
+  if (!make_tile_geom(ngrid, pick(8), pick(8), pick(32), &c->tiles)) return fail(kErrArg);
+  const size_t pbytes = sizeof(float) * (size_t)c->np;
+  if (hipMalloc(&c->pbuf, pbytes * 15) != hipSuccess) {
+    set_error("cannot allocate particle state");
+    return fail(kErrAlloc);
+  }
+  c->p.np = c->np;
+  for (int a = 0; a < 3; ++a) {
+    c->p.x[a] = c->pbuf + (0 + a) * c->np;
+    c->p.v[a] = c->pbuf + (3 + a) * c->np;
+    c->p.dx1[a] = c->pbuf + (6 + a) * c->np;
+    c->p.dx2[a] = c->pbuf + (9 + a) * c->np;
+    c->p.f[a] = c->pbuf + (12 + a) * c->np;
+  }
+  st = bin_alloc(&c->bins, c->tiles, c->np);
+  if (st) return fail(kErrAlloc);
+  if (hipMalloc(&c->d_mass, sizeof(double) * cola_context::kMassSlots) != hipSuccess) return fail(kErrAlloc);
+  if (hipMemset(c->d_mass, 0, sizeof(double) * cola_context::kMassSlots) != hipSuccess) return fail(kErrHip);
+  *out = c;
+  return kOk;
+}
+
+int cola_context_destroy(cola_context* c)
+{
+  if (!c) return kOk;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (const EventPair& ep : c->events) {
+    (void)hipEventDestroy(ep.a);
+    (void)hipEventDestroy(ep.b);
+  }
+  for (hipEvent_t e : c->free_events) (void)hipEventDestroy(e);
+  if (c->open_event) (void)hipEventDestroy(c->open_event);
+  c->pm_fft.destroy();
+  c->lpt_fft.destroy();
+  bin_free(&c->bins);
+  if (c->pbuf) (void)hipFree(c->pbuf);
+  if (c->density) (void)hipFree(c->density);
+  for (int a = 0; a < 3; ++a)
+    if (c->fgrid[a]) (void)hipFree(c->fgrid[a]);
+  if (c->d_mass) (void)hipFree(c->d_mass);
+  if (c->d_logk) (void)hipFree(c->d_logk);
+  if (c->d_logD) (void)hipFree(c->d_logD);
+  if (c->d_snap) (void)hipFree(c->d_snap);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return kOk;
+}
+
+int cola_synchronize(cola_context* c)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  return check_pending(c);
+}
+
+int cola_power_init(cola_context* c, const char* filename, double a_init, double sigma8, double omega_m,
+                    double omega_lambda)
+{
+  (void)a_init;
+  if (!valid(c) || !filename) return kErrArg;
+  if (c->power.read(filename) != 0) {
+    set_error(std::string("Error: unable to read input power spectrum: ") + filename);
+    return kErrPowerFile;
+  }
+  return finish_power_init(c, sigma8, omega_m, omega_lambda);
+}
+
+int cola_power_init_table(cola_context* c, const double* k, const double* pk, int n, double a_init, double sigma8,
+                          double omega_m, double omega_lambda)
+{
+  (void)a_init;
+  if (!valid(c) || !k || !pk || n < 2) return kErrArg;
+  c->power.set(k, pk, n);
+  return finish_power_init(c, sigma8, omega_m, omega_lambda);
+}
+
+int cola_eh98_nowiggle_table(double omega_m, double omega_b, double h, double ns, double sigma8, double kmin,
+                             double kmax, int n, double* k_out, double* p_out)
+{
+  if (n < 2 || !k_out || !p_out || !(kmin > 0) || !(kmax > kmin) || !(omega_m > 0) || !(h > 0) || !(sigma8 > 0)) {
+    set_error("cola_eh98_nowiggle_table: invalid arguments");
+    return kErrArg;
+  }
+  eh98_nowiggle_table(omega_m, omega_b, h, ns, sigma8, kmin, kmax, n, k_out, p_out);
+  return kOk;
+}
+
+double cola_power_spec(const cola_context* c, double k)
+{
+  if (!c || c->power.empty()) return 0.0;
+  return c->power(k);
+}
+
+double cola_growth_factor(const cola_context* c, double astart, double aend)
+{
+  if (!c) return 0.0;
+  return growth_factor(astart, aend, c->pow_omega_m, c->pow_omega_lambda);
+}
+
+double cola_power_sigma8(const cola_context* c) { return c ? c->sigma8_input : 0.0; }
+
+int cola_lpt_set_displacement(cola_context* c, double InitTime, double omega_m, int seed, double box)
+{
+  if (!valid(c)) return kErrArg;
+  if (c->power.empty() || !c->d_logk) {
+    set_error("cola_lpt_set_displacement: call cola_power_init first");
+    return kErrState;
+  }
+  const int N = c->nc;
+  const long long nz = N / 2 + 1;
+  const long long csize = (long long)N * N * nz;  // complex elements per grid
+  float2* work = nullptr;
+  COLA_HIP(hipMalloc(&work, sizeof(float2) * csize * 9));
+  float2* cd[3];
+  float2* cg[6];
+  for (int a = 0; a < 3; ++a) cd[a] = work + a * csize;
+  for (int g = 0; g < 6; ++g) cg[g] = work + (3 + g) * csize;
+  auto rg = [](float2* p) { return reinterpret_cast<float*>(p); };
+
+  std::vector<unsigned int> table((size_t)N * N);
+  lpt_seedtable(N, seed, table.data());
+  unsigned int* d_table = nullptr;
+  int st = kOk;
+  auto done = [&](int s) {
+    (void)hipStreamSynchronize(c->stream);
+    if (d_table) (void)hipFree(d_table);
+    (void)hipFree(work);
+    return s;
+  };
+  if (hipMalloc(&d_table, sizeof(unsigned int) * table.size()) != hipSuccess) return done(kErrAlloc);
+  if (hipMemcpyAsync(d_table, table.data(), sizeof(unsigned int) * table.size(), hipMemcpyHostToDevice, c->stream) !=
+      hipSuccess)
+    return done(kErrHip);
+
+  DevPower pw;
+  pw.logk = c->d_logk;
+  pw.logD = c->d_logD;
+  pw.n = (int)c->power.logk.size();
+  pw.norm = c->power.norm;
+  const double fac = std::pow(2 * kPiD / box, 1.5);  // lpt.c:166
+  phase_begin(c, kPhLptModes);
+  st = launch_mode_fill(N, d_table, pw, box, fac, cd[0], cd[1], cd[2], c->stream);
+  phase_end(c, kPhLptModes);
+  if (st) return done(st);
+
+  // lpt.c:358-403: i k_j dis_i for (ij) = 00 01 02 11 12 22, c2r each
+  static const int src[6] = {0, 0, 0, 1, 1, 2};
+  static const int dir[6] = {0, 1, 2, 1, 2, 2};
+  KOpParams kp;
+  kp.box = box;
+  for (int g = 0; g < 6; ++g) {
+    kp.axis = dir[g];
+    st = c->lpt_fft.c2r(cd[src[g]], rg(cg[g]), KOp::LptGrad, kp, c->stream);
+    if (st) return done(st);
+  }
+  phase_begin(c, kPhLptOther);
+  st = launch_lpt_source(N, rg(cg[0]), rg(cg[1]), rg(cg[2]), rg(cg[3]), rg(cg[4]), rg(cg[5]), c->stream);
+  phase_end(c, kPhLptOther);
+  if (st) return done(st);
+  st = c->lpt_fft.r2c(rg(cg[3]), c->stream);
+  if (st) return done(st);
+  // lpt.c:433-503: second-order displacements into cdigrad[0..2], ZA in place
+  for (int a = 0; a < 3; ++a) {
+    kp.axis = a;
+    st = c->lpt_fft.c2r(cg[3], rg(cg[a]), KOp::Lpt2, kp, c->stream);
+    if (st) return done(st);
+  }
+  for (int a = 0; a < 3; ++a) {
+    st = c->lpt_fft.c2r(nullptr, rg(cd[a]), KOp::None, kp, c->stream);
+    if (st) return done(st);
+  }
+  // lpt.c:507-561
+  LptParticleFactors f;
+  const double Omega = omega_m;
+  f.dx = box / N;
+  f.Dplus = 1.0 / growth_factor(InitTime, 1.0, c->pow_omega_m, c->pow_omega_lambda);
+  f.nmesh3_inv = 1.0 / std::pow((double)N, 3.0);
+  const double a = InitTime;
+  const double omega = Omega / (Omega + (1.0 - Omega) * a * a * a);
+  f.D2 = (float)(f.Dplus * f.Dplus) * std::pow(omega / Omega, -1.0 / 143.0);
+  f.D20 = std::pow(Omega, -1.0 / 143.0);
+  const float* disp[3] = {rg(cd[0]), rg(cd[1]), rg(cd[2])};
+  const float* disp2[3] = {rg(cg[0]), rg(cg[1]), rg(cg[2])};
+  phase_begin(c, kPhLptOther);
+  st = launch_lpt_particles(N, disp, disp2, f, c->p, c->stream);
+  phase_end(c, kPhLptOther);
+  if (st) return done(st);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return done(kErrHip);
+  c->a_x = InitTime;
+  c->a_v = 0.0f;
+  c->have_particles = true;
+  c->have_force = false;
+  return done(kOk);
+}
+
+int cola_lpt_get_local_nx(const cola_context* c) { return c ? c->nc : 0; }
+
+int cola_move_particles(cola_context* c)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  phase_begin(c, kPhWrap);
+  COLA_TRY(launch_wrap(c->p, c->boxf, c->stream));
+  phase_end(c, kPhWrap);
+  return kOk;
+}
+
+int cola_pm_calculate_forces(cola_context* c)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  COLA_TRY(pm_assign(c, false));
+  // the reference checks the mass every step before its FFT (pm.c:574)
+  COLA_TRY(check_pending(c));
+  COLA_TRY(pm_fields(c));
+  phase_begin(c, kPhReadout);
+  COLA_TRY(launch_readout(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], c->stream));
+  phase_end(c, kPhReadout);
+  c->have_force = true;
+  return kOk;
+}
+
+int cola_pm_assign(cola_context* c)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  COLA_TRY(pm_assign(c, false));
+  return check_pending(c);
+}
+
+int cola_kick(cola_context* c, float omega_m, float avel1)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  const KickFactors k = kick_factors(omega_m, c->a_x, c->a_v, avel1);
+  phase_begin(c, kPhKick);
+  COLA_TRY(launch_kick(c->p, k, c->stream));
+  phase_end(c, kPhKick);
+  c->a_v = avel1;
+  return kOk;
+}
+
+int cola_drift(cola_context* c, float omega_m, float apos1)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  const DriftFactors d = drift_factors(omega_m, c->a_x, c->a_v, apos1);
+  phase_begin(c, kPhDrift);
+  COLA_TRY(launch_drift(c->p, d, c->stream));
+  phase_end(c, kPhDrift);
+  c->a_x = apos1;
+  return kOk;
+}
+
+int cola_step(cola_context* c, float omega_m, float avel1, float apos1)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  COLA_TRY(pm_assign(c, true));
+  COLA_TRY(pm_fields(c));
+  const KickFactors k = kick_factors(omega_m, c->a_x, c->a_v, avel1);
+  // the drift starts from a_v = avel1 (cola.c:124 after the kick)
+  const DriftFactors d = drift_factors(omega_m, c->a_x, avel1, apos1);
+  phase_begin(c, kPhReadoutKickDrift);
+  COLA_TRY(launch_readout_kick_drift(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], k, d, c->stream));
+  phase_end(c, kPhReadoutKickDrift);
+  c->a_v = avel1;
+  c->a_x = apos1;
+  c->have_force = false;
+  if (c->mass_pending >= cola_context::kMassSlots / 2) return check_pending(c);
+  return kOk;
+}
+
+static int snapshot_common(cola_context* c, const SnapshotFactors& f, bool noncola, cola_particle_min* out,
+                           size_t np_out)
+{
+  if (!out || np_out < (size_t)c->np) {
+    set_error("snapshot buffer too small");
+    return kErrArg;
+  }
+  if (!c->d_snap) COLA_HIP(hipMalloc(&c->d_snap, sizeof(DevParticleMin) * c->np));
+  phase_begin(c, kPhSnapshot);
+  if (noncola)
+    COLA_TRY(launch_noncola_initial(c->p, f, c->d_snap, c->stream));
+  else
+    COLA_TRY(launch_snapshot(c->p, f, c->d_snap, c->stream));
+  phase_end(c, kPhSnapshot);
+  static_assert(sizeof(DevParticleMin) == sizeof(cola_particle_min), "snapshot record layout");
+  COLA_HIP(hipMemcpyAsync(out, c->d_snap, sizeof(DevParticleMin) * c->np, hipMemcpyDeviceToHost, c->stream));
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  return check_pending(c);
+}
+
+int cola_set_snapshot(cola_context* c, double aout, float omega_m, cola_particle_min* out, size_t np_out)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  if (!c->have_force) {
+    set_error("cola_set_snapshot needs the force of the current step (cola_pm_calculate_forces)");
+    return kErrState;
+  }
+  return snapshot_common(c, snapshot_factors(omega_m, c->a_x, c->a_v, aout), false, out, np_out);
+}
+
+int cola_set_noncola_initial(cola_context* c, float omega_m, cola_particle_min* out, size_t np_out)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  return snapshot_common(c, noncola_initial_factors(omega_m, c->a_x), true, out, np_out);
+}
+
+long long cola_np(const cola_context* c) { return c ? c->np : 0; }
+
+int cola_get_times(const cola_context* c, float* a_x, float* a_v)
+{
+  if (!c) return kErrArg;
+  if (a_x) *a_x = c->a_x;
+  if (a_v) *a_v = c->a_v;
+  return kOk;
+}
+
+int cola_set_times(cola_context* c, float a_x, float a_v)
+{
+  if (!valid(c)) return kErrArg;
+  c->a_x = a_x;
+  c->a_v = a_v;
+  return kOk;
+}
+
+int cola_get_particles(cola_context* c, cola_particle* out, size_t np_out)
+{
+  if (!valid(c) || !out || np_out < (size_t)c->np) return kErrArg;
+  COLA_TRY(require_particles(c));
+  std::vector<float> h((size_t)c->np * 12);
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  COLA_HIP(hipMemcpy(h.data(), c->pbuf, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+  const long long n = c->np;
+  for (long long i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      out[i].x[a] = h[(0 + a) * n + i];
+      out[i].v[a] = h[(3 + a) * n + i];
+      out[i].dx1[a] = h[(6 + a) * n + i];
+      out[i].dx2[a] = h[(9 + a) * n + i];
+    }
+    out[i].id = i + 1;
+  }
+  return check_pending(c);
+}
+
+int cola_set_particles(cola_context* c, const cola_particle* in, size_t np_in, float a_x, float a_v)
+{
+  if (!valid(c) || !in || np_in != (size_t)c->np) {
+    set_error("cola_set_particles: need exactly nc^3 particles in lattice-ID order");
+    return kErrArg;
+  }
+  const long long n = c->np;
+  std::vector<float> h((size_t)n * 12);
+  for (long long i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      h[(0 + a) * n + i] = in[i].x[a];
+      h[(3 + a) * n + i] = in[i].v[a];
+      h[(6 + a) * n + i] = in[i].dx1[a];
+      h[(9 + a) * n + i] = in[i].dx2[a];
+    }
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  COLA_HIP(hipMemcpy(c->pbuf, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  c->a_x = a_x;
+  c->a_v = a_v;
+  c->have_particles = true;
+  c->have_force = false;
+  return kOk;
+}
+
+int cola_get_forces(cola_context* c, float* out, size_t np_out)
+{
+  if (!valid(c) || !out || np_out < (size_t)c->np) return kErrArg;
+  if (!c->have_force) {
+    set_error("no force computed for the current state");
+    return kErrState;
+  }
+  const long long n = c->np;
+  std::vector<float> h((size_t)n * 3);
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  COLA_HIP(hipMemcpy(h.data(), c->p.f[0], sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+  for (long long i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) out[3 * i + a] = h[a * n + i];
+  return kOk;
+}
+
+int cola_get_density(cola_context* c, float* out, size_t n_out)
+{
+  if (!valid(c) || !out || n_out < (size_t)c->grid_floats) return kErrArg;
+  if (!c->have_grids || !c->density_is_real) {
+    set_error("the density grid is transformed in place by the force; call cola_pm_assign first");
+    return kErrState;
+  }
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  COLA_HIP(hipMemcpy(out, c->density, sizeof(float) * c->grid_floats, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+int cola_get_force_grid(cola_context* c, int axis, float* out, size_t n_out)
+{
+  if (!valid(c) || !out || axis < 0 || axis > 2 || n_out < (size_t)c->grid_floats) return kErrArg;
+  if (!c->have_grids) return kErrState;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  COLA_HIP(hipMemcpy(out, c->fgrid[axis], sizeof(float) * c->grid_floats, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+double cola_last_mass_sum(const cola_context* c) { return c ? c->last_mass : 0.0; }
+
+int cola_set_profiling(cola_context* c, int on)
+{
+  if (!valid(c)) return kErrArg;
+  c->profiling = on != 0;
+  return kOk;
+}
+
+int cola_phase_count(void) { return kNumPhases; }
+
+const char* cola_phase_name(int phase)
+{
+  if (phase < 0 || phase >= kNumPhases) return "";
+  return kPhaseNames[phase];
+}
+
+int cola_get_phase_stats(cola_context* c, double* ms, long long* launches, int n)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(collect_events(c));
+  for (int i = 0; i < n && i < kNumPhases; ++i) {
+    if (ms) ms[i] = c->phase_ms[i];
+    if (launches) launches[i] = c->phase_n[i];
+  }
+  return kOk;
+}
+
+int cola_reset_phase_stats(cola_context* c)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(collect_events(c));
+  for (int i = 0; i < kNumPhases; ++i) {
+    c->phase_ms[i] = 0;
+    c->phase_n[i] = 0;
+  }
+  return kOk;
+}
+
+int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
+{
+  if (!valid(c)) return kErrArg;
+  TileGeom t;
+  if (!make_tile_geom(c->ng, tx, ty, tz, &t)) {
+    set_error("deposit tile must divide ngrid and hold <= 4096 cells");
+    return kErrArg;
+  }
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  COLA_TRY(check_pending(c));
+  BinBuffers nb;
+  COLA_TRY(bin_alloc(&nb, t, c->np));
+  bin_free(&c->bins);
+  c->bins = nb;
+  c->tiles = t;
+  if (fft_log2_tile != c->fft_log2_tile) {
+    FftPlan a, b;
+    COLA_TRY(a.init(c->ng, fft_log2_tile));
+    if (b.init(c->nc, fft_log2_tile) != kOk) {
+      a.destroy();
+      return kErrUnsupported;
+    }
+    c->pm_fft.destroy();
+    c->lpt_fft.destroy();
+    c->pm_fft = a;
+    c->lpt_fft = b;
+    c->pm_fft.hook = fft_hook;
+    c->pm_fft.hook_user = c;
+    c->lpt_fft.hook = lpt_fft_hook;
+    c->lpt_fft.hook_user = c;
+    c->fft_log2_tile = fft_log2_tile;
+  }
+  return kOk;
+}
+
+static int standalone_fft(int n, float* data, bool forward)
+{
+  if (n <= 0 || n % 2 || !data) return kErrArg;
+  FftPlan plan;
+  COLA_TRY(plan.init(n));
+  const size_t floats = (size_t)n * n * 2 * (n / 2 + 1);
+  float* d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * floats) != hipSuccess) {
+    plan.destroy();
+    return kErrAlloc;
+  }
+  int st = hipMemcpy(d, data, sizeof(float) * floats, hipMemcpyHostToDevice) == hipSuccess ? kOk : kErrHip;
+  if (!st) st = forward ? plan.r2c(d, nullptr) : plan.c2r(nullptr, d, KOp::None, KOpParams{}, nullptr);
+  if (!st && hipDeviceSynchronize() != hipSuccess) st = kErrHip;
+  if (!st && hipMemcpy(data, d, sizeof(float) * floats, hipMemcpyDeviceToHost) != hipSuccess) st = kErrHip;
+  (void)hipFree(d);
+  plan.destroy();
+  return st;
+}
+
+int cola_fft_r2c_3d(int n, float* data) { return standalone_fft(n, data, true); }
+int cola_fft_c2r_3d(int n, float* data) { return standalone_fft(n, data, false); }
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Shared definitions of the HIP library: error plumbing and small device
+// helpers.  All device code is compiled with -ffp-contract=off so every float
+// and double operation rounds exactly where the reference's C code rounds.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+namespace cola {
+
+// Status codes reuse the reference's msg_abort codes (see cola_halo_amd.h).
+enum Status : int {
+  kOk = 0,
+  kErrAlloc = 4001,
+  kErrPowerFile = 3000,
+  kErrSigma8 = 3010,
+  kErrMass = 6040,
+  kErrArg = 9001,
+  kErrUnsupported = 9002,
+  kErrHip = 9003,
+  kErrState = 9004,
+};
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+// Return a status from the enclosing function if a HIP call fails.
+#define COLA_HIP(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::cola::set_error(std::string(#call) + ": " + hipGetErrorString(e_));             \
+      return ::cola::kErrHip;                                                           \
+    }                                                                                   \
+  } while (0)
+
+#define COLA_TRY(expr)         \
+  do {                         \
+    int s_ = (expr);           \
+    if (s_ != 0) return s_;    \
+  } while (0)
+
+__host__ __device__ inline float2 cmul(float2 a, float2 b)
+{
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+constexpr double kPiD = 3.14159265358979323846;
+
+}  // namespace cola

@@ -1,0 +1,87 @@
+// Particle-mesh force kernels (pm.c) and the COLA particle kernels (cola.c,
+// move.c wrap) on structure-of-arrays particle state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cosmo.h"
+
+namespace cola {
+
+// Device particle state, one float array per component (lattice-ID order;
+// id = index + 1, lpt.c:516,557).
+struct DevParticles {
+  long long np = 0;
+  float* x[3] = {nullptr, nullptr, nullptr};
+  float* v[3] = {nullptr, nullptr, nullptr};
+  float* dx1[3] = {nullptr, nullptr, nullptr};
+  float* dx2[3] = {nullptr, nullptr, nullptr};
+  float* f[3] = {nullptr, nullptr, nullptr};
+};
+
+// CIC geometry of pm.c on one rank.
+struct CicGeom {
+  int ng = 0;
+  float scale = 0.0f;    // pm.c:215  Ngrid / BoxSize
+  float wpar = 0.0f;     // pm.c:216  (Ngrid / nc)^3
+  float x_right = 0.0f;  // pm.c:422-423  ghost threshold L/Ng*Ng - L/Ng
+  float shift = 0.0f;    // pm.c:434  -BoxSize
+  long long row = 0;     // padded row length 2(ng/2+1)
+};
+CicGeom make_cic_geom(int ng, float boxsize, float wpar);
+
+// Tiling of the density grid for the binned deposit.
+struct TileGeom {
+  int tx = 8, ty = 8, tz = 32;
+  int ntx = 0, nty = 0, ntz = 0;
+  long long ntiles = 0;
+};
+bool make_tile_geom(int ng, int tx, int ty, int tz, TileGeom* t);
+
+struct BinBuffers {
+  int* counts = nullptr;       // ntiles, zero between steps
+  int* offsets = nullptr;      // ntiles + 1
+  int* block_sums = nullptr;   // scan scratch
+  float* sorted = nullptr;     // 3 floats per entry
+  long long capacity = 0;      // entries
+  int* overflow = nullptr;     // set when an index fell outside the grid or capacity
+};
+
+int bin_alloc(BinBuffers* b, const TileGeom& t, long long np);
+void bin_free(BinBuffers* b);
+
+// move.c:33-41
+int launch_wrap(const DevParticles& p, float boxsize, hipStream_t s);
+// Counting sort of particle positions into every tile their CIC stencil (or
+// their ghost copy's stencil) touches.  wrap_first folds move.c's wrap into
+// the counting pass.  Three launches: count, scan, scatter.
+int launch_bin_count(const DevParticles& p, const CicGeom& g, const TileGeom& t, BinBuffers& b, bool wrap_first,
+                     float boxsize, hipStream_t s);
+int launch_bin_scan(const TileGeom& t, BinBuffers& b, hipStream_t s);
+int launch_bin_scatter(const DevParticles& p, const CicGeom& g, const TileGeom& t, BinBuffers& b, hipStream_t s);
+// LDS-accumulated deposit, one workgroup per tile, no global atomics:
+// density = -1 + WPAR * sum of CIC weights (pm.c:209-279).  mass_sum (zeroed
+// by the caller) receives the double sum over the grid (pm.c:535-562).
+int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& b, float* density, double* mass_sum,
+                         hipStream_t s);
+// pm.c:359-412 + 505-532: readout of the three force grids into p.f.
+int launch_readout(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy, const float* fz,
+                   hipStream_t s);
+// Readout fused with cola.c:104-112 kick and cola.c:142-149 drift.
+int launch_readout_kick_drift(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy,
+                              const float* fz, const KickFactors& k, const DriftFactors& d, hipStream_t s);
+int launch_kick(const DevParticles& p, const KickFactors& k, hipStream_t s);
+int launch_drift(const DevParticles& p, const DriftFactors& d, hipStream_t s);
+
+// Snapshot records (cola.c:388-410) / non-COLA initial velocities (cola.c:321-331)
+struct DevParticleMin {
+  float x[3];
+  float v[3];
+  long long id;
+};
+int launch_snapshot(const DevParticles& p, const SnapshotFactors& f, DevParticleMin* out, hipStream_t s);
+int launch_noncola_initial(const DevParticles& p, const SnapshotFactors& f, DevParticleMin* out, hipStream_t s);
+
+int launch_zero_doubles(double* p, int n, hipStream_t s);
+
+}  // namespace cola

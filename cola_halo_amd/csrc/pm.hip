@@ -1,0 +1,614 @@
+// PM deposit / readout and COLA particle kernels.
+//
+// Deposit design: the reference scatters 8 CIC weights per particle with
+// atomics into a grid initialised to -1 (pm.c:209-279).  Here each step
+// counting-sorts particle positions into every density tile their stencil
+// touches (a particle whose stencil crosses a tile face is listed in both
+// tiles), then one workgroup per tile accumulates its particles' weights in
+// LDS and writes every cell of the tile exactly once with coalesced stores.
+// No global atomics touch the grid, and the -1 initialisation costs no extra
+// pass.  The reference's x periodicity through ghost copies
+// (pm.c:416-502, 505-532) is reproduced exactly: a particle with
+// x > L - L/Ng also deposits (and reads back) through a copy at x - L whose
+// weights are recomputed from that shifted position.
+#include "common.h"
+#include "pm.h"
+
+#include <algorithm>
+
+namespace cola {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kScanChunk = 2048;  // counts per scan workgroup (8 per thread)
+
+
+// The 8 CIC cells and weights of one position (pm.c:229-276).  fn(ix, iy, iz, w)
+// is called for every cell whose x layer lies in [0, ng); y and z wrap.
+// Returns false if y or z fall below -1 cell, where the reference would index
+// outside its grid; nothing is emitted then.
+template <class Fn>
+__device__ inline bool cic_cells(float x, float y, float z, const CicGeom& g, Fn&& fn)
+{
+  const int ng = g.ng;
+  const float X = x * g.scale;
+  const float Y = y * g.scale;
+  const float Z = z * g.scale;
+  const int iI = (int)floorf(X);
+  int J = (int)Y;
+  int K = (int)Z;
+  const float D1 = X - ((float)iI);
+  const float D2 = Y - ((float)J);
+  const float D3 = Z - ((float)K);
+  const float T1 = 1.0f - D1;
+  const float T2 = 1.0f - D2;
+  const float T3 = 1.0f - D3;
+  const float T2W = T2 * g.wpar;
+  const float D2W = D2 * g.wpar;
+  if (J < 0 || K < 0) return false;
+  if (J >= ng) J = 0;
+  if (K >= ng) K = 0;
+  const int I1 = iI + 1;
+  int J1 = J + 1;
+  if (J1 >= ng) J1 = 0;
+  int K1 = K + 1;
+  if (K1 >= ng) K1 = 0;
+  if (0 <= iI && iI < ng) {
+    fn(iI, J, K, T3 * T1 * T2W);
+    fn(iI, J, K1, D3 * T1 * T2W);
+    fn(iI, J1, K, T3 * T1 * D2W);
+    fn(iI, J1, K1, D3 * T1 * D2W);
+  }
+  if (0 <= I1 && I1 < ng) {
+    fn(I1, J, K, T3 * D1 * T2W);
+    fn(I1, J, K1, D3 * D1 * T2W);
+    fn(I1, J1, K, T3 * D1 * D2W);
+    fn(I1, J1, K1, D3 * D1 * D2W);
+  }
+  return true;
+}
+
+// The particle and, beyond the ghost threshold, its copy at x - L.
+template <class Fn>
+__device__ inline bool cic_with_ghost(float x, float y, float z, const CicGeom& g, Fn&& fn)
+{
+  bool ok = cic_cells(x, y, z, g, fn);
+  if (x > g.x_right) ok = cic_cells(x + g.shift, y, z, g, fn) && ok;
+  return ok;
+}
+
+// Distinct tiles touched by a particle's cells (<= 2 per axis).
+struct TileSet {
+  int x[2], y[2], z[2];
+  int nx = 0, ny = 0, nz = 0;
+  __device__ inline void add(int* a, int& n, int v)
+  {
+    for (int i = 0; i < n; ++i)
+      if (a[i] == v) return;
+    if (n < 2) a[n++] = v;
+  }
+};
+
+__device__ inline bool tiles_of(float x, float y, float z, const CicGeom& g, const TileGeom& t, TileSet& s)
+{
+  return cic_with_ghost(x, y, z, g, [&](int i, int j, int k, float) {
+    s.add(s.x, s.nx, i / t.tx);
+    s.add(s.y, s.ny, j / t.ty);
+    s.add(s.z, s.nz, k / t.tz);
+  });
+}
+
+__device__ inline void wrap1(float& x, float L)
+{
+  if (x < 0.0f)
+    x += L;
+  else if (x >= L)
+    x -= L;
+}
+
+__global__ void __launch_bounds__(kThreads) k_wrap(DevParticles p, float L)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float x = p.x[a][i];
+    wrap1(x, L);
+    p.x[a][i] = x;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
+            float L)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+  float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
+  if (wrap_first) {
+    const float x0 = x, y0 = y, z0 = z;
+    wrap1(x, L);
+    wrap1(y, L);
+    wrap1(z, L);
+    if (x != x0) p.x[0][i] = x;
+    if (y != y0) p.x[1][i] = y;
+    if (z != z0) p.x[2][i] = z;
+  }
+  TileSet s;
+  if (!tiles_of(x, y, z, g, t, s)) atomicOr(overflow, 1);
+  for (int a = 0; a < s.nx; ++a)
+    for (int b = 0; b < s.ny; ++b)
+      for (int c = 0; c < s.nz; ++c) atomicAdd(&counts[((long long)s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c]], 1);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, const int* __restrict__ offsets,
+              float* __restrict__ sorted, long long capacity, int* __restrict__ overflow)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+  const float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
+  TileSet s;
+  tiles_of(x, y, z, g, t, s);
+  for (int a = 0; a < s.nx; ++a)
+    for (int b = 0; b < s.ny; ++b)
+      for (int c = 0; c < s.nz; ++c) {
+        const long long tile = ((long long)s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c];
+        // counts return to zero for the next step as the slots are claimed
+        const long long pos = (long long)offsets[tile] + atomicSub(&counts[tile], 1) - 1;
+        if (pos < capacity) {
+          sorted[3 * pos + 0] = x;
+          sorted[3 * pos + 1] = y;
+          sorted[3 * pos + 2] = z;
+        } else {
+          atomicOr(overflow, 2);
+        }
+      }
+}
+
+// ---- exclusive scan of the tile counts -------------------------------------
+__global__ void __launch_bounds__(kThreads) k_scan_reduce(const int* __restrict__ in, long long n, int* __restrict__ sums)
+{
+  __shared__ int red[kThreads];
+  const long long base = (long long)blockIdx.x * kScanChunk;
+  int s = 0;
+  for (int k = threadIdx.x; k < kScanChunk; k += kThreads) {
+    const long long i = base + k;
+    if (i < n) s += in[i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__device__ inline int block_exclusive_scan(int v, int* sh, int* total)
+{
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < kThreads; off <<= 1) {
+    const int add = tid >= off ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += add;
+    __syncthreads();
+  }
+  const int incl = sh[tid];
+  *total = sh[kThreads - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+// single workgroup: exclusive scan of the chunk sums in place
+__global__ void __launch_bounds__(kThreads) k_scan_top(int* __restrict__ sums, int nsums)
+{
+  __shared__ int sh[kThreads];
+  int carry = 0;
+  for (int base = 0; base < nsums; base += kThreads) {
+    const int i = base + threadIdx.x;
+    const int v = i < nsums ? sums[i] : 0;
+    int total;
+    const int ex = block_exclusive_scan(v, sh, &total);
+    if (i < nsums) sums[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_scan_final(const int* __restrict__ in, long long n, const int* __restrict__ sums, int* __restrict__ out)
+{
+  __shared__ int sh[kThreads];
+  const long long base = (long long)blockIdx.x * kScanChunk;
+  constexpr int per = kScanChunk / kThreads;
+  int v[per];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < per; ++k) {
+    const long long i = base + (long long)threadIdx.x * per + k;
+    v[k] = i < n ? in[i] : 0;
+    s += v[k];
+  }
+  int total;
+  int run = sums[blockIdx.x] + block_exclusive_scan(s, sh, &total);
+#pragma unroll
+  for (int k = 0; k < per; ++k) {
+    const long long i = base + (long long)threadIdx.x * per + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+    if (i == n - 1) out[n] = run;
+  }
+}
+
+// ---- deposit: one workgroup per tile ----------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const float* __restrict__ sorted,
+                float* __restrict__ density, double* __restrict__ mass_sum)
+{
+  extern __shared__ float cells[];
+  __shared__ double red[kThreads];
+  const long long tile = blockIdx.x;
+  const int tz = (int)(tile % t.ntz);
+  const int ty = (int)((tile / t.ntz) % t.nty);
+  const int tx = (int)(tile / ((long long)t.ntz * t.nty));
+  const int x0 = tx * t.tx, y0 = ty * t.ty, z0 = tz * t.tz;
+  const int ncell = t.tx * t.ty * t.tz;
+  for (int e = threadIdx.x; e < ncell; e += kThreads) cells[e] = -1.0f;
+  __syncthreads();
+  const int beg = offsets[tile], end = offsets[tile + 1];
+  for (int k = beg + threadIdx.x; k < end; k += kThreads) {
+    const float x = sorted[3LL * k + 0], y = sorted[3LL * k + 1], z = sorted[3LL * k + 2];
+    cic_with_ghost(x, y, z, g, [&](int i, int j, int kk, float w) {
+      const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0), lk = (unsigned)(kk - z0);
+      if (li < (unsigned)t.tx && lj < (unsigned)t.ty && lk < (unsigned)t.tz)
+        atomicAdd(&cells[(li * t.ty + lj) * t.tz + lk], w);
+    });
+  }
+  __syncthreads();
+  double s = 0.0;
+  for (int e = threadIdx.x; e < ncell; e += kThreads) {
+    const int lk = e % t.tz;
+    const int lj = (e / t.tz) % t.ty;
+    const int li = e / (t.tz * t.ty);
+    const float v = cells[e];
+    density[((long long)(x0 + li) * g.ng + (y0 + lj)) * g.row + (z0 + lk)] = v;
+    s += (double)v;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(mass_sum, red[0]);
+}
+
+// ---- readout (pm.c:359-412) -------------------------------------------------
+// One axis of the CIC gather at one position, term order of pm.c:397-410.
+struct Gather3 {
+  float f[3];
+};
+
+__device__ inline Gather3 cic_gather(float x, float y, float z, const CicGeom& g, const float* __restrict__ g0,
+                                     const float* __restrict__ g1, const float* __restrict__ g2)
+{
+  const int ng = g.ng;
+  const float X = x * g.scale;
+  const float Y = y * g.scale;
+  const float Z = z * g.scale;
+  int iI = (int)floorf(X);
+  int J = (int)Y;
+  int K = (int)Z;
+  const float D1 = X - (float)iI;
+  const float D2 = Y - (float)J;
+  const float D3 = Z - (float)K;
+  const float T1 = 1.0f - D1;
+  const float T2 = 1.0f - D2;
+  const float T3 = 1.0f - D3;
+  Gather3 r;
+  r.f[0] = r.f[1] = r.f[2] = 0.0f;
+  if (J < 0 || K < 0) return r;
+  if (J >= ng) J = 0;
+  if (K >= ng) K = 0;
+  const int I1 = iI + 1;
+  int J1 = J + 1;
+  if (J1 >= ng) J1 = 0;
+  int K1 = K + 1;
+  if (K1 >= ng) K1 = 0;
+  const float* grids[3] = {g0, g1, g2};
+  const long long row = g.row;
+  if (0 <= iI && iI < ng) {
+    const long long a = ((long long)iI * ng + J) * row, b = ((long long)iI * ng + J1) * row;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const float* m = grids[ax];
+      r.f[ax] += m[a + K] * T3 * T1 * T2 + m[a + K1] * D3 * T1 * T2 + m[b + K] * T3 * T1 * D2 + m[b + K1] * D3 * T1 * D2;
+    }
+  }
+  if (0 <= I1 && I1 < ng) {
+    const long long a = ((long long)I1 * ng + J) * row, b = ((long long)I1 * ng + J1) * row;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const float* m = grids[ax];
+      r.f[ax] += m[a + K] * T3 * D1 * T2 + m[a + K1] * D3 * D1 * T2 + m[b + K] * T3 * D1 * D2 + m[b + K1] * D3 * D1 * D2;
+    }
+  }
+  return r;
+}
+
+// Force at a particle: its own gather plus, beyond the ghost threshold, the
+// gather at its ghost position added afterwards (pm.c:526-531).
+__device__ inline Gather3 particle_force(float x, float y, float z, const CicGeom& g, const float* g0, const float* g1,
+                                         const float* g2)
+{
+  Gather3 f = cic_gather(x, y, z, g, g0, g1, g2);
+  if (x > g.x_right) {
+    const Gather3 h = cic_gather(x + g.shift, y, z, g, g0, g1, g2);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f.f[a] += h.f[a];
+  }
+  return f;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_readout(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
+          const float* __restrict__ g2)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+  const Gather3 f = particle_force(p.x[0][i], p.x[1][i], p.x[2][i], g, g0, g1, g2);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p.f[a][i] = f.f[a];
+}
+
+// cola.c:105-111 for one component
+__device__ inline float kick1(float v, float f, float dx1, float dx2, const KickFactors& k)
+{
+  const float ax = (float)(k.m15om * (double)f - (double)(1.0f * (dx1 * k.q1 + dx2 * k.q2)));
+  return v + ax * k.dda;
+}
+
+// cola.c:143-148 for one component
+__device__ inline float drift1(float x, float v, float dx1, float dx2, const DriftFactors& d)
+{
+  return x + (v * d.dyyy + 1.0f * (dx1 * d.da1 + dx2 * d.da2));
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
+                     const float* __restrict__ g2, KickFactors kf, DriftFactors df)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+  float x[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) x[a] = p.x[a][i];
+  const Gather3 f = particle_force(x[0], x[1], x[2], g, g0, g1, g2);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float d1 = p.dx1[a][i], d2 = p.dx2[a][i];
+    const float v = kick1(p.v[a][i], f.f[a], d1, d2, kf);
+    p.v[a][i] = v;
+    p.x[a][i] = drift1(x[a], v, d1, d2, df);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_kick(DevParticles p, KickFactors kf)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p.v[a][i] = kick1(p.v[a][i], p.f[a][i], p.dx1[a][i], p.dx2[a][i], kf);
+}
+
+__global__ void __launch_bounds__(kThreads) k_drift(DevParticles p, DriftFactors df)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) p.x[a][i] = drift1(p.x[a][i], p.v[a][i], p.dx1[a][i], p.dx2[a][i], df);
+}
+
+// cola.c:388-410
+__global__ void __launch_bounds__(kThreads) k_snapshot(DevParticles p, SnapshotFactors sf, DevParticleMin* out)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+  DevParticleMin o;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float x = p.x[a][i], v = p.v[a][i], d1 = p.dx1[a][i], d2 = p.dx2[a][i], f = p.f[a][i];
+    const float ax = (float)(sf.kick.m15om * (double)f - (double)(1.0f * (d1 * sf.kick.q1 + d2 * sf.kick.q2)));
+    o.v[a] = sf.vfac * (v + ax * sf.kick.dda + (d1 * sf.Dv + d2 * sf.Dv2) * 1.0f);
+    o.x[a] = x + v * sf.drift.dyyy + 1.0f * (d1 * sf.drift.da1 + d2 * sf.drift.da2);
+  }
+  o.id = i + 1;
+  out[i] = o;
+}
+
+// cola.c:321-330
+__global__ void __launch_bounds__(kThreads) k_noncola_initial(DevParticles p, SnapshotFactors sf, DevParticleMin* out)
+{
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= p.np) return;
+  DevParticleMin o;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    o.v[a] = sf.vfac * (p.dx1[a][i] * sf.Dv + p.dx2[a][i] * sf.Dv2);
+    o.x[a] = p.x[a][i];
+  }
+  o.id = i + 1;
+  out[i] = o;
+}
+
+__global__ void k_zero_doubles(double* p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+}  // namespace
+
+CicGeom make_cic_geom(int ng, float boxsize, float wpar)
+{
+  CicGeom g;
+  g.ng = ng;
+  g.scale = ((float)ng) / ((float)boxsize);
+  g.wpar = wpar;
+  const float eps = boxsize / ng;
+  g.x_right = boxsize / ng * (0 + ng) - eps;
+  g.shift = -boxsize;
+  g.row = 2LL * (ng / 2 + 1);
+  return g;
+}
+
+bool make_tile_geom(int ng, int tx, int ty, int tz, TileGeom* t)
+{
+  if (tx <= 0 || ty <= 0 || tz <= 0 || ng % tx || ng % ty || ng % tz) return false;
+  if ((long long)tx * ty * tz > 4096) return false;
+  t->tx = tx;
+  t->ty = ty;
+  t->tz = tz;
+  t->ntx = ng / tx;
+  t->nty = ng / ty;
+  t->ntz = ng / tz;
+  t->ntiles = (long long)t->ntx * t->nty * t->ntz;
+  return true;
+}
+
+int bin_alloc(BinBuffers* b, const TileGeom& t, long long np)
+{
+  bin_free(b);
+  const long long nsums = (t.ntiles + kScanChunk - 1) / kScanChunk;
+  b->capacity = 8 * np + 1024;
+  COLA_HIP(hipMalloc(&b->counts, sizeof(int) * t.ntiles));
+  COLA_HIP(hipMalloc(&b->offsets, sizeof(int) * (t.ntiles + 1)));
+  COLA_HIP(hipMalloc(&b->block_sums, sizeof(int) * (nsums + 1)));
+  COLA_HIP(hipMalloc(&b->sorted, sizeof(float) * 3 * b->capacity));
+  COLA_HIP(hipMalloc(&b->overflow, sizeof(int)));
+  COLA_HIP(hipMemset(b->counts, 0, sizeof(int) * t.ntiles));
+  COLA_HIP(hipMemset(b->overflow, 0, sizeof(int)));
+  return kOk;
+}
+
+void bin_free(BinBuffers* b)
+{
+  if (b->counts) (void)hipFree(b->counts);
+  if (b->offsets) (void)hipFree(b->offsets);
+  if (b->block_sums) (void)hipFree(b->block_sums);
+  if (b->sorted) (void)hipFree(b->sorted);
+  if (b->overflow) (void)hipFree(b->overflow);
+  *b = BinBuffers{};
+}
+
+int launch_wrap(const DevParticles& p, float boxsize, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_wrap, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, boxsize);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_bin_count(const DevParticles& p, const CicGeom& g, const TileGeom& t, BinBuffers& b, bool wrap_first,
+                     float boxsize, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, t, b.counts, b.overflow,
+                     wrap_first ? 1 : 0, boxsize);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_bin_scan(const TileGeom& t, BinBuffers& b, hipStream_t s)
+{
+  const long long n = t.ntiles;
+  const int nsums = (int)((n + kScanChunk - 1) / kScanChunk);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nsums), dim3(kThreads), 0, s, b.counts, n, b.block_sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kThreads), 0, s, b.block_sums, nsums);
+  hipLaunchKernelGGL(k_scan_final, dim3(nsums), dim3(kThreads), 0, s, b.counts, n, b.block_sums, b.offsets);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_bin_scatter(const DevParticles& p, const CicGeom& g, const TileGeom& t, BinBuffers& b, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, t, b.counts, b.offsets,
+                     b.sorted, b.capacity, b.overflow);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& b, float* density, double* mass_sum,
+                         hipStream_t s)
+{
+  const size_t lds = sizeof(float) * t.tx * t.ty * t.tz;
+  hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)t.ntiles), dim3(kThreads), lds, s, g, t, b.offsets, b.sorted,
+                     density, mass_sum);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_readout(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy, const float* fz,
+                   hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_readout, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_readout_kick_drift(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy,
+                              const float* fz, const KickFactors& k, const DriftFactors& d, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_readout_kick_drift, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz, k, d);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_kick(const DevParticles& p, const KickFactors& k, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_kick, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, k);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_drift(const DevParticles& p, const DriftFactors& d, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_drift, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, d);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_snapshot(const DevParticles& p, const SnapshotFactors& f, DevParticleMin* out, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_snapshot, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, f, out);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_noncola_initial(const DevParticles& p, const SnapshotFactors& f, DevParticleMin* out, hipStream_t s)
+{
+  if (p.np == 0) return kOk;
+  hipLaunchKernelGGL(k_noncola_initial, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, f, out);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_zero_doubles(double* p, int n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_zero_doubles, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace cola

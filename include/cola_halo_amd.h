@@ -1,0 +1,169 @@
+/*
+ * cola_halo_amd -- MI355X-native 2LPT / particle-mesh / COLA time stepping.
+ *
+ * C ABI of the HIP library libcola_halo_amd.so.  Plain pointers and sizes
+ * only.  The library owns all device state; host buffers cross the boundary
+ * only where the reference hands host arrays around (snapshot output,
+ * particle state for checkpoints and tests).
+ *
+ * Each entry point names the reference interface it replaces
+ * (junkoda/cola_halo, file:line).  The reference reports fatal errors with
+ * msg_abort(code) -> MPI_Abort (msg.c:36-45); here every entry point returns
+ * that same numeric code instead (0 = success) and cola_last_error() gives
+ * the message.  One calling thread per context; a context is not re-entrant.
+ */
+#ifndef COLA_HALO_AMD_H
+#define COLA_HALO_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* particle.h:6-12 -- 56-byte particle record (AoS, host side only) */
+typedef struct {
+  float x[3];
+  float dx1[3]; /* ZA displacement extrapolated to a=1 */
+  float dx2[3]; /* 2LPT displacement extrapolated to a=1 */
+  float v[3];   /* COLA-frame velocity */
+  long long id;
+} cola_particle;
+
+/* particle.h:24-28 -- 32-byte snapshot record */
+typedef struct {
+  float x[3];
+  float v[3]; /* km/s */
+  long long id;
+} cola_particle_min;
+
+typedef struct cola_context cola_context;
+
+/* Status codes reused from the reference's msg_abort codes. */
+enum {
+  COLA_OK = 0,
+  COLA_ERR_ALLOC = 4001,        /* pm.c:129-131 */
+  COLA_ERR_POWER_FILE = 3000,   /* power.c:68-71 */
+  COLA_ERR_SIGMA8 = 3010,       /* power.c:111-113 */
+  COLA_ERR_MASS = 6040,         /* pm.c:551-553 */
+  COLA_ERR_ARG = 9001,          /* invalid argument (no reference counterpart) */
+  COLA_ERR_UNSUPPORTED = 9002,  /* grid size with a prime factor > 7 */
+  COLA_ERR_HIP = 9003,          /* HIP runtime error */
+  COLA_ERR_STATE = 9004         /* call out of order (e.g. force before ICs) */
+};
+
+/* Last error message of the calling thread ("" if none). */
+const char* cola_last_error(void);
+
+/* Context: replaces comm_init (comm.h:8, called main.c:65), pm_init
+   (pm.h:6-9, main.c:100), lpt_init (lpt.h:6, main.c:86),
+   allocate_particles / allocate_shared_memory (mem.h:12-15, main.c:85-92).
+   nc particles per dimension, ngrid PM cells per dimension (ngrid/nc need not
+   be an integer: mass per particle is (ngrid/nc)^3, pm.c:216 generalised). */
+int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out);
+int cola_context_destroy(cola_context* ctx);
+int cola_synchronize(cola_context* ctx);
+
+/* power.c:36-60 power_init: reads a (k, P(k)) table, checks sigma8 within 5%
+   (returns 3010 beyond) and keeps Norm = 1. */
+int cola_power_init(cola_context* ctx, const char* filename, double a_init, double sigma8,
+                    double omega_m, double omega_lambda);
+/* Same, from arrays already in memory. */
+int cola_power_init_table(cola_context* ctx, const double* k, const double* pk, int n,
+                          double a_init, double sigma8, double omega_m, double omega_lambda);
+/* power.c:120-151 PowerSpec */
+double cola_power_spec(const cola_context* ctx, double k);
+/* power.c:199-202 GrowthFactor */
+double cola_growth_factor(const cola_context* ctx, double astart, double aend);
+/* sigma8 of the loaded table by power.c:103-118's integral */
+double cola_power_sigma8(const cola_context* ctx);
+
+/* Linear P(k) table for the inputs of the configurations: Eisenstein & Hu
+   (1998, ApJ 496, 605) no-wiggle transfer function (eqs. 26, 28-31, T_cmb =
+   2.728 K) times k^ns, tabulated at n log-spaced k in [kmin, kmax] h/Mpc and
+   normalised so that sigma8 evaluated by power.c's own integral
+   (power.c:103-118, 204-224) equals sigma8.  No reference counterpart (the
+   reference reads a CAMB table, power.c:62-101). */
+int cola_eh98_nowiggle_table(double omega_m, double omega_b, double h, double ns, double sigma8,
+                             double kmin, double kmax, int n, double* k_out, double* p_out);
+
+/* lpt.c:145-578 lpt_set_displacement (lpt.h:7, main.c:127): 2LPT initial
+   conditions for seed on the device; sets a_x = a_init, a_v = 0. */
+int cola_lpt_set_displacement(cola_context* ctx, double a_init, double omega_m, int seed, double boxsize);
+/* lpt.c:611-614 lpt_get_local_nx (one rank: nc) */
+int cola_lpt_get_local_nx(const cola_context* ctx);
+
+/* move.c:15-98 move_particles2 (move.h:7, main.c:167): on one rank only the
+   periodic wrap into [0, L) remains. */
+int cola_move_particles(cola_context* ctx);
+
+/* pm.c:565-596 pm_calculate_forces (pm.h:10, main.c:171): CIC deposit with
+   the reference's ghost-particle x periodicity, mass check (6040), r2c FFT,
+   Poisson gradient per axis, c2r FFT, CIC readout into the force array. */
+int cola_pm_calculate_forces(cola_context* ctx);
+
+/* cola.c:73-117 cola_kick (cola.h:7, main.c:195) and cola.c:119-153
+   cola_drift (cola.h:9, main.c:205); a_x / a_v advance as in the reference. */
+int cola_kick(cola_context* ctx, float omega_m, float avel1);
+int cola_drift(cola_context* ctx, float omega_m, float apos1);
+
+/* One full time step with no snapshot in it: wrap, force, kick, drift
+   (main.c:162-206 with the output checks false).  The readout, kick and drift
+   run as one fused kernel; results equal the separate calls bit for bit.
+   No single reference counterpart. */
+int cola_step(cola_context* ctx, float omega_m, float avel1, float apos1);
+
+/* cola.c:341-416 cola_set_snapshot (cola.h:18, main.c:346): kick/drift to
+   aout, add the 2LPT velocity, km/s; writes np records in lattice-ID order
+   into the caller's host array. */
+int cola_set_snapshot(cola_context* ctx, double aout, float omega_m, cola_particle_min* out, size_t np_out);
+/* cola.c:297-338 set_noncola_initial (cola.h:16, main.c:135) */
+int cola_set_noncola_initial(cola_context* ctx, float omega_m, cola_particle_min* out, size_t np_out);
+
+/* State access (Particles in particle.h:14-22 is caller-visible in the
+   reference; here it lives on the device). */
+long long cola_np(const cola_context* ctx);
+int cola_get_times(const cola_context* ctx, float* a_x, float* a_v);
+/* Particles.a_x / a_v (particle.h:17), set directly by the driver
+   (main.c:145 sets a_v = da after the initial conditions). */
+int cola_set_times(cola_context* ctx, float a_x, float a_v);
+int cola_get_particles(cola_context* ctx, cola_particle* out, size_t np_out);
+int cola_set_particles(cola_context* ctx, const cola_particle* in, size_t np_in, float a_x, float a_v);
+int cola_get_forces(cola_context* ctx, float* out, size_t np_out);          /* np x 3 */
+/* pm.c:209-279 PtoMesh + pm.c:535-562 check_total_density only: leaves the
+   real-space density -1 + rho on the device until the next force call. */
+int cola_pm_assign(cola_context* ctx);
+int cola_get_density(cola_context* ctx, float* out, size_t n_out);          /* padded real grid */
+int cola_get_force_grid(cola_context* ctx, int axis, float* out, size_t n_out);
+double cola_last_mass_sum(const cola_context* ctx);                          /* pm.c:535-562 */
+
+/* Device-time profiling with HIP events on the context's stream, replacing
+   the reference's wall-clock phase timers (timer.c:11-84).  Phases are
+   finer than the reference's categories: one per kernel family, named by
+   cola_phase_name (e.g. "deposit", "fft_x_inv", "readout_kick_drift").
+   cola_get_phase_stats fills up to n entries of accumulated milliseconds and
+   launch counts since the last reset; it synchronises the stream. */
+int cola_set_profiling(cola_context* ctx, int on);
+int cola_phase_count(void);
+const char* cola_phase_name(int phase);
+int cola_get_phase_stats(cola_context* ctx, double* ms, long long* launches, int n);
+int cola_reset_phase_stats(cola_context* ctx);
+
+/* Deposit tile shape (cells per tile along x, y, z; each must divide ngrid,
+   product <= 4096) and FFT tile width (log2 of the transforms per
+   workgroup, -1 = automatic).  Results do not depend on either; the
+   autotuner sweeps them for speed. */
+int cola_set_tiling(cola_context* ctx, int tx, int ty, int tz, int fft_log2_tile);
+
+/* Standalone 3-D FFTs on host arrays with FFTW's conventions (unnormalised;
+   c2r = c2c over x and y, then c2r over z reading only the real parts of the
+   kz = 0 and kz = n/2 elements), in place on the padded real layout
+   [n][n][2(n/2+1)].  These are the transforms pm.c:285,353 and
+   lpt.c:403,427,501-502 execute through FFTW. */
+int cola_fft_r2c_3d(int n, float* data);
+int cola_fft_c2r_3d(int n, float* data);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

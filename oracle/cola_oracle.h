@@ -1,0 +1,110 @@
+/*
+ * ORACLE -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, CPU-only restatement of junkoda/cola_halo's time-stepping hot
+ * path (2LPT initial conditions, particle-mesh force, COLA kick/drift) used as
+ * the checker for the HIP library in cola_halo_amd/.  Only tests/, the
+ * package's smoke() and bench.py's cpu_baseline leg may load it, and only to
+ * check or time against; the product never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Third-party
+ * arithmetic the reference calls is restated from its published algorithm:
+ *   - GSL gsl_rng_ranlxd1 / gsl_rng_uniform (Luescher's 48-bit RANLUX),
+ *   - GSL gsl_integration_qag (QUADPACK QAG with the GK41 / GK61 rules),
+ *   - GSL gsl_sf_hyperg_2F1 (Gauss hypergeometric series),
+ *   - FFTW3f unnormalised r2c / c2r (the discrete Fourier transform).
+ */
+#ifndef COLA_ORACLE_H
+#define COLA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* particle.h:6-12 */
+typedef struct {
+  float x[3];
+  float dx1[3];
+  float dx2[3];
+  float v[3];
+  long long id;
+} OrcParticle;
+
+/* particle.h:24-28 */
+typedef struct {
+  float x[3];
+  float v[3];
+  long long id;
+} OrcParticleMin;
+
+/* ---- RANLUX double (GSL ranlxd) ---- */
+typedef struct {
+  double x[12];
+  double carry;
+  int ir, jr, ir_old, pr;
+} OrcRanlxd;
+
+void orc_ranlxd_set(OrcRanlxd* r, unsigned long seed, int luxury);
+double orc_ranlxd_uniform(OrcRanlxd* r);
+void orc_seedtable(int nmesh, int seed, unsigned int* table);
+
+/* ---- power.c ---- */
+int orc_power_set_table(const double* k, const double* p, int n);
+int orc_power_read_file(const char* filename);
+double orc_powerspec(double k);
+double orc_tophat_sigma2(double R);
+int orc_power_check_sigma8(double sigma8, double* sigma8_input);
+double orc_growth_factor(double astart, double aend, double omega_m, double omega_lambda);
+
+/* ---- quadrature / special functions ---- */
+double orc_qag(double (*f)(double, void*), void* params, double a, double b,
+               double epsabs, double epsrel, int limit, int key, double* abserr);
+double orc_hyp2f1(double a, double b, double c, double z);
+
+/* ---- cola.c scalars (Om is cola.c's file-static float) ---- */
+float orc_growthDtemp(float a, float Om);
+float orc_growthD(float a, float Om);
+float orc_growthD2(float a, float Om);
+float orc_growthD2v(float a, float Om);
+float orc_Qfactor(float a, float Om);
+double orc_DprimeQ(double a, float nGrowth, float Om);
+double orc_Sq(double ai, double af, double aRef, float Om);
+double orc_Sphi(double ai, double af, double aRef, float Om);
+
+/* ---- FFT (unnormalised, FFTW sign conventions) ---- */
+void orc_fft_1d(int n, double* re_im, int sign);
+void orc_fft_r2c_3d(int n, float* data);   /* padded real in, complex out, in place */
+void orc_fft_c2r_3d(int n, float* data);   /* complex in, padded real out, in place */
+
+/* ---- lpt.c ---- */
+int orc_lpt_set_displacement(int nc, double a_init, double omega_m, int seed,
+                             double box, OrcParticle* particles,
+                             float* cdisp_out, float* disp_out, float* disp2_out);
+
+/* ---- pm.c / move.c ---- */
+void orc_wrap(OrcParticle* p, int np, float boxsize);
+void orc_pm_deposit(int ngrid, float boxsize, float wpar, const OrcParticle* p,
+                    int np, float* density);
+double orc_density_sum(int ngrid, const float* density);
+int orc_pm_calculate_forces(int ngrid, float boxsize, float wpar, long long np_total,
+                            const OrcParticle* p, int np, float* force,
+                            float* density_out, float* force_grid_out, double* mass_sum);
+
+/* ---- cola.c kernels ---- */
+void orc_kick(OrcParticle* p, const float* force, int np, float Om,
+              float* a_x, float* a_v, float avel1);
+void orc_drift(OrcParticle* p, int np, float Om, float* a_x, float* a_v, float apos1);
+void orc_set_snapshot(double aout, const OrcParticle* p, const float* force, int np,
+                      float Om, float a_x, float a_v, OrcParticleMin* out);
+void orc_set_noncola_initial(const OrcParticle* p, int np, float Om, float a_x,
+                             OrcParticleMin* out);
+
+/* ---- main.c schedule ---- */
+int orc_run(int nc, int ngrid, double boxsize, double omega_m, double a_final,
+            int ntimestep, int seed, const double* zout, int nout,
+            OrcParticleMin* snapshots, float* snapshot_a, int* nsnap_written);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

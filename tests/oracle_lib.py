@@ -1,0 +1,130 @@
+"""ctypes binding of the CPU oracle (oracle/libcola_oracle.so).
+
+Test infrastructure only: tests, the package's smoke check and bench.py's
+cpu_baseline leg use it as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "libcola_oracle.so")
+
+PARTICLE_DTYPE = np.dtype([("x", "<f4", 3), ("dx1", "<f4", 3), ("dx2", "<f4", 3),
+                           ("v", "<f4", 3), ("id", "<i8")])
+PARTICLE_MIN_DTYPE = np.dtype([("x", "<f4", 3), ("v", "<f4", 3), ("id", "<i8")])
+assert PARTICLE_DTYPE.itemsize == 56 and PARTICLE_MIN_DTYPE.itemsize == 32
+
+
+class Ranlxd(C.Structure):
+    _fields_ = [("x", C.c_double * 12), ("carry", C.c_double), ("ir", C.c_int),
+                ("jr", C.c_int), ("ir_old", C.c_int), ("pr", C.c_int)]
+
+
+def _ptr(a, ctype=C.c_float):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def load():
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"oracle library not built: {LIB_PATH} (run make -C oracle)")
+    lib = C.CDLL(LIB_PATH)
+    d, f, i, vp = C.c_double, C.c_float, C.c_int, C.c_void_p
+    sig = {
+        "orc_ranlxd_set": (None, [C.POINTER(Ranlxd), C.c_ulong, i]),
+        "orc_ranlxd_uniform": (d, [C.POINTER(Ranlxd)]),
+        "orc_seedtable": (None, [i, i, vp]),
+        "orc_power_set_table": (i, [vp, vp, i]),
+        "orc_power_read_file": (i, [C.c_char_p]),
+        "orc_powerspec": (d, [d]),
+        "orc_tophat_sigma2": (d, [d]),
+        "orc_power_check_sigma8": (i, [d, C.POINTER(d)]),
+        "orc_growth_factor": (d, [d, d, d, d]),
+        "orc_hyp2f1": (d, [d, d, d, d]),
+        "orc_growthDtemp": (f, [f, f]),
+        "orc_growthD": (f, [f, f]),
+        "orc_growthD2": (f, [f, f]),
+        "orc_growthD2v": (f, [f, f]),
+        "orc_Qfactor": (f, [f, f]),
+        "orc_DprimeQ": (d, [d, f, f]),
+        "orc_Sq": (d, [d, d, d, f]),
+        "orc_Sphi": (d, [d, d, d, f]),
+        "orc_fft_1d": (None, [i, vp, i]),
+        "orc_fft_r2c_3d": (None, [i, vp]),
+        "orc_fft_c2r_3d": (None, [i, vp]),
+        "orc_lpt_set_displacement": (i, [i, d, d, i, d, vp, vp, vp, vp]),
+        "orc_wrap": (None, [vp, i, f]),
+        "orc_pm_deposit": (None, [i, f, f, vp, i, vp]),
+        "orc_density_sum": (d, [i, vp]),
+        "orc_pm_calculate_forces": (i, [i, f, f, C.c_longlong, vp, i, vp, vp, vp, C.POINTER(d)]),
+        "orc_kick": (None, [vp, vp, i, f, C.POINTER(f), C.POINTER(f), f]),
+        "orc_drift": (None, [vp, i, f, C.POINTER(f), C.POINTER(f), f]),
+        "orc_set_snapshot": (None, [d, vp, vp, i, f, f, f, vp]),
+        "orc_set_noncola_initial": (None, [vp, i, f, f, vp]),
+        "orc_run": (i, [i, i, d, d, d, i, i, vp, i, vp, vp, C.POINTER(i)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = load()
+    return _LIB
+
+
+def padded_real_shape(n):
+    return (n, n, 2 * (n // 2 + 1))
+
+
+def seedtable(n, seed):
+    t = np.zeros(n * n, dtype=np.uint32)
+    lib().orc_seedtable(n, seed, t.ctypes.data)
+    return t.reshape(n, n)
+
+
+def set_power_table(k, p):
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    return lib().orc_power_set_table(k.ctypes.data, p.ctypes.data, len(k))
+
+
+def lpt(nc, a_init, omega_m, seed, box, want_grids=False):
+    parts = np.zeros(nc ** 3, dtype=PARTICLE_DTYPE)
+    cs = nc * nc * (nc // 2 + 1) * 2
+    cdisp = np.zeros(3 * cs, dtype=np.float32) if want_grids else None
+    disp = np.zeros(3 * cs, dtype=np.float32) if want_grids else None
+    disp2 = np.zeros(3 * cs, dtype=np.float32) if want_grids else None
+    st = lib().orc_lpt_set_displacement(nc, a_init, omega_m, seed, box, parts.ctypes.data,
+                                        cdisp.ctypes.data if want_grids else None,
+                                        disp.ctypes.data if want_grids else None,
+                                        disp2.ctypes.data if want_grids else None)
+    assert st == 0
+    if want_grids:
+        return parts, cdisp.reshape(3, -1), disp.reshape(3, -1), disp2.reshape(3, -1)
+    return parts
+
+
+def pm_forces(ngrid, box, wpar, parts, want_grids=False):
+    np_ = len(parts)
+    force = np.zeros((np_, 3), dtype=np.float32)
+    gsize = ngrid * ngrid * 2 * (ngrid // 2 + 1)
+    dens = np.zeros(gsize, dtype=np.float32) if want_grids else None
+    fgrid = np.zeros(3 * gsize, dtype=np.float32) if want_grids else None
+    msum = C.c_double(0)
+    st = lib().orc_pm_calculate_forces(ngrid, box, wpar, np_, parts.ctypes.data, np_, force.ctypes.data,
+                                       dens.ctypes.data if want_grids else None,
+                                       fgrid.ctypes.data if want_grids else None, C.byref(msum))
+    out = {"status": st, "force": force, "mass_sum": msum.value}
+    if want_grids:
+        out["density"] = dens.reshape(padded_real_shape(ngrid))
+        out["force_grid"] = fgrid.reshape((3,) + padded_real_shape(ngrid))
+    return out
