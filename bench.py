@@ -1,0 +1,203 @@
+"""Benchmark: particle-steps/s of plain COLA on one MI355X.
+
+Workload (the configuration the metric is quoted on): 256^3 particles,
+768^3 PM grid (pm_nc_factor 3), box 500 Mpc/h, 2LPT ICs from seed 1 at
+a = 0.1, linear-in-a steps of da = 0.1 to a = 1, EH98 no-wiggle P(k)
+(Om 0.273, Ob 0.046, h 0.705, ns 0.96, sigma8 0.8).
+
+A step is one COLA time step of main.c:162-206: wrap, PM force, kick, drift
+(fused readout/kick/drift); the tenth step of each 10-step run is the final
+force evaluation at a = 1 (the snapshot itself is outside the timed region,
+as in the reference's COLA timer category).  particle-steps = nc^3 x force
+evaluations.  Inputs are resident in HBM when the timed region starts.
+
+Prints one JSON line.  The roofline object is for the kernel with the most
+device time in the timed region, from HIP events on the context's stream;
+cpu_baseline times the CPU oracle (C restatement, OpenMP) on a bounded
+sample: one step of the same workload from the same initial conditions.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s on 1 MI355X at P(k) err ≤1% (k≤1 h/Mpc) vs plain fine grid"
+
+NC, NG, BOX, SEED, NSTEPS = 256, 768, 500.0, 1, 10
+OM = 0.273
+
+
+def kernel_bytes(phase, nc, ng):
+    """Algorithmic HBM bytes of one launch: each input array read once and
+    each output array written once (FP32), per the canonical per-phase model."""
+    n_p = nc ** 3
+    n_g = ng ** 3
+    n_c = ng * ng * (ng // 2 + 1)
+    table = {
+        "fft_z_r2c": 4 * n_g + 8 * n_c,
+        "fft_y_fwd": 16 * n_c,
+        "fft_x_fwd": 16 * n_c,
+        "fft_x_inv": 16 * n_c,   # reads delta_k, writes F_k (Poisson fused on load)
+        "fft_y_inv": 16 * n_c,
+        "fft_z_c2r": 8 * n_c + 4 * n_g,
+        "deposit": 4 * n_g + 12 * n_p,
+        "readout_kick_drift": 12 * n_g + 72 * n_p,
+        "bin_count": 12 * n_p,
+        "bin_scatter": 24 * n_p,
+    }
+    return table.get(phase)
+
+
+def cpu_baseline(ctx_parts, nc, ng, box, threads):
+    """Oracle on the host: one PM force + kick + drift of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    import ctypes as C
+
+    import oracle_lib as orc
+
+    parts = ctx_parts.copy()
+    wpar = float(np.float32((ng / nc) ** 3))
+    force = np.zeros((len(parts), 3), np.float32)
+    t0 = time.perf_counter()
+    lib = orc.lib()
+    lib.orc_wrap(parts.ctypes.data, len(parts), box)
+    st = lib.orc_pm_calculate_forces(ng, box, wpar, len(parts), parts.ctypes.data, len(parts), force.ctypes.data,
+                                     None, None, C.byref(C.c_double()))
+    ax, av = C.c_float(0.1), C.c_float(0.1)
+    Om = float(np.float32(OM))
+    lib.orc_kick(parts.ctypes.data, force.ctypes.data, len(parts), Om, C.byref(ax), C.byref(av), 0.15)
+    lib.orc_drift(parts.ctypes.data, len(parts), Om, C.byref(ax), C.byref(av), 0.2)
+    dt = time.perf_counter() - t0
+    return {"value": nc ** 3 / dt, "unit": "particle-steps/s", "cores": int(threads), "kind": "port",
+            "sample": f"1 step (wrap + PM force on {ng}^3 + kick + drift) of the {nc}^3 workload, "
+                      f"{dt:.1f} s, oracle status {st}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nc", type=int, default=NC)
+    ap.add_argument("--ng", type=int, default=NG)
+    ap.add_argument("--box", type=float, default=BOX)
+    ap.add_argument("--tile", type=str, default="", help="deposit tile tx,ty,tz")
+    ap.add_argument("--fft-log2-tile", type=int, default=-2, help="FFT transforms per workgroup (log2)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this build has a single MI355X; "
+              "the multi-GPU design (seed sharding, slab decomposition) is described in DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import cola_halo_amd as ch
+
+    nc, ng, box = args.nc, args.ng, args.box
+    da = 1.0 / NSTEPS
+    a_init = da
+    Om = float(np.float32(OM))
+    k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
+    ctx = ch.Context(nc, ng, box)
+    if args.tile or args.fft_log2_tile != -2:
+        tx, ty, tz = (int(v) for v in args.tile.split(",")) if args.tile else (8, 8, 32)
+        ctx.set_tiling(tx, ty, tz, args.fft_log2_tile if args.fft_log2_tile != -2 else -1)
+    ctx.power_init_table(k, p, a_init, 0.8, OM, 1.0 - OM)
+    t_ic = time.perf_counter()
+    ctx.lpt_set_displacement(a_init, OM, SEED)
+    ctx.set_times(a_init, da)  # main.c:145
+    ctx.synchronize()
+    t_ic = time.perf_counter() - t_ic
+    ctx.state_save()
+
+    def run_steps(nsteps):
+        forces = 0
+        for s in range(nsteps):
+            istep = s % NSTEPS + 1
+            if istep == 1 and s > 0:
+                ctx.state_restore()
+            if istep < NSTEPS:
+                ctx.step(Om, (istep + 0.5) * da, (istep + 1.0) * da)
+            else:
+                ctx.move_particles()
+                ctx.pm_calculate_forces()
+            forces += 1
+        return forces
+
+    # warmup, then back to the initial conditions
+    run_steps(args.warmup)
+    ctx.synchronize()
+    ctx.state_restore()
+    ctx.synchronize()
+
+    ctx.set_profiling(True)
+    ctx.reset_phase_stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    forces = run_steps(args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    stats = ctx.phase_stats()
+    ctx.set_profiling(False)
+    elapsed = t1 - t0
+    value = nc ** 3 * forces / elapsed
+
+    # roofline of the kernel with the most device time
+    dom, (dom_ms, dom_n) = max(stats.items(), key=lambda kv: kv[1][0])
+    nbytes = kernel_bytes(dom, nc, ng)
+    roof = None
+    if nbytes is not None and dom_n > 0:
+        avg_s = dom_ms / dom_n / 1e3
+        achieved = nbytes / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_ms": round(dom_ms / dom_n, 4), "bytes_per_launch": nbytes}
+    # whole step against the canonical bytes per step (156 Np + 32 Ng + 64 Nc)
+    n_c = ng * ng * (ng // 2 + 1)
+    b_step = 156 * nc ** 3 + 32 * ng ** 3 + 64 * n_c
+    step_frac = b_step / (elapsed / forces) / 1e9 / HBM_PEAK_GBS
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        ctx.state_restore()
+        parts = ctx.get_particles()
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(parts, nc, ng, box, threads)
+    ctx.close()
+
+    phases = {k2: round(v[0] / max(forces, 1), 3) for k2, v in stats.items() if v[1] > 0}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain COLA {nc}^3 particles, {ng}^3 PM grid, box {box:g} Mpc/h, 2LPT seed {SEED}, "
+                               f"a 0.1->1 in {NSTEPS} steps, EH98 no-wiggle P(k) sigma8 0.8",
+                   "nc": nc, "ngrid": ng, "boxsize": box, "seed": SEED, "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_canonical_frac": round(step_frac, 4),
+        "phase_ms_per_step": phases,
+        "ic_seconds": round(t_ic, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

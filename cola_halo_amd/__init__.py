@@ -73,6 +73,8 @@ _SIGNATURES = {
     "cola_get_density": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "cola_get_force_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "cola_last_mass_sum": (C.c_double, [C.c_void_p]),
+    "cola_state_save": (C.c_int, [C.c_void_p]),
+    "cola_state_restore": (C.c_int, [C.c_void_p]),
     "cola_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_phase_count": (C.c_int, []),
     "cola_phase_name": (C.c_char_p, [C.c_int]),
@@ -265,6 +267,12 @@ class Context:
         out = np.zeros(padded_real_shape(self.ngrid), np.float32)
         _check(lib().cola_get_force_grid(self._h, axis, out.ctypes.data, out.size))
         return out
+
+    def state_save(self):
+        _check(lib().cola_state_save(self._h))
+
+    def state_restore(self):
+        _check(lib().cola_state_restore(self._h))
 
     def last_mass_sum(self):
         return lib().cola_last_mass_sum(self._h)

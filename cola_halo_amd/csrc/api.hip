@@ -102,6 +102,8 @@ struct cola_context {
   bool density_is_real = false;
 
   DevParticleMin* d_snap = nullptr;
+  float* saved = nullptr;  // x, v, dx1, dx2 copy of cola_state_save
+  float saved_ax = 0, saved_av = 0;
 
   bool profiling = false;
   std::vector<EventPair> events;
@@ -349,9 +351,9 @@ int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
   c->lpt_fft.hook = lpt_fft_hook;
   c->lpt_fft.hook_user = c;
   c->cic = make_cic_geom(ngrid, c->boxf, c->wpar);
-  // default deposit tile 8 x 8 x 32 cells, shrunk to divisors of ngrid
+  // default deposit tile 8 x 8 x 32 cells, halved until it divides ngrid
   auto pick = [&](int want) {
-    for (int t = want; t > 1; --t)
+    for (int t = want; t > 1; t /= 2)
       if (ngrid % t == 0) return t;
     return 1;
   };
@@ -398,6 +400,7 @@ int cola_context_destroy(cola_context* c)
   if (c->d_logk) (void)hipFree(c->d_logk);
   if (c->d_logD) (void)hipFree(c->d_logD);
   if (c->d_snap) (void)hipFree(c->d_snap);
+  if (c->saved) (void)hipFree(c->saved);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return kOk;
@@ -767,6 +770,34 @@ int cola_get_force_grid(cola_context* c, int axis, float* out, size_t n_out)
 
 double cola_last_mass_sum(const cola_context* c) { return c ? c->last_mass : 0.0; }
 
+int cola_state_save(cola_context* c)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(require_particles(c));
+  const size_t bytes = sizeof(float) * 12 * (size_t)c->np;
+  if (!c->saved) COLA_HIP(hipMalloc(&c->saved, bytes));
+  COLA_HIP(hipMemcpyAsync(c->saved, c->pbuf, bytes, hipMemcpyDeviceToDevice, c->stream));
+  c->saved_ax = c->a_x;
+  c->saved_av = c->a_v;
+  return kOk;
+}
+
+int cola_state_restore(cola_context* c)
+{
+  if (!valid(c)) return kErrArg;
+  if (!c->saved) {
+    set_error("cola_state_restore: nothing saved");
+    return kErrState;
+  }
+  const size_t bytes = sizeof(float) * 12 * (size_t)c->np;
+  COLA_HIP(hipMemcpyAsync(c->pbuf, c->saved, bytes, hipMemcpyDeviceToDevice, c->stream));
+  c->a_x = c->saved_ax;
+  c->a_v = c->saved_av;
+  c->have_particles = true;
+  c->have_force = false;
+  return kOk;
+}
+
 int cola_set_profiling(cola_context* c, int on)
 {
   if (!valid(c)) return kErrArg;
@@ -809,7 +840,7 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
   if (!valid(c)) return kErrArg;
   TileGeom t;
   if (!make_tile_geom(c->ng, tx, ty, tz, &t)) {
-    set_error("deposit tile must divide ngrid and hold <= 4096 cells");
+    set_error("deposit tile sides must be powers of two dividing ngrid, <= 4096 cells");
     return kErrArg;
   }
   COLA_HIP(hipStreamSynchronize(c->stream));

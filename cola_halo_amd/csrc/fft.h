@@ -1,13 +1,17 @@
 // Hand-written Stockham FFT for the PM and LPT grids.
 //
 // A 3-D transform of the padded real layout [n][n][2(n/2+1)] runs as three
-// tile passes.  Each pass loads a tile of T independent 1-D transforms of
-// length n into LDS (layout [position][tile column], column fastest), runs
-// the radix passes in registers with one LDS exchange per radix, and writes
-// the tile back.  The z pass works on rows (two real rows packed into one
-// complex transform); the y and x passes work on strided columns, reading T
-// adjacent complex columns per row so every global access is a contiguous
-// T*8-byte segment.
+// tile passes, one per axis.  A workgroup owns T independent 1-D transforms
+// of length n:
+//   * the first radix pass loads its butterfly inputs straight from HBM into
+//     registers (all loads of a thread in flight at once),
+//   * the radix passes exchange data through one LDS tile (one write and one
+//     read per radix boundary), with the twiddle table staged in LDS,
+//   * the last radix pass stores its outputs straight from registers.
+// Column passes (y, x) give each thread a column index fastest, so every
+// global access is a contiguous T*8-byte segment of a row; the row pass (z)
+// gives each thread a position fastest, so it streams whole rows, two real
+// rows packed into one complex transform.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,14 +20,20 @@
 
 namespace cola {
 
-constexpr int kFftThreads = 256;
-constexpr int kFftEmax = 48;  // complex values one thread holds per pass
-constexpr int kFftMaxTile = kFftThreads * kFftEmax;  // n * T limit (96 KiB of LDS)
+constexpr int kFftMaxThreads = 1024;
+constexpr int kFftMaxTile = 12288;  // n * T complex values per workgroup (96 KiB)
+
+// One radix pass: radix R, L = product of the earlier radices, m = n / R,
+// twiddle stride n / (R L) and magic multipliers for division by L and m.
+struct FftPass {
+  int R = 0, L = 0, m = 0, twstep = 0;
+  unsigned magL = 0, magM = 0;
+};
 
 struct FftRadices {
   int n = 0;
   int npass = 0;
-  int radix[8] = {0};
+  FftPass pass[8];
 };
 
 // Multiplier applied while loading the first (x) pass of an inverse
@@ -41,18 +51,25 @@ struct KOpParams {
   double box = 0.0;   // lpt.c kvec = n * 2 * pi / Box
 };
 
-// Per-size plan: radix schedule, twiddles and tile widths of the three passes.
+// Launch shape of one axis pass: 2^log2T transforms per workgroup, threads
+// per workgroup, register slots per thread.
+struct FftPassGeom {
+  int log2T = 0, threads = 0, emax = 0;
+};
+
+// Per-size plan: radix schedule, twiddles and tile shapes of the three passes.
 struct FftPlan {
   int n = 0;
   FftRadices rad;
   float2* d_tw = nullptr;  // exp(-2 pi i k / n), k < n
-  int log2T_rows = 0, log2T_y = 0, log2T_x = 0;
+  FftPassGeom grows, gy, gx;
 
   // Optional per-pass profiling hook: pass 0 z r2c, 1 y fwd, 2 x fwd,
   // 3 x inv, 4 y inv, 5 z c2r; begin = 1 before the launch, 0 after.
   void (*hook)(void* user, int pass, int begin) = nullptr;
   void* hook_user = nullptr;
 
+  // force_log2T >= 0 fixes the tile width of all three passes.
   int init(int n, int force_log2T = -1);
   void destroy();
 
@@ -63,10 +80,11 @@ struct FftPlan {
   int c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hipStream_t s) const;
 };
 
-// Radix schedule for n = 2^a 3^b 5^c 7^d; returns false otherwise.
+// Radix schedule for n = 2^a 3^b 5^c; returns false otherwise.
 bool fft_factor(int n, FftRadices* r);
-// Largest tile width (log2) with n*T <= kFftMaxTile that keeps every radix
-// pass within the per-thread register budget; capped at cap_log2.
-int fft_tile_log2(const FftRadices& r, int cap_log2);
+// Register slots one thread needs for tile width 2^log2T with the given
+// workgroup size (0 if the tile exceeds kFftMaxTile or 24 slots).
+int fft_emax(const FftRadices& r, int log2T, int threads);
+FftPassGeom pass_geom(const FftRadices& r, int log2T);
 
 }  // namespace cola

@@ -9,7 +9,9 @@
 #include "common.h"
 #include "fft.h"
 
+#include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace cola {
@@ -112,67 +114,16 @@ __device__ inline void dft(float2* v)
   else dft8<DIR>(v);
 }
 
-// One radix-R Stockham pass over the LDS tile [n][T] (column fastest).
-template <int R, int DIR>
-__device__ inline void radix_pass(float2* lds, float2 (&reg)[kFftEmax], int n, int L, int log2T,
-                                  const float2* __restrict__ tw, int tid)
-{
-  constexpr int Q = kFftEmax / R;
-  const int m = n / R;
-  const int nb = m << log2T;
-  const int tmask = (1 << log2T) - 1;
-  const int twstep = n / (R * L);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int u = tid + q * kFftThreads;
-    if (u < nb) {
-      const int c = u & tmask, b = u >> log2T;
-#pragma unroll
-      for (int s = 0; s < R; ++s) reg[q * R + s] = lds[((b + s * m) << log2T) + c];
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int u = tid + q * kFftThreads;
-    if (u < nb) {
-      const int c = u & tmask, b = u >> log2T;
-      const int j = b % L, i = b / L;
-      float2 v[R];
-#pragma unroll
-      for (int s = 0; s < R; ++s) v[s] = reg[q * R + s];
-      if (L > 1) {
-#pragma unroll
-        for (int s = 1; s < R; ++s) {
-          float2 w = tw[s * j * twstep];
-          if (DIR > 0) w.y = -w.y;
-          v[s] = cmul(v[s], w);
-        }
-      }
-      dft<R, DIR>(v);
-      const int o = i * R * L + j;
-#pragma unroll
-      for (int t = 0; t < R; ++t) lds[((o + t * L) << log2T) + c] = v[t];
-    }
-  }
-  __syncthreads();
-}
+__device__ inline unsigned fdiv(unsigned x, unsigned d, unsigned mag) { return d == 1 ? x : __umulhi(x, mag); }
 
-template <int DIR>
-__device__ inline void run_passes(float2* lds, const FftRadices& P, const float2* tw, int log2T, int tid)
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Column tiles 2k and 2k+1 share every 128-byte line they read, so the
+// remap puts them on the same XCD, one dispatch round apart:
+// block 16q + 8h + x (XCD x) -> tile 16q + 2x + h.
+__device__ inline int xcd_pair_remap(int b, int nb)
 {
-  float2 reg[kFftEmax];
-  int L = 1;
-  for (int p = 0; p < P.npass; ++p) {
-    switch (P.radix[p]) {
-      case 2: radix_pass<2, DIR>(lds, reg, P.n, L, log2T, tw, tid); break;
-      case 3: radix_pass<3, DIR>(lds, reg, P.n, L, log2T, tw, tid); break;
-      case 4: radix_pass<4, DIR>(lds, reg, P.n, L, log2T, tw, tid); break;
-      case 5: radix_pass<5, DIR>(lds, reg, P.n, L, log2T, tw, tid); break;
-      default: radix_pass<8, DIR>(lds, reg, P.n, L, log2T, tw, tid); break;
-    }
-    L *= P.radix[p];
-  }
+  if (b >= (nb & ~15)) return b;
+  return 2 * (8 * (b >> 4) + (b & 7)) + ((b >> 3) & 1);
 }
 
 // Signed wave number of index i: the PM convention keeps the Nyquist index
@@ -210,156 +161,313 @@ __device__ inline float2 apply_kop(float2 v, int ix, int iy, int iz, int n, cons
   }
 }
 
-// Strided complex pass: element (r, c) of tile (blockIdx.x, batch blockIdx.y)
-// lives at batch*bs + r*sr + (c0 + c); columns >= ncols are padding.
-template <int DIR, KOp OP>
-__global__ void __launch_bounds__(kFftThreads)
-fft_cols_kernel(const float2* __restrict__ in, float2* __restrict__ out, FftRadices P,
-                const float2* __restrict__ tw, long long sr, long long bs, int ncols, int log2T,
-                int nz, KOpParams kp)
-{
-  extern __shared__ float2 lds[];
-  const int n = P.n, tid = threadIdx.x;
-  const int T = 1 << log2T;
-  const int c = tid & (T - 1);
-  const int col = (blockIdx.x << log2T) + c;
-  const bool valid = col < ncols;
-  const long long base = (long long)blockIdx.y * bs + col;
-  const int rstep = kFftThreads >> log2T;
-  // the x pass addresses columns as (y, kz) pairs of the half-spectrum
-  int iy = 0, iz = 0;
-  if constexpr (OP != KOp::None) {
-    iy = col / nz;
-    iz = col - iy * nz;
-  }
-  for (int r = tid >> log2T; r < n; r += rstep) {
-    float2 v = make_float2(0.0f, 0.0f);
-    if (valid) {
-      v = in[base + r * sr];
-      if constexpr (OP != KOp::None) v = apply_kop<OP>(v, r, iy, iz, n, kp);
-    }
-    lds[(r << log2T) + c] = v;
-  }
-  __syncthreads();
-  run_passes<DIR>(lds, P, tw, log2T, tid);
-  if (valid)
-    for (int r = tid >> log2T; r < n; r += rstep) out[base + r * sr] = lds[(r << log2T) + c];
-}
+enum Mode : int { kCols = 0, kR2C = 1, kC2R = 2 };
 
-// z pass, forward: real rows 2q, 2q+1 (stride rs_in floats) packed as one
-// complex transform, unpacked into half spectra (stride rs_out complex).
-__global__ void __launch_bounds__(kFftThreads)
-fft_rows_r2c_kernel(const float* __restrict__ in, float2* __restrict__ out, FftRadices P,
-                    const float2* __restrict__ tw, long long rs_in, long long rs_out, int npairs, int log2T)
-{
-  extern __shared__ float2 lds[];
-  const int n = P.n, tid = threadIdx.x;
-  const int T = 1 << log2T;
-  const int c = tid & (T - 1);
-  const int q = (blockIdx.x << log2T) + c;
-  const bool valid = q < npairs;
-  const int rstep = kFftThreads >> log2T;
-  const float* ra = in + (2LL * q) * rs_in;
-  const float* rb = ra + rs_in;
-  for (int r = tid >> log2T; r < n; r += rstep) {
-    float2 v = make_float2(0.0f, 0.0f);
-    if (valid) v = make_float2(ra[r], rb[r]);
-    lds[(r << log2T) + c] = v;
-  }
-  __syncthreads();
-  run_passes<-1>(lds, P, tw, log2T, tid);
-  if (valid) {
-    float2* oa = out + (2LL * q) * rs_out;
-    float2* ob = oa + rs_out;
-    for (int k = tid >> log2T; k <= n / 2; k += rstep) {
-      const float2 z = lds[(k << log2T) + c];
-      const int km = k == 0 ? 0 : n - k;
-      const float2 zm = lds[(km << log2T) + c];
-      // A = (Z_k + conj Z_{n-k}) / 2,  B = (Z_k - conj Z_{n-k}) / (2i)
-      oa[k] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-      ob[k] = make_float2(0.5f * (z.y + zm.y), 0.5f * (zm.x - z.x));
-    }
-  }
-}
+struct TileArgs {
+  FftRadices P;
+  int log2T;             // transforms per workgroup (columns or row pairs)
+  int tpb;               // kCols: tiles per batch (grid is 1-D: batch-major)
+  int ncols;             // valid columns (kCols) or row pairs (kR2C / kC2R)
+  long long sr;          // kCols: stride between positions of one transform
+  long long bs;          // kCols: batch stride (blockIdx.y)
+  long long rs_in;       // kR2C: real row stride (floats); kC2R: complex row stride
+  long long rs_out;      // kR2C: complex row stride; kC2R: real row stride (floats)
+  int nz;                // x pass: columns are (y, kz) pairs, nz = n/2+1
+  unsigned magNz;
+  KOpParams kp;
+};
 
-// z pass, inverse: half spectra of rows 2q, 2q+1 extended by Hermitian
-// symmetry (imaginary parts of kz = 0 and n/2 dropped), Z = A + iB, one
-// complex inverse transform, real part -> row 2q, imaginary part -> row 2q+1.
-__global__ void __launch_bounds__(kFftThreads)
-fft_rows_c2r_kernel(const float2* __restrict__ in, float* __restrict__ out, FftRadices P,
-                    const float2* __restrict__ tw, long long rs_in, long long rs_out, int npairs, int log2T)
+// One workgroup: T transforms of length n.  MODE kCols: column c of the tile
+// is global column c0 + c, positions r at stride sr; LDS [r][c].  kR2C/kC2R:
+// transform c packs real rows 2(c0+c), 2(c0+c)+1; LDS [c][r] padded by one.
+template <int DIR, int MODE, KOp OP, int EMAX>
+__global__ void __launch_bounds__(kFftMaxThreads)
+fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __restrict__ tw, TileArgs A)
 {
-  extern __shared__ float2 lds[];
-  const int n = P.n, tid = threadIdx.x, h = n / 2;
+  extern __shared__ float2 smem[];
+  const int n = A.P.n;
+  const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
+  const int log2T = A.log2T;
   const int T = 1 << log2T;
-  const int c = tid & (T - 1);
-  const int q = (blockIdx.x << log2T) + c;
-  const bool valid = q < npairs;
-  const int rstep = kFftThreads >> log2T;
-  const float2* ia = in + (2LL * q) * rs_in;
-  const float2* ib = ia + rs_in;
-  for (int r = tid >> log2T; r < n; r += rstep) {
-    float2 v = make_float2(0.0f, 0.0f);
-    if (valid) {
-      float2 a, b;
-      if (r <= h) {
-        a = ia[r];
-        b = ib[r];
-        if (r == 0 || r == h) {
-          a.y = 0.0f;
-          b.y = 0.0f;
+  int c0, batch = 0;
+  if (MODE == kCols) {
+    const int lb = xcd_pair_remap(blockIdx.x, gridDim.x);
+    batch = lb / A.tpb;
+    c0 = (lb - batch * A.tpb) << log2T;
+  } else {
+    c0 = blockIdx.x << log2T;
+  }
+  float2* tws = smem;
+  float2* lds = smem + n;
+  for (int k = tid; k < n; k += nthr) tws[k] = tw[k];
+
+  auto lidx = [&](int r, int c) { return MODE == kCols ? (r << log2T) + c : c * (n + 1) + r; };
+  auto split = [&](int u, const FftPass& p, int& c, int& b) {
+    if (MODE == kCols) {
+      c = u & (T - 1);
+      b = u >> log2T;
+    } else {
+      c = (int)fdiv((unsigned)u, (unsigned)p.m, p.magM);
+      b = u - c * p.m;
+    }
+  };
+
+  using Tile = float2[EMAX];
+
+  // ---- pass 0: inputs straight from global memory
+  auto load_global = [&](auto Rc, Tile& reg) {
+    constexpr int R = decltype(Rc)::value;
+    const FftPass& p = A.P.pass[0];
+    const int nb = p.m << log2T;
+#pragma unroll
+    for (int q = 0; q < EMAX / R; ++q) {
+      const int u = tid + q * nthr;
+      int c = 0, b = 0;
+      split(u, p, c, b);
+      const bool live = u < nb && c0 + c < A.ncols;
+      if (MODE == kCols) {
+        // uniform base per leg s (scalar registers) + 32-bit lane offset b*sr + c
+        const float2* in = static_cast<const float2*>(in_) + (long long)batch * A.bs + c0;
+        const unsigned lane = (unsigned)b * (unsigned)A.sr + (unsigned)c;
+        int iy = 0, iz = 0;
+        if constexpr (OP != KOp::None) {
+          const int col = c0 + c;
+          iy = (int)fdiv((unsigned)col, (unsigned)A.nz, A.magNz);
+          iz = col - iy * A.nz;
+        }
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const float2* leg = in + (long long)(s * p.m) * A.sr;
+          reg[q * R + s] = live ? leg[lane] : make_float2(0.0f, 0.0f);
+        }
+        // the operator runs after all of this butterfly's loads are issued
+        if constexpr (OP != KOp::None) {
+#pragma unroll
+          for (int s = 0; s < R; ++s) reg[q * R + s] = apply_kop<OP>(reg[q * R + s], b + s * p.m, iy, iz, n, A.kp);
+        }
+      } else if (MODE == kR2C) {
+        const float* in = static_cast<const float*>(in_) + (2LL * c0) * A.rs_in;
+        const unsigned lane = (unsigned)(2 * c) * (unsigned)A.rs_in + (unsigned)b;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const float* leg = in + s * p.m;
+          reg[q * R + s] = live ? make_float2(leg[lane], leg[lane + (unsigned)A.rs_in]) : make_float2(0.0f, 0.0f);
         }
       } else {
-        a = ia[n - r];
-        b = ib[n - r];
-        a.y = -a.y;
-        b.y = -b.y;
+        const float2* in = static_cast<const float2*>(in_) + (2LL * c0) * A.rs_in;
+        const unsigned rowa = (unsigned)(2 * c) * (unsigned)A.rs_in;
+        const unsigned rowb = rowa + (unsigned)A.rs_in;
+        const int h = n / 2;
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const int r = b + s * p.m;
+          float2 v = make_float2(0.0f, 0.0f);
+          if (live) {
+            // Hermitian extension of both half spectra; Im of kz = 0, n/2 dropped
+            const unsigned rr = (unsigned)(r <= h ? r : n - r);
+            float2 a = in[rowa + rr], bb = in[rowb + rr];
+            if (r == 0 || r == h) {
+              a.y = 0.0f;
+              bb.y = 0.0f;
+            } else if (r > h) {
+              a.y = -a.y;
+              bb.y = -bb.y;
+            }
+            v = make_float2(a.x - bb.y, a.y + bb.x);
+          }
+          reg[q * R + s] = v;
+        }
       }
-      v = make_float2(a.x - b.y, a.y + b.x);
     }
-    lds[(r << log2T) + c] = v;
+  };
+
+  auto compute = [&](auto Rc, int pi, Tile& reg) {
+    constexpr int R = decltype(Rc)::value;
+    const FftPass& p = A.P.pass[pi];
+    const int nb = p.m << log2T;
+#pragma unroll
+    for (int q = 0; q < EMAX / R; ++q) {
+      const int u = tid + q * nthr;
+      if (u < nb) {
+        int c, b;
+        split(u, p, c, b);
+        if (p.L > 1) {
+          const int i = (int)fdiv((unsigned)b, (unsigned)p.L, p.magL);
+          const int j = b - i * p.L;
+#pragma unroll
+          for (int s = 1; s < R; ++s) {
+            float2 w = tws[s * j * p.twstep];
+            if (DIR > 0) w.y = -w.y;
+            reg[q * R + s] = cmul(reg[q * R + s], w);
+          }
+        }
+        dft<R, DIR>(&reg[q * R]);
+      }
+    }
+  };
+
+  // outputs of pass pi to LDS at positions (b div L) R L + t L + (b mod L)
+  auto store_lds = [&](auto Rc, int pi, Tile& reg) {
+    constexpr int R = decltype(Rc)::value;
+    const FftPass& p = A.P.pass[pi];
+    const int nb = p.m << log2T;
+#pragma unroll
+    for (int q = 0; q < EMAX / R; ++q) {
+      const int u = tid + q * nthr;
+      if (u < nb) {
+        int c, b;
+        split(u, p, c, b);
+        const int i = (int)fdiv((unsigned)b, (unsigned)p.L, p.magL);
+        const int j = b - i * p.L;
+        const int o = i * R * p.L + j;
+#pragma unroll
+        for (int t = 0; t < R; ++t) lds[lidx(o + t * p.L, c)] = reg[q * R + t];
+      }
+    }
+  };
+
+  auto load_lds = [&](auto Rc, int pi, Tile& reg) {
+    constexpr int R = decltype(Rc)::value;
+    const FftPass& p = A.P.pass[pi];
+    const int nb = p.m << log2T;
+#pragma unroll
+    for (int q = 0; q < EMAX / R; ++q) {
+      const int u = tid + q * nthr;
+      if (u < nb) {
+        int c, b;
+        split(u, p, c, b);
+#pragma unroll
+        for (int s = 0; s < R; ++s) reg[q * R + s] = lds[lidx(b + s * p.m, c)];
+      }
+    }
+  };
+
+  auto store_global = [&](auto Rc, int pi, Tile& reg) {
+    constexpr int R = decltype(Rc)::value;
+    const FftPass& p = A.P.pass[pi];
+    const int nb = p.m << log2T;
+#pragma unroll
+    for (int q = 0; q < EMAX / R; ++q) {
+      const int u = tid + q * nthr;
+      int c = 0, b = 0;
+      split(u, p, c, b);
+      if (u < nb && c0 + c < A.ncols) {
+        // last pass: L = n / R = m, so b div L = 0 and the outputs sit at b + t*m
+        if (MODE == kCols) {
+          float2* out = static_cast<float2*>(out_) + (long long)batch * A.bs + c0;
+          const unsigned lane = (unsigned)b * (unsigned)A.sr + (unsigned)c;
+#pragma unroll
+          for (int t = 0; t < R; ++t) out[(long long)(t * p.m) * A.sr + lane] = reg[q * R + t];
+        } else {
+          float* out = static_cast<float*>(out_) + (2LL * c0) * A.rs_out;
+          const unsigned lane = (unsigned)(2 * c) * (unsigned)A.rs_out + (unsigned)b;
+#pragma unroll
+          for (int t = 0; t < R; ++t) {
+            const float2 z = reg[q * R + t];
+            float* leg = out + t * p.m;
+            leg[lane] = z.x;
+            leg[lane + (unsigned)A.rs_out] = z.y;
+          }
+        }
+      }
+    }
+  };
+
+  auto with_radix = [&](int R, auto&& f) {
+    switch (R) {
+      case 2: f(std::integral_constant<int, 2>{}); break;
+      case 3: f(std::integral_constant<int, 3>{}); break;
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      case 5: f(std::integral_constant<int, 5>{}); break;
+      default: f(std::integral_constant<int, 8>{}); break;
+    }
+  };
+
+  // Each pass keeps its butterflies in a register tile local to its radix
+  // case: inputs from HBM (first pass) or LDS, outputs to LDS or HBM (last).
+  const int np_ = A.P.npass;
+  for (int pi = 0; pi < np_; ++pi) {
+    with_radix(A.P.pass[pi].R, [&](auto Rc) {
+      Tile reg;
+      const bool last = pi == np_ - 1;
+      if (pi == 0) {
+        load_global(Rc, reg);
+      } else {
+        load_lds(Rc, pi, reg);
+        __syncthreads();
+      }
+      compute(Rc, pi, reg);
+      if (!last || MODE == kR2C) {
+        store_lds(Rc, pi, reg);
+        __syncthreads();
+      } else {
+        store_global(Rc, pi, reg);
+      }
+    });
   }
-  __syncthreads();
-  run_passes<1>(lds, P, tw, log2T, tid);
-  if (valid) {
-    float* oa = out + (2LL * q) * rs_out;
-    float* ob = oa + rs_out;
-    for (int r = tid >> log2T; r < n; r += rstep) {
-      const float2 z = lds[(r << log2T) + c];
-      oa[r] = z.x;
-      ob[r] = z.y;
+  if (MODE == kR2C) {
+    // unpack the two real rows: A = (Z_k + conj Z_{n-k}) / 2, B = (Z_k - conj Z_{n-k}) / (2i)
+    float2* out = static_cast<float2*>(out_);
+    const int h = n / 2;
+    for (int c = 0; c < T; ++c) {
+      if (c0 + c >= A.ncols) break;
+      float2* oa = out + (2LL * (c0 + c)) * A.rs_out;
+      float2* ob = oa + A.rs_out;
+      for (int k = tid; k <= h; k += nthr) {
+        const float2 z = lds[lidx(k, c)];
+        const float2 zm = lds[lidx(k == 0 ? 0 : n - k, c)];
+        oa[k] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+        ob[k] = make_float2(0.5f * (z.y + zm.y), 0.5f * (zm.x - z.x));
+      }
     }
-    // the two padding floats of each real row are written as zero
-    if ((tid >> log2T) == 0) {
+  } else {
+    if (MODE == kC2R && tid < T && c0 + tid < A.ncols) {
+      // the two padding floats of each real row are written as zero
+      float* oa = static_cast<float*>(out_) + (2LL * (c0 + tid)) * A.rs_out;
       oa[n] = 0.0f;
       oa[n + 1] = 0.0f;
-      ob[n] = 0.0f;
-      ob[n + 1] = 0.0f;
+      oa[A.rs_out + n] = 0.0f;
+      oa[A.rs_out + n + 1] = 0.0f;
     }
   }
 }
 
-template <class K>
-int set_lds_limit(K kernel)
+unsigned magic_for(int d)
 {
-  COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               kFftMaxTile * (int)sizeof(float2)));
+  if (d <= 1) return 0;
+  return (unsigned)(((1ULL << 32) + (unsigned long long)d - 1) / (unsigned long long)d);
+}
+
+size_t tile_lds_bytes(int mode, int n, int log2T)
+{
+  const size_t data = mode == kCols ? ((size_t)n << log2T) : ((size_t)(n + 1) << log2T);
+  return sizeof(float2) * (data + n);
+}
+
+template <int DIR, int MODE, KOp OP, int E>
+int launch_e(dim3 grid, int threads, size_t lds, hipStream_t s, const void* in, void* out, const float2* tw,
+             const TileArgs& A)
+{
+  static bool attr = false;
+  if (!attr) {
+    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_tile<DIR, MODE, OP, E>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  hipLaunchKernelGGL((fft_tile<DIR, MODE, OP, E>), grid, dim3(threads), lds, s, in, out, tw, A);
   return kOk;
 }
 
-int ensure_attributes()
+template <int DIR, int MODE, KOp OP>
+int launch_tile(const FftPassGeom& g, dim3 grid, hipStream_t s, const void* in, void* out, const float2* tw,
+                const TileArgs& A)
 {
-  static int done = 0;
-  if (done) return kOk;
-  COLA_TRY(set_lds_limit(fft_cols_kernel<-1, KOp::None>));
-  COLA_TRY(set_lds_limit(fft_cols_kernel<1, KOp::None>));
-  COLA_TRY(set_lds_limit(fft_cols_kernel<1, KOp::PmPoisson>));
-  COLA_TRY(set_lds_limit(fft_cols_kernel<1, KOp::LptGrad>));
-  COLA_TRY(set_lds_limit(fft_cols_kernel<1, KOp::Lpt2>));
-  COLA_TRY(set_lds_limit(fft_rows_r2c_kernel));
-  COLA_TRY(set_lds_limit(fft_rows_c2r_kernel));
-  done = 1;
-  return kOk;
+  const size_t lds = tile_lds_bytes(MODE, A.P.n, g.log2T);
+  if (g.emax <= 8) return launch_e<DIR, MODE, OP, 8>(grid, g.threads, lds, s, in, out, tw, A);
+  if (g.emax <= 12) return launch_e<DIR, MODE, OP, 12>(grid, g.threads, lds, s, in, out, tw, A);
+  if (g.emax <= 16) return launch_e<DIR, MODE, OP, 16>(grid, g.threads, lds, s, in, out, tw, A);
+  return launch_e<DIR, MODE, OP, 24>(grid, g.threads, lds, s, in, out, tw, A);
 }
 
 int grid_x(long long ncols, int log2T) { return (int)((ncols + (1LL << log2T) - 1) >> log2T); }
@@ -371,19 +479,26 @@ bool fft_factor(int n, FftRadices* r)
   *r = FftRadices{};
   r->n = n;
   int m = n;
-  std::vector<int> rad;
-  while (m % 8 == 0) {
-    rad.push_back(8);
-    m /= 8;
-  }
-  while (m % 4 == 0) {
-    rad.push_back(4);
-    m /= 4;
-  }
+  int twos = 0;
   while (m % 2 == 0) {
-    rad.push_back(2);
     m /= 2;
+    ++twos;
   }
+  // powers of two as radix-8 passes; a leftover 2 becomes 4 x 4 instead of 8 x 2
+  std::vector<int> rad;
+  int n8 = twos / 3, n4 = 0, n2 = 0;
+  if (twos % 3 == 2) n4 = 1;
+  if (twos % 3 == 1) {
+    if (n8 > 0) {
+      --n8;
+      n4 = 2;
+    } else {
+      n2 = 1;
+    }
+  }
+  for (int i = 0; i < n8; ++i) rad.push_back(8);
+  for (int i = 0; i < n4; ++i) rad.push_back(4);
+  for (int i = 0; i < n2; ++i) rad.push_back(2);
   while (m % 3 == 0) {
     rad.push_back(3);
     m /= 3;
@@ -394,45 +509,84 @@ bool fft_factor(int n, FftRadices* r)
   }
   if (m != 1 || rad.size() > 8 || n < 2) return false;
   r->npass = (int)rad.size();
-  for (size_t i = 0; i < rad.size(); ++i) r->radix[i] = rad[i];
+  int L = 1;
+  for (size_t i = 0; i < rad.size(); ++i) {
+    FftPass& p = r->pass[i];
+    p.R = rad[i];
+    p.L = L;
+    p.m = n / p.R;
+    p.twstep = n / (p.R * L);
+    p.magL = magic_for(L);
+    p.magM = magic_for(p.m);
+    L *= p.R;
+  }
   return true;
 }
 
-int fft_tile_log2(const FftRadices& r, int cap_log2)
+int fft_emax(const FftRadices& r, int log2T, int threads)
 {
-  for (int l = cap_log2; l >= 0; --l) {
-    const long long T = 1LL << l;
-    if (r.n * T > kFftMaxTile) continue;
-    bool ok = true;
-    for (int p = 0; p < r.npass; ++p) {
-      const int R = r.radix[p];
-      const long long nb = (r.n / R) * T;
-      if ((nb + kFftThreads - 1) / kFftThreads > kFftEmax / R) ok = false;
-    }
-    if (ok) return l;
+  const long long T = 1LL << log2T;
+  if (r.n * T > kFftMaxTile) return 0;
+  int e = 0;
+  for (int p = 0; p < r.npass; ++p) {
+    const long long nb = (long long)r.pass[p].m * T;
+    const int q = (int)((nb + threads - 1) / threads);
+    e = std::max(e, q * r.pass[p].R);
   }
-  return -1;
+  return e <= 24 ? e : 0;
+}
+
+// Threads for a tile: the fewest (a multiple of 64, at most
+// kFftMaxThreads) that keep every thread within 8 register slots, else within
+// the smallest slot count reachable.
+FftPassGeom pass_geom(const FftRadices& r, int log2T)
+{
+  FftPassGeom best;
+  best.log2T = log2T;
+  const long long elems = (long long)r.n << log2T;
+  const int t0 = (int)std::max<long long>(64, std::min<long long>(kFftMaxThreads, ((elems / 8 + 63) / 64) * 64));
+  for (int t = t0; t <= kFftMaxThreads; t += 64) {
+    const int e = fft_emax(r, log2T, t);
+    if (e == 0) continue;
+    if (best.emax == 0 || e < best.emax) {
+      best.threads = t;
+      best.emax = e;
+    }
+    if (e <= 8) break;
+  }
+  return best;
+}
+
+// log2 of the widest tile with n*T <= 6144 complex values (48 KiB of LDS)
+static int cap_log2(int n)
+{
+  int l = 0;
+  while (((long long)n << (l + 1)) <= 6144) ++l;
+  return l;
 }
 
 int FftPlan::init(int n_, int force_log2T)
 {
   n = n_;
   if (n % 2 != 0 || !fft_factor(n, &rad)) return kErrUnsupported;
-  const int base = fft_tile_log2(rad, 5);
-  if (base < 0) return kErrUnsupported;
-  log2T_rows = log2T_y = log2T_x = base;
-  if (force_log2T >= 0) {
-    const int l = fft_tile_log2(rad, force_log2T);
-    if (l < 0) return kErrUnsupported;
-    log2T_rows = log2T_y = log2T_x = l;
-  }
-  // no wider tile than there are columns
-  auto fit = [](int l, long long ncols) {
+  // widest power-of-two tile with n*T <= 6144 (48 KiB of LDS), no wider than
+  // the columns, and a register tile of at most 24 values per thread
+  auto pick = [&](int want, long long ncols) -> FftPassGeom {
+    int l = want;
     while (l > 0 && (1LL << (l - 1)) >= ncols) --l;
-    return l;
+    while (l >= 0) {
+      FftPassGeom g = pass_geom(rad, l);
+      if (g.emax > 0) return g;
+      --l;
+    }
+    return FftPassGeom{};
   };
-  log2T_y = fit(log2T_y, n / 2 + 1);
-  log2T_rows = fit(log2T_rows, (long long)n * n / 2);
+  // columns: 8 adjacent complex values (64 B, 128 B per XCD-paired tiles);
+  // rows: 2 row pairs per workgroup (measured fastest on MI355X at n = 768)
+  gx = pick(force_log2T >= 0 ? force_log2T : std::min(3, cap_log2(n)), (long long)n * (n / 2 + 1));
+  gy = pick(force_log2T >= 0 ? force_log2T : std::min(3, cap_log2(n)), n / 2 + 1);
+  grows = pick(force_log2T >= 0 ? force_log2T : 1, (long long)n * n / 2);
+  if (gx.emax == 0 || gy.emax == 0 || grows.emax == 0) return kErrUnsupported;
   std::vector<float2> tw(n);
   for (int k = 0; k < n; ++k) {
     const double a = -2.0 * kPiD * (double)k / (double)n;
@@ -440,7 +594,7 @@ int FftPlan::init(int n_, int force_log2T)
   }
   COLA_HIP(hipMalloc(&d_tw, sizeof(float2) * n));
   COLA_HIP(hipMemcpy(d_tw, tw.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-  return ensure_attributes();
+  return kOk;
 }
 
 void FftPlan::destroy()
@@ -454,35 +608,38 @@ int FftPlan::r2c(float* data, hipStream_t s) const
   const long long nz = n / 2 + 1;
   const int npairs = n * n / 2;
   float2* cdata = reinterpret_cast<float2*>(data);
-  const dim3 blk(kFftThreads);
   auto mark = [&](int pass, int begin) {
     if (hook) hook(hook_user, pass, begin);
   };
+  TileArgs A;
+  A.P = rad;
   // z: rows
-  {
-    const size_t lds = sizeof(float2) * ((size_t)n << log2T_rows);
-    mark(0, 1);
-    hipLaunchKernelGGL(fft_rows_r2c_kernel, dim3(grid_x(npairs, log2T_rows)), blk, lds, s, data, cdata, rad, d_tw,
-                       2 * nz, nz, npairs, log2T_rows);
-    mark(0, 0);
-  }
+  A.log2T = grows.log2T;
+  A.ncols = npairs;
+  A.rs_in = 2 * nz;
+  A.rs_out = nz;
+  mark(0, 1);
+  COLA_TRY((launch_tile<-1, kR2C, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, data, cdata, d_tw, A)));
+  mark(0, 0);
   // y: columns kz of each x plane
-  {
-    const size_t lds = sizeof(float2) * ((size_t)n << log2T_y);
-    mark(1, 1);
-    hipLaunchKernelGGL((fft_cols_kernel<-1, KOp::None>), dim3(grid_x(nz, log2T_y), n), blk, lds, s, cdata, cdata,
-                       rad, d_tw, nz, (long long)n * nz, (int)nz, log2T_y, (int)nz, KOpParams{});
-    mark(1, 0);
-  }
+  A.log2T = gy.log2T;
+  A.ncols = (int)nz;
+  A.sr = nz;
+  A.bs = (long long)n * nz;
+  A.tpb = grid_x(nz, gy.log2T);
+  mark(1, 1);
+  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gy, dim3(A.tpb * n), s, cdata, cdata, d_tw, A)));
+  mark(1, 0);
   // x: columns (y, kz)
-  {
-    const size_t lds = sizeof(float2) * ((size_t)n << log2T_x);
-    const long long ncols = (long long)n * nz;
-    mark(2, 1);
-    hipLaunchKernelGGL((fft_cols_kernel<-1, KOp::None>), dim3(grid_x(ncols, log2T_x), 1), blk, lds, s, cdata, cdata,
-                       rad, d_tw, ncols, 0LL, (int)ncols, log2T_x, (int)nz, KOpParams{});
-    mark(2, 0);
-  }
+  const long long ncols = (long long)n * nz;
+  A.log2T = gx.log2T;
+  A.ncols = (int)ncols;
+  A.sr = ncols;
+  A.bs = 0;
+  A.tpb = grid_x(ncols, gx.log2T);
+  mark(2, 1);
+  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gx, dim3(A.tpb), s, cdata, cdata, d_tw, A)));
+  mark(2, 0);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
@@ -492,50 +649,56 @@ int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hi
   const long long nz = n / 2 + 1;
   const int npairs = n * n / 2;
   float2* cout = reinterpret_cast<float2*>(out);
-  const dim3 blk(kFftThreads);
   auto mark = [&](int pass, int begin) {
     if (hook) hook(hook_user, pass, begin);
   };
-  {
-    const size_t lds = sizeof(float2) * ((size_t)n << log2T_x);
-    const long long ncols = (long long)n * nz;
-    const dim3 grd(grid_x(ncols, log2T_x), 1);
-    const float2* src = in_k ? in_k : cout;
-    mark(3, 1);
-    switch (op) {
-      case KOp::PmPoisson:
-        hipLaunchKernelGGL((fft_cols_kernel<1, KOp::PmPoisson>), grd, blk, lds, s, src, cout, rad, d_tw, ncols, 0LL,
-                           (int)ncols, log2T_x, (int)nz, kp);
-        break;
-      case KOp::LptGrad:
-        hipLaunchKernelGGL((fft_cols_kernel<1, KOp::LptGrad>), grd, blk, lds, s, src, cout, rad, d_tw, ncols, 0LL,
-                           (int)ncols, log2T_x, (int)nz, kp);
-        break;
-      case KOp::Lpt2:
-        hipLaunchKernelGGL((fft_cols_kernel<1, KOp::Lpt2>), grd, blk, lds, s, src, cout, rad, d_tw, ncols, 0LL,
-                           (int)ncols, log2T_x, (int)nz, kp);
-        break;
-      default:
-        hipLaunchKernelGGL((fft_cols_kernel<1, KOp::None>), grd, blk, lds, s, src, cout, rad, d_tw, ncols, 0LL,
-                           (int)ncols, log2T_x, (int)nz, kp);
-        break;
-    }
-    mark(3, 0);
+  TileArgs A;
+  A.P = rad;
+  A.kp = kp;
+  A.nz = (int)nz;
+  A.magNz = magic_for((int)nz);
+  // x: columns (y, kz), k-space operator applied on load
+  const long long ncols = (long long)n * nz;
+  A.log2T = gx.log2T;
+  A.ncols = (int)ncols;
+  A.sr = ncols;
+  A.bs = 0;
+  A.tpb = grid_x(ncols, gx.log2T);
+  const float2* src = in_k ? in_k : cout;
+  const dim3 grid(A.tpb);
+  mark(3, 1);
+  switch (op) {
+    case KOp::PmPoisson:
+      COLA_TRY((launch_tile<1, kCols, KOp::PmPoisson>(gx, grid, s, src, cout, d_tw, A)));
+      break;
+    case KOp::LptGrad:
+      COLA_TRY((launch_tile<1, kCols, KOp::LptGrad>(gx, grid, s, src, cout, d_tw, A)));
+      break;
+    case KOp::Lpt2:
+      COLA_TRY((launch_tile<1, kCols, KOp::Lpt2>(gx, grid, s, src, cout, d_tw, A)));
+      break;
+    default:
+      COLA_TRY((launch_tile<1, kCols, KOp::None>(gx, grid, s, src, cout, d_tw, A)));
+      break;
   }
-  {
-    const size_t lds = sizeof(float2) * ((size_t)n << log2T_y);
-    mark(4, 1);
-    hipLaunchKernelGGL((fft_cols_kernel<1, KOp::None>), dim3(grid_x(nz, log2T_y), n), blk, lds, s, cout, cout, rad,
-                       d_tw, nz, (long long)n * nz, (int)nz, log2T_y, (int)nz, KOpParams{});
-    mark(4, 0);
-  }
-  {
-    const size_t lds = sizeof(float2) * ((size_t)n << log2T_rows);
-    mark(5, 1);
-    hipLaunchKernelGGL(fft_rows_c2r_kernel, dim3(grid_x(npairs, log2T_rows)), blk, lds, s, cout, out, rad, d_tw, nz,
-                       2 * nz, npairs, log2T_rows);
-    mark(5, 0);
-  }
+  mark(3, 0);
+  // y
+  A.log2T = gy.log2T;
+  A.ncols = (int)nz;
+  A.sr = nz;
+  A.bs = (long long)n * nz;
+  A.tpb = grid_x(nz, gy.log2T);
+  mark(4, 1);
+  COLA_TRY((launch_tile<1, kCols, KOp::None>(gy, dim3(A.tpb * n), s, cout, cout, d_tw, A)));
+  mark(4, 0);
+  // z: rows
+  A.log2T = grows.log2T;
+  A.ncols = npairs;
+  A.rs_in = nz;
+  A.rs_out = 2 * nz;
+  mark(5, 1);
+  COLA_TRY((launch_tile<1, kC2R, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, cout, out, d_tw, A)));
+  mark(5, 0);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

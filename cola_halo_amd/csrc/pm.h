@@ -30,9 +30,10 @@ struct CicGeom {
 };
 CicGeom make_cic_geom(int ng, float boxsize, float wpar);
 
-// Tiling of the density grid for the binned deposit.
+// Tiling of the density grid for the binned deposit (power-of-two tiles).
 struct TileGeom {
   int tx = 8, ty = 8, tz = 32;
+  int lx = 3, ly = 3, lz = 5;
   int ntx = 0, nty = 0, ntz = 0;
   long long ntiles = 0;
 };
@@ -45,6 +46,7 @@ struct BinBuffers {
   float* sorted = nullptr;     // 3 floats per entry
   long long capacity = 0;      // entries
   int* overflow = nullptr;     // set when an index fell outside the grid or capacity
+  double* tile_sums = nullptr; // per-tile density sums of the last deposit
 };
 
 int bin_alloc(BinBuffers* b, const TileGeom& t, long long np);

@@ -119,27 +119,67 @@ __global__ void __launch_bounds__(kThreads) k_wrap(DevParticles p, float L)
   }
 }
 
+// Wavefront-aggregated counter update: lanes holding the same key (lattice
+// neighbours usually fall in one tile) elect a leader that applies one atomic
+// for all of them.  delta = +1 counts, -1 claims slots; returns, per lane, the
+// counter value before this lane's unit (for -1: the claimed slot + 1).
+__device__ inline int wave_atomic(int* counters, long long key, bool active, int delta)
+{
+  const int lane = __lane_id();
+  int result = 0;
+  unsigned long long pending = __ballot(active);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const long long lkey = __shfl(key, leader);
+    const unsigned long long mask = __ballot(active && key == lkey) & pending;
+    const int cnt = __popcll(mask);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&counters[lkey], delta * cnt);
+    base = __shfl(base, leader);
+    if ((mask >> lane) & 1ULL) {
+      const int rank = __popcll(mask & ((1ULL << lane) - 1ULL));
+      result = base + delta * rank;
+    }
+    pending &= ~mask;
+  }
+  return result;
+}
+
+__device__ inline void tile_keys(const TileSet& s, const TileGeom& t, long long keys[8], int& nk)
+{
+  nk = 0;
+  for (int a = 0; a < s.nx; ++a)
+    for (int b = 0; b < s.ny; ++b)
+      for (int c = 0; c < s.nz; ++c) keys[nk++] = ((long long)s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c];
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
             float L)
 {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (i >= p.np) return;
-  float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
-  if (wrap_first) {
-    const float x0 = x, y0 = y, z0 = z;
-    wrap1(x, L);
-    wrap1(y, L);
-    wrap1(z, L);
-    if (x != x0) p.x[0][i] = x;
-    if (y != y0) p.x[1][i] = y;
-    if (z != z0) p.x[2][i] = z;
+  const bool live = i < p.np;
+  long long keys[8];
+  int nk = 0;
+  if (live) {
+    float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
+    if (wrap_first) {
+      const float x0 = x, y0 = y, z0 = z;
+      wrap1(x, L);
+      wrap1(y, L);
+      wrap1(z, L);
+      if (x != x0) p.x[0][i] = x;
+      if (y != y0) p.x[1][i] = y;
+      if (z != z0) p.x[2][i] = z;
+    }
+    TileSet ts;
+    if (!tiles_of(x, y, z, g, t, ts)) atomicOr(overflow, 1);
+    tile_keys(ts, t, keys, nk);
   }
-  TileSet s;
-  if (!tiles_of(x, y, z, g, t, s)) atomicOr(overflow, 1);
-  for (int a = 0; a < s.nx; ++a)
-    for (int b = 0; b < s.ny; ++b)
-      for (int c = 0; c < s.nz; ++c) atomicAdd(&counts[((long long)s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c]], 1);
+  // the slot loop is uniform across the wavefront (up to the wave's max)
+  int kmax = nk;
+  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  for (int k = 0; k < kmax; ++k) wave_atomic(counts, k < nk ? keys[k] : 0, k < nk, 1);
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -147,24 +187,36 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
               float* __restrict__ sorted, long long capacity, int* __restrict__ overflow)
 {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  if (i >= p.np) return;
-  const float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
-  TileSet s;
-  tiles_of(x, y, z, g, t, s);
-  for (int a = 0; a < s.nx; ++a)
-    for (int b = 0; b < s.ny; ++b)
-      for (int c = 0; c < s.nz; ++c) {
-        const long long tile = ((long long)s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c];
-        // counts return to zero for the next step as the slots are claimed
-        const long long pos = (long long)offsets[tile] + atomicSub(&counts[tile], 1) - 1;
-        if (pos < capacity) {
-          sorted[3 * pos + 0] = x;
-          sorted[3 * pos + 1] = y;
-          sorted[3 * pos + 2] = z;
-        } else {
-          atomicOr(overflow, 2);
-        }
+  const bool live = i < p.np;
+  long long keys[8];
+  int nk = 0;
+  float x = 0, y = 0, z = 0;
+  if (live) {
+    x = p.x[0][i];
+    y = p.x[1][i];
+    z = p.x[2][i];
+    TileSet ts;
+    tiles_of(x, y, z, g, t, ts);
+    tile_keys(ts, t, keys, nk);
+  }
+  int kmax = nk;
+  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  for (int k = 0; k < kmax; ++k) {
+    const bool act = k < nk;
+    const long long key = act ? keys[k] : 0;
+    // counts return to zero for the next step as the slots are claimed
+    const int before = wave_atomic(counts, key, act, -1);
+    if (act) {
+      const long long pos = (long long)offsets[key] + before - 1;
+      if (pos < capacity) {
+        sorted[3 * pos + 0] = x;
+        sorted[3 * pos + 1] = y;
+        sorted[3 * pos + 2] = z;
+      } else {
+        atomicOr(overflow, 2);
       }
+    }
+  }
 }
 
 // ---- exclusive scan of the tile counts -------------------------------------
@@ -244,18 +296,24 @@ k_scan_final(const int* __restrict__ in, long long n, const int* __restrict__ su
 }
 
 // ---- deposit: one workgroup per tile ----------------------------------------
+__device__ inline double wave_sum(double v)
+{
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const float* __restrict__ sorted,
-                float* __restrict__ density, double* __restrict__ mass_sum)
+                float* __restrict__ density, double* __restrict__ tile_sums)
 {
   extern __shared__ float cells[];
-  __shared__ double red[kThreads];
+  __shared__ double wsum[kThreads / 64];
   const long long tile = blockIdx.x;
   const int tz = (int)(tile % t.ntz);
   const int ty = (int)((tile / t.ntz) % t.nty);
   const int tx = (int)(tile / ((long long)t.ntz * t.nty));
-  const int x0 = tx * t.tx, y0 = ty * t.ty, z0 = tz * t.tz;
-  const int ncell = t.tx * t.ty * t.tz;
+  const int x0 = tx << t.lx, y0 = ty << t.ly, z0 = tz << t.lz;
+  const int ncell = 1 << (t.lx + t.ly + t.lz);
   for (int e = threadIdx.x; e < ncell; e += kThreads) cells[e] = -1.0f;
   __syncthreads();
   const int beg = offsets[tile], end = offsets[tile + 1];
@@ -264,26 +322,48 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
     cic_with_ghost(x, y, z, g, [&](int i, int j, int kk, float w) {
       const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0), lk = (unsigned)(kk - z0);
       if (li < (unsigned)t.tx && lj < (unsigned)t.ty && lk < (unsigned)t.tz)
-        atomicAdd(&cells[(li * t.ty + lj) * t.tz + lk], w);
+        atomicAdd(&cells[(((li << t.ly) + lj) << t.lz) + lk], w);
     });
   }
   __syncthreads();
   double s = 0.0;
+  const int zmask = t.tz - 1, ymask = t.ty - 1;
   for (int e = threadIdx.x; e < ncell; e += kThreads) {
-    const int lk = e % t.tz;
-    const int lj = (e / t.tz) % t.ty;
-    const int li = e / (t.tz * t.ty);
+    const int lk = e & zmask;
+    const int lj = (e >> t.lz) & ymask;
+    const int li = e >> (t.lz + t.ly);
     const float v = cells[e];
     density[((long long)(x0 + li) * g.ng + (y0 + lj)) * g.row + (z0 + lk)] = v;
     s += (double)v;
   }
-  red[threadIdx.x] = s;
+  s = wave_sum(s);
+  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
+    tile_sums[tile] = tot;
   }
-  if (threadIdx.x == 0) atomicAdd(mass_sum, red[0]);
+}
+
+// pm.c:535-562: sum of the per-tile sums into *out (zeroed by the caller)
+__global__ void __launch_bounds__(kThreads) k_sum_doubles(const double* __restrict__ v, long long n, double* out)
+{
+  __shared__ double wsum[kThreads / 64];
+  const long long base = (long long)blockIdx.x * kThreads * 16;
+  double s = 0.0;
+  for (int k = 0; k < 16; ++k) {
+    const long long i = base + (long long)k * kThreads + threadIdx.x;
+    if (i < n) s += v[i];
+  }
+  s = wave_sum(s);
+  if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
+    atomicAdd(out, tot);
+  }
 }
 
 // ---- readout (pm.c:359-412) -------------------------------------------------
@@ -467,13 +547,25 @@ CicGeom make_cic_geom(int ng, float boxsize, float wpar)
   return g;
 }
 
+static int log2_exact(int v)
+{
+  int l = 0;
+  while ((1 << l) < v) ++l;
+  return (1 << l) == v ? l : -1;
+}
+
 bool make_tile_geom(int ng, int tx, int ty, int tz, TileGeom* t)
 {
   if (tx <= 0 || ty <= 0 || tz <= 0 || ng % tx || ng % ty || ng % tz) return false;
   if ((long long)tx * ty * tz > 4096) return false;
+  const int lx = log2_exact(tx), ly = log2_exact(ty), lz = log2_exact(tz);
+  if (lx < 0 || ly < 0 || lz < 0) return false;
   t->tx = tx;
   t->ty = ty;
   t->tz = tz;
+  t->lx = lx;
+  t->ly = ly;
+  t->lz = lz;
   t->ntx = ng / tx;
   t->nty = ng / ty;
   t->ntz = ng / tz;
@@ -491,6 +583,7 @@ int bin_alloc(BinBuffers* b, const TileGeom& t, long long np)
   COLA_HIP(hipMalloc(&b->block_sums, sizeof(int) * (nsums + 1)));
   COLA_HIP(hipMalloc(&b->sorted, sizeof(float) * 3 * b->capacity));
   COLA_HIP(hipMalloc(&b->overflow, sizeof(int)));
+  COLA_HIP(hipMalloc(&b->tile_sums, sizeof(double) * t.ntiles));
   COLA_HIP(hipMemset(b->counts, 0, sizeof(int) * t.ntiles));
   COLA_HIP(hipMemset(b->overflow, 0, sizeof(int)));
   return kOk;
@@ -503,6 +596,7 @@ void bin_free(BinBuffers* b)
   if (b->block_sums) (void)hipFree(b->block_sums);
   if (b->sorted) (void)hipFree(b->sorted);
   if (b->overflow) (void)hipFree(b->overflow);
+  if (b->tile_sums) (void)hipFree(b->tile_sums);
   *b = BinBuffers{};
 }
 
@@ -549,7 +643,10 @@ int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& 
 {
   const size_t lds = sizeof(float) * t.tx * t.ty * t.tz;
   hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)t.ntiles), dim3(kThreads), lds, s, g, t, b.offsets, b.sorted,
-                     density, mass_sum);
+                     density, b.tile_sums);
+  const long long per = (long long)kThreads * 16;
+  hipLaunchKernelGGL(k_sum_doubles, dim3((unsigned)((t.ntiles + per - 1) / per)), dim3(kThreads), 0, s, b.tile_sums,
+                     t.ntiles, mass_sum);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

@@ -137,6 +137,13 @@ int cola_get_density(cola_context* ctx, float* out, size_t n_out);          /* p
 int cola_get_force_grid(cola_context* ctx, int axis, float* out, size_t n_out);
 double cola_last_mass_sum(const cola_context* ctx);                          /* pm.c:535-562 */
 
+/* Device-side copy of the particle state and clocks, and its restore: a
+   resume point between steps that never leaves HBM.  No reference
+   counterpart (the reference has no checkpoint; its GADGET output lacks dx1,
+   dx2 and the COLA-frame velocity). */
+int cola_state_save(cola_context* ctx);
+int cola_state_restore(cola_context* ctx);
+
 /* Device-time profiling with HIP events on the context's stream, replacing
    the reference's wall-clock phase timers (timer.c:11-84).  Phases are
    finer than the reference's categories: one per kernel family, named by
