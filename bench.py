@@ -92,7 +92,16 @@ def main():
     ap.add_argument("--box", type=float, default=BOX)
     ap.add_argument("--tile", type=str, default="", help="deposit tile tx,ty,tz")
     ap.add_argument("--fft-log2-tile", type=int, default=-2, help="FFT transforms per workgroup (log2)")
+    ap.add_argument("--xcd-group", type=int, default=0, help="FFT column tiles per XCD group")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the plain copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import cola_halo_amd as ch
+
+        print(json.dumps({"copy_probe_mib": args.copy_probe,
+                          "gbps": ch.bandwidth_probe(args.copy_probe << 20, 5)}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this build has a single MI355X; "
               "the multi-GPU design (seed sharding, slab decomposition) is described in DESIGN.md",
@@ -110,6 +119,8 @@ def main():
     if args.tile or args.fft_log2_tile != -2:
         tx, ty, tz = (int(v) for v in args.tile.split(",")) if args.tile else (8, 8, 32)
         ctx.set_tiling(tx, ty, tz, args.fft_log2_tile if args.fft_log2_tile != -2 else -1)
+    if args.xcd_group:
+        ctx.set_fft_xcd_group(args.xcd_group)
     ctx.power_init_table(k, p, a_init, 0.8, OM, 1.0 - OM)
     t_ic = time.perf_counter()
     ctx.lpt_set_displacement(a_init, OM, SEED)

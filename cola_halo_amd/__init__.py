@@ -81,8 +81,10 @@ _SIGNATURES = {
     "cola_get_phase_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "cola_reset_phase_stats": (C.c_int, [C.c_void_p]),
     "cola_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cola_set_fft_xcd_group": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_eh98_nowiggle_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
+    "cola_bandwidth_probe": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "cola_fft_r2c_3d": (C.c_int, [C.c_int, C.c_void_p]),
     "cola_fft_c2r_3d": (C.c_int, [C.c_int, C.c_void_p]),
 }
@@ -125,6 +127,13 @@ def eh98_table(omega_m=0.273, omega_b=0.046, h=0.705, ns=0.96, sigma8=0.8, kmin=
     _check(lib().cola_eh98_nowiggle_table(omega_m, omega_b, h, ns, sigma8, kmin, kmax, n, k.ctypes.data,
                                           p.ctypes.data))
     return k, p
+
+
+def bandwidth_probe(nbytes=1 << 30, reps=20):
+    """Copy bandwidth of a plain streaming kernel on the device (GB/s)."""
+    g = C.c_double()
+    _check(lib().cola_bandwidth_probe(nbytes, reps, C.byref(g)))
+    return g.value
 
 
 def padded_real_shape(n):
@@ -293,6 +302,9 @@ class Context:
 
     def set_tiling(self, tx, ty, tz, fft_log2_tile=-1):
         _check(lib().cola_set_tiling(self._h, tx, ty, tz, fft_log2_tile))
+
+    def set_fft_xcd_group(self, group):
+        _check(lib().cola_set_fft_xcd_group(self._h, group))
 
 
 def run_cola(ctx, omega_m, a_final, nsteps, seed, zout, fused=True, on_snapshot=None):

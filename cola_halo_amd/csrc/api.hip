@@ -870,6 +870,56 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
   return kOk;
 }
 
+}  // extern "C"
+
+namespace {
+__global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst, long long n)
+{
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" {
+
+int cola_bandwidth_probe(size_t bytes, int reps, double* gbps)
+{
+  if (bytes < 16 || reps < 1 || !gbps) return kErrArg;
+  const long long n = (long long)(bytes / 16);
+  float4 *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  COLA_HIP(hipMalloc(&a, 16 * n));
+  if (hipMalloc(&b, 16 * n) != hipSuccess) {
+    (void)hipFree(a);
+    return kErrAlloc;
+  }
+  int st = kOk;
+  if (hipMemset(a, 0, 16 * n) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    st = kErrHip;
+  if (!st) {
+    const unsigned blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, nullptr, a, b, n);
+    (void)hipEventRecord(e1, nullptr);
+    float ms = 0;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = kErrHip;
+    *gbps = 2.0 * 16.0 * (double)n * reps / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return st;
+}
+
+int cola_set_fft_xcd_group(cola_context* c, int group)
+{
+  if (!valid(c) || !(group == 1 || group == 2 || group == 4 || group == 8)) return kErrArg;
+  c->pm_fft.xgroup = group;
+  c->lpt_fft.xgroup = group;
+  return kOk;
+}
+
 static int standalone_fft(int n, float* data, bool forward)
 {
   if (n <= 0 || n % 2 || !data) return kErrArg;

@@ -63,6 +63,7 @@ struct FftPlan {
   FftRadices rad;
   float2* d_tw = nullptr;  // exp(-2 pi i k / n), k < n
   FftPassGeom grows, gy, gx;
+  int xgroup = 8;  // adjacent column tiles per XCD dispatch group (measured best)
 
   // Optional per-pass profiling hook: pass 0 z r2c, 1 y fwd, 2 x fwd,
   // 3 x inv, 4 y inv, 5 z c2r; begin = 1 before the launch, 0 after.

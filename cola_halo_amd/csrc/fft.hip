@@ -117,13 +117,15 @@ __device__ inline void dft(float2* v)
 __device__ inline unsigned fdiv(unsigned x, unsigned d, unsigned mag) { return d == 1 ? x : __umulhi(x, mag); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Column tiles 2k and 2k+1 share every 128-byte line they read, so the
-// remap puts them on the same XCD, one dispatch round apart:
-// block 16q + 8h + x (XCD x) -> tile 16q + 2x + h.
-__device__ inline int xcd_pair_remap(int b, int nb)
+// L2.  Adjacent column tiles share the 128-byte lines they read, so groups of
+// G adjacent tiles go to the same XCD in consecutive dispatch rounds:
+// block 8G q + 8h + x (XCD x, h < G) -> tile 8G q + G x + h.
+__device__ inline int xcd_group_remap(int b, int nb, int G)
 {
-  if (b >= (nb & ~15)) return b;
-  return 2 * (8 * (b >> 4) + (b & 7)) + ((b >> 3) & 1);
+  const int span = 8 * G;
+  if (G <= 1 || b >= nb - nb % span) return b;
+  const int q = b / span, r = b - q * span;
+  return span * q + G * (r & 7) + (r >> 3);
 }
 
 // Signed wave number of index i: the PM convention keeps the Nyquist index
@@ -167,6 +169,8 @@ struct TileArgs {
   FftRadices P;
   int log2T;             // transforms per workgroup (columns or row pairs)
   int tpb;               // kCols: tiles per batch (grid is 1-D: batch-major)
+  int tpb_pad;           // kCols: tpb rounded up to a multiple of the XCD group
+  int xgroup;            // kCols: adjacent tiles per XCD group
   int ncols;             // valid columns (kCols) or row pairs (kR2C / kC2R)
   long long sr;          // kCols: stride between positions of one transform
   long long bs;          // kCols: batch stride (blockIdx.y)
@@ -192,9 +196,11 @@ fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __
   const int T = 1 << log2T;
   int c0, batch = 0;
   if (MODE == kCols) {
-    const int lb = xcd_pair_remap(blockIdx.x, gridDim.x);
-    batch = lb / A.tpb;
-    c0 = (lb - batch * A.tpb) << log2T;
+    const int lb = xcd_group_remap(blockIdx.x, gridDim.x, A.xgroup);
+    batch = lb / A.tpb_pad;
+    const int tile = lb - batch * A.tpb_pad;
+    if (tile >= A.tpb) return;  // padding workgroup (before any barrier)
+    c0 = tile << log2T;
   } else {
     c0 = blockIdx.x << log2T;
   }
@@ -472,6 +478,13 @@ int launch_tile(const FftPassGeom& g, dim3 grid, hipStream_t s, const void* in, 
 
 int grid_x(long long ncols, int log2T) { return (int)((ncols + (1LL << log2T) - 1) >> log2T); }
 
+void set_groups(TileArgs& A, int tpb, int group)
+{
+  A.tpb = tpb;
+  A.xgroup = group;
+  A.tpb_pad = (tpb + group - 1) / group * group;
+}
+
 }  // namespace
 
 bool fft_factor(int n, FftRadices* r)
@@ -626,9 +639,9 @@ int FftPlan::r2c(float* data, hipStream_t s) const
   A.ncols = (int)nz;
   A.sr = nz;
   A.bs = (long long)n * nz;
-  A.tpb = grid_x(nz, gy.log2T);
+  set_groups(A, grid_x(nz, gy.log2T), xgroup);
   mark(1, 1);
-  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gy, dim3(A.tpb * n), s, cdata, cdata, d_tw, A)));
+  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gy, dim3(A.tpb_pad * n), s, cdata, cdata, d_tw, A)));
   mark(1, 0);
   // x: columns (y, kz)
   const long long ncols = (long long)n * nz;
@@ -636,9 +649,9 @@ int FftPlan::r2c(float* data, hipStream_t s) const
   A.ncols = (int)ncols;
   A.sr = ncols;
   A.bs = 0;
-  A.tpb = grid_x(ncols, gx.log2T);
+  set_groups(A, grid_x(ncols, gx.log2T), xgroup);
   mark(2, 1);
-  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gx, dim3(A.tpb), s, cdata, cdata, d_tw, A)));
+  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gx, dim3(A.tpb_pad), s, cdata, cdata, d_tw, A)));
   mark(2, 0);
   COLA_HIP(hipGetLastError());
   return kOk;
@@ -663,9 +676,9 @@ int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hi
   A.ncols = (int)ncols;
   A.sr = ncols;
   A.bs = 0;
-  A.tpb = grid_x(ncols, gx.log2T);
+  set_groups(A, grid_x(ncols, gx.log2T), xgroup);
   const float2* src = in_k ? in_k : cout;
-  const dim3 grid(A.tpb);
+  const dim3 grid(A.tpb_pad);
   mark(3, 1);
   switch (op) {
     case KOp::PmPoisson:
@@ -687,9 +700,9 @@ int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hi
   A.ncols = (int)nz;
   A.sr = nz;
   A.bs = (long long)n * nz;
-  A.tpb = grid_x(nz, gy.log2T);
+  set_groups(A, grid_x(nz, gy.log2T), xgroup);
   mark(4, 1);
-  COLA_TRY((launch_tile<1, kCols, KOp::None>(gy, dim3(A.tpb * n), s, cout, cout, d_tw, A)));
+  COLA_TRY((launch_tile<1, kCols, KOp::None>(gy, dim3(A.tpb_pad * n), s, cout, cout, d_tw, A)));
   mark(4, 0);
   // z: rows
   A.log2T = grows.log2T;

@@ -161,6 +161,14 @@ int cola_reset_phase_stats(cola_context* ctx);
    workgroup, -1 = automatic).  Results do not depend on either; the
    autotuner sweeps them for speed. */
 int cola_set_tiling(cola_context* ctx, int tx, int ty, int tz, int fft_log2_tile);
+/* Adjacent FFT column tiles dispatched to one XCD (1, 2, 4 or 8). */
+int cola_set_fft_xcd_group(cola_context* ctx, int group);
+
+/* Device copy of `bytes` bytes by a plain streaming kernel, repeated reps
+   times; returns the mean copy bandwidth (read + write bytes per second,
+   GB/s) in *gbps.  Used to calibrate hardware byte counters and as the
+   achievable-bandwidth reference.  No reference counterpart. */
+int cola_bandwidth_probe(size_t bytes, int reps, double* gbps);
 
 /* Standalone 3-D FFTs on host arrays with FFTW's conventions (unnormalised;
    c2r = c2c over x and y, then c2r over z reading only the real parts of the
