@@ -85,6 +85,13 @@ _SIGNATURES = {
     "cola_eh98_nowiggle_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "cola_bandwidth_probe": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "cola_lpt_seedtable": (C.c_int, [C.c_int, C.c_int, C.c_void_p]),
+    "cola_power_table_sigma8": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "cola_power_table_eval": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    "cola_power_file_sigma8": (C.c_int, [C.c_char_p, C.POINTER(C.c_double)]),
+    "cola_ranlxd1_uniforms": (C.c_int, [C.c_ulong, C.c_int, C.c_void_p]),
+    "cola_growth_scalars": (C.c_int, [C.c_float, C.c_float, C.c_void_p]),
+    "cola_step_factors": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_void_p]),
     "cola_fft_r2c_3d": (C.c_int, [C.c_int, C.c_void_p]),
     "cola_fft_c2r_3d": (C.c_int, [C.c_int, C.c_void_p]),
 }
@@ -127,6 +134,59 @@ def eh98_table(omega_m=0.273, omega_b=0.046, h=0.705, ns=0.96, sigma8=0.8, kmin=
     _check(lib().cola_eh98_nowiggle_table(omega_m, omega_b, h, ns, sigma8, kmin, kmax, n, k.ctypes.data,
                                           p.ctypes.data))
     return k, p
+
+
+def power_table_sigma8(k, pk):
+    """sigma8 of a (k, P) table by power.c's integral (host code)."""
+    k = np.ascontiguousarray(k, np.float64)
+    pk = np.ascontiguousarray(pk, np.float64)
+    s = C.c_double()
+    _check(lib().cola_power_table_sigma8(k.ctypes.data, pk.ctypes.data, len(k), C.byref(s)))
+    return s.value
+
+
+def power_table_eval(k, pk, kq):
+    """power.c PowerSpec of a (k, P) table at the wave numbers kq (host code)."""
+    k = np.ascontiguousarray(k, np.float64)
+    pk = np.ascontiguousarray(pk, np.float64)
+    kq = np.ascontiguousarray(np.atleast_1d(kq), np.float64)
+    out = np.zeros(len(kq))
+    _check(lib().cola_power_table_eval(k.ctypes.data, pk.ctypes.data, len(k), kq.ctypes.data, len(kq),
+                                       out.ctypes.data))
+    return out
+
+
+def power_file_sigma8(filename):
+    s = C.c_double()
+    _check(lib().cola_power_file_sigma8(os.fsencode(filename), C.byref(s)))
+    return s.value
+
+
+def seedtable(nmesh, seed):
+    """lpt.c:172-207 seed table (host code of the library)."""
+    t = np.zeros(nmesh * nmesh, np.uint32)
+    _check(lib().cola_lpt_seedtable(nmesh, seed, t.ctypes.data))
+    return t.reshape(nmesh, nmesh)
+
+
+def ranlxd1_uniforms(seed, n):
+    out = np.zeros(n, np.float64)
+    _check(lib().cola_ranlxd1_uniforms(seed, n, out.ctypes.data))
+    return out
+
+
+def growth_scalars(omega_m, a):
+    """(growthD, growthD2, growthD2v, Qfactor, DprimeQ) of cola.c at a."""
+    out = np.zeros(5)
+    _check(lib().cola_growth_scalars(omega_m, a, out.ctypes.data))
+    return out
+
+
+def step_factors(omega_m, a_x, a_v, avel1, apos1):
+    """(dda, q1, q2, dyyy, da1, da2) of cola_kick / cola_drift."""
+    out = np.zeros(6)
+    _check(lib().cola_step_factors(omega_m, a_x, a_v, avel1, apos1, out.ctypes.data))
+    return out
 
 
 def bandwidth_probe(nbytes=1 << 30, reps=20):

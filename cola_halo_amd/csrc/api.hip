@@ -445,6 +445,77 @@ int cola_eh98_nowiggle_table(double omega_m, double omega_b, double h, double ns
   return kOk;
 }
 
+int cola_power_table_sigma8(const double* k, const double* pk, int n, double* sigma8)
+{
+  if (!k || !pk || n < 2 || !sigma8) return kErrArg;
+  PowerTable t;
+  t.set(k, pk, n);
+  *sigma8 = t.sigma8();
+  return kOk;
+}
+
+int cola_power_table_eval(const double* k, const double* pk, int n, const double* kq, int nq, double* out)
+{
+  if (!k || !pk || n < 2 || (nq > 0 && (!kq || !out))) return kErrArg;
+  PowerTable t;
+  t.set(k, pk, n);
+  for (int i = 0; i < nq; ++i) out[i] = t(kq[i]);
+  return kOk;
+}
+
+int cola_power_file_sigma8(const char* filename, double* sigma8)
+{
+  if (!filename || !sigma8) return kErrArg;
+  PowerTable t;
+  if (t.read(filename) != 0) {
+    set_error(std::string("Error: unable to read input power spectrum: ") + filename);
+    return kErrPowerFile;
+  }
+  if (t.logk.size() < 2) return kErrPowerFile;
+  *sigma8 = t.sigma8();
+  return kOk;
+}
+
+int cola_lpt_seedtable(int nmesh, int seed, unsigned int* table)
+{
+  if (nmesh <= 0 || !table) return kErrArg;
+  lpt_seedtable(nmesh, seed, table);
+  return kOk;
+}
+
+int cola_ranlxd1_uniforms(unsigned long seed, int n, double* out)
+{
+  if (n < 0 || (n > 0 && !out)) return kErrArg;
+  ranlux_uniforms(seed, n, out);
+  return kOk;
+}
+
+int cola_growth_scalars(float omega_m, float a, double* out)
+{
+  if (!out) return kErrArg;
+  const ColaScalars s(omega_m);
+  out[0] = s.growthD(a);
+  out[1] = s.growthD2(a);
+  out[2] = s.growthD2v(a);
+  out[3] = s.Qfactor(a);
+  out[4] = s.DprimeQ(a, 1.0f);
+  return kOk;
+}
+
+int cola_step_factors(float omega_m, float a_x, float a_v, float avel1, float apos1, double* out)
+{
+  if (!out) return kErrArg;
+  const KickFactors k = kick_factors(omega_m, a_x, a_v, avel1);
+  const DriftFactors d = drift_factors(omega_m, a_x, avel1, apos1);
+  out[0] = k.dda;
+  out[1] = k.q1;
+  out[2] = k.q2;
+  out[3] = d.dyyy;
+  out[4] = d.da1;
+  out[5] = d.da2;
+  return kOk;
+}
+
 double cola_power_spec(const cola_context* c, double k)
 {
   if (!c || c->power.empty()) return 0.0;

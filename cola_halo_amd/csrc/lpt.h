@@ -9,6 +9,8 @@ namespace cola {
 
 // lpt.c:172-207 seed table of one RANLUX stream (host, sequential).
 void lpt_seedtable(int nmesh, int seed, unsigned int* table);
+// first n uniforms of the stream seeded with seed (host)
+void ranlux_uniforms(unsigned long long seed, int n, double* out);
 
 // Device copy of the power table (power.c:26-29) for the mode fill.
 struct DevPower {

@@ -279,6 +279,13 @@ void lpt_seedtable(int N, int seed, unsigned int* t)
   }
 }
 
+void ranlux_uniforms(unsigned long long seed, int n, double* out)
+{
+  Ranlux48 r;
+  r.seed(seed);
+  for (int i = 0; i < n; ++i) out[i] = r.uniform();
+}
+
 int launch_mode_fill(int n, const unsigned int* d_seedtable, const DevPower& pw, double box, double fac,
                      float2* cd0, float2* cd1, float2* cd2, hipStream_t s)
 {

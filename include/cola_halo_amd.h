@@ -78,6 +78,13 @@ double cola_growth_factor(const cola_context* ctx, double astart, double aend);
 /* sigma8 of the loaded table by power.c:103-118's integral */
 double cola_power_sigma8(const cola_context* ctx);
 
+/* Host-only power-table helpers (no device needed): sigma8 of a (k, P)
+   table by power.c:103-118's integral; PowerSpec (power.c:120-151) of that
+   table at nq wave numbers; the same read from a file (3000 if unreadable). */
+int cola_power_table_sigma8(const double* k, const double* pk, int n, double* sigma8);
+int cola_power_table_eval(const double* k, const double* pk, int n, const double* kq, int nq, double* out);
+int cola_power_file_sigma8(const char* filename, double* sigma8);
+
 /* Linear P(k) table for the inputs of the configurations: Eisenstein & Hu
    (1998, ApJ 496, 605) no-wiggle transfer function (eqs. 26, 28-31, T_cmb =
    2.728 K) times k^ns, tabulated at n log-spaced k in [kmin, kmax] h/Mpc and
@@ -86,6 +93,19 @@ double cola_power_sigma8(const cola_context* ctx);
    reference reads a CAMB table, power.c:62-101). */
 int cola_eh98_nowiggle_table(double omega_m, double omega_b, double h, double ns, double sigma8,
                              double kmin, double kmax, int n, double* k_out, double* p_out);
+
+/* lpt.c:172-207: the seed table of the white-noise columns (nmesh^2 entries). */
+int cola_lpt_seedtable(int nmesh, int seed, unsigned int* table);
+/* The first n uniforms of the RANLUX stream seeded with seed (luxury 202,
+   GSL ranlxd1 semantics as lpt.c:172-233 uses it). */
+int cola_ranlxd1_uniforms(unsigned long seed, int n, double* out);
+/* cola.c scalars at a, Om = (float)omega_m: out[0] growthD, out[1] growthD2,
+   out[2] growthD2v, out[3] Qfactor, out[4] DprimeQ(a, 1) (cola.c:156-234). */
+int cola_growth_scalars(float omega_m, float a, double* out);
+/* Per-step factors of cola_kick / cola_drift (cola.c:77-91, 123-134):
+   out[0] dda, out[1] q1, out[2] q2, out[3] dyyy, out[4] da1, out[5] da2;
+   the drift starts from a_v = avel1. */
+int cola_step_factors(float omega_m, float a_x, float a_v, float avel1, float apos1, double* out);
 
 /* lpt.c:145-578 lpt_set_displacement (lpt.h:7, main.c:127): 2LPT initial
    conditions for seed on the device; sets a_x = a_init, a_v = 0. */
