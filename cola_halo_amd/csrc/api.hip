@@ -40,6 +40,7 @@ enum Phase : int {
   kPhFftXInv,
   kPhFftYInv,
   kPhFftZC2R,
+  kPhFftXFused,
   kPhReadout,
   kPhReadoutKickDrift,
   kPhKick,
@@ -52,7 +53,7 @@ enum Phase : int {
 };
 const char* const kPhaseNames[kNumPhases] = {
     "wrap",      "bin_count",  "bin_scan",   "bin_scatter", "deposit",   "fft_z_r2c",     "fft_y_fwd",
-    "fft_x_fwd", "fft_x_inv",  "fft_y_inv",  "fft_z_c2r",   "readout",   "readout_kick_drift", "kick",
+    "fft_x_fwd", "fft_x_inv",  "fft_y_inv",  "fft_z_c2r",   "fft_x_fused", "readout",   "readout_kick_drift", "kick",
     "drift",     "snapshot",   "lpt_modes",  "lpt_fft",     "lpt_other"};
 
 struct EventPair {
@@ -166,7 +167,7 @@ int collect_events(cola_context* c)
 void fft_hook(void* user, int pass, int begin)
 {
   cola_context* c = static_cast<cola_context*>(user);
-  static const int map[6] = {kPhFftZR2C, kPhFftYFwd, kPhFftXFwd, kPhFftXInv, kPhFftYInv, kPhFftZC2R};
+  static const int map[7] = {kPhFftZR2C, kPhFftYFwd, kPhFftXFwd, kPhFftXInv, kPhFftYInv, kPhFftZC2R, kPhFftXFused};
   if (begin)
     phase_begin(c, map[pass]);
   else
@@ -253,16 +254,12 @@ int pm_assign(cola_context* c, bool wrap_first)
 // the c2r into the three force grids.
 int pm_fields(cola_context* c)
 {
-  COLA_TRY(c->pm_fft.r2c(c->density, c->stream));
-  c->density_is_real = false;
   const float scale = 2.0 * kPiD / c->boxf;  // pm.c:315
   KOpParams kp;
   kp.f1 = -1.0 / std::pow((double)c->ng, 3.0) / scale;  // pm.c:317
-  for (int a = 0; a < 3; ++a) {
-    kp.axis = a;
-    COLA_TRY(c->pm_fft.c2r(reinterpret_cast<const float2*>(c->density), c->fgrid[a], KOp::PmPoisson, kp, c->stream));
-  }
-  return kOk;
+  c->density_is_real = false;
+  float* const out[3] = {c->fgrid[0], c->fgrid[1], c->fgrid[2]};
+  return c->pm_fft.forces(c->density, out, kp, c->stream);
 }
 
 bool valid(const cola_context* c) { return c != nullptr; }

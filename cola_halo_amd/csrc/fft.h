@@ -66,7 +66,7 @@ struct FftPlan {
   int xgroup = 8;  // adjacent column tiles per XCD dispatch group (measured best)
 
   // Optional per-pass profiling hook: pass 0 z r2c, 1 y fwd, 2 x fwd,
-  // 3 x inv, 4 y inv, 5 z c2r; begin = 1 before the launch, 0 after.
+  // 3 x inv, 4 y inv, 5 z c2r, 6 fused x; begin = 1 before the launch, 0 after.
   void (*hook)(void* user, int pass, int begin) = nullptr;
   void* hook_user = nullptr;
 
@@ -79,6 +79,13 @@ struct FftPlan {
   // c2c over x (optionally out of place from in_k with a k-space operator),
   // then y, then c2r over z, in place on out.
   int c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hipStream_t s) const;
+  // The PM force fields in one sequence (pm.c:282-355): r2c of the padded
+  // real density (in place; destroyed), then for each axis the Poisson
+  // gradient and c2r into out[axis].  The forward x pass, the three Poisson
+  // operators and the three inverse x passes run as one fused kernel
+  // (hook pass 6); delta(k) is never written to memory.
+  int forces(float* density, float* const out[3], const KOpParams& kp, hipStream_t s) const;
+  bool can_fuse() const;
 };
 
 // Radix schedule for n = 2^a 3^b 5^c; returns false otherwise.

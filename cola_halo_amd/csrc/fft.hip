@@ -439,6 +439,198 @@ fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __
   }
 }
 
+// ---- fused x pass: forward x FFT, then per axis Poisson + inverse x FFT ----
+// Column layout helpers (LDS [r][c], column fastest).
+struct ColCtx {
+  float2* lds;
+  const float2* tws;
+  int tid, nthr, log2T;
+};
+
+template <int R, int EMAX, int DIR>
+__device__ inline void col_compute(float2 (&reg)[EMAX], const FftPass& p, const ColCtx& x)
+{
+  const int nb = p.m << x.log2T;
+#pragma unroll
+  for (int q = 0; q < EMAX / R; ++q) {
+    const int u = x.tid + q * x.nthr;
+    if (u < nb) {
+      const int b = u >> x.log2T;
+      if (p.L > 1) {
+        const int i = (int)fdiv((unsigned)b, (unsigned)p.L, p.magL);
+        const int j = b - i * p.L;
+#pragma unroll
+        for (int s = 1; s < R; ++s) {
+          float2 w = x.tws[s * j * p.twstep];
+          if (DIR > 0) w.y = -w.y;
+          reg[q * R + s] = cmul(reg[q * R + s], w);
+        }
+      }
+      dft<R, DIR>(&reg[q * R]);
+    }
+  }
+}
+
+template <int R, int EMAX>
+__device__ inline void col_store_lds(const float2 (&reg)[EMAX], const FftPass& p, const ColCtx& x)
+{
+  const int nb = p.m << x.log2T;
+#pragma unroll
+  for (int q = 0; q < EMAX / R; ++q) {
+    const int u = x.tid + q * x.nthr;
+    if (u < nb) {
+      const int c = u & ((1 << x.log2T) - 1), b = u >> x.log2T;
+      const int i = (int)fdiv((unsigned)b, (unsigned)p.L, p.magL);
+      const int j = b - i * p.L;
+      const int o = i * R * p.L + j;
+#pragma unroll
+      for (int t = 0; t < R; ++t) x.lds[((o + t * p.L) << x.log2T) + c] = reg[q * R + t];
+    }
+  }
+}
+
+template <int R, int EMAX>
+__device__ inline void col_load_lds(float2 (&reg)[EMAX], const FftPass& p, const ColCtx& x)
+{
+  const int nb = p.m << x.log2T;
+#pragma unroll
+  for (int q = 0; q < EMAX / R; ++q) {
+    const int u = x.tid + q * x.nthr;
+    if (u < nb) {
+      const int c = u & ((1 << x.log2T) - 1), b = u >> x.log2T;
+#pragma unroll
+      for (int s = 0; s < R; ++s) reg[q * R + s] = x.lds[((b + s * p.m) << x.log2T) + c];
+    }
+  }
+}
+
+template <class F>
+__device__ inline void radix_case(int R, F&& f)
+{
+  switch (R) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 5: f(std::integral_constant<int, 5>{}); break;
+    default: f(std::integral_constant<int, 8>{}); break;
+  }
+}
+
+// One workgroup: T columns (y, kz) of the half spectrum after the forward z
+// and y passes.  Forward x FFT (plan A.P, radix R_LAST last), then for each
+// axis the PM Poisson operator (pm.c:322-346) and the inverse x FFT with the
+// radices in reverse order (A.P2, radix R_LAST first): the forward's last
+// butterflies are exactly the inverse's first, so delta(k) stays in registers
+// and never goes to memory.  Outputs: the three x-inverted force spectra.
+template <int R_LAST, int EMAX>
+__global__ void __launch_bounds__(kFftMaxThreads)
+fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __restrict__ out1,
+            float2* __restrict__ out2, const float2* __restrict__ tw, TileArgs A, FftRadices Pinv)
+{
+  extern __shared__ float2 smem[];
+  const int n = A.P.n;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int log2T = A.log2T;
+  const int lb = xcd_group_remap(blockIdx.x, gridDim.x, A.xgroup);
+  if (lb >= A.tpb) return;  // padding workgroup (before any barrier)
+  const int c0 = lb << log2T;
+  ColCtx x{smem + n, smem, tid, nthr, log2T};
+  for (int k = tid; k < n; k += nthr) smem[k] = tw[k];
+  using Tile = float2[EMAX];
+  const int np_ = A.P.npass;
+  const int tmask = (1 << log2T) - 1;
+
+  // forward pass 0 from global memory
+  radix_case(A.P.pass[0].R, [&](auto Rc) {
+    constexpr int R = decltype(Rc)::value;
+    const FftPass& p = A.P.pass[0];
+    const int nb = p.m << log2T;
+    Tile reg;
+#pragma unroll
+    for (int q = 0; q < EMAX / R; ++q) {
+      const int u = tid + q * nthr;
+      const int c = u & tmask, b = u >> log2T;
+      const bool live = u < nb && c0 + c < A.ncols;
+      const float2* base = in + c0;
+      const unsigned lane = (unsigned)b * (unsigned)A.sr + (unsigned)c;
+#pragma unroll
+      for (int s = 0; s < R; ++s)
+        reg[q * R + s] = live ? base[(long long)(s * p.m) * A.sr + lane] : make_float2(0.0f, 0.0f);
+    }
+    col_compute<R, EMAX, -1>(reg, p, x);
+    col_store_lds<R, EMAX>(reg, p, x);
+  });
+  __syncthreads();
+  for (int pi = 1; pi < np_ - 1; ++pi) {
+    radix_case(A.P.pass[pi].R, [&](auto Rc) {
+      constexpr int R = decltype(Rc)::value;
+      Tile reg;
+      col_load_lds<R, EMAX>(reg, A.P.pass[pi], x);
+      __syncthreads();
+      col_compute<R, EMAX, -1>(reg, A.P.pass[pi], x);
+      col_store_lds<R, EMAX>(reg, A.P.pass[pi], x);
+    });
+    __syncthreads();
+  }
+  // forward last pass: delta(k) of this tile stays in registers
+  Tile keep;
+  const FftPass& pl = A.P.pass[np_ - 1];
+  col_load_lds<R_LAST, EMAX>(keep, pl, x);
+  __syncthreads();
+  col_compute<R_LAST, EMAX, -1>(keep, pl, x);
+
+  float2* outs[3] = {out0, out1, out2};
+  for (int axis = 0; axis < 3; ++axis) {
+    KOpParams kp = A.kp;
+    kp.axis = axis;
+    Tile reg;
+    // inverse pass 0 (L = 1): operator on the forward outputs b + t*m
+#pragma unroll
+    for (int q = 0; q < EMAX / R_LAST; ++q) {
+      const int u = tid + q * nthr;
+      const int c = u & tmask, b = u >> log2T;
+      const int col = c0 + c;
+      const int iy = (int)fdiv((unsigned)col, (unsigned)A.nz, A.magNz);
+      const int iz = col - iy * A.nz;
+#pragma unroll
+      for (int t = 0; t < R_LAST; ++t)
+        reg[q * R_LAST + t] = apply_kop<KOp::PmPoisson>(keep[q * R_LAST + t], b + t * pl.m, iy, iz, n, kp);
+    }
+    col_compute<R_LAST, EMAX, 1>(reg, Pinv.pass[0], x);
+    col_store_lds<R_LAST, EMAX>(reg, Pinv.pass[0], x);
+    __syncthreads();
+    for (int pi = 1; pi < np_; ++pi) {
+      const bool last = pi == np_ - 1;
+      radix_case(Pinv.pass[pi].R, [&](auto Rc) {
+        constexpr int R = decltype(Rc)::value;
+        const FftPass& p = Pinv.pass[pi];
+        Tile r2;
+        col_load_lds<R, EMAX>(r2, p, x);
+        __syncthreads();
+        col_compute<R, EMAX, 1>(r2, p, x);
+        if (!last) {
+          col_store_lds<R, EMAX>(r2, p, x);
+        } else {
+          // last pass: outputs at b + t*m
+          const int nb = p.m << log2T;
+          float2* out = outs[axis] + c0;
+#pragma unroll
+          for (int q = 0; q < EMAX / R; ++q) {
+            const int u = tid + q * nthr;
+            const int c = u & tmask, b = u >> log2T;
+            if (u < nb && c0 + c < A.ncols) {
+              const unsigned lane = (unsigned)b * (unsigned)A.sr + (unsigned)c;
+#pragma unroll
+              for (int t = 0; t < R; ++t) out[(long long)(t * p.m) * A.sr + lane] = r2[q * R + t];
+            }
+          }
+        }
+      });
+      if (!last) __syncthreads();
+    }
+  }
+}
+
 unsigned magic_for(int d)
 {
   if (d <= 1) return 0;
@@ -653,6 +845,136 @@ int FftPlan::r2c(float* data, hipStream_t s) const
   mark(2, 1);
   COLA_TRY((launch_tile<-1, kCols, KOp::None>(gx, dim3(A.tpb_pad), s, cdata, cdata, d_tw, A)));
   mark(2, 0);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+// The same radices in reverse order (L, m, twiddle stride recomputed).
+static FftRadices reversed(const FftRadices& r)
+{
+  FftRadices v;
+  v.n = r.n;
+  v.npass = r.npass;
+  int L = 1;
+  for (int i = 0; i < r.npass; ++i) {
+    FftPass& p = v.pass[i];
+    p.R = r.pass[r.npass - 1 - i].R;
+    p.L = L;
+    p.m = r.n / p.R;
+    p.twstep = r.n / (p.R * L);
+    p.magL = magic_for(L);
+    p.magM = magic_for(p.m);
+    L *= p.R;
+  }
+  return v;
+}
+
+template <int RL, int E>
+static int launch_fused_e(const FftPassGeom& g, int grid, hipStream_t s, const float2* in, float2* o0, float2* o1,
+                          float2* o2, const float2* tw, const TileArgs& A, const FftRadices& Pinv)
+{
+  static bool attr = false;
+  if (!attr) {
+    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_x_fused<RL, E>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  hipLaunchKernelGGL((fft_x_fused<RL, E>), dim3(grid), dim3(g.threads), tile_lds_bytes(kCols, A.P.n, g.log2T), s, in,
+                     o0, o1, o2, tw, A, Pinv);
+  return kOk;
+}
+
+template <int RL>
+static int launch_fused_r(const FftPassGeom& g, int grid, hipStream_t s, const float2* in, float2* o0, float2* o1,
+                          float2* o2, const float2* tw, const TileArgs& A, const FftRadices& Pinv)
+{
+  if (g.emax <= 8) return launch_fused_e<RL, 8>(g, grid, s, in, o0, o1, o2, tw, A, Pinv);
+  return launch_fused_e<RL, 12>(g, grid, s, in, o0, o1, o2, tw, A, Pinv);
+}
+
+bool FftPlan::can_fuse() const
+{
+  if (rad.npass < 2 || gx.emax > 12) return false;
+  const int RL = rad.pass[rad.npass - 1].R;
+  return RL == 3 || RL == 4 || RL == 5 || RL == 8;
+}
+
+int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hipStream_t s) const
+{
+  if (!can_fuse()) {
+    COLA_TRY(r2c(density, s));
+    KOpParams k = kp;
+    for (int a = 0; a < 3; ++a) {
+      k.axis = a;
+      COLA_TRY(c2r(reinterpret_cast<const float2*>(density), out[a], KOp::PmPoisson, k, s));
+    }
+    return kOk;
+  }
+  const long long nz = n / 2 + 1;
+  const int npairs = n * n / 2;
+  float2* cdata = reinterpret_cast<float2*>(density);
+  auto mark = [&](int pass, int begin) {
+    if (hook) hook(hook_user, pass, begin);
+  };
+  TileArgs A;
+  A.P = rad;
+  A.kp = kp;
+  A.nz = (int)nz;
+  A.magNz = magic_for((int)nz);
+  // z and y forward
+  A.log2T = grows.log2T;
+  A.ncols = npairs;
+  A.rs_in = 2 * nz;
+  A.rs_out = nz;
+  mark(0, 1);
+  COLA_TRY((launch_tile<-1, kR2C, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, density, cdata, d_tw, A)));
+  mark(0, 0);
+  A.log2T = gy.log2T;
+  A.ncols = (int)nz;
+  A.sr = nz;
+  A.bs = (long long)n * nz;
+  set_groups(A, grid_x(nz, gy.log2T), xgroup);
+  mark(1, 1);
+  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gy, dim3(A.tpb_pad * n), s, cdata, cdata, d_tw, A)));
+  mark(1, 0);
+  // fused x: forward, Poisson per axis, inverse
+  const long long ncols = (long long)n * nz;
+  A.log2T = gx.log2T;
+  A.ncols = (int)ncols;
+  A.sr = ncols;
+  A.bs = 0;
+  set_groups(A, grid_x(ncols, gx.log2T), xgroup);
+  const FftRadices Pinv = reversed(rad);
+  float2* o[3] = {reinterpret_cast<float2*>(out[0]), reinterpret_cast<float2*>(out[1]),
+                  reinterpret_cast<float2*>(out[2])};
+  mark(6, 1);
+  switch (rad.pass[rad.npass - 1].R) {
+    case 3: COLA_TRY(launch_fused_r<3>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
+    case 4: COLA_TRY(launch_fused_r<4>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
+    case 5: COLA_TRY(launch_fused_r<5>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
+    default: COLA_TRY(launch_fused_r<8>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
+  }
+  mark(6, 0);
+  // per axis: y inverse, z c2r
+  for (int a = 0; a < 3; ++a) {
+    TileArgs B;
+    B.P = rad;
+    B.log2T = gy.log2T;
+    B.ncols = (int)nz;
+    B.sr = nz;
+    B.bs = (long long)n * nz;
+    set_groups(B, grid_x(nz, gy.log2T), xgroup);
+    mark(4, 1);
+    COLA_TRY((launch_tile<1, kCols, KOp::None>(gy, dim3(B.tpb_pad * n), s, o[a], o[a], d_tw, B)));
+    mark(4, 0);
+    B.log2T = grows.log2T;
+    B.ncols = npairs;
+    B.rs_in = nz;
+    B.rs_out = 2 * nz;
+    mark(5, 1);
+    COLA_TRY((launch_tile<1, kC2R, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, o[a], out[a], d_tw, B)));
+    mark(5, 0);
+  }
   COLA_HIP(hipGetLastError());
   return kOk;
 }
