@@ -45,6 +45,7 @@ def kernel_bytes(phase, nc, ng):
         "fft_y_fwd": 16 * n_c,
         "fft_x_fwd": 16 * n_c,
         "fft_x_inv": 16 * n_c,   # reads delta_k, writes F_k (Poisson fused on load)
+        "fft_x_fused": 32 * n_c,  # reads delta_k once, writes the three F_k
         "fft_y_inv": 16 * n_c,
         "fft_z_c2r": 8 * n_c + 4 * n_g,
         "deposit": 4 * n_g + 12 * n_p,
@@ -53,6 +54,25 @@ def kernel_bytes(phase, nc, ng):
         "bin_scatter": 24 * n_p,
     }
     return table.get(phase)
+
+
+def measured_traffic(kernel, nc, ng):
+    """HBM bytes per launch of `kernel` from the calibrated rocprofv3 counter
+    table (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a copy kernel of known size), or None when the table
+    does not cover this kernel at this workload."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for entry in table.get("workloads", []):
+        if entry.get("nc") == nc and entry.get("ngrid") == ng:
+            k = entry.get("kernels", {}).get(kernel)
+            if k is not None:
+                return int(k["fetch_bytes"] + k["write_bytes"])
+    return None
 
 
 def cpu_baseline(ctx_parts, nc, ng, box, threads):
@@ -169,7 +189,7 @@ def main():
         avg_s = dom_ms / dom_n / 1e3
         achieved = nbytes / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, nc, ng), "kernel": dom,
                 "avg_launch_ms": round(dom_ms / dom_n, 4), "bytes_per_launch": nbytes}
     # whole step against the canonical bytes per step (156 Np + 32 Ng + 64 Nc)
     n_c = ng * ng * (ng // 2 + 1)

@@ -73,6 +73,19 @@ _SIGNATURES = {
     "cola_get_density": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "cola_get_force_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "cola_last_mass_sum": (C.c_double, [C.c_void_p]),
+    "cola_read_parameters": (C.c_int, [C.c_char_p, C.c_void_p]),
+    "cola_confirm_parameters": (C.c_int, [C.c_void_p]),
+    "cola_write_gadget": (C.c_int, [C.c_char_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int]),
+    "cola_write_coarse_grid": (C.c_int, [C.c_char_p, C.c_float, C.c_int, C.c_void_p]),
+    "cola_fof_find_halos": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.POINTER(C.c_longlong)]),
+    "cola_fof_get_halos": (C.c_int, [C.c_void_p, C.c_void_p, C.c_longlong]),
+    "cola_fof_get_labels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "cola_get_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "cola_write_halos": (C.c_int, [C.c_char_p, C.c_void_p, C.c_longlong]),
+    "cola_coarse_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "cola_power_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_state_save": (C.c_int, [C.c_void_p]),
     "cola_state_restore": (C.c_int, [C.c_void_p]),
     "cola_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
@@ -221,6 +234,71 @@ def fft_c2r_3d(spec):
     return buf[:, :, :n].copy()
 
 
+HALO_DTYPE = np.dtype([("nfof", "<i4"), ("x", "<f4", 3), ("v", "<f4", 3)])  # cola_halo, 28 bytes
+
+
+def write_halos(filename, halos):
+    """fof.c:1166-1168 text lines."""
+    halos = np.ascontiguousarray(halos, HALO_DTYPE)
+    _check(lib().cola_write_halos(os.fsencode(filename), halos.ctypes.data, len(halos)))
+
+
+PARAM_MAX_ZOUT = 256
+PARAM_MAX_STRING = 1024
+_PARAM_STRINGS = ("power_spectrum_filename", "fof_filename", "snapshot_filename", "subsample_filename",
+                  "cgrid_filename", "init_filename")
+
+
+class Params(C.Structure):
+    """cola_params (parameters.h:4-49 with fixed-size storage)."""
+    _fields_ = ([("nc", C.c_int), ("pm_nc_factor", C.c_int), ("np_alloc_factor", C.c_double),
+                 ("ntimestep", C.c_int), ("random_seed", C.c_int), ("nrealization", C.c_int),
+                 ("loglevel", C.c_int), ("a_final", C.c_double), ("boxsize", C.c_double),
+                 ("omega_m", C.c_double), ("sigma8", C.c_double), ("h", C.c_double),
+                 ("n_zout", C.c_int), ("zout", C.c_double * PARAM_MAX_ZOUT)]
+                + [f for name in _PARAM_STRINGS
+                   for f in ((name, C.c_char * PARAM_MAX_STRING), ("strlen_" + name, C.c_int))]
+                + [("fof_linking_factor", C.c_double), ("subsample_factor", C.c_double),
+                   ("cgrid_nc", C.c_int), ("write_longid", C.c_int)])
+
+    def as_dict(self):
+        """Plain values; an absent string key is None."""
+        out = {}
+        for name, _ in self._fields_:
+            if name.startswith("strlen_") or name == "n_zout":
+                continue
+            if name == "zout":
+                out["zout"] = [self.zout[i] for i in range(self.n_zout)]
+            elif name in _PARAM_STRINGS:
+                n = getattr(self, "strlen_" + name)
+                out[name] = None if n == 0 else getattr(self, name).decode()
+            else:
+                out[name] = getattr(self, name)
+        return out
+
+
+def read_parameters(filename):
+    """read_param.c semantics; raises ColaError with the reference's abort code."""
+    p = Params()
+    _check(lib().cola_read_parameters(os.fsencode(filename), C.byref(p)))
+    return p
+
+
+def confirm_parameters(p):
+    _check(lib().cola_confirm_parameters(C.byref(p)))
+
+
+def write_gadget(filename, snap, a, boxsize, omega_m, h, long_id=False):
+    snap = np.ascontiguousarray(snap, PARTICLE_MIN_DTYPE)
+    _check(lib().cola_write_gadget(os.fsencode(filename), snap.ctypes.data, len(snap), a, boxsize, omega_m, h,
+                                   1 if long_id else 0))
+
+
+def write_coarse_grid(filename, boxsize, delta):
+    delta = np.ascontiguousarray(delta, np.float32)
+    _check(lib().cola_write_coarse_grid(os.fsencode(filename), boxsize, delta.shape[0], delta.ctypes.data))
+
+
 class Context:
     """One realization's device state: the reference's Particles, PM grids,
     FFT plans and the a_x / a_v clocks (main.c:52-111 set-up)."""
@@ -345,6 +423,48 @@ class Context:
 
     def last_mass_sum(self):
         return lib().cola_last_mass_sum(self._h)
+
+    # friends-of-friends
+    def fof_find_halos(self, ll, nfof_min=32):
+        """FoF on the last snapshot (wraps its positions).  Returns the halo
+        records (HALO_DTYPE), ordered by the smallest member index."""
+        n = C.c_longlong(0)
+        _check(lib().cola_fof_find_halos(self._h, ll, nfof_min, C.byref(n)))
+        out = np.zeros(n.value, HALO_DTYPE)
+        _check(lib().cola_fof_get_halos(self._h, out.ctypes.data, len(out)))
+        return out
+
+    def fof_labels(self):
+        out = np.zeros(self.np, np.int32)
+        _check(lib().cola_fof_get_labels(self._h, out.ctypes.data, len(out)))
+        return out
+
+    def get_snapshot(self):
+        out = np.zeros(self.np, PARTICLE_MIN_DTYPE)
+        _check(lib().cola_get_snapshot(self._h, out.ctypes.data, len(out)))
+        return out
+
+    # output grids
+    _SOURCES = {"particles": 0, "snapshot": 1}
+
+    def coarse_grid(self, n, source="snapshot"):
+        """coarse_grid.c: delta = rho n^3 / np - 1 on an n^3 CIC grid (periodic),
+        C order.  Returns (delta[n, n, n], (sum of rho, sum of delta))."""
+        out = np.empty((n, n, n), np.float32)
+        sums = np.zeros(2, np.float64)
+        _check(lib().cola_coarse_grid(self._h, n, self._SOURCES[source], out.ctypes.data, sums.ctypes.data))
+        return out, (float(sums[0]), float(sums[1]))
+
+    def power_spectrum(self, n, kmin, kmax, nbins, deconvolve=False, subtract_shot=False, source="particles"):
+        """Shell-averaged P(k) of the particles on an n^3 CIC grid.  Returns
+        (mean k, mean P, modes) per bin."""
+        k = np.zeros(nbins)
+        p = np.zeros(nbins)
+        m = np.zeros(nbins)
+        flags = (1 if deconvolve else 0) | (2 if subtract_shot else 0)
+        _check(lib().cola_power_spectrum(self._h, n, self._SOURCES[source], kmin, kmax, nbins, flags,
+                                         k.ctypes.data, p.ctypes.data, m.ctypes.data))
+        return k, p, m
 
     # profiling / tuning
     def set_profiling(self, on=True):

@@ -12,7 +12,9 @@
 #include <vector>
 
 #include "../../include/cola_halo_amd.h"
+#include "analysis.h"
 #include "common.h"
+#include "fof.h"
 #include "cosmo.h"
 #include "fft.h"
 #include "lpt.h"
@@ -103,8 +105,17 @@ struct cola_context {
   bool density_is_real = false;
 
   DevParticleMin* d_snap = nullptr;
+  bool have_snapshot = false;
   float* saved = nullptr;  // x, v, dx1, dx2 copy of cola_state_save
   float saved_ax = 0, saved_av = 0;
+
+  // Output grids (coarse density grid, power spectrum), sized on first use.
+  float* aux_grid = nullptr;
+  long long aux_floats = 0;
+  FftPlan aux_fft;
+  int aux_fft_n = 0;
+  double* d_aux = nullptr;  // 2 sums + 3 * kMaxShellBins shell accumulators
+  FofResult fof;            // last friends-of-friends catalogue
 
   bool profiling = false;
   std::vector<EventPair> events;
@@ -398,6 +409,10 @@ int cola_context_destroy(cola_context* c)
   if (c->d_logD) (void)hipFree(c->d_logD);
   if (c->d_snap) (void)hipFree(c->d_snap);
   if (c->saved) (void)hipFree(c->saved);
+  c->aux_fft.destroy();
+  fof_free(&c->fof);
+  if (c->aux_grid) (void)hipFree(c->aux_grid);
+  if (c->d_aux) (void)hipFree(c->d_aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return kOk;
@@ -713,6 +728,7 @@ static int snapshot_common(cola_context* c, const SnapshotFactors& f, bool nonco
   else
     COLA_TRY(launch_snapshot(c->p, f, c->d_snap, c->stream));
   phase_end(c, kPhSnapshot);
+  c->have_snapshot = true;
   static_assert(sizeof(DevParticleMin) == sizeof(cola_particle_min), "snapshot record layout");
   COLA_HIP(hipMemcpyAsync(out, c->d_snap, sizeof(DevParticleMin) * c->np, hipMemcpyDeviceToHost, c->stream));
   COLA_HIP(hipStreamSynchronize(c->stream));
@@ -947,7 +963,185 @@ __global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ s
 }
 }  // namespace
 
+namespace {
+
+int position_source(cola_context* c, int source, PosSource* src)
+{
+  if (source == COLA_SOURCE_PARTICLES) {
+    COLA_TRY(require_particles(c));
+    for (int a = 0; a < 3; ++a) src->x[a] = c->p.x[a];
+    src->stride = 1;
+  } else if (source == COLA_SOURCE_SNAPSHOT) {
+    if (!c->have_snapshot) {
+      set_error("no snapshot: call cola_set_snapshot or cola_set_noncola_initial first");
+      return kErrState;
+    }
+    const float* base = reinterpret_cast<const float*>(c->d_snap);
+    for (int a = 0; a < 3; ++a) src->x[a] = base + a;
+    src->stride = sizeof(DevParticleMin) / sizeof(float);
+  } else {
+    set_error("unknown position source");
+    return kErrArg;
+  }
+  src->np = c->np;
+  return kOk;
+}
+
+int ensure_aux(cola_context* c, long long floats)
+{
+  if (!c->d_aux) COLA_HIP(hipMalloc(&c->d_aux, sizeof(double) * (2 + 3 * kMaxShellBins)));
+  if (c->aux_floats >= floats) return kOk;
+  if (c->aux_grid) (void)hipFree(c->aux_grid);
+  c->aux_grid = nullptr;
+  c->aux_floats = 0;
+  if (hipMalloc(&c->aux_grid, sizeof(float) * floats) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("unable to allocate the output grid");
+    return kErrAlloc;
+  }
+  c->aux_floats = floats;
+  return kOk;
+}
+
+// coarse_grid.c:49-68 on the device: CIC assignment, then delta = rho n^3 / np - 1.
+int deposit_contrast(cola_context* c, const PosSource& src, int n, long long row)
+{
+  COLA_TRY(launch_cic_periodic(src, n, c->boxf, row, c->aux_grid, c->stream));
+  COLA_TRY(launch_fill_zero(c->d_aux, 2, c->stream));
+  const float nbar_inv = (double)n * n * n / (double)c->np;  // coarse_grid.c:56
+  return launch_density_contrast(n, row, nbar_inv, c->aux_grid, c->d_aux, c->stream);
+}
+
+}  // namespace
+
 extern "C" {
+
+int cola_coarse_grid(cola_context* c, int n, int source, float* delta_out, double* sums_out)
+{
+  if (!valid(c) || n < 1 || n > 4096 || !delta_out) return kErrArg;
+  PosSource src;
+  COLA_TRY(position_source(c, source, &src));
+  const long long ncell = (long long)n * n * n;
+  COLA_TRY(ensure_aux(c, ncell));
+  COLA_TRY(deposit_contrast(c, src, n, n));
+  double sums[2] = {0, 0};
+  COLA_HIP(hipMemcpyAsync(delta_out, c->aux_grid, sizeof(float) * ncell, hipMemcpyDeviceToHost, c->stream));
+  COLA_HIP(hipMemcpyAsync(sums, c->d_aux, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  if (sums_out) {
+    sums_out[0] = sums[0];
+    sums_out[1] = sums[1];
+  }
+  COLA_TRY(check_pending(c));
+  if (!(std::fabs(sums[0] - (double)c->np) < 0.1 * sums[0])) {  // coarse_grid.c:84
+    set_error("coarse grid: assigned mass differs from the particle count by 10% or more");
+    return kErrState;
+  }
+  return kOk;
+}
+
+int cola_power_spectrum(cola_context* c, int n, int source, double kmin, double kmax, int nbins, int flags,
+                        double* k_out, double* p_out, double* nmodes_out)
+{
+  if (!valid(c) || n < 2 || !k_out || !p_out || !nmodes_out) return kErrArg;
+  if (nbins < 1 || nbins > kMaxShellBins || !(kmax > kmin) || !(kmin >= 0.0) || (flags & ~3)) {
+    set_error("power spectrum: need 1 <= nbins <= 2048, 0 <= kmin < kmax and flags in {0..3}");
+    return kErrArg;
+  }
+  PosSource src;
+  COLA_TRY(position_source(c, source, &src));
+  if (c->aux_fft_n != n) {
+    c->aux_fft.destroy();
+    c->aux_fft_n = 0;
+    const int st = c->aux_fft.init(n, c->fft_log2_tile);
+    if (st != kOk) {
+      if (st == kErrUnsupported) set_error("power spectrum: the grid size must be even and of the form 2^a 3^b 5^c");
+      return st;
+    }
+    c->aux_fft_n = n;
+  }
+  const long long row = 2LL * (n / 2 + 1);
+  COLA_TRY(ensure_aux(c, (long long)n * n * row));
+  COLA_TRY(deposit_contrast(c, src, n, row));
+  COLA_TRY(c->aux_fft.r2c(c->aux_grid, c->stream));
+  ShellBins b;
+  b.kmin = kmin;
+  b.dk = (kmax - kmin) / nbins;
+  b.kf = 2.0 * kPiD / c->boxsize;
+  const double vol = c->boxsize * c->boxsize * c->boxsize;
+  const double nn = (double)n * n * n;
+  b.norm = vol / (nn * nn);
+  b.nbins = nbins;
+  b.deconvolve = flags & COLA_PK_DECONVOLVE_CIC;
+  double* acc = c->d_aux + 2;
+  COLA_TRY(launch_fill_zero(acc, 3LL * nbins, c->stream));
+  COLA_TRY(launch_power_shells(n, reinterpret_cast<const float2*>(c->aux_grid), b, acc, c->stream));
+  std::vector<double> h(3 * (size_t)nbins);
+  COLA_HIP(hipMemcpyAsync(h.data(), acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nbins; ++i) {
+    const double cnt = h[3 * i + 2];
+    nmodes_out[i] = cnt;
+    k_out[i] = cnt > 0 ? h[3 * i] / cnt : 0.0;
+    p_out[i] = cnt > 0 ? h[3 * i + 1] / cnt : 0.0;
+    if (cnt > 0 && (flags & COLA_PK_SUBTRACT_SHOT)) p_out[i] -= vol / (double)c->np;
+  }
+  return check_pending(c);
+}
+
+int cola_fof_find_halos(cola_context* c, float ll, int nfof_min, long long* nhalo_out)
+{
+  if (!valid(c) || !nhalo_out || nfof_min < 1) return kErrArg;
+  if (!c->have_snapshot) {
+    set_error("FoF runs on the snapshot: call cola_set_snapshot or cola_set_noncola_initial first");
+    return kErrState;
+  }
+  FofInput in;
+  in.base = reinterpret_cast<float*>(c->d_snap);
+  in.stride = sizeof(DevParticleMin) / sizeof(float);
+  in.np = c->np;
+  in.boxsize = c->boxf;
+  in.ll = ll;
+  in.nfof_min = nfof_min;
+  COLA_TRY(fof_run(in, &c->fof, c->stream));
+  *nhalo_out = (long long)c->fof.halos.size();
+  return check_pending(c);
+}
+
+int cola_fof_get_halos(cola_context* c, cola_halo* out, long long n)
+{
+  if (!valid(c) || (n > 0 && !out)) return kErrArg;
+  if (n < (long long)c->fof.halos.size()) {
+    set_error("halo buffer too small");
+    return kErrArg;
+  }
+  std::copy(c->fof.halos.begin(), c->fof.halos.end(), out);
+  return kOk;
+}
+
+int cola_fof_get_labels(cola_context* c, int* out, size_t n)
+{
+  if (!valid(c) || !out) return kErrArg;
+  if (!c->fof.d_label || n < (size_t)c->fof.np) {
+    set_error(c->fof.d_label ? "label buffer too small" : "no FoF result: call cola_fof_find_halos first");
+    return c->fof.d_label ? kErrArg : kErrState;
+  }
+  COLA_HIP(hipMemcpyAsync(out, c->fof.d_label, sizeof(int) * c->fof.np, hipMemcpyDeviceToHost, c->stream));
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+int cola_get_snapshot(cola_context* c, cola_particle_min* out, size_t np_out)
+{
+  if (!valid(c) || !out || np_out < (size_t)c->np) return kErrArg;
+  if (!c->have_snapshot) {
+    set_error("no snapshot: call cola_set_snapshot or cola_set_noncola_initial first");
+    return kErrState;
+  }
+  COLA_HIP(hipMemcpyAsync(out, c->d_snap, sizeof(DevParticleMin) * c->np, hipMemcpyDeviceToHost, c->stream));
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
 
 int cola_bandwidth_probe(size_t bytes, int reps, double* gbps)
 {

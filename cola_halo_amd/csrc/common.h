@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <string>
 
+#include "errors.h"
+
 namespace cola {
 
 // Status codes reuse the reference's msg_abort codes (see cola_halo_amd.h).
@@ -22,9 +24,6 @@ enum Status : int {
   kErrHip = 9003,
   kErrState = 9004,
 };
-
-void set_error(const std::string& msg);
-const char* last_error();
 
 // Return a status from the enclosing function if a HIP call fails.
 #define COLA_HIP(call)                                                                  \

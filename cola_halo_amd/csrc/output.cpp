@@ -1,0 +1,194 @@
+// Output files of the driver: GADGET-1 snapshots (write.c:14-107) and the
+// coarse density grid (coarse_grid.c:46-91).  Host only; the records come
+// from the device in lattice-ID order.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cola_halo_amd.h"
+#include "errors.h"
+
+namespace {
+
+using cola::set_error;
+
+// GADGET-1 header, 256 bytes (the file format of gadget_file.h:4-23).
+struct GadgetHeader {
+  int32_t np[6];
+  double mass[6];
+  double time;
+  double redshift;
+  int32_t flag_sfr;
+  int32_t flag_feedback;
+  uint32_t np_total[6];
+  int32_t flag_cooling;
+  int32_t num_files;
+  double boxsize;
+  double omega0;
+  double omega_lambda;
+  double hubble_param;
+  int32_t flag_stellarage;
+  int32_t flag_metals;
+  uint32_t np_total_highword[6];
+  int32_t flag_entropy_instead_u;
+  char fill[60];
+};
+static_assert(sizeof(GadgetHeader) == 256, "GADGET-1 header is 256 bytes");
+
+class File {
+ public:
+  explicit File(const char* name) : fp_(std::fopen(name, "wb")) {}
+  ~File()
+  {
+    if (fp_) std::fclose(fp_);
+  }
+  bool ok() const { return fp_ != nullptr && good_; }
+  void put(const void* data, size_t bytes)
+  {
+    if (fp_ && good_ && bytes > 0) good_ = std::fwrite(data, 1, bytes, fp_) == bytes;
+  }
+  void marker(int32_t len) { put(&len, sizeof(len)); }
+  bool close()
+  {
+    const bool closed = fp_ && std::fclose(fp_) == 0;
+    fp_ = nullptr;
+    return closed && good_;
+  }
+
+ private:
+  FILE* fp_;
+  bool good_ = true;
+};
+
+// Stream one record block in bounded chunks through a staging buffer.
+template <typename T, typename F>
+void put_block(File& f, long long np, int per_particle, F fill)
+{
+  constexpr long long kChunk = 1 << 16;
+  std::vector<T> buf(kChunk * per_particle);
+  for (long long i0 = 0; i0 < np; i0 += kChunk) {
+    const long long n = std::min(kChunk, np - i0);
+    for (long long i = 0; i < n; ++i) fill(i0 + i, &buf[i * per_particle]);
+    f.put(buf.data(), sizeof(T) * n * per_particle);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cola_write_gadget(const char* filename, const cola_particle_min* p, long long np, float a, float boxsize_f,
+                      float omega_m_f, float h_f, int use_long_id)
+{
+  if (!filename || (np > 0 && !p) || np < 0) return COLA_ERR_ARG;
+  // The record-length markers are 4-byte ints (write.c:61-100).
+  const long long id_bytes = use_long_id ? 8 : 4;
+  if (np * 12 > INT32_MAX || np * id_bytes > INT32_MAX) {
+    set_error("GADGET-1 record longer than 2^31 bytes: too many particles for one file");
+    return COLA_ERR_ARG;
+  }
+  File f(filename);
+  if (!f.ok()) {
+    set_error(std::string("Error: Unable to write to file: ") + filename);
+    return 9000;
+  }
+  const double h = h_f;
+  const double boxsize = boxsize_f;
+  const double omega_m = omega_m_f;
+  const long long np_total = np;  // one rank
+
+  GadgetHeader header;
+  std::memset(&header, 0, sizeof(header));
+  const double rho_crit = 27.7455;  // write.c:45
+  const double m = omega_m * rho_crit * std::pow(boxsize, 3.0) / np_total;
+  header.np[1] = (int32_t)np;
+  header.mass[1] = m;
+  header.time = a;
+  header.redshift = 1.0 / header.time - 1;
+  header.np_total[1] = (uint32_t)np_total;
+  header.np_total_highword[1] = (uint32_t)(np_total >> 32);
+  header.num_files = 1;
+  header.boxsize = boxsize;
+  header.omega0 = omega_m;
+  header.omega_lambda = 1.0 - omega_m;
+  header.hubble_param = h;
+
+  f.marker(sizeof(GadgetHeader));
+  f.put(&header, sizeof(header));
+  f.marker(sizeof(GadgetHeader));
+
+  const int32_t blk_vec = (int32_t)(np * 12);
+  f.marker(blk_vec);
+  put_block<float>(f, np, 3, [&](long long i, float* o) {
+    for (int k = 0; k < 3; ++k) o[k] = p[i].x[k];
+  });
+  f.marker(blk_vec);
+
+  const float vfac = 1.0 / std::sqrt((double)a);  // write.c:74 (double sqrt), GADGET velocity convention
+  f.marker(blk_vec);
+  put_block<float>(f, np, 3, [&](long long i, float* o) {
+    for (int k = 0; k < 3; ++k) o[k] = vfac * p[i].v[k];
+  });
+  f.marker(blk_vec);
+
+  const int32_t blk_id = (int32_t)(np * id_bytes);
+  f.marker(blk_id);
+  if (use_long_id)
+    put_block<unsigned long long>(f, np, 1, [&](long long i, unsigned long long* o) { *o = p[i].id; });
+  else
+    put_block<unsigned int>(f, np, 1, [&](long long i, unsigned int* o) { *o = (unsigned int)p[i].id; });
+  f.marker(blk_id);
+
+  if (!f.close()) {
+    set_error(std::string("Error: write failed: ") + filename);
+    return 9000;
+  }
+  return COLA_OK;
+}
+
+int cola_write_coarse_grid(const char* filename, float boxsize, int n, const float* delta)
+{
+  if (!filename || n < 1 || !delta) return COLA_ERR_ARG;
+  File f(filename);
+  if (!f.ok()) {
+    set_error(std::string("Unable to write grid to file: ") + filename);
+    return 10021;  // coarse_grid.c:41-43
+  }
+  f.put(&boxsize, sizeof(float));
+  f.put(&n, sizeof(int));
+  f.put(delta, sizeof(float) * (size_t)n * n * n);
+  f.put(&n, sizeof(int));
+  if (!f.close()) {
+    set_error(std::string("write failed: ") + filename);
+    return 10021;
+  }
+  return COLA_OK;
+}
+
+int cola_write_halos(const char* filename, const cola_halo* halos, long long n)
+{
+  if (!filename || (n > 0 && !halos) || n < 0) return COLA_ERR_ARG;
+  FILE* fp = std::fopen(filename, "w");
+  if (!fp) {
+    set_error(std::string("Error: Unable to write halo to file ") + filename);
+    return 8110;  // fof.c:1148-1150
+  }
+  bool ok = true;
+  for (long long i = 0; i < n && ok; ++i) {
+    const cola_halo& h = halos[i];
+    ok = std::fprintf(fp, "%d %e %e %e %e %e %e\n", h.nfof, h.x[0], h.x[1], h.x[2], h.v[0], h.v[1], h.v[2]) > 0;
+  }
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok) {
+    set_error(std::string("write failed: ") + filename);
+    return 8110;
+  }
+  return COLA_OK;
+}
+
+}  // extern "C"

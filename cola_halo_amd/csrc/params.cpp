@@ -1,0 +1,297 @@
+// Parameter file reader of the driver: read_param.c:26-327 and
+// confirm_param.c:8-131 restated on a fixed-size struct, with the reference's
+// syntax, defaults, quirks and abort codes (returned instead of MPI_Abort).
+#include <cctype>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/cola_halo_amd.h"
+#include "errors.h"
+
+namespace {
+
+using cola::set_error;
+
+// read_param.c:26-39: everything from "--" or the first newline on is dropped.
+void remove_comments(char* str)
+{
+  for (char* p = str; *p != '\0'; p++) {
+    if (p[0] == '-' && p[1] == '-') {
+      *p = '\0';
+      return;
+    }
+    if (*p == '\n') {
+      *p = '\0';
+      return;
+    }
+  }
+}
+
+char* skip_spaces(char* p)
+{
+  while (*p == ' ' || *p == '\t') p++;
+  return p;
+}
+
+// read_param.c:47-59: the name ends at a blank or '=' (a tab belongs to the
+// name), and the character that ended it is skipped.  When the line ends
+// right after the name the reference reads past the terminator; here the
+// value is then empty.
+char* get_name(char* p, char* name, size_t cap, bool* too_long)
+{
+  p = skip_spaces(p);
+  size_t n = 0;
+  *too_long = false;
+  while (*p != ' ' && *p != '=' && *p != '\0') {
+    if (n + 1 < cap)
+      name[n++] = *p;
+    else
+      *too_long = true;
+    p++;
+  }
+  name[n] = '\0';
+  return *p == '\0' ? p : p + 1;
+}
+
+char* get_operator(char* p)
+{
+  p = skip_spaces(p);
+  while (*p == '=') p++;
+  return p;
+}
+
+// read_param.c:73-93: a number must start with a digit, otherwise it reads 0
+// (so "-1", "+2" and ".5" are all 0).
+int get_int(char* p)
+{
+  p = skip_spaces(p);
+  return isdigit((unsigned char)*p) ? atoi(p) : 0;
+}
+
+double get_double(char* p)
+{
+  p = skip_spaces(p);
+  return isdigit((unsigned char)*p) ? atof(p) : 0.0;
+}
+
+// read_param.c:95-120: the text between a leading '"' and the next '"' (or the
+// end of the line); strlen counts the terminating NUL.
+int get_string(char* p, char* out, int* len)
+{
+  const std::string original = p;
+  p = skip_spaces(p);
+  if (*p != '"') {
+    set_error("Error: string value not starting from \": " + original);
+    return 1003;
+  }
+  p++;
+  int n = 0;
+  while (*p != '"' && *p != '\0' && *p != '\n') {
+    if (n + 1 >= COLA_PARAM_MAX_STRING) {
+      set_error("Error: string value too long: " + original);
+      return 1003;
+    }
+    out[n++] = *p++;
+  }
+  out[n] = '\0';
+  *len = n + 1;
+  return 0;
+}
+
+// read_param.c:122-136
+int get_bool(char* p, int* out)
+{
+  p = skip_spaces(p);
+  if (*p == 't' || *p == 'T' || *p == '1') {
+    *out = 1;
+    return 0;
+  }
+  if (*p == 'f' || *p == 'F' || *p == '0') {
+    *out = 0;
+    return 0;
+  }
+  set_error(std::string("Error: unknown bool value ") + p);
+  return 1002;
+}
+
+// read_param.c:139-187: "{a, b, ...}".  A token ends at ',', '}', ' ' or the
+// end of the line and must start with a digit, '-', '+' or '.', so a blank
+// before a comma ends the token and the empty token that follows aborts.
+int get_numarray(char* p, double* arr, int cap, int* len)
+{
+  const std::string original = p;
+  while (*p == ' ' || *p == '\t') p++;
+  int n = 0;
+  if (*p == '{') {
+    p++;
+    do {
+      char numbuf[64];
+      int q = 0;
+      p = skip_spaces(p);
+      while (*p != ',' && *p != '}' && *p != ' ' && *p != '\0') {
+        if (q + 1 >= (int)sizeof(numbuf)) {
+          set_error("Error: unable to parse array: " + original);
+          return 1004;
+        }
+        numbuf[q++] = *p++;
+      }
+      numbuf[q] = '\0';
+      const char c0 = numbuf[0];
+      if (!isdigit((unsigned char)c0) && c0 != '-' && c0 != '+' && c0 != '.') {
+        set_error("Error: unable to parse array: " + original);
+        return 1004;
+      }
+      if (n >= cap) {
+        set_error("Error: more output redshifts than the parameter struct holds: " + original);
+        return 1004;
+      }
+      arr[n++] = atof(numbuf);
+      if (*p == ',') {
+        p++;
+      } else if (*p == '}') {
+        *len = n;
+        return 0;
+      }
+    } while (*p != '\0');
+  }
+  set_error("Error in parsing the array: " + original);
+  return 1003;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cola_read_parameters(const char* filename, cola_params* param)
+{
+  if (!filename || !param) return COLA_ERR_ARG;
+  FILE* fp = fopen(filename, "r");
+  if (!fp) {
+    set_error(std::string("Error: Unable to read parameter file: ") + filename);
+    return 1001;  // read_param.c:198-200
+  }
+  std::memset(param, 0, sizeof(*param));
+  // read_param.c:228-251
+  param->omega_m = -1.0;
+  param->sigma8 = -1.0;
+  param->h = -1.0;
+  param->boxsize = -1.0;
+  param->a_final = 1.0;
+  param->pm_nc_factor = 3;
+  param->random_seed = 1;
+  param->nrealization = 1;
+  param->loglevel = 0;
+  param->ntimestep = 0;
+  param->n_zout = 0;
+  param->np_alloc_factor = 1.25;
+  param->fof_linking_factor = 0.2;
+  param->subsample_factor = 0.0;
+  param->cgrid_nc = 0;
+  param->write_longid = 0;
+
+  char buf[1024];
+  char name[64];
+  int st = 0;
+  while (st == 0 && fgets(buf, 1023, fp)) {
+    remove_comments(buf);
+    bool too_long = false;
+    char* p = get_name(buf, name, sizeof(name), &too_long);
+    if (name[0] == '\0') continue;
+    p = get_operator(p);
+    const std::string key = name;
+    if (too_long)
+      st = -1;
+    else if (key == "nc")
+      param->nc = get_int(p);
+    else if (key == "boxsize")
+      param->boxsize = get_double(p);
+    else if (key == "a_final")
+      param->a_final = get_double(p);
+    else if (key == "ntimestep")
+      param->ntimestep = get_int(p);
+    else if (key == "output_redshifts")
+      st = get_numarray(p, param->zout, COLA_PARAM_MAX_ZOUT, &param->n_zout);
+    else if (key == "random_seed")
+      param->random_seed = get_int(p);
+    else if (key == "omega_m")
+      param->omega_m = get_double(p);
+    else if (key == "nrealization")
+      param->nrealization = get_int(p);
+    else if (key == "h")
+      param->h = get_double(p);
+    else if (key == "sigma8")
+      param->sigma8 = get_double(p);
+    else if (key == "pm_nc_factor")
+      param->pm_nc_factor = get_int(p);
+    else if (key == "np_alloc_factor")
+      param->np_alloc_factor = get_double(p);
+    else if (key == "loglevel")
+      param->loglevel = get_int(p);
+    else if (key == "powerspectrum")
+      st = get_string(p, param->power_spectrum_filename, &param->strlen_power_spectrum_filename);
+    else if (key == "fof")
+      st = get_string(p, param->fof_filename, &param->strlen_fof_filename);
+    else if (key == "linking_factor")
+      param->fof_linking_factor = get_double(p);
+    else if (key == "snapshot")
+      st = get_string(p, param->snapshot_filename, &param->strlen_snapshot_filename);
+    else if (key == "subsample")
+      st = get_string(p, param->subsample_filename, &param->strlen_subsample_filename);
+    else if (key == "subsample_factor")
+      param->subsample_factor = get_double(p);
+    else if (key == "coarse_grid")
+      st = get_string(p, param->cgrid_filename, &param->strlen_cgrid_filename);
+    else if (key == "coarse_grid_nc")
+      param->cgrid_nc = get_int(p);
+    else if (key == "initial")
+      st = get_string(p, param->init_filename, &param->strlen_init_filename);
+    else if (key == "write_longid")
+      st = get_bool(p, &param->write_longid);
+    else
+      st = -1;
+    if (st == -1) {
+      set_error("Error: unknown variable name in parameter file= " + key);
+      st = 1007;  // read_param.c:315-316
+    }
+  }
+  fclose(fp);
+  return st;
+}
+
+// confirm_param.c:8-131 (the checks; the driver prints the values).
+int cola_confirm_parameters(const cola_params* p)
+{
+  if (!p) return COLA_ERR_ARG;
+  struct Check {
+    bool bad;
+    int code;
+    const char* msg;
+  };
+  const Check checks[] = {
+      {p->nc <= 0, 11001, "nc is not given is the parameter file"},
+      {p->boxsize <= 0.0, 11002, "boxsize is not given in the parameter file"},
+      {p->ntimestep <= 0, 11003, "ntimestep is not given in the parameter file"},
+      {p->n_zout == 0, 11004, "output_redshifts are not given in the parameter file."},
+      {p->omega_m <= 0.0, 11005, "omega_m is not given in the parameter file."},
+      {p->h <= 0.0, 11006, "h is not given in the parameter file"},
+      {p->sigma8 <= 0.0, 11007, "sigma8 is not given in the parameter file"},
+      {p->strlen_power_spectrum_filename == 0, 11008, "powerspectrum is not given in the parameter file"},
+      {p->strlen_power_spectrum_filename == 1, 11018, "Error: powerspectrum filename is empty"},
+      {p->strlen_fof_filename == 1, 11019, "Error: fof_filename is empty"},
+      {p->strlen_snapshot_filename == 1, 11020, "Error: snapshot filename is empty."},
+      {p->strlen_subsample_filename == 1, 10021, "Error: subsample filename is empty."},
+      {p->strlen_cgrid_filename == 1, 10022, "coarse_grid filename is empty."},
+      {p->strlen_init_filename == 1, 10023, "init filename is empty."},
+  };
+  for (const Check& c : checks) {
+    if (c.bad) {
+      set_error(c.msg);
+      return c.code;
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

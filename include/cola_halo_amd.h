@@ -176,6 +176,133 @@ const char* cola_phase_name(int phase);
 int cola_get_phase_stats(cola_context* ctx, double* ms, long long* launches, int n);
 int cola_reset_phase_stats(cola_context* ctx);
 
+/* ---- Output grids ---------------------------------------------------------
+   Positions are read where they already are on the device: the particle
+   state (COLA_SOURCE_PARTICLES) or the records of the last cola_set_snapshot
+   / cola_set_noncola_initial (COLA_SOURCE_SNAPSHOT, what the reference's
+   snapshot_time outputs use, main.c:346-382). */
+#define COLA_SOURCE_PARTICLES 0
+#define COLA_SOURCE_SNAPSHOT 1
+
+/* Replaces coarse_grid2 (coarse_grid.c:17-93 with cic_mass_assignment,
+   coarse_grid.c:95-121) up to the file write: CIC density with periodic wrap
+   and no WPAR on an n^3 grid, converted to delta = rho * n^3 / np - 1 (float),
+   returned in C order [ix][iy][iz] in delta_out (n^3 floats).  sums_out, if
+   not NULL, receives the sum of rho and the sum of delta (coarse_grid.c:64-66).
+   Returns COLA_ERR_STATE where the reference asserts
+   abs(sum - np) < 0.1 sum (coarse_grid.c:84).  The device summation order
+   differs from the reference's serial loop: cells agree to float rounding. */
+int cola_coarse_grid(cola_context* ctx, int n, int source, float* delta_out, double* sums_out);
+
+/* Shell-averaged power spectrum of the particle distribution (no reference
+   counterpart; the CIC assignment is cola_coarse_grid's): delta on an n^3
+   grid (n even, 2^a 3^b 5^c), r2c FFT, P(k) = L^3 abs(delta_k / n^3)^2 over
+   all modes but k = 0, averaged in nbins linear bins of abs(k) on
+   [kmin, kmax) (h/Mpc).  Per bin: mean abs(k), mean P ((Mpc/h)^3) and the
+   number of modes (0 for an empty bin, whose k and P are 0).  flags:
+   COLA_PK_DECONVOLVE_CIC divides each mode by the CIC window squared,
+   COLA_PK_SUBTRACT_SHOT subtracts L^3 / np from each bin. */
+#define COLA_PK_DECONVOLVE_CIC 1
+#define COLA_PK_SUBTRACT_SHOT 2
+int cola_power_spectrum(cola_context* ctx, int n, int source, double kmin, double kmax, int nbins, int flags,
+                        double* k_out, double* p_out, double* nmodes_out);
+
+/* ---- Friends-of-friends halos -------------------------------------------
+   One halo of the FoF catalogue: the fields of one line of fof.c:1166-1168
+   ("%d %e %e %e %e %e %e": nfof, centre, mean velocity in km/s). */
+typedef struct {
+  int nfof;
+  float x[3];
+  float v[3];
+} cola_halo;
+
+/* Replaces fof_find_halos (fof.c:1035-1084) and the gathering half of
+   fof_write_halos (fof.c:1086-1164) on one rank, on the device copy of the
+   last snapshot: the records' positions are first wrapped into [0, L)
+   (move.c:33-41, the move_particles2_min of fof.c:1044), then every pair
+   with squared distance below ll^2 under the periodic minimum image is
+   linked (fof.c:586-587), and groups of at least nfof_min particles
+   (fof.c:224 uses 32) become halos: centre = x0 + mean of the folded offsets
+   from x0 wrapped into [0, L), velocity = mean record velocity
+   (fof.c:1152-1163).  Halos are ordered by the smallest record index of
+   their group and x0 is that record's position (the reference lists halos
+   in tree-walk order and takes x0 from the walk's first particle).  The
+   driver passes ll = (float)(linking_factor * boxsize / nc) (main.c:353). */
+int cola_fof_find_halos(cola_context* ctx, float ll, int nfof_min, long long* nhalo_out);
+int cola_fof_get_halos(cola_context* ctx, cola_halo* out, long long n);
+/* Per snapshot record: the smallest record index of its group when the group
+   is a halo, else -1 (no reference counterpart; for tests). */
+int cola_fof_get_labels(cola_context* ctx, int* out, size_t n);
+/* The device snapshot records as they are now (after cola_fof_find_halos,
+   with positions wrapped, as the reference writes them after FoF). */
+int cola_get_snapshot(cola_context* ctx, cola_particle_min* out, size_t np_out);
+
+/* fof.c:1147-1175: the halo text file, one "%d %e %e %e %e %e %e" line per
+   halo.  Returns 8110 when the file cannot be opened. */
+int cola_write_halos(const char* filename, const cola_halo* halos, long long n);
+
+/* ---- Driver-side files (host only) -------------------------------------- */
+
+/* Parameters of the driver, parameters.h:4-49 with fixed-size storage.  A
+   string key that is absent has strlen_* = 0; otherwise strlen_* counts the
+   terminating NUL, as in the reference (so "" gives 1). */
+#define COLA_PARAM_MAX_ZOUT 256
+#define COLA_PARAM_MAX_STRING 1024
+typedef struct {
+  int nc;
+  int pm_nc_factor;
+  double np_alloc_factor;
+  int ntimestep;
+  int random_seed;
+  int nrealization;
+  int loglevel;
+  double a_final;
+  double boxsize;
+  double omega_m, sigma8, h;
+  int n_zout;
+  double zout[COLA_PARAM_MAX_ZOUT];
+  char power_spectrum_filename[COLA_PARAM_MAX_STRING];
+  int strlen_power_spectrum_filename;
+  char fof_filename[COLA_PARAM_MAX_STRING];
+  int strlen_fof_filename;
+  char snapshot_filename[COLA_PARAM_MAX_STRING];
+  int strlen_snapshot_filename;
+  char subsample_filename[COLA_PARAM_MAX_STRING];
+  int strlen_subsample_filename;
+  char cgrid_filename[COLA_PARAM_MAX_STRING];
+  int strlen_cgrid_filename;
+  char init_filename[COLA_PARAM_MAX_STRING];
+  int strlen_init_filename;
+  double fof_linking_factor;
+  double subsample_factor;
+  int cgrid_nc;
+  int write_longid;
+} cola_params;
+
+/* read_param.c:190-327 read_parameter_file (one rank; no broadcast): same
+   syntax ("name = value", "--" comments, {a, b} arrays, "strings",
+   t/T/1/f/F/0 booleans, numbers that must start with a digit), defaults and
+   abort codes: 1001 unreadable file, 1002 bad boolean, 1003 bad string or
+   array, 1004 bad array element, 1007 unknown name. */
+int cola_read_parameters(const char* filename, cola_params* out);
+/* confirm_param.c:8-131 confirm_parameters: returns its abort code (11001-11008,
+   11018-11020, 10021-10023) for a missing or empty required value, else 0. */
+int cola_confirm_parameters(const cola_params* p);
+
+/* write.c:14-107 write_snapshot for one rank: a GADGET-1 file with 4-byte
+   record markers, the 256-byte header (gadget_file.h:4-23; particle mass
+   omega_m * 27.7455 * L^3 / np, time a, redshift 1/a - 1, one file),
+   positions, velocities times 1/sqrt(a) and 4- or 8-byte IDs.  `filename`
+   is the full name (the reference appends ".<rank>").  a, boxsize, omega_m
+   and h are the Snapshot struct's floats (particle.h:30-42). Returns 9000
+   when the file cannot be opened (write.c:23-24). */
+int cola_write_gadget(const char* filename, const cola_particle_min* p, long long np, float a, float boxsize,
+                      float omega_m, float h, int use_long_id);
+
+/* coarse_grid.c:46-91 file layout: float boxsize, int n, n^3 float delta in
+   C order, int n.  Returns 10021 when the file cannot be opened. */
+int cola_write_coarse_grid(const char* filename, float boxsize, int n, const float* delta);
+
 /* Deposit tile shape (cells per tile along x, y, z; each must divide ngrid,
    product <= 4096) and FFT tile width (log2 of the transforms per
    workgroup, -1 = automatic).  Results do not depend on either; the

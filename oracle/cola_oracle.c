@@ -1185,6 +1185,279 @@ void orc_set_snapshot(const double aout, const OrcParticle* p, const float* f, i
 }
 
 /* ======================================================================
+ * coarse_grid.c:95-121 (cic_mass_assignment) and coarse_grid.c:59-70
+ * (density contrast of coarse_grid2), serial, in particle order.
+ * ====================================================================== */
+void orc_coarse_grid(const float* pos, long stride, int np, int nc, float boxsize, float* grid)
+{
+  const long ngrid = (long)nc * nc * nc;
+  for (long i = 0; i < ngrid; i++) grid[i] = 0.0f;
+  const float dx_inv = nc / boxsize;
+  for (int i = 0; i < np; i++) {
+    const float* x = pos + (long)i * stride;
+    int ix[3], ix0[3], ix1[3];
+    float w[3];
+    for (int j = 0; j < 3; ++j) {
+      ix[j] = (int)floor(x[j] * dx_inv);
+      w[j] = 1.0f - (x[j] * dx_inv - ix[j]);
+      ix0[j] = (ix[j] + nc) % nc;
+      ix1[j] = (ix[j] + 1 + nc) % nc;
+    }
+#define CG(a, b, c) grid[((long)(a) * nc + (b)) * nc + (c)]
+    CG(ix0[0], ix0[1], ix0[2]) += w[0] * w[1] * w[2];
+    CG(ix0[0], ix1[1], ix0[2]) += w[0] * (1 - w[1]) * w[2];
+    CG(ix0[0], ix0[1], ix1[2]) += w[0] * w[1] * (1 - w[2]);
+    CG(ix0[0], ix1[1], ix1[2]) += w[0] * (1 - w[1]) * (1 - w[2]);
+    CG(ix1[0], ix0[1], ix0[2]) += (1 - w[0]) * w[1] * w[2];
+    CG(ix1[0], ix1[1], ix0[2]) += (1 - w[0]) * (1 - w[1]) * w[2];
+    CG(ix1[0], ix0[1], ix1[2]) += (1 - w[0]) * w[1] * (1 - w[2]);
+    CG(ix1[0], ix1[1], ix1[2]) += (1 - w[0]) * (1 - w[1]) * (1 - w[2]);
+#undef CG
+  }
+}
+
+void orc_density_contrast(float* grid, int nc, long long np_total, double* sums)
+{
+  const long ngrid = (long)nc * nc * nc;
+  const float nbar_inv = (double)ngrid / np_total;
+  double sum = 0.0, delta_sum = 0.0;
+  for (long i = 0; i < ngrid; i++) {
+    sum += grid[i];
+    grid[i] = nbar_inv * grid[i] - 1.0f;
+    delta_sum += grid[i];
+  }
+  sums[0] = sum;
+  sums[1] = delta_sum;
+}
+
+/* Shell-averaged power spectrum.  Not a reference function: the estimator
+   is defined from the discrete Fourier transform itself (full complex
+   spectrum in double precision, every one of the n^3 modes visited once),
+   so it checks the half-spectrum bookkeeping of the device version. */
+static double sinc(double x) { return x == 0.0 ? 1.0 : sin(x) / x; }
+
+void orc_power_spectrum(const float* delta, int n, double boxsize, double kmin, double kmax, int nbins,
+                        int flags, double np_shot, double* k_out, double* p_out, double* nmodes_out)
+{
+  const long nn = (long)n * n * n;
+  double* g = malloc(sizeof(double) * 2 * nn);
+  for (long i = 0; i < nn; i++) {
+    g[2 * i] = delta[i];
+    g[2 * i + 1] = 0.0;
+  }
+  double* line = malloc(sizeof(double) * 2 * n);
+  const long stride[3] = {(long)n * n, n, 1};
+  for (int axis = 0; axis < 3; axis++) {
+    const long s = stride[axis];
+    for (long base = 0; base < nn; base++) {
+      /* visit each line once: its first element has index 0 along axis */
+      if ((base / s) % n != 0) continue;
+      for (int t = 0; t < n; t++) {
+        line[2 * t] = g[2 * (base + t * s)];
+        line[2 * t + 1] = g[2 * (base + t * s) + 1];
+      }
+      orc_fft_1d(n, line, -1);
+      for (int t = 0; t < n; t++) {
+        g[2 * (base + t * s)] = line[2 * t];
+        g[2 * (base + t * s) + 1] = line[2 * t + 1];
+      }
+    }
+  }
+  free(line);
+  for (int b = 0; b < nbins; b++) k_out[b] = p_out[b] = nmodes_out[b] = 0.0;
+  const double kf = 2.0 * M_PI / boxsize;
+  const double dk = (kmax - kmin) / nbins;
+  const double vol = boxsize * boxsize * boxsize;
+  const double norm = vol / ((double)nn * (double)nn);
+  for (int ix = 0; ix < n; ix++)
+    for (int iy = 0; iy < n; iy++)
+      for (int iz = 0; iz < n; iz++) {
+        const int m[3] = {ix <= n / 2 ? ix : ix - n, iy <= n / 2 ? iy : iy - n, iz <= n / 2 ? iz : iz - n};
+        const long long kk2 = (long long)m[0] * m[0] + (long long)m[1] * m[1] + (long long)m[2] * m[2];
+        if (kk2 == 0) continue;
+        const double k = kf * sqrt((double)kk2);
+        const double t = (k - kmin) / dk;
+        if (!(t >= 0.0) || t >= nbins) continue;
+        const int b = (int)floor(t);
+        const long idx = ((long)ix * n + iy) * n + iz;
+        double p = (g[2 * idx] * g[2 * idx] + g[2 * idx + 1] * g[2 * idx + 1]) * norm;
+        if (flags & 1) {
+          double w = 1.0;
+          for (int a = 0; a < 3; a++) {
+            const double s = sinc(M_PI * m[a] / n);
+            w *= s * s;
+          }
+          p /= w * w;
+        }
+        k_out[b] += k;
+        p_out[b] += p;
+        nmodes_out[b] += 1.0;
+      }
+  for (int b = 0; b < nbins; b++) {
+    if (nmodes_out[b] > 0) {
+      k_out[b] /= nmodes_out[b];
+      p_out[b] /= nmodes_out[b];
+      if (flags & 2) p_out[b] -= vol / np_shot;
+    }
+  }
+  free(g);
+}
+
+/* ======================================================================
+ * Friends-of-friends groups (the link rule of fof.c:586-587 on one rank),
+ * serial: cell lists of side >= ll and a union-find over record indices.
+ * ====================================================================== */
+static int uf_find(int* parent, int x)
+{
+  int r = x;
+  while (parent[r] != r) r = parent[r];
+  while (parent[x] != r) {
+    const int next = parent[x];
+    parent[x] = r;
+    x = next;
+  }
+  return r;
+}
+
+static float fold(float d, float box)
+{
+  const float half = 0.5f * box;
+  if (d > half)
+    d -= box;
+  else if (d < -half)
+    d += box;
+  return d;
+}
+
+typedef struct {
+  unsigned long long key; /* cell key << 32 | record index */
+} CellEntry;
+
+static int cmp_cell(const void* a, const void* b)
+{
+  const unsigned long long x = ((const CellEntry*)a)->key, y = ((const CellEntry*)b)->key;
+  return x < y ? -1 : (x > y);
+}
+
+/* first entry with cell key >= k */
+static long cell_lower(const CellEntry* e, long n, unsigned long long k)
+{
+  long lo = 0, hi = n;
+  while (lo < hi) {
+    const long mid = (lo + hi) / 2;
+    if ((e[mid].key >> 32) < k)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+int orc_fof(float* rec, long stride, int np, float box, float ll, int nfof_min, int* label, OrcHalo* halos)
+{
+  for (int i = 0; i < np; i++)
+    for (int a = 0; a < 3; a++) {
+      float* x = rec + (long)i * stride + a;
+      if (*x < 0.0f)
+        *x += box;
+      else if (*x >= box)
+        *x -= box;
+    }
+  /* cell side >= 1.01 ll, so float rounding of a cell index never separates a linked pair by two cells */
+  int nd = (int)floor((double)box / (1.01 * (double)ll));
+  if (nd > 1600) nd = 1600;
+  if (nd < 1) nd = 1;
+  CellEntry* ent = malloc(sizeof(CellEntry) * (np > 0 ? np : 1));
+  int* parent = malloc(sizeof(int) * (np > 0 ? np : 1));
+  int* cell = malloc(sizeof(int) * 3 * (np > 0 ? np : 1));
+  for (int i = 0; i < np; i++) {
+    unsigned long long c = 0;
+    for (int a = 0; a < 3; a++) {
+      int q = (int)floor((double)rec[(long)i * stride + a] / box * nd);
+      if (q >= nd) q = nd - 1;
+      if (q < 0) q = 0;
+      cell[3 * i + a] = q;
+      c = c * nd + q;
+    }
+    ent[i].key = (c << 32) | (unsigned)i;
+    parent[i] = i;
+  }
+  qsort(ent, np, sizeof(CellEntry), cmp_cell);
+  const float ll2 = ll * ll;
+  int offs[3], noff;
+  if (nd >= 3) {
+    offs[0] = -1, offs[1] = 0, offs[2] = 1, noff = 3;
+  } else {
+    offs[0] = 0, offs[1] = 1, noff = nd;
+  }
+  for (int i = 0; i < np; i++) {
+    const float* p = rec + (long)i * stride;
+    for (int a = 0; a < noff; a++)
+      for (int b = 0; b < noff; b++)
+        for (int c = 0; c < noff; c++) {
+          const unsigned long long cx = (cell[3 * i] + offs[a] + nd) % nd;
+          const unsigned long long cy = (cell[3 * i + 1] + offs[b] + nd) % nd;
+          const unsigned long long cz = (cell[3 * i + 2] + offs[c] + nd) % nd;
+          const unsigned long long k = (cx * nd + cy) * nd + cz;
+          for (long e = cell_lower(ent, np, k); e < np && (ent[e].key >> 32) == k; e++) {
+            const int j = (int)(ent[e].key & 0xffffffffu);
+            if (j <= i) continue;
+            const float* q = rec + (long)j * stride;
+            const float dx = fold(p[0] - q[0], box);
+            const float dy = fold(p[1] - q[1], box);
+            const float dz = fold(p[2] - q[2], box);
+            if (dx * dx + dy * dy + dz * dz < ll2) {
+              const int ri = uf_find(parent, i), rj = uf_find(parent, j);
+              if (ri != rj) parent[ri > rj ? ri : rj] = ri < rj ? ri : rj;
+            }
+          }
+        }
+  }
+  free(ent);
+  /* smallest index of a group = its root (links point to smaller roots) */
+  int* count = calloc(np, sizeof(int));
+  for (int i = 0; i < np; i++) count[uf_find(parent, i)]++;
+  int nhalo = 0;
+  int* slot = malloc(sizeof(int) * np);
+  for (int i = 0; i < np; i++) {
+    const int r = uf_find(parent, i);
+    label[i] = count[r] >= nfof_min ? r : -1;
+    slot[i] = -1;
+    if (count[r] >= nfof_min && r == i) slot[i] = nhalo++;
+  }
+  double* sums = calloc((size_t)6 * (nhalo > 0 ? nhalo : 1), sizeof(double));
+  for (int i = 0; i < np; i++) {
+    if (label[i] < 0) continue;
+    const int h = slot[label[i]];
+    const float* x0 = rec + (long)label[i] * stride;
+    const float* p = rec + (long)i * stride;
+    for (int a = 0; a < 3; a++) {
+      sums[6 * h + a] += (double)fold(p[a] - x0[a], box);
+      sums[6 * h + 3 + a] += (double)p[3 + a];
+    }
+  }
+  for (int i = 0; i < np; i++) {
+    if (slot[i] < 0) continue;
+    const int h = slot[i];
+    const float* x0 = rec + (long)i * stride;
+    halos[h].nfof = count[i];
+    for (int a = 0; a < 3; a++) {
+      float x = (float)((double)x0[a] + sums[6 * h + a] / count[i]);
+      if (x < 0.0f) x += box;
+      if (x >= box) x -= box;
+      halos[h].x[a] = x;
+      halos[h].v[a] = (float)(sums[6 * h + 3 + a] / count[i]);
+    }
+  }
+  free(parent);
+  free(cell);
+  free(count);
+  free(slot);
+  free(sums);
+  return nhalo;
+}
+
+/* ======================================================================
  * main.c:67-75, 117-207: linear-in-a leapfrog schedule for one realization.
  * snapshots must hold nout * nc^3 entries; the power table must be set.
  * ====================================================================== */

@@ -99,6 +99,36 @@ void orc_set_snapshot(double aout, const OrcParticle* p, const float* force, int
 void orc_set_noncola_initial(const OrcParticle* p, int np, float Om, float a_x,
                              OrcParticleMin* out);
 
+/* ---- coarse_grid.c and the shell power spectrum ---- */
+/* coarse_grid.c:95-121: zero grid[n^3] (C order), then add the CIC weights of
+   np positions (component a of particle i at pos[i*stride + a]), periodic. */
+void orc_coarse_grid(const float* pos, long stride, int np, int n, float boxsize, float* grid);
+/* coarse_grid.c:59-70: grid = nbar_inv * grid - 1 with nbar_inv = (float)(n^3 / np);
+   sums[0] = sum of the input cells, sums[1] = sum of the output cells. */
+void orc_density_contrast(float* grid, int n, long long np_total, double* sums);
+/* Shell-averaged power spectrum of a real grid delta[n^3] (C order) in a box
+   of side L: P(k) = L^3 |DFT(delta)(k) / n^3|^2 over all n^3 modes except
+   k = 0, in nbins linear bins of |k| on [kmin, kmax).  flags bit 0 divides
+   each mode by the CIC window squared, bit 1 subtracts L^3 / np_shot from the
+   bin mean.  Outputs per bin: mean |k|, mean P, number of modes. */
+void orc_power_spectrum(const float* delta, int n, double boxsize, double kmin, double kmax, int nbins,
+                        int flags, double np_shot, double* k_out, double* p_out, double* nmodes_out);
+
+/* ---- friends-of-friends (fof.c on one rank) ---- */
+typedef struct {
+  int nfof;
+  float x[3];
+  float v[3];
+} OrcHalo;
+/* Wraps rec positions into [0, L) in place (move.c:33-41), links every pair
+   with squared float distance below ll*ll under the periodic minimum image
+   (strict, fof.c:586-587), and for groups of >= nfof_min members writes
+   label[i] = smallest member index (else -1) and one halo per group in order
+   of that index: centre x0 + mean folded offset from x0 (x0 = the smallest
+   member's position), wrapped into [0, L); mean velocity (rec[i*stride+3..5]).
+   Returns the number of halos (halos must hold np entries). */
+int orc_fof(float* rec, long stride, int np, float boxsize, float ll, int nfof_min, int* label, OrcHalo* halos);
+
 /* ---- main.c schedule ---- */
 int orc_run(int nc, int ngrid, double boxsize, double omega_m, double a_final,
             int ntimestep, int seed, const double* zout, int nout,

@@ -63,6 +63,10 @@ def load():
         "orc_set_snapshot": (None, [d, vp, vp, i, f, f, f, vp]),
         "orc_set_noncola_initial": (None, [vp, i, f, f, vp]),
         "orc_run": (i, [i, i, d, d, d, i, i, vp, i, vp, vp, C.POINTER(i)]),
+        "orc_fof": (i, [vp, C.c_long, i, f, f, i, vp, vp]),
+        "orc_coarse_grid": (None, [vp, C.c_long, i, i, f, vp]),
+        "orc_density_contrast": (None, [vp, i, C.c_longlong, vp]),
+        "orc_power_spectrum": (None, [vp, i, d, d, d, i, i, d, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -128,3 +132,46 @@ def pm_forces(ngrid, box, wpar, parts, want_grids=False):
         out["density"] = dens.reshape(padded_real_shape(ngrid))
         out["force_grid"] = fgrid.reshape((3,) + padded_real_shape(ngrid))
     return out
+
+
+def coarse_grid(pos, n, box):
+    """coarse_grid.c:95-121 then :59-70 on positions pos[np, 3] (or a record
+    array with an "x" field).  Returns (delta[n, n, n], (sum rho, sum delta))."""
+    if pos.dtype.names:
+        stride = pos.dtype.itemsize // 4
+        base = pos
+    else:
+        base = np.ascontiguousarray(pos, dtype=np.float32)
+        stride = 3
+    grid = np.zeros(n ** 3, np.float32)
+    lib().orc_coarse_grid(base.ctypes.data, stride, len(pos), n, box, grid.ctypes.data)
+    sums = np.zeros(2)
+    lib().orc_density_contrast(grid.ctypes.data, n, len(pos), sums.ctypes.data)
+    return grid.reshape(n, n, n), (sums[0], sums[1])
+
+
+def power_spectrum(delta, box, kmin, kmax, nbins, flags=0, np_shot=1.0):
+    """Shell power spectrum of a real grid delta[n, n, n]."""
+    delta = np.ascontiguousarray(delta, dtype=np.float32)
+    n = delta.shape[0]
+    k = np.zeros(nbins)
+    p = np.zeros(nbins)
+    m = np.zeros(nbins)
+    lib().orc_power_spectrum(delta.ctypes.data, n, box, kmin, kmax, nbins, flags, float(np_shot), k.ctypes.data,
+                             p.ctypes.data, m.ctypes.data)
+    return k, p, m
+
+
+HALO_DTYPE = np.dtype([("nfof", "<i4"), ("x", "<f4", 3), ("v", "<f4", 3)])
+
+
+def fof(records, box, ll, nfof_min=32):
+    """Friends-of-friends on snapshot records (wrapped in place, like the
+    reference).  Returns (labels, halos)."""
+    assert records.dtype == PARTICLE_MIN_DTYPE and records.flags.c_contiguous
+    n = len(records)
+    label = np.zeros(n, np.int32)
+    halos = np.zeros(max(n, 1), HALO_DTYPE)
+    nh = lib().orc_fof(records.ctypes.data, PARTICLE_MIN_DTYPE.itemsize // 4, n, box, ll, nfof_min,
+                       label.ctypes.data, halos.ctypes.data)
+    return label, halos[:nh].copy()

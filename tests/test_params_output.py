@@ -1,0 +1,158 @@
+"""The driver's parameter reader against the reference's own (read_param.c,
+confirm_param.c, msg.c compiled from the reference tree; fixture
+tests/golden/params_golden.json, and the live binary oracle/_ref/ref_params
+when it is built), and the GADGET-1 / coarse-grid file writers against
+independent numpy readers of the documented layouts."""
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import cola_halo_amd as ch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = json.load(open(os.path.join(HERE, "golden", "params_golden.json")))["cases"]
+REF_BIN = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "ref_params")
+
+
+def _ours(text, confirm, tmp_path):
+    path = tmp_path / "param.lua"
+    path.write_bytes(text.encode())
+    try:
+        p = ch.read_parameters(str(path))
+        if confirm:
+            ch.confirm_parameters(p)
+    except ch.ColaError as e:
+        return e.code, None
+    d = p.as_dict()
+    d["n_zout"] = p.n_zout
+    for name in ("power_spectrum_filename", "fof_filename", "snapshot_filename", "subsample_filename",
+                 "cgrid_filename", "init_filename"):
+        d["strlen_" + name] = getattr(p, "strlen_" + name)
+    return 0, d
+
+
+@pytest.mark.parametrize("case", sorted(GOLDEN))
+def test_parameters_match_reference_reader(case, tmp_path):
+    g = GOLDEN[case]
+    rc, params = _ours(g["text"], g["confirm"], tmp_path)
+    assert rc == g["rc"], (case, rc, g["rc"])
+    if rc == 0:
+        ref = g["params"]
+        for key, val in ref.items():
+            assert params[key] == val, (case, key, params[key], val)
+        assert set(ref) <= set(params)
+
+
+@pytest.mark.skipif(not os.path.exists(REF_BIN), reason="reference reader not built (oracle/ref.mk)")
+def test_fixture_is_current_with_reference_binary(tmp_path):
+    """Re-run a few cases through the live reference build."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen", os.path.join(HERE, "golden", "gen_params_golden.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    for case in ("reference_param_lua", "array_blank_before_comma", "confirm_missing_power", "crlf_line_ends"):
+        got = gen.run_reference(GOLDEN[case]["text"], GOLDEN[case]["confirm"])
+        assert got["rc"] == GOLDEN[case]["rc"]
+        assert got["params"] == GOLDEN[case]["params"]
+
+
+def test_unreadable_parameter_file():
+    with pytest.raises(ch.ColaError) as e:
+        ch.read_parameters("/nonexistent/param.lua")
+    assert e.value.code == 1001
+
+
+# ---------------------------------------------------------------- GADGET-1
+def read_gadget(path):
+    """Independent reader of the GADGET-1 layout of write.c:14-107."""
+    raw = open(path, "rb").read()
+    off = 0
+
+    def record():
+        nonlocal off
+        (n,) = struct.unpack_from("<i", raw, off)
+        body = raw[off + 4: off + 4 + n]
+        (n2,) = struct.unpack_from("<i", raw, off + 4 + n)
+        assert n == n2
+        off += 8 + n
+        return body
+
+    h = record()
+    assert len(h) == 256
+    hdr = {
+        "np": struct.unpack_from("<6i", h, 0),
+        "mass": struct.unpack_from("<6d", h, 24),
+        "time": struct.unpack_from("<d", h, 72)[0],
+        "redshift": struct.unpack_from("<d", h, 80)[0],
+        "np_total": struct.unpack_from("<6I", h, 96),
+        "num_files": struct.unpack_from("<i", h, 124)[0],
+        "boxsize": struct.unpack_from("<d", h, 128)[0],
+        "omega0": struct.unpack_from("<d", h, 136)[0],
+        "omega_lambda": struct.unpack_from("<d", h, 144)[0],
+        "hubble": struct.unpack_from("<d", h, 152)[0],
+        "np_total_highword": struct.unpack_from("<6I", h, 168),
+    }
+    pos = np.frombuffer(record(), "<f4").reshape(-1, 3)
+    vel = np.frombuffer(record(), "<f4").reshape(-1, 3)
+    ids = record()
+    assert off == len(raw)
+    return hdr, pos, vel, ids
+
+
+@pytest.mark.parametrize("long_id", [False, True])
+def test_gadget_writer_layout(tmp_path, long_id):
+    rng = np.random.default_rng(3)
+    n = 1000
+    snap = np.zeros(n, ch.PARTICLE_MIN_DTYPE)
+    snap["x"] = rng.uniform(0, 100, (n, 3))
+    snap["v"] = rng.normal(0, 300, (n, 3))
+    snap["id"] = np.arange(1, n + 1) + (2 ** 33 if long_id else 0)
+    a, box, om, h = np.float32(0.5780347), np.float32(100.0), np.float32(0.273), np.float32(0.705)
+    path = str(tmp_path / "snp00100a.0")
+    ch.write_gadget(path, snap, a, box, om, h, long_id=long_id)
+    hdr, pos, vel, ids = read_gadget(path)
+    assert hdr["np"] == (0, n, 0, 0, 0, 0)
+    assert hdr["mass"][1] == float(om) * 27.7455 * float(box) ** 3 / n   # write.c:45-46
+    assert hdr["time"] == float(a)
+    assert hdr["redshift"] == 1.0 / float(a) - 1
+    assert hdr["np_total"] == (0, n, 0, 0, 0, 0)
+    assert hdr["np_total_highword"] == (0,) * 6
+    assert hdr["num_files"] == 1
+    assert hdr["boxsize"] == float(box)
+    assert hdr["omega0"] == float(om)
+    assert hdr["omega_lambda"] == 1.0 - float(om)
+    assert hdr["hubble"] == float(h)
+    assert np.array_equal(pos, snap["x"])
+    vfac = np.float32(1.0 / np.sqrt(np.float64(a)))   # write.c:74, double sqrt stored to float
+    assert np.array_equal(vel, (vfac * snap["v"]).astype(np.float32))
+    if long_id:
+        assert np.array_equal(np.frombuffer(ids, "<u8"), snap["id"].astype(np.uint64))
+    else:
+        assert np.array_equal(np.frombuffer(ids, "<u4"), snap["id"].astype(np.uint32))
+
+
+def test_gadget_writer_unwritable_path():
+    with pytest.raises(ch.ColaError) as e:
+        ch.write_gadget("/nonexistent/dir/snp.0", np.zeros(2, ch.PARTICLE_MIN_DTYPE), 1.0, 1.0, 0.3, 0.7)
+    assert e.value.code == 9000
+
+
+def test_coarse_grid_file_layout(tmp_path):
+    """tools/density_grid.c:1-28: float boxsize, int nc, nc^3 floats in C
+    order, int nc."""
+    n = 6
+    delta = np.random.default_rng(1).standard_normal((n, n, n)).astype(np.float32)
+    path = str(tmp_path / "grid00001a.b")
+    ch.write_coarse_grid(path, 250.0, delta)
+    raw = open(path, "rb").read()
+    assert len(raw) == 4 + 4 + 4 * n ** 3 + 4
+    assert struct.unpack_from("<f", raw, 0)[0] == 250.0
+    assert struct.unpack_from("<i", raw, 4)[0] == n
+    assert np.array_equal(np.frombuffer(raw, "<f4", n ** 3, 8).reshape(n, n, n), delta)
+    assert struct.unpack_from("<i", raw, 8 + 4 * n ** 3)[0] == n
+    with pytest.raises(ch.ColaError) as e:
+        ch.write_coarse_grid("/nonexistent/grid.b", 1.0, delta)
+    assert e.value.code == 10021
