@@ -83,6 +83,10 @@ _SIGNATURES = {
     "cola_fof_get_labels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "cola_get_snapshot": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "cola_write_halos": (C.c_int, [C.c_char_p, C.c_void_p, C.c_longlong]),
+    "cola_subsampler_create": (C.c_int, [C.c_double, C.c_uint, C.POINTER(C.c_void_p)]),
+    "cola_subsampler_destroy": (C.c_int, [C.c_void_p]),
+    "cola_write_subsample": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
+                                       C.c_float, C.c_float, C.POINTER(C.c_longlong)]),
     "cola_coarse_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "cola_power_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -241,6 +245,29 @@ def write_halos(filename, halos):
     """fof.c:1166-1168 text lines."""
     halos = np.ascontiguousarray(halos, HALO_DTYPE)
     _check(lib().cola_write_halos(os.fsencode(filename), halos.ctypes.data, len(halos)))
+
+
+class Subsampler:
+    """subsample.c: RANLUX stream seeded with 2 * seed, kept across outputs."""
+
+    def __init__(self, factor, seed):
+        self._h = C.c_void_p()
+        _check(lib().cola_subsampler_create(factor, seed, C.byref(self._h)))
+
+    def write(self, filename, snap, a, boxsize, omega_m, h):
+        snap = np.ascontiguousarray(snap, PARTICLE_MIN_DTYPE)
+        n = C.c_longlong(0)
+        _check(lib().cola_write_subsample(self._h, os.fsencode(filename), snap.ctypes.data, len(snap), a, boxsize,
+                                          omega_m, h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self._h:
+            lib().cola_subsampler_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 PARAM_MAX_ZOUT = 256

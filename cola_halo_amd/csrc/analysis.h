@@ -15,16 +15,18 @@ struct PosSource {
   long long np = 0;
 };
 
-// coarse_grid.c:95-121 on an n^3 grid with rows of `row` floats (n for the
-// C-order file layout, 2(n/2+1) for an in-place r2c): zero the grid, then add
-// every particle's eight CIC weights with periodic wrap (float atomics, so the
-// summation order, and with it the last bit, is not the reference's serial one).
-int launch_cic_periodic(const PosSource& src, int n, float boxsize, long long row, float* grid, hipStream_t s);
+// coarse_grid.c:95-121 on an n^3 grid: zero `fix` (n^3 cells, C order),
+// then add every particle's eight CIC weights with periodic wrap as 32.32
+// fixed-point integers (exact for weights >= 2^-9), so the sums do not depend
+// on the order of the atomics and the grid is bit-reproducible.
+int launch_cic_periodic(const PosSource& src, int n, float boxsize, unsigned long long* fix, hipStream_t s);
 
-// coarse_grid.c:59-70 over the n^3 cells: grid = nbar_inv * grid - 1.
-// sums[0] += sum of the input cells, sums[1] += sum of the output cells
-// (sums zeroed by the caller).
-int launch_density_contrast(int n, long long row, float nbar_inv, float* grid, double* sums, hipStream_t s);
+// coarse_grid.c:59-70: rho = (float) fixed-point sum, grid = nbar_inv * rho - 1
+// (float, as the reference), written to a grid with rows of `row` floats (n
+// for the C-order file layout, 2(n/2+1) for an in-place r2c).  sums[0] += sum
+// of rho, sums[1] += sum of delta (sums zeroed by the caller).
+int launch_density_contrast(int n, long long row, float nbar_inv, const unsigned long long* fix, float* grid,
+                            double* sums, hipStream_t s);
 
 // Shell bins of |k| = kf * |m| on [kmin, kmin + nbins * dk).
 struct ShellBins {

@@ -15,13 +15,21 @@ __device__ inline double wave_sum_d(double v)
   return v;
 }
 
+constexpr float kFix = 4294967296.0f;  // 2^32
+constexpr double kUnfix = 1.0 / 4294967296.0;
+
+__device__ inline void add_fix(unsigned long long* cell, float w)
+{
+  atomicAdd(cell, (unsigned long long)rintf(w * kFix));
+}
+
 // coarse_grid.c:95-121.  One thread per particle; the index arithmetic and
 // the weight products follow the reference term by term: X = x * dx_inv in
 // float, ix = floor(X), w = 1 - (X - ix), neighbours (ix + nc) % nc and
 // (ix + 1 + nc) % nc (computed here with a modulo that also holds below -nc,
 // where the reference would index out of bounds).
 __global__ void __launch_bounds__(kThreads)
-k_cic_periodic(PosSource src, int n, float dx_inv, long long row, float* __restrict__ grid)
+k_cic_periodic(PosSource src, int n, float dx_inv, unsigned long long* __restrict__ fix)
 {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i >= src.np) return;
@@ -36,32 +44,31 @@ k_cic_periodic(PosSource src, int n, float dx_inv, long long row, float* __restr
     i0[j] = m;
     i1[j] = m + 1 == n ? 0 : m + 1;
   }
-  const long long r00 = ((long long)i0[0] * n + i0[1]) * row, r01 = ((long long)i0[0] * n + i1[1]) * row;
-  const long long r10 = ((long long)i1[0] * n + i0[1]) * row, r11 = ((long long)i1[0] * n + i1[1]) * row;
-  unsafeAtomicAdd(&grid[r00 + i0[2]], w[0] * w[1] * w[2]);
-  unsafeAtomicAdd(&grid[r01 + i0[2]], w[0] * (1 - w[1]) * w[2]);
-  unsafeAtomicAdd(&grid[r00 + i1[2]], w[0] * w[1] * (1 - w[2]));
-  unsafeAtomicAdd(&grid[r01 + i1[2]], w[0] * (1 - w[1]) * (1 - w[2]));
-  unsafeAtomicAdd(&grid[r10 + i0[2]], (1 - w[0]) * w[1] * w[2]);
-  unsafeAtomicAdd(&grid[r11 + i0[2]], (1 - w[0]) * (1 - w[1]) * w[2]);
-  unsafeAtomicAdd(&grid[r10 + i1[2]], (1 - w[0]) * w[1] * (1 - w[2]));
-  unsafeAtomicAdd(&grid[r11 + i1[2]], (1 - w[0]) * (1 - w[1]) * (1 - w[2]));
+  const long long r00 = ((long long)i0[0] * n + i0[1]) * n, r01 = ((long long)i0[0] * n + i1[1]) * n;
+  const long long r10 = ((long long)i1[0] * n + i0[1]) * n, r11 = ((long long)i1[0] * n + i1[1]) * n;
+  add_fix(&fix[r00 + i0[2]], w[0] * w[1] * w[2]);
+  add_fix(&fix[r01 + i0[2]], w[0] * (1 - w[1]) * w[2]);
+  add_fix(&fix[r00 + i1[2]], w[0] * w[1] * (1 - w[2]));
+  add_fix(&fix[r01 + i1[2]], w[0] * (1 - w[1]) * (1 - w[2]));
+  add_fix(&fix[r10 + i0[2]], (1 - w[0]) * w[1] * w[2]);
+  add_fix(&fix[r11 + i0[2]], (1 - w[0]) * (1 - w[1]) * w[2]);
+  add_fix(&fix[r10 + i1[2]], (1 - w[0]) * w[1] * (1 - w[2]));
+  add_fix(&fix[r11 + i1[2]], (1 - w[0]) * (1 - w[1]) * (1 - w[2]));
 }
 
 // coarse_grid.c:63-68: delta = nbar_inv * rho - 1 (float), with the double
-// sums of rho and delta.  Cell c of the n^3 grid sits at (c / n) * row + c % n.
+// sums of rho and delta.  Cell c of the n^3 grid goes to (c / n) * row + c % n.
 __global__ void __launch_bounds__(kThreads)
-k_density_contrast(long long ncell, int n, long long row, float nbar_inv, float* __restrict__ grid,
-                   double* __restrict__ sums)
+k_density_contrast(long long ncell, int n, long long row, float nbar_inv, const unsigned long long* __restrict__ fix,
+                   float* __restrict__ grid, double* __restrict__ sums)
 {
   __shared__ double part[2][kThreads / 64];
   double s_in = 0.0, s_out = 0.0;
   const long long stride = (long long)gridDim.x * kThreads;
   for (long long c = (long long)blockIdx.x * kThreads + threadIdx.x; c < ncell; c += stride) {
-    const long long a = (c / n) * row + c % n;
-    const float rho = grid[a];
+    const float rho = (float)((double)fix[c] * kUnfix);
     const float d = nbar_inv * rho - 1.0f;
-    grid[a] = d;
+    grid[(c / n) * row + c % n] = d;
     s_in += (double)rho;
     s_out += (double)d;
   }
@@ -159,26 +166,30 @@ int launch_fill_zero(double* p, long long n, hipStream_t s)
   return kOk;
 }
 
-int launch_cic_periodic(const PosSource& src, int n, float boxsize, long long row, float* grid, hipStream_t s)
+int launch_cic_periodic(const PosSource& src, int n, float boxsize, unsigned long long* fix, hipStream_t s)
 {
-  if (n < 1 || row < n) {
+  if (n < 1) {
     set_error("coarse grid: invalid grid size");
     return kErrArg;
   }
-  COLA_TRY(launch_fill_zero(grid, (long long)n * n * row, s));
+  const long long ncell = (long long)n * n * n;
+  hipLaunchKernelGGL(k_fill_zero<unsigned long long>, dim3(capped_blocks(ncell, kThreads * 8LL, 65536)),
+                     dim3(kThreads), 0, s, fix, ncell);
+  COLA_HIP(hipGetLastError());
   if (src.np == 0) return kOk;
   const float dx_inv = n / boxsize;  // coarse_grid.c:99
   hipLaunchKernelGGL(k_cic_periodic, dim3((unsigned)((src.np + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, src,
-                     n, dx_inv, row, grid);
+                     n, dx_inv, fix);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
 
-int launch_density_contrast(int n, long long row, float nbar_inv, float* grid, double* sums, hipStream_t s)
+int launch_density_contrast(int n, long long row, float nbar_inv, const unsigned long long* fix, float* grid,
+                            double* sums, hipStream_t s)
 {
   const long long ncell = (long long)n * n * n;
   hipLaunchKernelGGL(k_density_contrast, dim3(capped_blocks(ncell, kThreads * 16LL, 8192)), dim3(kThreads), 0, s,
-                     ncell, n, row, nbar_inv, grid, sums);
+                     ncell, n, row, nbar_inv, fix, grid, sums);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

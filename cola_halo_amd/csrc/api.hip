@@ -112,6 +112,8 @@ struct cola_context {
   // Output grids (coarse density grid, power spectrum), sized on first use.
   float* aux_grid = nullptr;
   long long aux_floats = 0;
+  unsigned long long* aux_fix = nullptr;  // fixed-point CIC sums, n^3
+  long long aux_fix_cells = 0;
   FftPlan aux_fft;
   int aux_fft_n = 0;
   double* d_aux = nullptr;  // 2 sums + 3 * kMaxShellBins shell accumulators
@@ -412,6 +414,7 @@ int cola_context_destroy(cola_context* c)
   c->aux_fft.destroy();
   fof_free(&c->fof);
   if (c->aux_grid) (void)hipFree(c->aux_grid);
+  if (c->aux_fix) (void)hipFree(c->aux_fix);
   if (c->d_aux) (void)hipFree(c->d_aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -987,29 +990,41 @@ int position_source(cola_context* c, int source, PosSource* src)
   return kOk;
 }
 
-int ensure_aux(cola_context* c, long long floats)
+int ensure_aux(cola_context* c, long long floats, long long cells)
 {
   if (!c->d_aux) COLA_HIP(hipMalloc(&c->d_aux, sizeof(double) * (2 + 3 * kMaxShellBins)));
-  if (c->aux_floats >= floats) return kOk;
-  if (c->aux_grid) (void)hipFree(c->aux_grid);
-  c->aux_grid = nullptr;
-  c->aux_floats = 0;
-  if (hipMalloc(&c->aux_grid, sizeof(float) * floats) != hipSuccess) {
-    (void)hipGetLastError();
-    set_error("unable to allocate the output grid");
-    return kErrAlloc;
+  if (c->aux_floats < floats) {
+    if (c->aux_grid) (void)hipFree(c->aux_grid);
+    c->aux_grid = nullptr;
+    c->aux_floats = 0;
+    if (hipMalloc(&c->aux_grid, sizeof(float) * floats) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("unable to allocate the output grid");
+      return kErrAlloc;
+    }
+    c->aux_floats = floats;
   }
-  c->aux_floats = floats;
+  if (c->aux_fix_cells < cells) {
+    if (c->aux_fix) (void)hipFree(c->aux_fix);
+    c->aux_fix = nullptr;
+    c->aux_fix_cells = 0;
+    if (hipMalloc(&c->aux_fix, sizeof(unsigned long long) * cells) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("unable to allocate the output grid");
+      return kErrAlloc;
+    }
+    c->aux_fix_cells = cells;
+  }
   return kOk;
 }
 
 // coarse_grid.c:49-68 on the device: CIC assignment, then delta = rho n^3 / np - 1.
 int deposit_contrast(cola_context* c, const PosSource& src, int n, long long row)
 {
-  COLA_TRY(launch_cic_periodic(src, n, c->boxf, row, c->aux_grid, c->stream));
+  COLA_TRY(launch_cic_periodic(src, n, c->boxf, c->aux_fix, c->stream));
   COLA_TRY(launch_fill_zero(c->d_aux, 2, c->stream));
   const float nbar_inv = (double)n * n * n / (double)c->np;  // coarse_grid.c:56
-  return launch_density_contrast(n, row, nbar_inv, c->aux_grid, c->d_aux, c->stream);
+  return launch_density_contrast(n, row, nbar_inv, c->aux_fix, c->aux_grid, c->d_aux, c->stream);
 }
 
 }  // namespace
@@ -1022,7 +1037,7 @@ int cola_coarse_grid(cola_context* c, int n, int source, float* delta_out, doubl
   PosSource src;
   COLA_TRY(position_source(c, source, &src));
   const long long ncell = (long long)n * n * n;
-  COLA_TRY(ensure_aux(c, ncell));
+  COLA_TRY(ensure_aux(c, ncell, ncell));
   COLA_TRY(deposit_contrast(c, src, n, n));
   double sums[2] = {0, 0};
   COLA_HIP(hipMemcpyAsync(delta_out, c->aux_grid, sizeof(float) * ncell, hipMemcpyDeviceToHost, c->stream));
@@ -1061,7 +1076,7 @@ int cola_power_spectrum(cola_context* c, int n, int source, double kmin, double 
     c->aux_fft_n = n;
   }
   const long long row = 2LL * (n / 2 + 1);
-  COLA_TRY(ensure_aux(c, (long long)n * n * row));
+  COLA_TRY(ensure_aux(c, (long long)n * n * row, (long long)n * n * n));
   COLA_TRY(deposit_contrast(c, src, n, row));
   COLA_TRY(c->aux_fft.r2c(c->aux_grid, c->stream));
   ShellBins b;

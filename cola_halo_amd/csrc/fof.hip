@@ -217,12 +217,18 @@ k_fof_slots(int nhalo, const int* __restrict__ heads, const int* __restrict__ ro
 }
 
 // fof.c:284-324 halo_particle: offsets from x0 with the periodic fold of
-// HalfBoxSize, and the velocity sum, accumulated in double per halo.
+// HalfBoxSize, and the velocity sum.  Sums are fixed-point integers (offsets
+// in units of 2^-32, velocities of 2^-24 km/s), exact for all but the
+// smallest values and independent of the order of the atomics, so the
+// catalogue is bit-reproducible.
+constexpr float kFixX = 4294967296.0f;  // 2^32
+constexpr float kFixV = 16777216.0f;    // 2^24
+
 __global__ void __launch_bounds__(kThreads)
 k_fof_accumulate(long long np, const float* __restrict__ base, long long stride, float box,
                  const int* __restrict__ rootof, const int* __restrict__ slot, const int* __restrict__ heads,
                  const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-                 double* __restrict__ sums)
+                 unsigned long long* __restrict__ sums)
 {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i >= np) return;
@@ -238,8 +244,8 @@ k_fof_accumulate(long long np, const float* __restrict__ base, long long stride,
       dx -= box;
     else if (dx < -half)
       dx += box;
-    unsafeAtomicAdd(&sums[6LL * h + a], (double)dx);
-    unsafeAtomicAdd(&sums[6LL * h + 3 + a], (double)base[i * stride + 3 + a]);
+    atomicAdd(&sums[6LL * h + a], (unsigned long long)(long long)rintf(dx * kFixX));
+    atomicAdd(&sums[6LL * h + 3 + a], (unsigned long long)(long long)rintf(base[i * stride + 3 + a] * kFixV));
   }
 }
 
@@ -247,7 +253,8 @@ k_fof_accumulate(long long np, const float* __restrict__ base, long long stride,
 __global__ void __launch_bounds__(kThreads)
 k_fof_finalize(int nhalo, const int* __restrict__ heads, const int* __restrict__ rootof,
                const int* __restrict__ count, const float* __restrict__ px, const float* __restrict__ py,
-               const float* __restrict__ pz, const double* __restrict__ sums, float box, cola_halo* __restrict__ out)
+               const float* __restrict__ pz, const unsigned long long* __restrict__ sums, float box,
+               cola_halo* __restrict__ out)
 {
   const int h = blockIdx.x * kThreads + threadIdx.x;
   if (h >= nhalo) return;
@@ -257,11 +264,13 @@ k_fof_finalize(int nhalo, const int* __restrict__ heads, const int* __restrict__
   cola_halo o;
   o.nfof = n;
   for (int a = 0; a < 3; ++a) {
-    float x = (float)((double)x0[a] + sums[6LL * h + a] / n);
+    const double dsum = (double)(long long)sums[6LL * h + a] / (double)kFixX;
+    const double vsum = (double)(long long)sums[6LL * h + 3 + a] / (double)kFixV;
+    float x = (float)((double)x0[a] + dsum / n);
     if (x < 0.0f) x += box;
     if (x >= box) x -= box;
     o.x[a] = x;
-    o.v[a] = (float)(sums[6LL * h + 3 + a] / n);
+    o.v[a] = (float)(vsum / n);
   }
   out[h] = o;
 }
@@ -389,11 +398,12 @@ int fof_run(const FofInput& in, FofResult* out, hipStream_t s)
   COLA_HIP(hipStreamSynchronize(s));
   if (nhalo == 0) return kOk;
 
-  double* sums;
+  unsigned long long* sums;
   cola_halo* d_halos;
   COLA_TRY(sc.get(&sums, 6LL * nhalo));
   COLA_TRY(sc.get(&d_halos, nhalo));
-  hipLaunchKernelGGL(k_fill<double>, dim3(blocks(6LL * nhalo)), dim3(kThreads), 0, s, sums, 6LL * nhalo, 0.0);
+  hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(blocks(6LL * nhalo)), dim3(kThreads), 0, s, sums, 6LL * nhalo,
+                     0ULL);
   hipLaunchKernelGGL(k_fof_slots, dim3(blocks(nhalo)), dim3(kThreads), 0, s, nhalo, heads, rootof, slot);
   hipLaunchKernelGGL(k_fof_accumulate, dim3(blocks(np)), dim3(kThreads), 0, s, np, in.base, in.stride, in.boxsize,
                      rootof, slot, heads, px, py, pz, sums);

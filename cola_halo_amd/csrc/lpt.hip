@@ -10,6 +10,7 @@
 // the three spectra need no clearing pass.
 #include "common.h"
 #include "lpt.h"
+#include "ranlux.h"
 
 #include <cmath>
 
@@ -278,6 +279,21 @@ void lpt_seedtable(int N, int seed, unsigned int* t)
     for (int j = 0; j < i + 1; j++) t[(N - 1 - j) * N + (N - 1 - i)] = U();
   }
 }
+
+struct RanluxStream {
+  Ranlux48 r;
+};
+
+RanluxStream* ranlux_new(unsigned long long seed)
+{
+  RanluxStream* s = new RanluxStream;
+  s->r.seed(seed);
+  return s;
+}
+
+double ranlux_next(RanluxStream* s) { return s->r.uniform(); }
+
+void ranlux_delete(RanluxStream* s) { delete s; }
 
 void ranlux_uniforms(unsigned long long seed, int n, double* out)
 {

@@ -12,6 +12,7 @@
 
 #include "../../include/cola_halo_amd.h"
 #include "errors.h"
+#include "ranlux.h"
 
 namespace {
 
@@ -144,6 +145,71 @@ int cola_write_gadget(const char* filename, const cola_particle_min* p, long lon
     put_block<unsigned int>(f, np, 1, [&](long long i, unsigned int* o) { *o = (unsigned int)p[i].id; });
   f.marker(blk_id);
 
+  if (!f.close()) {
+    set_error(std::string("Error: write failed: ") + filename);
+    return 9000;
+  }
+  return COLA_OK;
+}
+
+struct cola_subsampler {
+  cola::RanluxStream* rng = nullptr;
+  double factor = 0.0;
+};
+
+int cola_subsampler_create(double factor, unsigned int seed, cola_subsampler** out)
+{
+  if (!out || !(factor >= 0.0)) return COLA_ERR_ARG;
+  cola_subsampler* s = new cola_subsampler;
+  s->factor = factor;
+  s->rng = cola::ranlux_new(2ULL * seed);  // subsample.c:26, this_node = 0
+  *out = s;
+  return COLA_OK;
+}
+
+int cola_subsampler_destroy(cola_subsampler* s)
+{
+  if (!s) return COLA_OK;
+  cola::ranlux_delete(s->rng);
+  delete s;
+  return COLA_OK;
+}
+
+int cola_write_subsample(cola_subsampler* s, const char* filename, const cola_particle_min* p, long long np, float a,
+                         float boxsize, float omega_m, float h, long long* nsub_out)
+{
+  if (!s || !filename || (np > 0 && !p) || np < 0) return COLA_ERR_ARG;
+  std::vector<float> sel;
+  for (long long i = 0; i < np; ++i) {
+    if (cola::ranlux_next(s->rng) < s->factor) {  // subsample.c:131
+      for (int k = 0; k < 3; ++k) sel.push_back(p[i].x[k]);
+      for (int k = 0; k < 3; ++k) sel.push_back(p[i].v[k]);
+    }
+  }
+  const long long nsub = (long long)sel.size() / 6;
+  if (nsub_out) *nsub_out = nsub;
+  if (nsub > INT32_MAX) {
+    set_error("subsample larger than 2^31 records");
+    return COLA_ERR_ARG;
+  }
+  const double rho_crit = 27.7455;
+  float header[6];
+  header[0] = boxsize;
+  header[1] = omega_m * rho_crit * std::pow((double)boxsize, 3.0) / np;  // subsample.c:150-151
+  header[2] = omega_m;
+  header[3] = h;
+  header[4] = a;
+  header[5] = 1.0 / a - 1.0;
+  const int32_t n32 = (int32_t)nsub;
+  File f(filename);
+  if (!f.ok()) {
+    set_error(std::string("Error: Unable to write to file: ") + filename);
+    return 9000;
+  }
+  f.put(header, sizeof(header));
+  f.put(&n32, sizeof(n32));
+  f.put(sel.data(), sizeof(float) * sel.size());
+  f.put(&n32, sizeof(n32));
   if (!f.close()) {
     set_error(std::string("Error: write failed: ") + filename);
     return 9000;

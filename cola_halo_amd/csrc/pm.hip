@@ -302,11 +302,20 @@ __device__ inline double wave_sum(double v)
   return v;
 }
 
+// Each CIC weight (pm.c's float products, WPAR included) is added to its
+// LDS cell as a 32.32 fixed-point integer: w * 2^32 is an exact integer for
+// every w >= 2^-9 and is rounded at 2^-33 below, so a cell holds the exact
+// sum of its weights whatever the order of the atomics, and the density
+// -1 + sum is rounded once.  The grid is therefore bit-reproducible (the
+// reference's serial float sum differs from it by float rounding only).
+constexpr float kFix = 4294967296.0f;  // 2^32
+constexpr double kUnfix = 1.0 / 4294967296.0;
+
 __global__ void __launch_bounds__(kThreads)
 k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const float* __restrict__ sorted,
                 float* __restrict__ density, double* __restrict__ tile_sums)
 {
-  extern __shared__ float cells[];
+  extern __shared__ unsigned long long cells[];
   __shared__ double wsum[kThreads / 64];
   const long long tile = blockIdx.x;
   const int tz = (int)(tile % t.ntz);
@@ -314,7 +323,7 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
   const int tx = (int)(tile / ((long long)t.ntz * t.nty));
   const int x0 = tx << t.lx, y0 = ty << t.ly, z0 = tz << t.lz;
   const int ncell = 1 << (t.lx + t.ly + t.lz);
-  for (int e = threadIdx.x; e < ncell; e += kThreads) cells[e] = -1.0f;
+  for (int e = threadIdx.x; e < ncell; e += kThreads) cells[e] = 0ULL;
   __syncthreads();
   const int beg = offsets[tile], end = offsets[tile + 1];
   for (int k = beg + threadIdx.x; k < end; k += kThreads) {
@@ -322,7 +331,7 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
     cic_with_ghost(x, y, z, g, [&](int i, int j, int kk, float w) {
       const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0), lk = (unsigned)(kk - z0);
       if (li < (unsigned)t.tx && lj < (unsigned)t.ty && lk < (unsigned)t.tz)
-        atomicAdd(&cells[(((li << t.ly) + lj) << t.lz) + lk], w);
+        atomicAdd(&cells[(((li << t.ly) + lj) << t.lz) + lk], (unsigned long long)rintf(w * kFix));
     });
   }
   __syncthreads();
@@ -332,7 +341,7 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
     const int lk = e & zmask;
     const int lj = (e >> t.lz) & ymask;
     const int li = e >> (t.lz + t.ly);
-    const float v = cells[e];
+    const float v = (float)(-1.0 + (double)cells[e] * kUnfix);  // pm.c:220 density starts at -1
     density[((long long)(x0 + li) * g.ng + (y0 + lj)) * g.row + (z0 + lk)] = v;
     s += (double)v;
   }
@@ -641,7 +650,7 @@ int launch_bin_scatter(const DevParticles& p, const CicGeom& g, const TileGeom& 
 int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& b, float* density, double* mass_sum,
                          hipStream_t s)
 {
-  const size_t lds = sizeof(float) * t.tx * t.ty * t.tz;
+  const size_t lds = sizeof(unsigned long long) * t.tx * t.ty * t.tz;
   hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)t.ntiles), dim3(kThreads), lds, s, g, t, b.offsets, b.sorted,
                      density, b.tile_sums);
   const long long per = (long long)kThreads * 16;

@@ -299,6 +299,21 @@ int cola_confirm_parameters(const cola_params* p);
 int cola_write_gadget(const char* filename, const cola_particle_min* p, long long np, float a, float boxsize,
                       float omega_m, float h, int use_long_id);
 
+/* Random particle subsample (subsample.c).  The generator is a RANLUX
+   (ranlxd1) stream seeded with 2 * seed (subsample_init, subsample.c:19-29,
+   rank 0) that persists across outputs and realizations. */
+typedef struct cola_subsampler cola_subsampler;
+int cola_subsampler_create(double factor, unsigned int seed, cola_subsampler** out);
+int cola_subsampler_destroy(cola_subsampler* s);
+/* subsample.c:115-159 write_random_sabsample + write.c:254-311
+   write_particles_binary_mpi on one rank: record i is kept when the next
+   uniform draw is < factor (one draw per record, in record order).  File:
+   6 floats (boxsize, particle mass omega_m 27.7455 L^3 / np, omega_m, h, a,
+   redshift), int n, n x {float x[3], v[3]}, int n.  *nsub_out (may be NULL)
+   receives n.  Returns 9000 when the file cannot be written. */
+int cola_write_subsample(cola_subsampler* s, const char* filename, const cola_particle_min* p, long long np,
+                         float a, float boxsize, float omega_m, float h, long long* nsub_out);
+
 /* coarse_grid.c:46-91 file layout: float boxsize, int n, n^3 float delta in
    C order, int n.  Returns 10021 when the file cannot be opened. */
 int cola_write_coarse_grid(const char* filename, float boxsize, int n, const float* delta);

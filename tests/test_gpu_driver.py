@@ -1,0 +1,123 @@
+"""The cola_halo driver program end to end on the GPU: a parameter file in the
+reference's syntax, two output redshifts, every output kind, checked file by
+file against the same run driven from Python through the library."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import cola_halo_amd as ch
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(os.path.dirname(HERE), "cola_halo_amd", "cola_halo")
+PK_FILE = os.path.join(HERE, "golden", "linear_matterpower.dat")
+
+PARAM = """-- cola code parameter file (test)
+nc = 32
+boxsize = 64.0
+random_seed= 7
+nrealization= 2
+ntimestep= 10
+a_final= 1.0
+output_redshifts= {{1.0, 0.0}}
+omega_m = 0.273
+h       = 0.705
+sigma8  = 0.812
+pm_nc_factor= 3
+loglevel=2
+powerspectrum= "{pk}"
+fof= "{d}/fof"
+linking_factor= 0.2
+snapshot= "{d}/snp"
+subsample= "{d}/sub"
+subsample_factor= 0.05
+coarse_grid= "{d}/grid"
+coarse_grid_nc= 16
+initial= "{d}/init"
+write_longid= false
+"""
+
+
+def _gadget(path):
+    raw = open(path, "rb").read()
+    off = 0
+    recs = []
+    while off < len(raw):
+        (n,) = struct.unpack_from("<i", raw, off)
+        recs.append(raw[off + 4: off + 4 + n])
+        off += 8 + n
+    time = struct.unpack_from("<d", recs[0], 72)[0]
+    return time, np.frombuffer(recs[1], "<f4").reshape(-1, 3), np.frombuffer(recs[2], "<f4").reshape(-1, 3), \
+        np.frombuffer(recs[3], "<u4")
+
+
+def _wrap(x, box):
+    b = np.float32(box)
+    return np.where(x < 0, x + b, np.where(x >= b, x - b, x)).astype(np.float32)
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_outputs_match_library_run(tmp_path):
+    print("driver run", flush=True)
+    param = tmp_path / "param.lua"
+    param.write_text(PARAM.format(pk=PK_FILE, d=tmp_path))
+    r = subprocess.run([DRIVER, str(param)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "Timestep 10/10" in r.stdout
+
+    nc, box, om = 32, 64.0, 0.273
+    Om = np.float32(om)
+    sub = ch.Subsampler(0.05, 7)
+    with ch.Context(nc, 3 * nc, box) as ctx:
+        ctx.power_init(PK_FILE, 0.1, 0.812, om, 1 - om)
+        for seed in (7, 8):
+            # initial condition file (main.c:131-137)
+            ctx.lpt_set_displacement(0.1, om, seed)
+            init = ctx.set_noncola_initial(Om)
+            t, pos, vel, ids = _gadget(tmp_path / f"init_{seed:05d}.0")
+            assert t == np.float32(0.1)
+            assert np.array_equal(pos, init["x"]) and np.array_equal(ids, init["id"].astype(np.uint32))
+
+            snaps = []
+
+            def on_snapshot(a, s):
+                ll = float(np.float32(0.2 * np.float32(box) / nc))
+                halos = ctx.fof_find_halos(ll, 32)
+                snaps.append((a, ctx.get_snapshot(), halos, ctx.coarse_grid(16)[0]))
+
+            ch.run_cola(ctx, om, 1.0, 10, seed, [1.0, 0.0], on_snapshot=on_snapshot)
+            assert len(snaps) == 2
+            for iout, (a, snap, halos, grid) in enumerate(snaps):
+                sfx = "ab"[iout]
+                t, pos, vel, ids = _gadget(tmp_path / f"snp{seed:05d}{sfx}.0")
+                assert t == np.float32(a)
+                assert np.array_equal(pos, snap["x"])                      # wrapped by FoF, as the reference
+                assert np.array_equal(pos, _wrap(snap["x"], box))
+                vfac = np.float32(1.0 / np.sqrt(np.float64(np.float32(a))))
+                assert np.array_equal(vel, (vfac * snap["v"]).astype(np.float32))
+                assert np.array_equal(ids, np.arange(1, nc ** 3 + 1, dtype=np.uint32))
+                lines = open(tmp_path / f"fof{seed:05d}{sfx}.txt").read().splitlines()
+                assert len(lines) == len(halos) > 0
+                for line, h in zip(lines, halos):
+                    assert line == "%d %e %e %e %e %e %e" % (h["nfof"], *map(float, h["x"]), *map(float, h["v"]))
+                raw = open(tmp_path / f"grid{seed:05d}{sfx}.b", "rb").read()
+                assert np.array_equal(np.frombuffer(raw, "<f4", 16 ** 3, 8).reshape(16, 16, 16), grid)
+                sub_path = str(tmp_path / f"check_sub{seed}{sfx}.b")
+                sub.write(sub_path, snap, np.float32(a), np.float32(box), Om, np.float32(0.705))
+                assert open(sub_path, "rb").read() == open(tmp_path / f"sub{seed:05d}{sfx}.b", "rb").read()
+    sub.close()
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_abort_codes(tmp_path):
+    param = tmp_path / "param.lua"
+    param.write_text("nc = 16\nfoo = 1\n")
+    r = subprocess.run([DRIVER, str(param)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1007 & 0xFF and "unknown variable" in r.stderr
+    param.write_text(PARAM.format(pk="/nonexistent/pk.dat", d=tmp_path))
+    r = subprocess.run([DRIVER, str(param)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 3000 & 0xFF

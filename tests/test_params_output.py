@@ -156,3 +156,36 @@ def test_coarse_grid_file_layout(tmp_path):
     with pytest.raises(ch.ColaError) as e:
         ch.write_coarse_grid("/nonexistent/grid.b", 1.0, delta)
     assert e.value.code == 10021
+
+
+def test_subsample_selection_and_layout(tmp_path):
+    """subsample.c:115-159: record i is kept when the next ranlxd1 uniform of
+    the stream seeded with 2 * seed is below the factor; the stream carries
+    over to the next output."""
+    import oracle_lib as orc
+    rng = np.random.default_rng(4)
+    n, factor, seed = 5000, 0.05, 100
+    snap = np.zeros(n, ch.PARTICLE_MIN_DTYPE)
+    snap["x"] = rng.uniform(0, 100, (n, 3))
+    snap["v"] = rng.normal(0, 300, (n, 3))
+    sub = ch.Subsampler(factor, seed)
+    r = orc.Ranlxd()
+    orc.lib().orc_ranlxd_set(r, 2 * seed, 202)
+    a, box, om, h = np.float32(0.5), np.float32(100.0), np.float32(0.273), np.float32(0.705)
+    for k in range(2):  # two outputs from one stream
+        path = str(tmp_path / f"sub00100{'ab'[k]}.b")
+        nsub = sub.write(path, snap, a, box, om, h)
+        keep = np.array([orc.lib().orc_ranlxd_uniform(r) < factor for _ in range(n)])
+        raw = open(path, "rb").read()
+        hdr = np.frombuffer(raw, "<f4", 6)
+        assert hdr[0] == box and hdr[2] == om and hdr[3] == h and hdr[4] == a
+        assert hdr[1] == np.float32(float(om) * 27.7455 * float(box) ** 3 / n)
+        assert hdr[5] == np.float32(1.0 / float(a) - 1.0)
+        cnt = struct.unpack_from("<i", raw, 24)[0]
+        assert cnt == nsub == keep.sum()
+        body = np.frombuffer(raw, "<f4", 6 * cnt, 28).reshape(cnt, 6)
+        assert np.array_equal(body[:, :3], snap["x"][keep])
+        assert np.array_equal(body[:, 3:], snap["v"][keep])
+        assert struct.unpack_from("<i", raw, 28 + 24 * cnt)[0] == cnt
+        assert len(raw) == 32 + 24 * cnt
+    sub.close()
