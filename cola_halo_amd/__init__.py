@@ -87,6 +87,7 @@ _SIGNATURES = {
     "cola_subsampler_destroy": (C.c_int, [C.c_void_p]),
     "cola_write_subsample": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.POINTER(C.c_longlong)]),
+    "cola_fft_time": (C.c_int, [C.c_int, C.c_int, C.c_void_p]),
     "cola_coarse_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "cola_power_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -1217,6 +1217,66 @@ static int standalone_fft(int n, float* data, bool forward)
   return st;
 }
 
+__global__ void k_fill_pattern(float* d, long long n, unsigned seed)
+{
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  unsigned h = (unsigned)i * 2654435761u ^ seed;
+  h ^= h >> 15;
+  h *= 2246822519u;
+  h ^= h >> 13;
+  d[i] = (float)(h & 0xffffff) * (1.0f / 16777216.0f) - 0.5f;
+}
+
+int cola_fft_time(int n, int reps, double* ms_out)
+{
+  if (n <= 0 || n % 2 || reps < 1 || !ms_out) return kErrArg;
+  FftPlan plan;
+  COLA_TRY(plan.init(n));
+  const long long floats = (long long)n * n * 2 * (n / 2 + 1);
+  float* buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // work, pristine copy, 3 force grids
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = kOk;
+  for (int b = 0; b < 5 && !st; ++b)
+    if (hipMalloc(&buf[b], sizeof(float) * floats) != hipSuccess) st = kErrAlloc;
+  if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = kErrHip;
+  if (!st) {
+    hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)((floats + 255) / 256)), dim3(256), 0, nullptr, buf[1], floats,
+                       12345u);
+    KOpParams kp;
+    kp.f1 = 1.0f;
+    float* const out[3] = {buf[2], buf[3], buf[4]};
+    double tot[3] = {0, 0, 0};
+    for (int r = 0; r <= reps && !st; ++r) {  // r = 0 warms up
+      for (int which = 0; which < 3 && !st; ++which) {
+        if (hipMemcpyAsync(buf[0], buf[1], sizeof(float) * floats, hipMemcpyDeviceToDevice, nullptr) != hipSuccess) {
+          st = kErrHip;
+          break;
+        }
+        (void)hipEventRecord(e0, nullptr);
+        if (which == 0)
+          st = plan.r2c(buf[0], nullptr);
+        else if (which == 1)
+          st = plan.c2r(nullptr, buf[0], KOp::None, KOpParams{}, nullptr);
+        else
+          st = plan.forces(buf[0], out, kp, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0;
+        if (!st && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          st = kErrHip;
+        if (r > 0) tot[which] += ms;
+      }
+    }
+    for (int w = 0; w < 3; ++w) ms_out[w] = tot[w] / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (float* b : buf)
+    if (b) (void)hipFree(b);
+  plan.destroy();
+  return st;
+}
+
 int cola_fft_r2c_3d(int n, float* data) { return standalone_fft(n, data, true); }
 int cola_fft_c2r_3d(int n, float* data) { return standalone_fft(n, data, false); }
 

@@ -338,6 +338,12 @@ int cola_bandwidth_probe(size_t bytes, int reps, double* gbps);
    [n][n][2(n/2+1)].  These are the transforms pm.c:285,353 and
    lpt.c:403,427,501-502 execute through FFTW. */
 int cola_fft_r2c_3d(int n, float* data);
+/* Device time of this library's transforms on an n^3 grid resident in HBM
+   (mean over reps after one warm-up, HIP events on the stream): ms_out[0]
+   r2c, ms_out[1] c2r, ms_out[2] the PM force sequence (r2c, then for each of
+   three axes the Poisson gradient and a c2r into its own grid).  For
+   comparison with other FFT libraries; no reference counterpart. */
+int cola_fft_time(int n, int reps, double* ms_out);
 int cola_fft_c2r_3d(int n, float* data);
 
 #ifdef __cplusplus

@@ -48,7 +48,9 @@ def test_coarse_grid_matches_oracle(eh_table, nc, n, box):
         g, sums = ctx.coarse_grid(n, source="particles")
     ref, rsums = orc.coarse_grid(parts["x"], n, box)
     assert np.all(np.abs(g - ref) <= 1e-5 * (1 + np.abs(ref)))
-    assert sums[0] == pytest.approx(rsums[0], rel=1e-9)
+    # each device cell is its exact weight sum rounded once; the oracle
+    # rounds after every add: the totals agree to float rounding
+    assert sums[0] == pytest.approx(rsums[0], rel=1e-6)
     assert sums[1] == pytest.approx(rsums[1], abs=1e-6 * n ** 3)
 
 
