@@ -196,6 +196,27 @@ def main():
     b_step = 156 * nc ** 3 + 32 * ng ** 3 + 64 * n_c
     step_frac = b_step / (elapsed / forces) / 1e9 / HBM_PEAK_GBS
 
+    # output side on the final state (a = 1), outside the timed region
+    outputs = None
+    if args.steps % NSTEPS == 0:
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.set_snapshot(1.0, Om)
+        t_snap = time.perf_counter() - t
+        ll = float(np.float32(0.2 * np.float32(box) / nc))  # main.c:353, linking factor 0.2
+        t = time.perf_counter()
+        halos = ctx.fof_find_halos(ll, 32)
+        t_fof = time.perf_counter() - t
+        t = time.perf_counter()
+        ctx.coarse_grid(nc)
+        t_grid = time.perf_counter() - t
+        t = time.perf_counter()
+        ctx.power_spectrum(ng, 0.0, 1.0, 50, deconvolve=True)
+        t_pk = time.perf_counter() - t
+        outputs = {"snapshot_to_host_ms": round(t_snap * 1e3, 2), "fof_ms": round(t_fof * 1e3, 2),
+                   "halos": int(len(halos)), f"coarse_grid_{nc}_ms": round(t_grid * 1e3, 2),
+                   f"power_spectrum_{ng}_ms": round(t_pk * 1e3, 2)}
+
     cpu = None
     if not args.no_cpu_baseline:
         ctx.state_restore()
@@ -225,6 +246,7 @@ def main():
         "step_canonical_frac": round(step_frac, 4),
         "phase_ms_per_step": phases,
         "ic_seconds": round(t_ic, 3),
+        "outputs": outputs,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
