@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--tile", type=str, default="", help="deposit tile tx,ty,tz")
     ap.add_argument("--fft-log2-tile", type=int, default=-2, help="FFT transforms per workgroup (log2)")
     ap.add_argument("--xcd-group", type=int, default=0, help="FFT column tiles per XCD group")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of tuning them on the initial conditions")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -148,6 +150,13 @@ def main():
     ctx.synchronize()
     t_ic = time.perf_counter() - t_ic
     ctx.state_save()
+    # launch-shape tuning on the initial state, outside the timed region (the
+    # reference plans its FFTs with FFTW_MEASURE at pm_init, pm.c:139-147)
+    tuning = None
+    if not args.no_autotune and not (args.tile or args.fft_log2_tile != -2 or args.xcd_group):
+        tuning = ctx.autotune(reps=3)
+        ctx.state_restore()
+        ctx.synchronize()
 
     def run_steps(nsteps):
         forces = 0
@@ -246,6 +255,7 @@ def main():
         "step_canonical_frac": round(step_frac, 4),
         "phase_ms_per_step": phases,
         "ic_seconds": round(t_ic, 3),
+        "tuning": tuning,
         "outputs": outputs,
         "cpu_baseline": cpu,
     }

@@ -87,6 +87,8 @@ _SIGNATURES = {
     "cola_subsampler_destroy": (C.c_int, [C.c_void_p]),
     "cola_write_subsample": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.POINTER(C.c_longlong)]),
+    "cola_set_fft_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "cola_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cola_fft_time": (C.c_int, [C.c_int, C.c_int, C.c_void_p]),
     "cola_coarse_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "cola_power_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
@@ -269,6 +271,17 @@ class Subsampler:
 
     def __del__(self):
         self.close()
+
+
+class Tuning(C.Structure):
+    """cola_tuning: the launch shapes the autotuner chose."""
+    _fields_ = [("tile", C.c_int * 3), ("fft_log2_cols", C.c_int), ("fft_log2_rows", C.c_int),
+                ("xcd_group", C.c_int), ("ms_before", C.c_double), ("ms_after", C.c_double)]
+
+    def as_dict(self):
+        return {"tile": list(self.tile), "fft_log2_cols": self.fft_log2_cols, "fft_log2_rows": self.fft_log2_rows,
+                "xcd_group": self.xcd_group, "ms_before": round(self.ms_before, 4),
+                "ms_after": round(self.ms_after, 4)}
 
 
 PARAM_MAX_ZOUT = 256
@@ -510,6 +523,16 @@ class Context:
 
     def set_tiling(self, tx, ty, tz, fft_log2_tile=-1):
         _check(lib().cola_set_tiling(self._h, tx, ty, tz, fft_log2_tile))
+
+    def set_fft_tiles(self, log2_cols=-1, log2_rows=-1, xcd_group=8):
+        _check(lib().cola_set_fft_tiles(self._h, log2_cols, log2_rows, xcd_group))
+
+    def autotune(self, reps=3):
+        """Pick the fastest deposit tile / FFT launch shapes for the current
+        particle state; returns the choice (results do not depend on it)."""
+        t = Tuning()
+        _check(lib().cola_autotune(self._h, reps, C.byref(t)))
+        return t.as_dict()
 
     def set_fft_xcd_group(self, group):
         _check(lib().cola_set_fft_xcd_group(self._h, group))

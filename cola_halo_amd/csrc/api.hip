@@ -82,7 +82,8 @@ struct cola_context {
   long long grid_floats = 0;
 
   FftPlan pm_fft, lpt_fft;
-  int fft_log2_tile = -1;
+  int fft_log2_tile = -1;  // column passes (cola_set_tiling sets both)
+  int fft_log2_rows = -1;
   CicGeom cic;
   TileGeom tiles;
   BinBuffers bins;
@@ -922,14 +923,18 @@ int cola_reset_phase_stats(cola_context* c)
   return kOk;
 }
 
-int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
+}  // extern "C"
+
+namespace {
+
+int set_deposit_tiles(cola_context* c, int tx, int ty, int tz)
 {
-  if (!valid(c)) return kErrArg;
   TileGeom t;
   if (!make_tile_geom(c->ng, tx, ty, tz, &t)) {
     set_error("deposit tile sides must be powers of two dividing ngrid, <= 4096 cells");
     return kErrArg;
   }
+  if (t.tx == c->tiles.tx && t.ty == c->tiles.ty && t.tz == c->tiles.tz && c->bins.counts) return kOk;
   COLA_HIP(hipStreamSynchronize(c->stream));
   COLA_TRY(check_pending(c));
   BinBuffers nb;
@@ -937,13 +942,26 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
   bin_free(&c->bins);
   c->bins = nb;
   c->tiles = t;
-  if (fft_log2_tile != c->fft_log2_tile) {
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_TRY(set_deposit_tiles(c, tx, ty, tz));
+  if (fft_log2_tile != c->fft_log2_tile || fft_log2_tile != c->fft_log2_rows) {
     FftPlan a, b;
-    COLA_TRY(a.init(c->ng, fft_log2_tile));
-    if (b.init(c->nc, fft_log2_tile) != kOk) {
+    COLA_TRY(a.init(c->ng, fft_log2_tile, fft_log2_tile));
+    if (b.init(c->nc, fft_log2_tile, fft_log2_tile) != kOk) {
       a.destroy();
       return kErrUnsupported;
     }
+    a.xgroup = c->pm_fft.xgroup;
+    b.xgroup = c->lpt_fft.xgroup;
     c->pm_fft.destroy();
     c->lpt_fft.destroy();
     c->pm_fft = a;
@@ -952,9 +970,126 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
     c->pm_fft.hook_user = c;
     c->lpt_fft.hook = lpt_fft_hook;
     c->lpt_fft.hook_user = c;
+    c->fft_log2_rows = fft_log2_tile;
     c->fft_log2_tile = fft_log2_tile;
   }
   return kOk;
+}
+
+int cola_set_fft_tiles(cola_context* c, int log2_cols, int log2_rows, int xcd_group)
+{
+  if (!valid(c) || log2_cols < -1 || log2_cols > 6 || log2_rows < -1 || log2_rows > 6 || xcd_group < 1 ||
+      xcd_group > 64 || (xcd_group & (xcd_group - 1)))
+    return kErrArg;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  if (log2_cols != c->fft_log2_tile || log2_rows != c->fft_log2_rows) {
+    FftPlan a;
+    const int st = a.init(c->ng, log2_cols, log2_rows);
+    if (st != kOk) {
+      set_error("FFT tile widths not supported for this grid");
+      return st;
+    }
+    c->pm_fft.destroy();
+    c->pm_fft = a;
+    c->pm_fft.hook = fft_hook;
+    c->pm_fft.hook_user = c;
+    c->fft_log2_tile = log2_cols;
+    c->fft_log2_rows = log2_rows;
+  }
+  c->pm_fft.xgroup = xcd_group;
+  return kOk;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Mean device time of `reps` PM force fields (deposit + transforms) of the
+// current particle state.
+int time_pm_fields(cola_context* c, int reps, double* ms)
+{
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  COLA_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return kErrHip;
+  }
+  int st = pm_assign(c, false);
+  if (!st) st = pm_fields(c);
+  if (!st && hipEventRecord(e0, c->stream) != hipSuccess) st = kErrHip;
+  for (int r = 0; r < reps && !st; ++r) {
+    st = pm_assign(c, false);
+    if (!st) st = pm_fields(c);
+  }
+  float t = 0;
+  if (!st && (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+    st = kErrHip;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!st) st = check_pending(c);
+  *ms = t / reps;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cola_autotune(cola_context* c, int reps, cola_tuning* out)
+{
+  if (!valid(c) || reps < 1 || !out) return kErrArg;
+  COLA_TRY(require_particles(c));
+  const bool prof = c->profiling;
+  c->profiling = false;
+  COLA_TRY(launch_wrap(c->p, c->boxf, c->stream));  // move.c:33-41, as every step begins
+  cola_tuning best;
+  best.tile[0] = c->tiles.tx;
+  best.tile[1] = c->tiles.ty;
+  best.tile[2] = c->tiles.tz;
+  best.fft_log2_cols = c->fft_log2_tile;
+  best.fft_log2_rows = c->fft_log2_rows;
+  best.xcd_group = c->pm_fft.xgroup;
+  int st = time_pm_fields(c, reps, &best.ms_before);
+  best.ms_after = best.ms_before;
+  auto apply = [&](const cola_tuning& t) -> int {
+    int r = set_deposit_tiles(c, t.tile[0], t.tile[1], t.tile[2]);
+    if (!r) r = cola_set_fft_tiles(c, t.fft_log2_cols, t.fft_log2_rows, t.xcd_group);
+    return r;
+  };
+  auto trial = [&](cola_tuning t) {
+    if (st) return;
+    if (apply(t) != kOk) return;  // shape not supported for this grid: skip it
+    double ms = 0;
+    st = time_pm_fields(c, reps, &ms);
+    if (!st && ms < best.ms_after) {
+      best = t;
+      best.ms_after = ms;
+    }
+  };
+  // coordinate descent: deposit tile, then FFT column / row tiles, then XCD group
+  static const int tiles[][3] = {{8, 8, 32}, {16, 8, 32}, {8, 16, 32}, {16, 16, 16}, {8, 8, 64}, {4, 8, 64}, {16, 16, 32}};
+  for (const auto& tl : tiles) {
+    cola_tuning t = best;
+    t.tile[0] = tl[0], t.tile[1] = tl[1], t.tile[2] = tl[2];
+    trial(t);
+  }
+  for (int cols = 2; cols <= 4; ++cols)
+    for (int rows = 0; rows <= 2; ++rows) {
+      cola_tuning t = best;
+      t.fft_log2_cols = cols;
+      t.fft_log2_rows = rows;
+      trial(t);
+    }
+  for (int g : {4, 8, 16, 64}) {
+    cola_tuning t = best;
+    t.xcd_group = g;
+    trial(t);
+  }
+  if (!st) st = apply(best);
+  c->profiling = prof;
+  *out = best;
+  return st;
 }
 
 }  // extern "C"
@@ -1191,7 +1326,7 @@ int cola_bandwidth_probe(size_t bytes, int reps, double* gbps)
 
 int cola_set_fft_xcd_group(cola_context* c, int group)
 {
-  if (!valid(c) || !(group == 1 || group == 2 || group == 4 || group == 8)) return kErrArg;
+  if (!valid(c) || group < 1 || group > 64 || (group & (group - 1))) return kErrArg;
   c->pm_fft.xgroup = group;
   c->lpt_fft.xgroup = group;
   return kOk;

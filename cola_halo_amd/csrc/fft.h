@@ -70,8 +70,10 @@ struct FftPlan {
   void (*hook)(void* user, int pass, int begin) = nullptr;
   void* hook_user = nullptr;
 
-  // force_log2T >= 0 fixes the tile width of all three passes.
-  int init(int n, int force_log2T = -1);
+  // log2T_cols >= 0 fixes the tile width (log2 transforms per workgroup) of
+  // the y and x column passes, log2T_rows >= 0 that of the z row passes
+  // (row pairs); -1 picks the measured defaults.
+  int init(int n, int log2T_cols = -1, int log2T_rows = -1);
   void destroy();
 
   // In place on the padded layout: real -> half-spectrum [x][y][kz].

@@ -770,7 +770,7 @@ static int cap_log2(int n)
   return l;
 }
 
-int FftPlan::init(int n_, int force_log2T)
+int FftPlan::init(int n_, int log2T_cols, int log2T_rows)
 {
   n = n_;
   if (n % 2 != 0 || !fft_factor(n, &rad)) return kErrUnsupported;
@@ -788,9 +788,9 @@ int FftPlan::init(int n_, int force_log2T)
   };
   // columns: 8 adjacent complex values (64 B, 128 B per XCD-paired tiles);
   // rows: 2 row pairs per workgroup (measured fastest on MI355X at n = 768)
-  gx = pick(force_log2T >= 0 ? force_log2T : std::min(3, cap_log2(n)), (long long)n * (n / 2 + 1));
-  gy = pick(force_log2T >= 0 ? force_log2T : std::min(3, cap_log2(n)), n / 2 + 1);
-  grows = pick(force_log2T >= 0 ? force_log2T : 1, (long long)n * n / 2);
+  gx = pick(log2T_cols >= 0 ? log2T_cols : std::min(3, cap_log2(n)), (long long)n * (n / 2 + 1));
+  gy = pick(log2T_cols >= 0 ? log2T_cols : std::min(3, cap_log2(n)), n / 2 + 1);
+  grows = pick(log2T_rows >= 0 ? log2T_rows : 1, (long long)n * n / 2);
   if (gx.emax == 0 || gy.emax == 0 || grows.emax == 0) return kErrUnsupported;
   std::vector<float2> tw(n);
   for (int k = 0; k < n; ++k) {

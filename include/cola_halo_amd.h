@@ -323,8 +323,26 @@ int cola_write_coarse_grid(const char* filename, float boxsize, int n, const flo
    workgroup, -1 = automatic).  Results do not depend on either; the
    autotuner sweeps them for speed. */
 int cola_set_tiling(cola_context* ctx, int tx, int ty, int tz, int fft_log2_tile);
-/* Adjacent FFT column tiles dispatched to one XCD (1, 2, 4 or 8). */
+/* Adjacent FFT column tiles dispatched to one XCD (a power of two, 1 to 64). */
 int cola_set_fft_xcd_group(cola_context* ctx, int group);
+/* Launch shape of the PM transforms: log2 of the transforms per workgroup of
+   the y/x column passes and of the z row passes (-1 = default), and the XCD
+   group.  Results do not depend on them. */
+int cola_set_fft_tiles(cola_context* ctx, int log2_cols, int log2_rows, int xcd_group);
+
+/* Autotuner (no reference counterpart): times the PM force fields (deposit +
+   transforms, `reps` per candidate) of the current particle state for deposit
+   tile shapes, FFT column / row tile widths and XCD groups by coordinate
+   descent, keeps the fastest and reports it.  No result depends on the choice:
+   the deposit sums in order-independent fixed point and every transform
+   computes each column with the same arithmetic whatever the tile width.
+   Wraps the positions first (move.c:33-41), as every step does. */
+typedef struct {
+  int tile[3];
+  int fft_log2_cols, fft_log2_rows, xcd_group;
+  double ms_before, ms_after; /* mean PM force-field time of the start and the chosen shape */
+} cola_tuning;
+int cola_autotune(cola_context* ctx, int reps, cola_tuning* out);
 
 /* Device copy of `bytes` bytes by a plain streaming kernel, repeated reps
    times; returns the mean copy bandwidth (read + write bytes per second,

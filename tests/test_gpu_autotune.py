@@ -1,0 +1,44 @@
+"""The autotuner changes launch shapes only: forces and a full run are bit for
+bit the same before and after tuning, and every tuned shape is accepted."""
+import numpy as np
+import pytest
+
+import cola_halo_amd as ch
+
+pytestmark = pytest.mark.gpu
+
+OM = 0.273
+
+
+def test_autotune_keeps_results(eh_table):
+    print("autotune 64^3 / 192^3", flush=True)
+    k, p = eh_table
+    with ch.Context(64, 192, 100.0) as ctx:
+        ctx.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+        ctx.lpt_set_displacement(0.1, OM, 3)
+        ctx.state_save()
+        ctx.move_particles()
+        ctx.pm_calculate_forces()
+        f0 = ctx.get_forces()
+        tuned = ctx.autotune(reps=2)
+        assert tuned["ms_after"] <= tuned["ms_before"]
+        assert all(t in (4, 8, 16, 32, 64) for t in tuned["tile"])
+        ctx.state_restore()
+        ctx.move_particles()
+        ctx.pm_calculate_forces()
+        f1 = ctx.get_forces()
+    assert np.array_equal(f0, f1)
+
+
+@pytest.mark.parametrize("cols,rows,group", [(2, 0, 4), (3, 1, 8), (4, 2, 64)])
+def test_fft_launch_shapes_agree_bit_for_bit(eh_table, cols, rows, group):
+    k, p = eh_table
+    out = []
+    for shape in (None, (cols, rows, group)):
+        with ch.Context(32, 96, 100.0) as ctx:
+            ctx.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+            if shape:
+                ctx.set_fft_tiles(*shape)
+            (a, snap), = ch.run_cola(ctx, OM, 1.0, 10, 5, [0.0])
+            out.append(snap)
+    assert np.array_equal(out[0], out[1])
