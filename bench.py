@@ -101,6 +101,66 @@ def cpu_baseline(ctx_parts, nc, ng, box, threads):
                       f"{dt:.1f} s, oracle status {st}"}
 
 
+ENSEMBLE = [(96, 288, 187.5)] * 16 + [(128, 384, 250.0)] * 16 + [(160, 480, 312.5)] * 16
+
+
+def ensemble_bench(args):
+    """48 uneven realizations (16 x 96^3/288^3, 16 x 128^3/384^3,
+    16 x 160^3/480^3, seeds 1-48, 10 steps each): the step loops run one
+    realization after another, then interleaved across all contexts with
+    graph-captured steps.  Prints one JSON line."""
+    import cola_halo_amd as ch
+
+    k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
+    da = 1.0 / NSTEPS
+    Om = float(np.float32(OM))
+    ctxs = [ch.Context(nc, ng, box) for nc, ng, box in ENSEMBLE]
+    for c in ctxs:
+        c.power_init_table(k, p, da, 0.8, OM, 1.0 - OM)
+
+    def prepare(graph):
+        for i, c in enumerate(ctxs):
+            c.set_step_graph(graph)
+            c.lpt_set_displacement(da, OM, i + 1)
+            c.set_times(c.times()[0], da)
+        for c in ctxs:
+            c.synchronize()
+
+    def finish():
+        for c in ctxs:
+            c.move_particles()
+            c.pm_calculate_forces()
+        for c in ctxs:
+            c.synchronize()
+
+    particles = sum(nc ** 3 for nc, _, _ in ENSEMBLE)
+    out = {}
+    for mode in ("sequential", "batched_graph"):
+        prepare(mode != "sequential")
+        t0 = time.perf_counter()
+        if mode == "sequential":
+            for c in ctxs:
+                for istep in range(1, NSTEPS):
+                    c.step(Om, (istep + 0.5) * da, (istep + 1.0) * da)
+                c.move_particles()
+                c.pm_calculate_forces()
+                c.synchronize()
+        else:
+            for istep in range(1, NSTEPS):
+                ch.step_batch(ctxs, Om, [(istep + 0.5) * da] * len(ctxs), [(istep + 1.0) * da] * len(ctxs))
+            finish()
+        dt = time.perf_counter() - t0
+        out[mode] = {"seconds": round(dt, 4), "particle_steps_per_s": round(particles * NSTEPS / dt, 1)}
+    for c in ctxs:
+        c.close()
+    print(json.dumps({"metric": "ensemble particle-steps/s, 48 uneven realizations", "value":
+                      out["batched_graph"]["particle_steps_per_s"], "unit": "particle-steps/s", "n_gpus": 1,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "16 x 96^3/288^3 L187.5 + 16 x 128^3/384^3 L250 + 16 x 160^3/480^3 "
+                                             "L312.5, seeds 1-48, 10 steps", "particles": particles},
+                      "modes": out}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +173,8 @@ def main():
     ap.add_argument("--tile", type=str, default="", help="deposit tile tx,ty,tz")
     ap.add_argument("--fft-log2-tile", type=int, default=-2, help="FFT transforms per workgroup (log2)")
     ap.add_argument("--xcd-group", type=int, default=0, help="FFT column tiles per XCD group")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="time the 48-realization ensemble (sequential vs batched graph steps) instead")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of tuning them on the initial conditions")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -123,6 +185,9 @@ def main():
 
         print(json.dumps({"copy_probe_mib": args.copy_probe,
                           "gbps": ch.bandwidth_probe(args.copy_probe << 20, 5)}))
+        return
+    if args.ensemble:
+        ensemble_bench(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this build has a single MI355X; "

@@ -87,6 +87,8 @@ _SIGNATURES = {
     "cola_subsampler_destroy": (C.c_int, [C.c_void_p]),
     "cola_write_subsample": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.POINTER(C.c_longlong)]),
+    "cola_set_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "cola_step_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]),
     "cola_set_fft_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "cola_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cola_fft_time": (C.c_int, [C.c_int, C.c_int, C.c_void_p]),
@@ -524,6 +526,9 @@ class Context:
     def set_tiling(self, tx, ty, tz, fft_log2_tile=-1):
         _check(lib().cola_set_tiling(self._h, tx, ty, tz, fft_log2_tile))
 
+    def set_step_graph(self, on=True):
+        _check(lib().cola_set_step_graph(self._h, 1 if on else 0))
+
     def set_fft_tiles(self, log2_cols=-1, log2_rows=-1, xcd_group=8):
         _check(lib().cola_set_fft_tiles(self._h, log2_cols, log2_rows, xcd_group))
 
@@ -536,6 +541,42 @@ class Context:
 
     def set_fft_xcd_group(self, group):
         _check(lib().cola_set_fft_xcd_group(self._h, group))
+
+
+def step_batch(ctxs, omega_m, avel1, apos1):
+    """cola_step on every context, back to back on their own streams."""
+    n = len(ctxs)
+    hs = (C.c_void_p * n)(*[c._h for c in ctxs])
+    a1 = np.ascontiguousarray(avel1, np.float32)
+    p1 = np.ascontiguousarray(apos1, np.float32)
+    _check(lib().cola_step_batch(hs, n, omega_m, a1.ctypes.data, p1.ctypes.data))
+
+
+def run_cola_batch(ctxs, omega_m, a_final, nsteps, seeds, a_out_final=True):
+    """Realizations of the same schedule (main.c:117-207, one output at
+    a_final) on several contexts, their steps interleaved so the GPU overlaps
+    them.  Returns one snapshot per context."""
+    da = a_final / nsteps
+    a_init = da
+    Om = float(np.float32(omega_m))
+    for ctx, seed in zip(ctxs, seeds):
+        ctx.lpt_set_displacement(a_init, omega_m, seed)
+        ctx.set_times(ctx.times()[0], da)  # main.c:145
+    aout = float(np.float32(a_final))
+    for istep in range(1, nsteps + 1):
+        avel0, apos0 = (istep - 0.5) * da, istep * da
+        avel1, apos1 = (istep + 0.5) * da, (istep + 1.0) * da
+        if not (avel0 <= aout <= avel1):
+            step_batch(ctxs, Om, [avel1] * len(ctxs), [apos1] * len(ctxs))
+            continue
+        snaps = []
+        for ctx in ctxs:
+            ctx.move_particles()
+            ctx.pm_calculate_forces()
+        for ctx in ctxs:
+            snaps.append(ctx.set_snapshot(aout, Om))
+        return snaps
+    return []
 
 
 def run_cola(ctx, omega_m, a_final, nsteps, seed, zout, fused=True, on_snapshot=None):

@@ -90,6 +90,10 @@ struct cola_context {
 
   static constexpr int kMassSlots = 64;
   double* d_mass = nullptr;
+  StepParams* h_prm = nullptr;  // pinned ring, one entry per mass slot
+  StepParams* d_prm = nullptr;  // the values of the step in flight
+  bool graph_mode = false;
+  hipGraphExec_t step_graph = nullptr;  // captured fused step
   int mass_slot = 0;     // next slot
   int mass_pending = 0;  // slots written since the last check
   double last_mass = 0;
@@ -241,13 +245,27 @@ int alloc_grids(cola_context* c)
 
 // pm.c:209-279: binning and the tiled deposit into c->density.  wrap_first
 // folds move_particles2 into the binning pass.
-int pm_assign(cola_context* c, bool wrap_first)
+// Claims the next mass-check slot and sends this step's values to the device
+// (through the pinned ring entry of the slot, which the host does not touch
+// again before the stream has passed it: the ring is drained every 62 slots).
+int step_params(cola_context* c, const KickFactors* k, const DriftFactors* d)
 {
-  COLA_TRY(alloc_grids(c));
+  if (c->mass_pending >= cola_context::kMassSlots - 2) COLA_TRY(check_pending(c));
   const int slot = c->mass_slot;
   c->mass_slot = (c->mass_slot + 1) % cola_context::kMassSlots;
   c->mass_pending += 1;
-  COLA_TRY(launch_zero_doubles(c->d_mass + slot, 1, c->stream));
+  StepParams& hp = c->h_prm[slot];
+  hp.slot = slot;
+  if (k) hp.k = *k;
+  if (d) hp.d = *d;
+  COLA_HIP(hipMemcpyAsync(c->d_prm, &hp, sizeof(StepParams), hipMemcpyHostToDevice, c->stream));
+  return kOk;
+}
+
+// pm.c:209-279 + 535-562 kernels (captured into the step graph as they are).
+int assign_kernels(cola_context* c, bool wrap_first)
+{
+  COLA_TRY(launch_zero_mass(c->d_mass, c->d_prm, c->stream));
   phase_begin(c, kPhBinCount);
   COLA_TRY(launch_bin_count(c->p, c->cic, c->tiles, c->bins, wrap_first, c->boxf, c->stream));
   phase_end(c, kPhBinCount);
@@ -258,8 +276,16 @@ int pm_assign(cola_context* c, bool wrap_first)
   COLA_TRY(launch_bin_scatter(c->p, c->cic, c->tiles, c->bins, c->stream));
   phase_end(c, kPhBinScatter);
   phase_begin(c, kPhDeposit);
-  COLA_TRY(launch_deposit_tiles(c->cic, c->tiles, c->bins, c->density, c->d_mass + slot, c->stream));
+  COLA_TRY(launch_deposit_tiles(c->cic, c->tiles, c->bins, c->density, c->d_mass, c->d_prm, c->stream));
   phase_end(c, kPhDeposit);
+  return kOk;
+}
+
+int pm_assign(cola_context* c, bool wrap_first)
+{
+  COLA_TRY(alloc_grids(c));
+  COLA_TRY(step_params(c, nullptr, nullptr));
+  COLA_TRY(assign_kernels(c, wrap_first));
   c->density_is_real = true;
   return kOk;
 }
@@ -386,6 +412,10 @@ int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
   if (st) return fail(kErrAlloc);
   if (hipMalloc(&c->d_mass, sizeof(double) * cola_context::kMassSlots) != hipSuccess) return fail(kErrAlloc);
   if (hipMemset(c->d_mass, 0, sizeof(double) * cola_context::kMassSlots) != hipSuccess) return fail(kErrHip);
+  if (hipHostMalloc(&c->h_prm, sizeof(StepParams) * cola_context::kMassSlots) != hipSuccess) return fail(kErrAlloc);
+  std::memset(c->h_prm, 0, sizeof(StepParams) * cola_context::kMassSlots);
+  if (hipMalloc(&c->d_prm, sizeof(StepParams)) != hipSuccess) return fail(kErrAlloc);
+  if (hipMemset(c->d_prm, 0, sizeof(StepParams)) != hipSuccess) return fail(kErrHip);
   *out = c;
   return kOk;
 }
@@ -408,6 +438,9 @@ int cola_context_destroy(cola_context* c)
   for (int a = 0; a < 3; ++a)
     if (c->fgrid[a]) (void)hipFree(c->fgrid[a]);
   if (c->d_mass) (void)hipFree(c->d_mass);
+  if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+  if (c->h_prm) (void)hipHostFree(c->h_prm);
+  if (c->d_prm) (void)hipFree(c->d_prm);
   if (c->d_logk) (void)hipFree(c->d_logk);
   if (c->d_logD) (void)hipFree(c->d_logD);
   if (c->d_snap) (void)hipFree(c->d_snap);
@@ -699,22 +732,93 @@ int cola_drift(cola_context* c, float omega_m, float apos1)
   return kOk;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The kernels of one fused step after step_params: wrap + deposit, the force
+// fields, readout + kick + drift.
+int step_kernels(cola_context* c)
+{
+  COLA_TRY(assign_kernels(c, true));
+  COLA_TRY(pm_fields(c));
+  phase_begin(c, kPhReadoutKickDrift);
+  COLA_TRY(launch_readout_kick_drift(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], c->d_prm, c->stream));
+  phase_end(c, kPhReadoutKickDrift);
+  return kOk;
+}
+
+void drop_step_graph(cola_context* c)
+{
+  if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
+  c->step_graph = nullptr;
+}
+
+// Captures step_kernels once; every later step replays it after sending its
+// values (step_params).  Per-phase profiling is off inside the graph.
+int capture_step_graph(cola_context* c)
+{
+  const bool prof = c->profiling;
+  c->profiling = false;
+  COLA_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int st = step_kernels(c);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->profiling = prof;
+  if (st != kOk || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (st == kOk) set_error(std::string("step capture: ") + hipGetErrorString(e));
+    return st != kOk ? st : kErrHip;
+  }
+  const hipError_t ei = hipGraphInstantiate(&c->step_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    c->step_graph = nullptr;
+    set_error(std::string("step graph instantiate: ") + hipGetErrorString(ei));
+    return kErrHip;
+  }
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cola_step(cola_context* c, float omega_m, float avel1, float apos1)
 {
   if (!valid(c)) return kErrArg;
   COLA_TRY(require_particles(c));
-  COLA_TRY(pm_assign(c, true));
-  COLA_TRY(pm_fields(c));
+  COLA_TRY(alloc_grids(c));
   const KickFactors k = kick_factors(omega_m, c->a_x, c->a_v, avel1);
   // the drift starts from a_v = avel1 (cola.c:124 after the kick)
   const DriftFactors d = drift_factors(omega_m, c->a_x, avel1, apos1);
-  phase_begin(c, kPhReadoutKickDrift);
-  COLA_TRY(launch_readout_kick_drift(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], k, d, c->stream));
-  phase_end(c, kPhReadoutKickDrift);
+  if (c->graph_mode && !c->step_graph) COLA_TRY(capture_step_graph(c));
+  COLA_TRY(step_params(c, &k, &d));
+  if (c->graph_mode)
+    COLA_HIP(hipGraphLaunch(c->step_graph, c->stream));
+  else
+    COLA_TRY(step_kernels(c));
+  c->density_is_real = false;
   c->a_v = avel1;
   c->a_x = apos1;
   c->have_force = false;
   if (c->mass_pending >= cola_context::kMassSlots / 2) return check_pending(c);
+  return kOk;
+}
+
+int cola_step_batch(cola_context* const* ctxs, int n, float omega_m, const float* avel1, const float* apos1)
+{
+  if (n < 0 || (n > 0 && (!ctxs || !avel1 || !apos1))) return kErrArg;
+  for (int i = 0; i < n; ++i) COLA_TRY(cola_step(ctxs[i], omega_m, avel1[i], apos1[i]));
+  return kOk;
+}
+
+int cola_set_step_graph(cola_context* c, int on)
+{
+  if (!valid(c)) return kErrArg;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  c->graph_mode = on != 0;
+  if (!c->graph_mode) drop_step_graph(c);
   return kOk;
 }
 
@@ -936,6 +1040,7 @@ int set_deposit_tiles(cola_context* c, int tx, int ty, int tz)
   }
   if (t.tx == c->tiles.tx && t.ty == c->tiles.ty && t.tz == c->tiles.tz && c->bins.counts) return kOk;
   COLA_HIP(hipStreamSynchronize(c->stream));
+  drop_step_graph(c);
   COLA_TRY(check_pending(c));
   BinBuffers nb;
   COLA_TRY(bin_alloc(&nb, t, c->np));
@@ -982,6 +1087,7 @@ int cola_set_fft_tiles(cola_context* c, int log2_cols, int log2_rows, int xcd_gr
       xcd_group > 64 || (xcd_group & (xcd_group - 1)))
     return kErrArg;
   COLA_HIP(hipStreamSynchronize(c->stream));
+  drop_step_graph(c);
   if (log2_cols != c->fft_log2_tile || log2_rows != c->fft_log2_rows) {
     FftPlan a;
     const int st = a.init(c->ng, log2_cols, log2_rows);
@@ -1327,6 +1433,8 @@ int cola_bandwidth_probe(size_t bytes, int reps, double* gbps)
 int cola_set_fft_xcd_group(cola_context* c, int group)
 {
   if (!valid(c) || group < 1 || group > 64 || (group & (group - 1))) return kErrArg;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  drop_step_graph(c);
   c->pm_fft.xgroup = group;
   c->lpt_fft.xgroup = group;
   return kOk;

@@ -61,17 +61,29 @@ int launch_bin_count(const DevParticles& p, const CicGeom& g, const TileGeom& t,
                      float boxsize, hipStream_t s);
 int launch_bin_scan(const TileGeom& t, BinBuffers& b, hipStream_t s);
 int launch_bin_scatter(const DevParticles& p, const CicGeom& g, const TileGeom& t, BinBuffers& b, hipStream_t s);
+// Per-step values the step kernels read from device memory (one small
+// host-to-device copy per step), so that a captured step graph replays with
+// new values: the mass-check ring slot and the kick / drift factors.
+struct StepParams {
+  int slot;
+  int pad;
+  KickFactors k;
+  DriftFactors d;
+};
+
 // LDS-accumulated deposit, one workgroup per tile, no global atomics:
-// density = -1 + WPAR * sum of CIC weights (pm.c:209-279).  mass_sum (zeroed
-// by the caller) receives the double sum over the grid (pm.c:535-562).
-int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& b, float* density, double* mass_sum,
-                         hipStream_t s);
+// density = -1 + WPAR * sum of CIC weights (pm.c:209-279).  ring[prm->slot]
+// (zeroed by launch_zero_mass) receives the double sum over the grid
+// (pm.c:535-562).
+int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& b, float* density, double* ring,
+                         const StepParams* prm, hipStream_t s);
+int launch_zero_mass(double* ring, const StepParams* prm, hipStream_t s);
 // pm.c:359-412 + 505-532: readout of the three force grids into p.f.
 int launch_readout(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy, const float* fz,
                    hipStream_t s);
 // Readout fused with cola.c:104-112 kick and cola.c:142-149 drift.
 int launch_readout_kick_drift(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy,
-                              const float* fz, const KickFactors& k, const DriftFactors& d, hipStream_t s);
+                              const float* fz, const StepParams* prm, hipStream_t s);
 int launch_kick(const DevParticles& p, const KickFactors& k, hipStream_t s);
 int launch_drift(const DevParticles& p, const DriftFactors& d, hipStream_t s);
 

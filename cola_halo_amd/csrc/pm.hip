@@ -355,9 +355,11 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
   }
 }
 
-// pm.c:535-562: sum of the per-tile sums into *out (zeroed by the caller)
-__global__ void __launch_bounds__(kThreads) k_sum_doubles(const double* __restrict__ v, long long n, double* out)
+// pm.c:535-562: sum of the per-tile sums into ring[prm->slot] (zeroed first)
+__global__ void __launch_bounds__(kThreads)
+k_sum_doubles(const double* __restrict__ v, long long n, double* ring, const StepParams* __restrict__ prm)
 {
+  double* const out = ring + prm->slot;
   __shared__ double wsum[kThreads / 64];
   const long long base = (long long)blockIdx.x * kThreads * 16;
   double s = 0.0;
@@ -468,10 +470,12 @@ __device__ inline float drift1(float x, float v, float dx1, float dx2, const Dri
 
 __global__ void __launch_bounds__(kThreads)
 k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
-                     const float* __restrict__ g2, KickFactors kf, DriftFactors df)
+                     const float* __restrict__ g2, const StepParams* __restrict__ prm)
 {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   if (i >= p.np) return;
+  const KickFactors kf = prm->k;
+  const DriftFactors df = prm->d;
   float x[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) x[a] = p.x[a][i];
@@ -537,6 +541,11 @@ __global__ void k_zero_doubles(double* p, int n)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0.0;
+}
+
+__global__ void k_zero_mass(double* ring, const StepParams* __restrict__ prm)
+{
+  if (threadIdx.x == 0) ring[prm->slot] = 0.0;
 }
 
 unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -647,15 +656,15 @@ int launch_bin_scatter(const DevParticles& p, const CicGeom& g, const TileGeom& 
   return kOk;
 }
 
-int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& b, float* density, double* mass_sum,
-                         hipStream_t s)
+int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& b, float* density, double* ring,
+                         const StepParams* prm, hipStream_t s)
 {
   const size_t lds = sizeof(unsigned long long) * t.tx * t.ty * t.tz;
   hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)t.ntiles), dim3(kThreads), lds, s, g, t, b.offsets, b.sorted,
                      density, b.tile_sums);
   const long long per = (long long)kThreads * 16;
   hipLaunchKernelGGL(k_sum_doubles, dim3((unsigned)((t.ntiles + per - 1) / per)), dim3(kThreads), 0, s, b.tile_sums,
-                     t.ntiles, mass_sum);
+                     t.ntiles, ring, prm);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
@@ -670,10 +679,10 @@ int launch_readout(const DevParticles& p, const CicGeom& g, const float* fx, con
 }
 
 int launch_readout_kick_drift(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy,
-                              const float* fz, const KickFactors& k, const DriftFactors& d, hipStream_t s)
+                              const float* fz, const StepParams* prm, hipStream_t s)
 {
   if (p.np == 0) return kOk;
-  hipLaunchKernelGGL(k_readout_kick_drift, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz, k, d);
+  hipLaunchKernelGGL(k_readout_kick_drift, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz, prm);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
@@ -706,6 +715,13 @@ int launch_noncola_initial(const DevParticles& p, const SnapshotFactors& f, DevP
 {
   if (p.np == 0) return kOk;
   hipLaunchKernelGGL(k_noncola_initial, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, f, out);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_zero_mass(double* ring, const StepParams* prm, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_zero_mass, dim3(1), dim3(64), 0, s, ring, prm);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

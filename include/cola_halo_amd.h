@@ -133,6 +133,19 @@ int cola_drift(cola_context* ctx, float omega_m, float apos1);
    No single reference counterpart. */
 int cola_step(cola_context* ctx, float omega_m, float avel1, float apos1);
 
+/* Graph mode for cola_step (no reference counterpart): the step's kernels are
+   captured into a HIP graph on the first call and replayed afterwards; the
+   per-step values (kick / drift factors, mass-check slot) reach the kernels
+   through one small host-to-device copy before each replay.  Results equal
+   the eager path bit for bit.  Changing tiles or FFT shapes drops the graph. */
+int cola_set_step_graph(cola_context* ctx, int on);
+
+/* Batched realizations (no reference counterpart): one cola_step on each of n
+   contexts (sizes may differ), issued back to back on their own streams so
+   the GPU overlaps the realizations' kernels; per-context avel1 / apos1.
+   Equals n separate cola_step calls bit for bit. */
+int cola_step_batch(cola_context* const* ctxs, int n, float omega_m, const float* avel1, const float* apos1);
+
 /* cola.c:341-416 cola_set_snapshot (cola.h:18, main.c:346): kick/drift to
    aout, add the 2LPT velocity, km/s; writes np records in lattice-ID order
    into the caller's host array. */
