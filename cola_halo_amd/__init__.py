@@ -87,6 +87,22 @@ _SIGNATURES = {
     "cola_subsampler_destroy": (C.c_int, [C.c_void_p]),
     "cola_write_subsample": (C.c_int, [C.c_void_p, C.c_char_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.POINTER(C.c_longlong)]),
+    "cola_resnet_nparam": (C.c_int, []),
+    "cola_resnet_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(C.c_void_p)]),
+    "cola_resnet_destroy": (C.c_int, [C.c_void_p]),
+    "cola_resnet_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cola_resnet_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cola_resnet_get_scales": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "cola_resnet_set_scales": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    "cola_resnet_forward_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
+                                            C.c_void_p]),
+    "cola_copy_particles": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cola_resnet_add_pair": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cola_resnet_num_samples": (C.c_int, [C.c_void_p]),
+    "cola_resnet_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint, C.c_void_p]),
+    "cola_resnet_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "cola_set_residual_net": (C.c_int, [C.c_void_p, C.c_void_p]),
     "cola_set_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_step_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]),
     "cola_set_fft_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
@@ -284,6 +300,81 @@ class Tuning(C.Structure):
         return {"tile": list(self.tile), "fft_log2_cols": self.fft_log2_cols, "fft_log2_rows": self.fft_log2_rows,
                 "xcd_group": self.xcd_group, "ms_before": round(self.ms_before, 4),
                 "ms_after": round(self.ms_after, 4)}
+
+
+class ResidualNet:
+    """The learned coarse-to-fine force residual (cola_resnet_*)."""
+
+    def __init__(self, n, seed=1):
+        self.n = n
+        self._h = C.c_void_p()
+        _check(lib().cola_resnet_create(n, seed, C.byref(self._h)))
+
+    @staticmethod
+    def nparam():
+        return lib().cola_resnet_nparam()
+
+    def params(self):
+        out = np.zeros(self.nparam(), np.float32)
+        _check(lib().cola_resnet_get_params(self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparam()
+        _check(lib().cola_resnet_set_params(self._h, p.ctypes.data))
+
+    def scales(self):
+        a, b = C.c_float(), C.c_float()
+        _check(lib().cola_resnet_get_scales(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_scales(self, in_scale, out_scale):
+        _check(lib().cola_resnet_set_scales(self._h, in_scale, out_scale))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(x.shape[:3] + (3,), np.float32)
+        _check(lib().cola_resnet_forward_host(self._h, x.ctypes.data, y.ctypes.data))
+        return y
+
+    def gradient(self, x, target):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        loss = C.c_double()
+        g = np.zeros(self.nparam(), np.float32)
+        _check(lib().cola_resnet_gradient_host(self._h, x.ctypes.data, t.ctypes.data, C.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def add_pair(self, coarse, fine):
+        _check(lib().cola_resnet_add_pair(self._h, coarse._h, fine._h))
+
+    def num_samples(self):
+        return lib().cola_resnet_num_samples(self._h)
+
+    def train(self, ntrain, epochs, lr=1e-3, seed=1):
+        out = np.zeros(epochs)
+        _check(lib().cola_resnet_train(self._h, ntrain, epochs, lr, seed, out.ctypes.data))
+        return out
+
+    def eval(self, first, count):
+        loss = C.c_double()
+        _check(lib().cola_resnet_eval(self._h, first, count, C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            lib().cola_resnet_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 PARAM_MAX_ZOUT = 256
@@ -525,6 +616,12 @@ class Context:
 
     def set_tiling(self, tx, ty, tz, fft_log2_tile=-1):
         _check(lib().cola_set_tiling(self._h, tx, ty, tz, fft_log2_tile))
+
+    def copy_particles_from(self, other):
+        _check(lib().cola_copy_particles(self._h, other._h))
+
+    def set_residual_net(self, net):
+        _check(lib().cola_set_residual_net(self._h, net._h if net is not None else None))
 
     def set_step_graph(self, on=True):
         _check(lib().cola_set_step_graph(self._h, 1 if on else 0))

@@ -15,6 +15,7 @@
 #include "analysis.h"
 #include "common.h"
 #include "fof.h"
+#include "resnet.h"
 #include "cosmo.h"
 #include "fft.h"
 #include "lpt.h"
@@ -93,6 +94,7 @@ struct cola_context {
   StepParams* h_prm = nullptr;  // pinned ring, one entry per mass slot
   StepParams* d_prm = nullptr;  // the values of the step in flight
   bool graph_mode = false;
+  ResNet* net = nullptr;  // learned force residual added to the force grids (not owned)
   hipGraphExec_t step_graph = nullptr;  // captured fused step
   int mass_slot = 0;     // next slot
   int mass_pending = 0;  // slots written since the last check
@@ -256,6 +258,7 @@ int step_params(cola_context* c, const KickFactors* k, const DriftFactors* d)
   c->mass_pending += 1;
   StepParams& hp = c->h_prm[slot];
   hp.slot = slot;
+  hp.a = c->a_x;
   if (k) hp.k = *k;
   if (d) hp.d = *d;
   COLA_HIP(hipMemcpyAsync(c->d_prm, &hp, sizeof(StepParams), hipMemcpyHostToDevice, c->stream));
@@ -297,9 +300,16 @@ int pm_fields(cola_context* c)
   const float scale = 2.0 * kPiD / c->boxf;  // pm.c:315
   KOpParams kp;
   kp.f1 = -1.0 / std::pow((double)c->ng, 3.0) / scale;  // pm.c:317
-  c->density_is_real = false;
   float* const out[3] = {c->fgrid[0], c->fgrid[1], c->fgrid[2]};
-  return c->pm_fft.forces(c->density, out, kp, c->stream);
+  if (c->net) {
+    // the net reads the real-space density before the r2c overwrites it
+    COLA_TRY(resnet_features(c->net, c->density, c->cic.row, &c->d_prm->a, c->stream));
+    COLA_TRY(resnet_forward(c->net, c->stream));
+  }
+  c->density_is_real = false;
+  COLA_TRY(c->pm_fft.forces(c->density, out, kp, c->stream));
+  if (c->net) COLA_TRY(resnet_add_residual(c->net, out, c->cic.row, c->stream));
+  return kOk;
 }
 
 bool valid(const cola_context* c) { return c != nullptr; }
@@ -1396,6 +1406,220 @@ int cola_get_snapshot(cola_context* c, cola_particle_min* out, size_t np_out)
   }
   COLA_HIP(hipMemcpyAsync(out, c->d_snap, sizeof(DevParticleMin) * c->np, hipMemcpyDeviceToHost, c->stream));
   COLA_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+int cola_copy_particles(cola_context* dst, cola_context* src)
+{
+  if (!valid(dst) || !valid(src) || dst->np != src->np) return kErrArg;
+  COLA_TRY(require_particles(src));
+  COLA_HIP(hipStreamSynchronize(src->stream));
+  COLA_HIP(hipStreamSynchronize(dst->stream));
+  COLA_HIP(hipMemcpyAsync(dst->pbuf, src->pbuf, sizeof(float) * 12 * (size_t)src->np, hipMemcpyDeviceToDevice,
+                          dst->stream));
+  dst->a_x = src->a_x;
+  dst->a_v = src->a_v;
+  dst->have_particles = true;
+  dst->have_force = false;
+  return kOk;
+}
+
+}  // extern "C"
+
+struct cola_resnet {
+  ResNet r;
+  hipStream_t stream = nullptr;
+};
+
+extern "C" {
+
+int cola_resnet_nparam(void) { return kNetParams; }
+
+int cola_resnet_create(int n, unsigned int seed, cola_resnet** out)
+{
+  if (!out) return kErrArg;
+  *out = nullptr;
+  cola_resnet* h = new cola_resnet;
+  int st = hipStreamCreate(&h->stream) == hipSuccess ? kOk : kErrHip;
+  if (!st) st = resnet_init(&h->r, n, seed);
+  if (!st) st = hipStreamSynchronize(h->stream) == hipSuccess ? kOk : kErrHip;
+  if (st) {
+    resnet_free(&h->r);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return st;
+  }
+  *out = h;
+  return kOk;
+}
+
+int cola_resnet_destroy(cola_resnet* h)
+{
+  if (!h) return kOk;
+  (void)hipStreamSynchronize(h->stream);
+  resnet_free(&h->r);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return kOk;
+}
+
+int cola_resnet_get_params(cola_resnet* h, float* out)
+{
+  if (!h || !out) return kErrArg;
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_HIP(hipMemcpy(out, h->r.d_param, sizeof(float) * kNetParams, hipMemcpyDeviceToHost));
+  return kOk;
+}
+
+int cola_resnet_set_params(cola_resnet* h, const float* in)
+{
+  if (!h || !in) return kErrArg;
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_HIP(hipMemcpy(h->r.d_param, in, sizeof(float) * kNetParams, hipMemcpyHostToDevice));
+  COLA_TRY(resnet_pack(&h->r, h->stream));
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+int cola_resnet_get_scales(cola_resnet* h, float* in_scale, float* out_scale)
+{
+  if (!h) return kErrArg;
+  if (in_scale) *in_scale = h->r.in_scale;
+  if (out_scale) *out_scale = h->r.out_scale;
+  return kOk;
+}
+
+int cola_resnet_set_scales(cola_resnet* h, float in_scale, float out_scale)
+{
+  if (!h || !(out_scale > 0.0f) || !(in_scale > 0.0f)) return kErrArg;
+  h->r.in_scale = in_scale;
+  h->r.out_scale = out_scale;
+  return kOk;
+}
+
+namespace {
+
+// host x [n^3][8] -> device input (bf16)
+int upload_input(cola_resnet* h, const float* x)
+{
+  const long long nc = (long long)h->r.n * h->r.n * h->r.n;
+  std::vector<unsigned short> xb(nc * 8);
+  for (long long i = 0; i < nc * 8; ++i) {
+    unsigned u;
+    std::memcpy(&u, &x[i], 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    xb[i] = (unsigned short)(u >> 16);
+  }
+  COLA_HIP(hipMemcpy(h->r.d_x, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+  return kOk;
+}
+
+}  // namespace
+
+int cola_resnet_forward_host(cola_resnet* h, const float* x, float* y)
+{
+  if (!h || !x || !y) return kErrArg;
+  const long long nc = (long long)h->r.n * h->r.n * h->r.n;
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(upload_input(h, x));
+  COLA_TRY(resnet_forward(&h->r, h->stream));
+  std::vector<float> yd(3 * nc);
+  COLA_HIP(hipMemcpyAsync(yd.data(), h->r.d_y, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost, h->stream));
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  for (long long cell = 0; cell < nc; ++cell)
+    for (int a = 0; a < 3; ++a) y[cell * 3 + a] = yd[a * nc + cell];
+  return kOk;
+}
+
+int cola_resnet_gradient_host(cola_resnet* h, const float* x, const float* target, double* loss, float* grad)
+{
+  if (!h || !x || !target || !loss || !grad) return kErrArg;
+  const long long nc = (long long)h->r.n * h->r.n * h->r.n;
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(upload_input(h, x));
+  std::vector<float> t(3 * nc);
+  for (long long cell = 0; cell < nc; ++cell)
+    for (int a = 0; a < 3; ++a) t[a * nc + cell] = target[cell * 3 + a] * h->r.out_scale;
+  float* dt = nullptr;
+  COLA_HIP(hipMalloc(&dt, sizeof(float) * 3 * nc));
+  int st = hipMemcpy(dt, t.data(), sizeof(float) * 3 * nc, hipMemcpyHostToDevice) == hipSuccess ? kOk : kErrHip;
+  if (!st) st = resnet_forward(&h->r, h->stream);
+  if (!st) st = resnet_backward(&h->r, nullptr, dt, h->stream);
+  if (!st && hipMemcpyAsync(loss, h->r.d_loss, sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess)
+    st = kErrHip;
+  if (!st && hipMemcpyAsync(grad, h->r.d_grad, sizeof(float) * kNetParams, hipMemcpyDeviceToHost, h->stream) !=
+                 hipSuccess)
+    st = kErrHip;
+  if (!st && hipStreamSynchronize(h->stream) != hipSuccess) st = kErrHip;
+  (void)hipFree(dt);
+  return st;
+}
+
+int cola_resnet_add_pair(cola_resnet* h, cola_context* coarse, cola_context* fine)
+{
+  if (!h || !valid(coarse) || !valid(fine)) return kErrArg;
+  if (coarse->ng != h->r.n || fine->ng != 2 * coarse->ng || fine->np != coarse->np ||
+      fine->boxsize != coarse->boxsize) {
+    set_error("force pair: need coarse ngrid == net grid, fine ngrid == 2 coarse, same particles and box");
+    return kErrArg;
+  }
+  COLA_TRY(require_particles(coarse));
+  COLA_TRY(require_particles(fine));
+  ResNet* const saved = coarse->net;
+  coarse->net = nullptr;  // the pair holds the plain coarse force
+  int st = pm_assign(coarse, true);
+  if (!st) st = hipStreamSynchronize(coarse->stream) == hipSuccess ? kOk : kErrHip;
+  float* dcopy = nullptr;
+  // keep the coarse real-space density: pm_fields transforms it in place
+  if (!st && hipMalloc(&dcopy, sizeof(float) * coarse->grid_floats) != hipSuccess) st = kErrAlloc;
+  if (!st && hipMemcpyAsync(dcopy, coarse->density, sizeof(float) * coarse->grid_floats, hipMemcpyDeviceToDevice,
+                            coarse->stream) != hipSuccess)
+    st = kErrHip;
+  if (!st) st = pm_fields(coarse);
+  coarse->net = saved;
+  if (!st) st = pm_assign(fine, true);
+  if (!st) st = pm_fields(fine);
+  if (!st && (hipStreamSynchronize(fine->stream) != hipSuccess || hipStreamSynchronize(coarse->stream) != hipSuccess))
+    st = kErrHip;
+  if (!st) st = check_pending(coarse);
+  if (!st) st = check_pending(fine);
+  if (!st) {
+    const float* const cf[3] = {coarse->fgrid[0], coarse->fgrid[1], coarse->fgrid[2]};
+    const float* const ff[3] = {fine->fgrid[0], fine->fgrid[1], fine->fgrid[2]};
+    st = resnet_store_pair(&h->r, dcopy, coarse->cic.row, cf, ff, fine->cic.row, coarse->a_x, h->stream);
+  }
+  if (!st && hipStreamSynchronize(h->stream) != hipSuccess) st = kErrHip;
+  if (dcopy) (void)hipFree(dcopy);
+  coarse->have_force = false;
+  fine->have_force = false;
+  return st;
+}
+
+int cola_resnet_num_samples(const cola_resnet* h) { return h ? (int)h->r.s_feat.size() : 0; }
+
+int cola_resnet_train(cola_resnet* h, int ntrain, int epochs, float lr, unsigned int seed, double* epoch_loss)
+{
+  if (!h) return kErrArg;
+  return resnet_train(&h->r, ntrain, epochs, lr, seed, epoch_loss, h->stream);
+}
+
+int cola_resnet_eval(cola_resnet* h, int first, int count, double* loss)
+{
+  if (!h || !loss) return kErrArg;
+  return resnet_eval(&h->r, first, count, loss, h->stream);
+}
+
+int cola_set_residual_net(cola_context* c, cola_resnet* h)
+{
+  if (!valid(c)) return kErrArg;
+  if (h && h->r.n != c->ng) {
+    set_error("residual net grid differs from the context's PM grid");
+    return kErrArg;
+  }
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  if (h) COLA_HIP(hipStreamSynchronize(h->stream));
+  drop_step_graph(c);
+  c->net = h ? &h->r : nullptr;
   return kOk;
 }
 

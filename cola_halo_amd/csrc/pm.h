@@ -63,10 +63,10 @@ int launch_bin_scan(const TileGeom& t, BinBuffers& b, hipStream_t s);
 int launch_bin_scatter(const DevParticles& p, const CicGeom& g, const TileGeom& t, BinBuffers& b, hipStream_t s);
 // Per-step values the step kernels read from device memory (one small
 // host-to-device copy per step), so that a captured step graph replays with
-// new values: the mass-check ring slot and the kick / drift factors.
+// new values: the mass-check ring slot, the time and the kick / drift factors.
 struct StepParams {
   int slot;
-  int pad;
+  float a;  // scale factor of the positions the force is computed at
   KickFactors k;
   DriftFactors d;
 };

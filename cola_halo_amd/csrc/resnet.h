@@ -1,0 +1,73 @@
+// Learned force residual: a 4-layer 3-D convolution network on the coarse PM
+// grid (16 channels, 3^3 kernels, periodic), input asinh(delta) and the scale
+// factor a, output the 3-component force residual between a coarse PM grid
+// and a grid twice as fine.  Activations and weights in bfloat16, products
+// on MFMA (v_mfma_f32_16x16x32_bf16) with fp32 accumulation; master weights
+// and Adam moments in fp32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace cola {
+
+constexpr int kNetLayers = 4;
+constexpr int kNetCin[kNetLayers] = {8, 16, 16, 16};
+constexpr int kNetCout[kNetLayers] = {16, 16, 16, 3};
+constexpr int kNetParams = 27 * 8 * 16 + 16 + 2 * (27 * 16 * 16 + 16) + 27 * 16 * 3 + 3;
+
+struct ResNet {
+  int n = 0;  // grid cells per axis (multiple of 16)
+  // fp32 master parameters (W1 b1 W2 b2 W3 b3 W4 b4; W[27][cin][cout]) and Adam moments
+  float* d_param = nullptr;
+  float* d_m = nullptr;
+  float* d_v = nullptr;
+  float* d_grad = nullptr;
+  long long adam_t = 0;
+  // MFMA B operands: forward (per layer) and data-gradient (layers 2-4) packs
+  unsigned short* d_wf[kNetLayers] = {nullptr, nullptr, nullptr, nullptr};
+  unsigned short* d_wb[kNetLayers] = {nullptr, nullptr, nullptr, nullptr};
+  // activations: input [n^3][8], h1..h3 [n^3][16] (bf16), output y [3][n^3] fp32
+  unsigned short* d_x = nullptr;
+  unsigned short* d_h[3] = {nullptr, nullptr, nullptr};
+  float* d_y = nullptr;
+  // gradients at layer outputs [n^3][16] (bf16), two buffers
+  unsigned short* d_g[2] = {nullptr, nullptr};
+  float* d_wpart = nullptr;  // per-wave partial weight gradients
+  int wpart_waves = 0;
+  double* d_loss = nullptr;
+  float* d_a = nullptr;      // scale factor of standalone evaluations
+  float out_scale = 1.0f;    // residual = out_scale * y
+  float in_scale = 1.0f;     // feature = asinh(delta) * in_scale
+
+  // training set: per sample the feature grid (bf16 [n^3]), the target
+  // residual (bf16 [3][n^3]) and the scale factor a
+  std::vector<unsigned short*> s_feat, s_target;
+  std::vector<float> s_a;
+};
+
+int resnet_init(ResNet* r, int n, unsigned seed);
+void resnet_free(ResNet* r);
+// fp32 master parameters -> bf16 MFMA packs
+int resnet_pack(ResNet* r, hipStream_t s);
+// input features from a density grid (rows of `row` floats) and the scale
+// factor *d_a (device memory, so a captured step reads each step's value)
+int resnet_features(ResNet* r, const float* delta, long long row, const float* d_a, hipStream_t s);
+// stores a training pair: features of the coarse density, target = fine force
+// at the coincident points (fine cell 2i) minus coarse force, and a
+int resnet_store_pair(ResNet* r, const float* coarse_delta, long long crow, const float* const coarse_f[3],
+                      const float* const fine_f[3], long long frow, float a, hipStream_t s);
+// mean loss over stored samples [first, first + count); train: epochs over
+// samples [0, ntrain) in shuffled order, one Adam step per sample
+int resnet_eval(ResNet* r, int first, int count, double* loss, hipStream_t s);
+int resnet_train(ResNet* r, int ntrain, int epochs, float lr, unsigned seed, double* epoch_loss, hipStream_t s);
+// forward from d_x; writes d_y (and keeps h1..h3)
+int resnet_forward(ResNet* r, hipStream_t s);
+// loss against target (bf16 [3][n^3] or fp32 [3][n^3]) and gradient into d_grad
+int resnet_backward(ResNet* r, const unsigned short* target_bf16, const float* target_f32, hipStream_t s);
+int resnet_adam(ResNet* r, float lr, hipStream_t s);
+// out[a] (rows of `row` floats) += out_scale * y[a]
+int resnet_add_residual(const ResNet* r, float* const out[3], long long row, hipStream_t s);
+
+}  // namespace cola

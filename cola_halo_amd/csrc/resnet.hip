@@ -1,0 +1,590 @@
+// Learned force residual on MFMA: implicit-GEMM 3-D convolutions in bfloat16
+// with fp32 accumulation (forward, data gradient, weight gradient), the loss,
+// Adam and the glue to the PM grids.
+//
+// Layout: activations channel-last, act[cell][16] (input [cell][8]), cell =
+// (x n + y) n + z, periodic.  A convolution is the GEMM
+//   out[cell][co] = sum over taps t and input channels ci of
+//                   in[cell + d_t][ci] W[t][ci][co],   K = 27 cin,
+// computed per wave as a 16-cell (along z) x 16-channel tile with
+// v_mfma_f32_16x16x32_bf16.  Its lanes hold (g = lane / 16, r = lane % 16):
+// A rows r, k = 8g..8g+7; B k = 8g..8g+7, column r; D rows 4g..4g+3,
+// column r.  With cin = 16 one K chunk of 32 covers two taps (8 channels per
+// lane group), with cin = 8 four taps, so every A operand is one 16-byte load
+// of 8 consecutive channels of one cell.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "common.h"
+#include "resnet.h"
+
+namespace cola {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+
+__host__ __device__ inline unsigned short f2bf(float f)
+{
+  unsigned u = __builtin_bit_cast(unsigned, f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);                                         // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
+
+__device__ inline bf16x8 load8(const unsigned short* p)
+{
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+__device__ inline long long cell_at(int x, int y, int z, int t, int sign, int n)
+{
+  const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+  return ((long long)wrapi(x + sign * dx, n) * n + wrapi(y + sign * dy, n)) * n + wrapi(z + sign * dz, n);
+}
+
+enum ConvMode { kFwdRelu = 0, kFwdLinear = 1, kBwdData = 2 };
+
+// One wave per 16-cell x 16-channel output tile, grid-stride over tiles; the
+// B operands of all K chunks stay in registers.  CIN = channels of the A
+// tensor (8 or 16).  kBwdData reads the A tensor at cell - d_t and masks the
+// result by the derivative of the previous layer's relu.
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(kThreads)
+k_conv(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
+       int cout, int n, unsigned short* __restrict__ out_bf, float* __restrict__ out_f,
+       const unsigned short* __restrict__ hprev)
+{
+  constexpr int kTpc = 32 / CIN;                 // taps per K chunk
+  constexpr int kChunks = (27 + kTpc - 1) / kTpc;
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  bf16x8 b[kChunks];
+#pragma unroll
+  for (int c = 0; c < kChunks; ++c) b[c] = load8(wpk + ((long long)c * 64 + lane) * 8);
+  const float bcol = (MODE == kBwdData || r >= cout) ? 0.0f : bias[r];
+  const int tile_tap = (8 * g) / CIN;           // tap within the chunk of this lane group
+  const int ci0 = (8 * g) % CIN;                // first channel of this lane group
+  const int sign = MODE == kBwdData ? -1 : 1;
+  const long long ntiles = (long long)n * n * (n / 16);
+  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
+  const long long wave = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  for (long long tile = wave; tile < ntiles; tile += nwaves) {
+    const int zt = (int)(tile % (n / 16));
+    const long long xy = tile / (n / 16);
+    const int y = (int)(xy % n), x = (int)(xy / n);
+    const int z = zt * 16 + r;
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+      const int t = c * kTpc + tile_tap;
+      bf16x8 a;
+      if (t < 27)
+        a = load8(in + cell_at(x, y, z, t, sign, n) * CIN + ci0);
+      else
+        a = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[c], acc, 0, 0, 0);
+    }
+    const long long row0 = ((long long)x * n + y) * n + zt * 16 + 4 * g;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const long long cell = row0 + i;
+      float v = acc[i] + bcol;
+      if (MODE == kFwdRelu) {
+        out_bf[cell * 16 + r] = f2bf(v > 0.0f ? v : 0.0f);
+      } else if (MODE == kFwdLinear) {
+        if (r < 3) out_f[(long long)r * n * n * n + cell] = v;
+      } else {
+        out_bf[cell * 16 + r] = f2bf(bf2f(hprev[cell * 16 + r]) > 0.0f ? v : 0.0f);
+      }
+    }
+  }
+}
+
+// Weight gradient of one layer: dW[t][ci][co] = sum_cells X[cell + d_t][ci]
+// G[cell][co] (M = ci, N = co, K = 32 cells along z per MFMA; all 27 taps
+// accumulate in registers) and db[co] = sum_cells G[cell][co].  Each wave
+// writes its partial sums to part[wave][27 * 256 + 16].
+template <int CIN>
+__global__ void __launch_bounds__(kThreads)
+k_conv_wgrad(const unsigned short* __restrict__ X, const unsigned short* __restrict__ G, int n,
+             float* __restrict__ part)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  floatx4 acc[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  float bsum = 0.0f;
+  const long long nblk = (long long)n * n * (n / 32);
+  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
+  const long long wave = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  for (long long blk = wave; blk < nblk; blk += nwaves) {
+    const int zb = (int)(blk % (n / 32));
+    const long long xy = blk / (n / 32);
+    const int y = (int)(xy % n), x = (int)(xy / n);
+    const int z0 = zb * 32 + 8 * g;
+    unsigned short bv[8];
+    const long long base = ((long long)x * n + y) * n + z0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      bv[j] = G[(base + j) * 16 + r];
+      bsum += bf2f(bv[j]);
+    }
+    bf16x8 bop;
+    __builtin_memcpy(&bop, bv, 16);
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      unsigned short av[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) av[j] = r < CIN ? X[cell_at(x, y, z0 + j, t, 1, n) * CIN + r] : (unsigned short)0;
+      bf16x8 aop;
+      __builtin_memcpy(&aop, av, 16);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop, bop, acc[t], 0, 0, 0);
+    }
+  }
+  float* p = part + wave * (27 * 256 + 16);
+#pragma unroll
+  for (int t = 0; t < 27; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) p[(t * 16 + 4 * g + i) * 16 + r] = acc[t][i];
+  // bias: lanes r, r + 16, r + 32, r + 48 hold the same channel
+  bsum += __shfl_xor(bsum, 16);
+  bsum += __shfl_xor(bsum, 32);
+  if (g == 0) p[27 * 256 + r] = bsum;
+}
+
+// Sum of the per-wave partials (fixed order) into the fp32 gradient of one
+// layer, in the master layout W[t][ci][co] then b[co].
+__global__ void k_wgrad_reduce(const float* __restrict__ part, int nwaves, int cin, int cout, float* __restrict__ grad)
+{
+  const int nw = 27 * cin * cout;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nw + cout) return;
+  int src;
+  if (p < nw) {
+    const int t = p / (cin * cout), ci = (p / cout) % cin, co = p % cout;
+    src = (t * 16 + ci) * 16 + co;
+  } else {
+    src = 27 * 256 + (p - nw);
+  }
+  float s = 0.0f;
+  for (int w = 0; w < nwaves; ++w) s += part[(long long)w * (27 * 256 + 16) + src];
+  grad[p] = s;
+}
+
+// fp32 master weights W[t][ci][co] -> forward B packs (k = tap-major over ci)
+// and data-gradient B packs (A = gradient with 16 channels co, column = ci).
+__global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned short* __restrict__ wf,
+                       unsigned short* __restrict__ wb)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int c = blockIdx.x;  // chunk
+  {
+    const int tpc = 32 / cin, chunks = (27 + tpc - 1) / tpc;
+    if (c < chunks)
+      for (int j = 0; j < 8; ++j) {
+        const int k = 8 * g + j, tap = c * tpc + k / cin, ci = k % cin;
+        const float v = (tap < 27 && col < cout) ? w[((long long)tap * cin + ci) * cout + col] : 0.0f;
+        wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
+      }
+  }
+  if (wb && c < 14)
+    for (int j = 0; j < 8; ++j) {
+      const int k = 8 * g + j, tap = 2 * c + k / 16, co = k % 16;
+      const float v = (tap < 27 && co < cout && col < cin) ? w[((long long)tap * cin + col) * cout + co] : 0.0f;
+      wb[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
+    }
+}
+
+// Input features: channel 0 = asinh(delta) * scale, channel 1 = a, rest 0.
+__global__ void k_features(const float* __restrict__ delta, long long row, int n, const float* __restrict__ pa,
+                           float scale, unsigned short* __restrict__ x)
+{
+  const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long nc = (long long)n * n * n;
+  if (cell >= nc) return;
+  const long long line = cell / n;
+  const float d = delta[line * row + cell % n];
+  unsigned short v[8] = {f2bf(asinhf(d) * scale), f2bf(*pa), 0, 0, 0, 0, 0, 0};
+  *reinterpret_cast<uint4*>(x + cell * 8) = *reinterpret_cast<uint4*>(v);
+}
+
+// Features of a stored sample (bf16 asinh(delta) * scale) and its a.
+__global__ void k_features_stored(const unsigned short* __restrict__ f, int n, float a, unsigned short* __restrict__ x)
+{
+  const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (cell >= (long long)n * n * n) return;
+  unsigned short v[8] = {f[cell], f2bf(a), 0, 0, 0, 0, 0, 0};
+  *reinterpret_cast<uint4*>(x + cell * 8) = *reinterpret_cast<uint4*>(v);
+}
+
+// Loss = mean (y - t / out_scale)^2 over cells and 3 components; gradient
+// G[cell][co] = 2 (y - t) / N for co < 3, 0 for the padding channels.
+__global__ void __launch_bounds__(kThreads)
+k_loss(const float* __restrict__ y, const unsigned short* __restrict__ tb, const float* __restrict__ tf, int n,
+       float inv_scale, unsigned short* __restrict__ G, double* __restrict__ loss)
+{
+  __shared__ double part[kThreads / 64];
+  const long long nc = (long long)n * n * n;
+  const double inv_n = 1.0 / (3.0 * (double)nc);
+  double s = 0.0;
+  const long long stride = (long long)gridDim.x * kThreads;
+  for (long long cell = (long long)blockIdx.x * kThreads + threadIdx.x; cell < nc; cell += stride) {
+    unsigned short gv[16] = {0};
+    for (int c = 0; c < 3; ++c) {
+      const long long i = (long long)c * nc + cell;
+      const float t = (tb ? bf2f(tb[i]) : tf[i]) * inv_scale;
+      const float d = y[i] - t;
+      s += (double)d * d;
+      gv[c] = f2bf((float)(2.0 * inv_n * d));
+    }
+    uint4* o = reinterpret_cast<uint4*>(G + cell * 16);
+    o[0] = *reinterpret_cast<uint4*>(gv);
+    o[1] = *reinterpret_cast<uint4*>(gv + 8);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) t += part[w];
+    unsafeAtomicAdd(loss, t * inv_n);
+  }
+}
+
+__global__ void k_zero1(double* p)
+{
+  if (threadIdx.x == 0) *p = 0.0;
+}
+
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, int np, float lr, float b1, float b2, float c1, float c2, float eps)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Training pair: feature asinh(delta) * scale (bf16) and target = fine force
+// at the coincident fine cell (2i, 2j, 2k) minus the coarse force (bf16).
+__global__ void k_store_pair(const float* __restrict__ cd, long long crow, const float* __restrict__ c0,
+                             const float* __restrict__ c1, const float* __restrict__ c2, const float* __restrict__ f0,
+                             const float* __restrict__ f1, const float* __restrict__ f2, long long frow, int n,
+                             float scale, unsigned short* __restrict__ feat, unsigned short* __restrict__ target)
+{
+  const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long nc = (long long)n * n * n;
+  if (cell >= nc) return;
+  const int z = (int)(cell % n);
+  const long long xy = cell / n;
+  const int y = (int)(xy % n), x = (int)(xy / n);
+  const long long ca = xy * crow + z;
+  const long long fa = ((long long)(2 * x) * (2 * n) + 2 * y) * frow + 2 * z;
+  feat[cell] = f2bf(asinhf(cd[ca]) * scale);
+  target[cell] = f2bf(f0[fa] - c0[ca]);
+  target[nc + cell] = f2bf(f1[fa] - c1[ca]);
+  target[2 * nc + cell] = f2bf(f2[fa] - c2[ca]);
+}
+
+__global__ void __launch_bounds__(kThreads) k_sumsq(const unsigned short* __restrict__ t, long long n, double* out)
+{
+  __shared__ double part[kThreads / 64];
+  double s = 0.0;
+  const long long stride = (long long)gridDim.x * kThreads;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+    const double v = bf2f(t[i]);
+    s += v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) tot += part[w];
+    unsafeAtomicAdd(out, tot);
+  }
+}
+
+__global__ void k_add_residual(const float* __restrict__ y, int n, float scale, float* __restrict__ o0,
+                               float* __restrict__ o1, float* __restrict__ o2, long long row)
+{
+  const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long nc = (long long)n * n * n;
+  if (cell >= nc) return;
+  const long long at = (cell / n) * row + cell % n;
+  o0[at] += scale * y[cell];
+  o1[at] += scale * y[nc + cell];
+  o2[at] += scale * y[2 * nc + cell];
+}
+
+long long param_offset(int layer)
+{
+  long long off = 0;
+  for (int l = 0; l < layer; ++l) off += 27LL * kNetCin[l] * kNetCout[l] + kNetCout[l];
+  return off;
+}
+
+unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+constexpr int kConvBlocks = 2048;
+constexpr int kWgradBlocks = 512;
+
+template <typename T>
+int dev_alloc(T** p, long long count)
+{
+  if (hipMalloc(p, sizeof(T) * count) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    set_error("learned residual: unable to allocate device memory");
+    return kErrAlloc;
+  }
+  return kOk;
+}
+
+}  // namespace
+
+int resnet_init(ResNet* r, int n, unsigned seed)
+{
+  if (n < 32 || n % 32 != 0) {
+    set_error("learned residual: the grid must be a multiple of 32 cells per axis");
+    return kErrArg;
+  }
+  r->n = n;
+  const long long nc = (long long)n * n * n;
+  COLA_TRY(dev_alloc(&r->d_param, kNetParams));
+  COLA_TRY(dev_alloc(&r->d_m, kNetParams));
+  COLA_TRY(dev_alloc(&r->d_v, kNetParams));
+  COLA_TRY(dev_alloc(&r->d_grad, kNetParams));
+  for (int l = 0; l < kNetLayers; ++l) {
+    COLA_TRY(dev_alloc(&r->d_wf[l], 14LL * 64 * 8));
+    COLA_TRY(dev_alloc(&r->d_wb[l], 14LL * 64 * 8));
+  }
+  COLA_TRY(dev_alloc(&r->d_x, nc * 8));
+  for (int l = 0; l < 3; ++l) COLA_TRY(dev_alloc(&r->d_h[l], nc * 16));
+  COLA_TRY(dev_alloc(&r->d_y, 3 * nc));
+  COLA_TRY(dev_alloc(&r->d_loss, 1));
+  COLA_TRY(dev_alloc(&r->d_a, 1));
+  r->wpart_waves = kWgradBlocks * (kThreads / 64);
+  // He-normal initial weights (std sqrt(2 / fan_in)), zero biases and moments
+  std::vector<float> p(kNetParams, 0.0f);
+  std::mt19937 gen(seed);
+  for (int l = 0; l < kNetLayers; ++l) {
+    std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (27.0f * kNetCin[l])) * (l == kNetLayers - 1 ? 0.1f : 1.0f));
+    const long long off = param_offset(l);
+    for (long long i = 0; i < 27LL * kNetCin[l] * kNetCout[l]; ++i) p[off + i] = nd(gen);
+  }
+  COLA_HIP(hipMemcpy(r->d_param, p.data(), sizeof(float) * kNetParams, hipMemcpyHostToDevice));
+  COLA_HIP(hipMemset(r->d_m, 0, sizeof(float) * kNetParams));
+  COLA_HIP(hipMemset(r->d_v, 0, sizeof(float) * kNetParams));
+  return resnet_pack(r, nullptr);
+}
+
+void resnet_free(ResNet* r)
+{
+  auto fr = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  fr(r->d_param), fr(r->d_m), fr(r->d_v), fr(r->d_grad);
+  for (int l = 0; l < kNetLayers; ++l) fr(r->d_wf[l]), fr(r->d_wb[l]);
+  fr(r->d_x), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a);
+  for (int l = 0; l < 3; ++l) fr(r->d_h[l]);
+  for (int l = 0; l < 2; ++l) fr(r->d_g[l]);
+  for (auto p : r->s_feat) fr(p);
+  for (auto p : r->s_target) fr(p);
+  *r = ResNet{};
+}
+
+int resnet_pack(ResNet* r, hipStream_t s)
+{
+  for (int l = 0; l < kNetLayers; ++l) {
+    hipLaunchKernelGGL(k_pack, dim3(14), dim3(64), 0, s, r->d_param + param_offset(l), kNetCin[l], kNetCout[l],
+                       r->d_wf[l], l > 0 ? r->d_wb[l] : nullptr);
+  }
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int resnet_features(ResNet* r, const float* delta, long long row, const float* d_a, hipStream_t s)
+{
+  const long long nc = (long long)r->n * r->n * r->n;
+  hipLaunchKernelGGL(k_features, dim3(grid_cells(nc)), dim3(kThreads), 0, s, delta, row, r->n, d_a, r->in_scale,
+                     r->d_x);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int resnet_forward(ResNet* r, hipStream_t s)
+{
+  const int n = r->n;
+  hipLaunchKernelGGL((k_conv<8, kFwdRelu>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_x, r->d_wf[0],
+                     r->d_param + param_offset(0) + 27 * 8 * 16, 16, n, r->d_h[0], nullptr, nullptr);
+  hipLaunchKernelGGL((k_conv<16, kFwdRelu>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_h[0], r->d_wf[1],
+                     r->d_param + param_offset(1) + 27 * 16 * 16, 16, n, r->d_h[1], nullptr, nullptr);
+  hipLaunchKernelGGL((k_conv<16, kFwdRelu>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_h[1], r->d_wf[2],
+                     r->d_param + param_offset(2) + 27 * 16 * 16, 16, n, r->d_h[2], nullptr, nullptr);
+  hipLaunchKernelGGL((k_conv<16, kFwdLinear>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_h[2], r->d_wf[3],
+                     r->d_param + param_offset(3) + 27 * 16 * 3, 3, n, nullptr, r->d_y, nullptr);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStream_t s)
+{
+  const int n = r->n;
+  const long long nc = (long long)n * n * n;
+  if (!r->d_g[0]) {
+    COLA_TRY(dev_alloc(&r->d_g[0], nc * 16));
+    COLA_TRY(dev_alloc(&r->d_g[1], nc * 16));
+    COLA_TRY(dev_alloc(&r->d_wpart, (long long)r->wpart_waves * (27 * 256 + 16)));
+  }
+  hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, r->d_loss);
+  hipLaunchKernelGGL(k_loss, dim3(std::min<long long>(4096, grid_cells(nc))), dim3(kThreads), 0, s, r->d_y, tb, tf,
+                     n, 1.0f / r->out_scale, r->d_g[0], r->d_loss);
+  const unsigned short* X[kNetLayers] = {r->d_x, r->d_h[0], r->d_h[1], r->d_h[2]};
+  unsigned short* G = r->d_g[0];
+  unsigned short* Gp = r->d_g[1];
+  for (int l = kNetLayers - 1; l >= 0; --l) {
+    if (kNetCin[l] == 8)
+      hipLaunchKernelGGL(k_conv_wgrad<8>, dim3(kWgradBlocks), dim3(kThreads), 0, s, X[l], G, n, r->d_wpart);
+    else
+      hipLaunchKernelGGL(k_conv_wgrad<16>, dim3(kWgradBlocks), dim3(kThreads), 0, s, X[l], G, n, r->d_wpart);
+    const int np = 27 * kNetCin[l] * kNetCout[l] + kNetCout[l];
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((np + 255) / 256), dim3(256), 0, s, r->d_wpart, r->wpart_waves,
+                       kNetCin[l], kNetCout[l], r->d_grad + param_offset(l));
+    if (l == 0) break;
+    // gradient at the output of layer l - 1: data gradient of layer l masked by relu'(h_{l-1})
+    hipLaunchKernelGGL((k_conv<16, kBwdData>), dim3(kConvBlocks), dim3(kThreads), 0, s, G, r->d_wb[l], nullptr, 0,
+                       n, Gp, nullptr, r->d_h[l - 1]);
+    std::swap(G, Gp);
+  }
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int resnet_adam(ResNet* r, float lr, hipStream_t s)
+{
+  const float b1 = 0.9f, b2 = 0.999f;
+  r->adam_t += 1;
+  const float c1 = 1.0f - std::pow(b1, (float)r->adam_t), c2 = 1.0f - std::pow(b2, (float)r->adam_t);
+  hipLaunchKernelGGL(k_adam, dim3((kNetParams + 255) / 256), dim3(256), 0, s, r->d_param, r->d_m, r->d_v, r->d_grad,
+                     kNetParams, lr, b1, b2, c1, c2, 1e-8f);
+  COLA_HIP(hipGetLastError());
+  return resnet_pack(r, s);
+}
+
+int resnet_store_pair(ResNet* r, const float* coarse_delta, long long crow, const float* const cf[3],
+                      const float* const ff[3], long long frow, float a, hipStream_t s)
+{
+  const long long nc = (long long)r->n * r->n * r->n;
+  unsigned short *feat = nullptr, *target = nullptr;
+  COLA_TRY(dev_alloc(&feat, nc));
+  if (dev_alloc(&target, 3 * nc) != kOk) {
+    (void)hipFree(feat);
+    return kErrAlloc;
+  }
+  r->s_feat.push_back(feat);
+  r->s_target.push_back(target);
+  r->s_a.push_back(a);
+  hipLaunchKernelGGL(k_store_pair, dim3(grid_cells(nc)), dim3(kThreads), 0, s, coarse_delta, crow, cf[0], cf[1], cf[2],
+                     ff[0], ff[1], ff[2], frow, r->n, r->in_scale, feat, target);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+namespace {
+
+int run_sample(ResNet* r, int i, bool train, float lr, double* loss, hipStream_t s)
+{
+  const long long nc = (long long)r->n * r->n * r->n;
+  hipLaunchKernelGGL(k_features_stored, dim3(grid_cells(nc)), dim3(kThreads), 0, s, r->s_feat[i], r->n, r->s_a[i],
+                     r->d_x);
+  COLA_TRY(resnet_forward(r, s));
+  COLA_TRY(resnet_backward(r, r->s_target[i], nullptr, s));
+  if (train) COLA_TRY(resnet_adam(r, lr, s));
+  COLA_HIP(hipMemcpyAsync(loss, r->d_loss, sizeof(double), hipMemcpyDeviceToHost, s));
+  COLA_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+// out_scale = RMS of the stored targets (so a zero prediction has loss 1)
+int set_out_scale(ResNet* r, int count, hipStream_t s)
+{
+  const long long nc = (long long)r->n * r->n * r->n;
+  hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, r->d_loss);
+  for (int i = 0; i < count; ++i)
+    hipLaunchKernelGGL(k_sumsq, dim3(2048), dim3(kThreads), 0, s, r->s_target[i], 3 * nc, r->d_loss);
+  double sq = 0.0;
+  COLA_HIP(hipMemcpyAsync(&sq, r->d_loss, sizeof(double), hipMemcpyDeviceToHost, s));
+  COLA_HIP(hipStreamSynchronize(s));
+  const double rms = std::sqrt(sq / (3.0 * (double)nc * count));
+  r->out_scale = rms > 0.0 ? (float)rms : 1.0f;
+  return kOk;
+}
+
+}  // namespace
+
+int resnet_eval(ResNet* r, int first, int count, double* loss, hipStream_t s)
+{
+  if (first < 0 || count < 1 || first + count > (int)r->s_feat.size()) {
+    set_error("learned residual: sample range out of bounds");
+    return kErrArg;
+  }
+  double tot = 0.0;
+  for (int i = first; i < first + count; ++i) {
+    double l = 0.0;
+    COLA_TRY(run_sample(r, i, false, 0.0f, &l, s));
+    tot += l;
+  }
+  *loss = tot / count;
+  return kOk;
+}
+
+int resnet_train(ResNet* r, int ntrain, int epochs, float lr, unsigned seed, double* epoch_loss, hipStream_t s)
+{
+  if (ntrain < 1 || ntrain > (int)r->s_feat.size() || epochs < 1) {
+    set_error("learned residual: need 1 <= ntrain <= stored samples and epochs >= 1");
+    return kErrArg;
+  }
+  if (r->adam_t == 0) COLA_TRY(set_out_scale(r, ntrain, s));
+  std::vector<int> order(ntrain);
+  for (int i = 0; i < ntrain; ++i) order[i] = i;
+  std::mt19937 gen(seed);
+  for (int e = 0; e < epochs; ++e) {
+    std::shuffle(order.begin(), order.end(), gen);
+    double tot = 0.0;
+    for (int i : order) {
+      double l = 0.0;
+      COLA_TRY(run_sample(r, i, true, lr, &l, s));
+      tot += l;
+    }
+    if (epoch_loss) epoch_loss[e] = tot / ntrain;
+  }
+  return kOk;
+}
+
+int resnet_add_residual(const ResNet* r, float* const out[3], long long row, hipStream_t s)
+{
+  const long long nc = (long long)r->n * r->n * r->n;
+  hipLaunchKernelGGL(k_add_residual, dim3(grid_cells(nc)), dim3(kThreads), 0, s, r->d_y, r->n, r->out_scale, out[0],
+                     out[1], out[2], row);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace cola

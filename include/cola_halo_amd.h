@@ -254,6 +254,46 @@ int cola_get_snapshot(cola_context* ctx, cola_particle_min* out, size_t np_out);
    halo.  Returns 8110 when the file cannot be opened. */
 int cola_write_halos(const char* filename, const cola_halo* halos, long long n);
 
+/* ---- Learned force residual (no reference counterpart) --------------------
+   A 4-layer periodic 3-D conv net on the coarse PM grid (8 -> 16 -> 16 -> 16
+   -> 3 channels, 3^3 kernels, relu between layers), input asinh(delta) *
+   in_scale and the scale factor a, output the force residual between the
+   coarse grid and a grid twice as fine, in units of out_scale.  bfloat16
+   weights and activations on MFMA with fp32 accumulation; fp32 master
+   weights and Adam.  Parameters (cola_resnet_nparam floats) are W1 b1 W2 b2
+   W3 b3 W4 b4 with W[27][cin][cout], tap t = 9 (dx+1) + 3 (dy+1) + (dz+1). */
+typedef struct cola_resnet cola_resnet;
+int cola_resnet_nparam(void);
+int cola_resnet_create(int n, unsigned int seed, cola_resnet** out); /* n: multiple of 32 */
+int cola_resnet_destroy(cola_resnet* net);
+int cola_resnet_get_params(cola_resnet* net, float* out);
+int cola_resnet_set_params(cola_resnet* net, const float* in);
+int cola_resnet_get_scales(cola_resnet* net, float* in_scale, float* out_scale);
+int cola_resnet_set_scales(cola_resnet* net, float in_scale, float out_scale);
+/* Host arrays (tests): x [n^3][8] channel-last input (rounded to bfloat16),
+   y [n^3][3] output; the gradient entry point also returns the mean squared
+   error against target [n^3][3] (in output units) and its gradient with
+   respect to every parameter. */
+int cola_resnet_forward_host(cola_resnet* net, const float* x, float* y);
+int cola_resnet_gradient_host(cola_resnet* net, const float* x, const float* target, double* loss, float* grad);
+/* Device copy of the particle state of src into dst (same particle count). */
+int cola_copy_particles(cola_context* dst, cola_context* src);
+/* Training pair from two contexts holding the same particles: coarse (the
+   net's grid) and fine (twice the cells per axis).  Stores asinh(delta) of
+   the coarse density and the residual = fine force at the coincident cells
+   minus the coarse force, with the scale factor a_x of the coarse context. */
+int cola_resnet_add_pair(cola_resnet* net, cola_context* coarse, cola_context* fine);
+int cola_resnet_num_samples(const cola_resnet* net);
+/* Adam, one step per stored sample, epochs over samples [0, ntrain) in a
+   shuffled order; out_scale is set to the targets' RMS on the first call.
+   epoch_loss (may be NULL) receives the mean loss per epoch (1 = predicting
+   zero). */
+int cola_resnet_train(cola_resnet* net, int ntrain, int epochs, float lr, unsigned int seed, double* epoch_loss);
+int cola_resnet_eval(cola_resnet* net, int first, int count, double* loss);
+/* Attach (net) or detach (NULL) the net: every force of the context then adds
+   out_scale * net(delta, a) to the three force grids before the readout. */
+int cola_set_residual_net(cola_context* ctx, cola_resnet* net);
+
 /* ---- Driver-side files (host only) -------------------------------------- */
 
 /* Parameters of the driver, parameters.h:4-49 with fixed-size storage.  A

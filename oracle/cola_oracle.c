@@ -1458,6 +1458,193 @@ int orc_fof(float* rec, long stride, int np, float box, float ll, int nfof_min, 
 }
 
 /* ======================================================================
+ * Learned force residual: the conv net in double precision (no reference
+ * counterpart; the checker of the device's BF16 MFMA kernels).
+ * ====================================================================== */
+double orc_bf16_round(double x)
+{
+  float f = (float)x;
+  unsigned u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return x; /* inf / nan */
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+void orc_conv3d(int n, int cin, int cout, const double* in, const double* w, const double* b, double* out)
+{
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < (long)n * n * n; c++) {
+    const int x = (int)(c / ((long)n * n)), y = (int)((c / n) % n), z = (int)(c % n);
+    double acc[16];
+    for (int o = 0; o < cout; o++) acc[o] = b ? b[o] : 0.0;
+    for (int t = 0; t < 27; t++) {
+      const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+      const long s = (((long)((x + dx + n) % n) * n + (y + dy + n) % n) * n + (z + dz + n) % n);
+      for (int i = 0; i < cin; i++) {
+        const double v = in[s * cin + i];
+        if (v == 0.0) continue;
+        const double* wr = w + ((long)t * cin + i) * cout;
+        for (int o = 0; o < cout; o++) acc[o] += v * wr[o];
+      }
+    }
+    for (int o = 0; o < cout; o++) out[c * cout + o] = acc[o];
+  }
+}
+
+/* gradient of a conv with respect to its input: gin[s][i] = sum_t sum_o g[s - t][o] w[t][i][o] */
+static void conv3d_back_data(int n, int cin, int cout, const double* g, const double* w, double* gin)
+{
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < (long)n * n * n; c++) {
+    const int x = (int)(c / ((long)n * n)), y = (int)((c / n) % n), z = (int)(c % n);
+    double acc[16] = {0};
+    for (int t = 0; t < 27; t++) {
+      const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+      const long d = (((long)((x - dx + n) % n) * n + (y - dy + n) % n) * n + (z - dz + n) % n);
+      for (int i = 0; i < cin; i++) {
+        const double* wr = w + ((long)t * cin + i) * cout;
+        double s = 0.0;
+        for (int o = 0; o < cout; o++) s += g[d * cout + o] * wr[o];
+        acc[i] += s;
+      }
+    }
+    for (int i = 0; i < cin; i++) gin[c * cin + i] = acc[i];
+  }
+}
+
+/* gradient with respect to the weights and bias (accumulated into gw, gb) */
+static void conv3d_back_weights(int n, int cin, int cout, const double* in, const double* g, double* gw, double* gb)
+{
+  const long nc = (long)n * n * n;
+  for (int t = 0; t < 27; t++) {
+    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    for (int i = 0; i < cin; i++)
+      for (int o = 0; o < cout; o++) {
+        double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+        for (long c = 0; c < nc; c++) {
+          const int x = (int)(c / ((long)n * n)), y = (int)((c / n) % n), z = (int)(c % n);
+          const long src = (((long)((x + dx + n) % n) * n + (y + dy + n) % n) * n + (z + dz + n) % n);
+          s += in[src * cin + i] * g[c * cout + o];
+        }
+        gw[((long)t * cin + i) * cout + o] += s;
+      }
+  }
+  for (int o = 0; o < cout; o++) {
+    double s = 0.0;
+    for (long c = 0; c < nc; c++) s += g[c * cout + o];
+    gb[o] += s;
+  }
+}
+
+typedef struct {
+  const double *w[4], *b[4];
+} NetView;
+
+static const int kNetCin[4] = {8, 16, 16, 16};
+static const int kNetCout[4] = {16, 16, 16, 3};
+
+static NetView net_view(const double* p)
+{
+  NetView v;
+  for (int l = 0; l < 4; l++) {
+    v.w[l] = p;
+    p += 27 * kNetCin[l] * kNetCout[l];
+    v.b[l] = p;
+    p += kNetCout[l];
+  }
+  return v;
+}
+
+static void round_all(double* a, long n, int on)
+{
+  if (!on) return;
+  for (long i = 0; i < n; i++) a[i] = orc_bf16_round(a[i]);
+}
+
+/* forward keeping the activations: h[0] = input (rounded), h[1..3] after relu, y */
+static double net_forward_keep(int n, const double* params, const double* x, const double* target, double* y,
+                               double** h, int rb)
+{
+  const long nc = (long)n * n * n;
+  double* pr = malloc(sizeof(double) * ORC_NET_NPARAM);
+  memcpy(pr, params, sizeof(double) * ORC_NET_NPARAM);
+  round_all(pr, ORC_NET_NPARAM, rb);
+  const NetView v = net_view(pr);
+  memcpy(h[0], x, sizeof(double) * nc * 8);
+  round_all(h[0], nc * 8, rb);
+  for (int l = 0; l < 3; l++) {
+    orc_conv3d(n, kNetCin[l], kNetCout[l], h[l], v.w[l], v.b[l], h[l + 1]);
+    for (long i = 0; i < nc * 16; i++) h[l + 1][i] = h[l + 1][i] > 0.0 ? h[l + 1][i] : 0.0;
+    round_all(h[l + 1], nc * 16, rb);
+  }
+  orc_conv3d(n, 16, 3, h[3], v.w[3], v.b[3], y);
+  free(pr);
+  if (!target) return 0.0;
+  double loss = 0.0;
+  for (long i = 0; i < nc * 3; i++) loss += (y[i] - target[i]) * (y[i] - target[i]);
+  return loss / (double)(nc * 3);
+}
+
+double orc_net_forward(int n, const double* params, const double* x, const double* target, double* y, int rb)
+{
+  const long nc = (long)n * n * n;
+  double* h[4];
+  h[0] = malloc(sizeof(double) * nc * 8);
+  for (int l = 1; l < 4; l++) h[l] = malloc(sizeof(double) * nc * 16);
+  const double loss = net_forward_keep(n, params, x, target, y, h, rb);
+  for (int l = 0; l < 4; l++) free(h[l]);
+  return loss;
+}
+
+double orc_net_gradient(int n, const double* params, const double* x, const double* target, double* grad, int rb)
+{
+  const long nc = (long)n * n * n;
+  double* h[4];
+  h[0] = malloc(sizeof(double) * nc * 8);
+  for (int l = 1; l < 4; l++) h[l] = malloc(sizeof(double) * nc * 16);
+  double* y = malloc(sizeof(double) * nc * 3);
+  const double loss = net_forward_keep(n, params, x, target, y, h, rb);
+  double* pr = malloc(sizeof(double) * ORC_NET_NPARAM);
+  memcpy(pr, params, sizeof(double) * ORC_NET_NPARAM);
+  round_all(pr, ORC_NET_NPARAM, rb);
+  const NetView v = net_view(pr);
+  for (long i = 0; i < ORC_NET_NPARAM; i++) grad[i] = 0.0;
+  double* gw[4];
+  double* gb[4];
+  {
+    double* p = grad;
+    for (int l = 0; l < 4; l++) {
+      gw[l] = p;
+      p += 27 * kNetCin[l] * kNetCout[l];
+      gb[l] = p;
+      p += kNetCout[l];
+    }
+  }
+  double* g = malloc(sizeof(double) * nc * 16);
+  double* gprev = malloc(sizeof(double) * nc * 16);
+  for (long i = 0; i < nc * 3; i++) g[i] = 2.0 * (y[i] - target[i]) / (double)(nc * 3);
+  for (int l = 3; l >= 0; l--) {
+    conv3d_back_weights(n, kNetCin[l], kNetCout[l], h[l], g, gw[l], gb[l]);
+    if (l == 0) break;
+    conv3d_back_data(n, kNetCin[l], kNetCout[l], g, v.w[l], gprev);
+    for (long i = 0; i < nc * 16; i++) gprev[i] = h[l][i] > 0.0 ? gprev[i] : 0.0;
+    double* t = g;
+    g = gprev;
+    gprev = t;
+  }
+  free(g);
+  free(gprev);
+  free(y);
+  free(pr);
+  for (int l = 0; l < 4; l++) free(h[l]);
+  return loss;
+}
+
+/* ======================================================================
  * main.c:67-75, 117-207: linear-in-a leapfrog schedule for one realization.
  * snapshots must hold nout * nc^3 entries; the power table must be set.
  * ====================================================================== */

@@ -129,6 +129,28 @@ typedef struct {
    Returns the number of halos (halos must hold np entries). */
 int orc_fof(float* rec, long stride, int np, float boxsize, float ll, int nfof_min, int* label, OrcHalo* halos);
 
+/* ---- learned force residual: 3-D periodic conv net, double precision ----
+   Layers (taps t = 9 (dx+1) + 3 (dy+1) + (dz+1), input at (x+dx, y+dy, z+dz)
+   with periodic wrap; activations channel-last [cell][channel]):
+     h1 = relu(conv(x, W1[27][8][16]) + b1)      x: 8 input channels
+     h2 = relu(conv(h1, W2[27][16][16]) + b2)
+     h3 = relu(conv(h2, W3[27][16][16]) + b3)
+     y  = conv(h3, W4[27][16][3]) + b4
+   Parameters are packed in that order (W1, b1, W2, b2, W3, b3, W4, b4):
+   ORC_NET_NPARAM doubles.  Loss = mean over cells and 3 components of
+   (y - target)^2.  round_bf16 != 0 rounds the input, the weights and every
+   stored activation to bfloat16 (round to nearest even) the way the device
+   kernels store them; the arithmetic stays double. */
+#define ORC_NET_NPARAM (27 * 8 * 16 + 16 + 2 * (27 * 16 * 16 + 16) + 27 * 16 * 3 + 3)
+double orc_bf16_round(double x);
+void orc_conv3d(int n, int cin, int cout, const double* in, const double* w, const double* b, double* out);
+/* forward; y: [n^3][3]; returns the loss if target != NULL (else 0) */
+double orc_net_forward(int n, const double* params, const double* x, const double* target, double* y,
+                       int round_bf16);
+/* loss and its gradient with respect to every parameter (grad: ORC_NET_NPARAM) */
+double orc_net_gradient(int n, const double* params, const double* x, const double* target, double* grad,
+                        int round_bf16);
+
 /* ---- main.c schedule ---- */
 int orc_run(int nc, int ngrid, double boxsize, double omega_m, double a_final,
             int ntimestep, int seed, const double* zout, int nout,

@@ -63,6 +63,10 @@ def load():
         "orc_set_snapshot": (None, [d, vp, vp, i, f, f, f, vp]),
         "orc_set_noncola_initial": (None, [vp, i, f, f, vp]),
         "orc_run": (i, [i, i, d, d, d, i, i, vp, i, vp, vp, C.POINTER(i)]),
+        "orc_bf16_round": (d, [d]),
+        "orc_conv3d": (None, [i, i, i, vp, vp, vp, vp]),
+        "orc_net_forward": (d, [i, vp, vp, vp, vp, i]),
+        "orc_net_gradient": (d, [i, vp, vp, vp, vp, i]),
         "orc_fof": (i, [vp, C.c_long, i, f, f, i, vp, vp]),
         "orc_coarse_grid": (None, [vp, C.c_long, i, i, f, vp]),
         "orc_density_contrast": (None, [vp, i, C.c_longlong, vp]),
@@ -175,3 +179,45 @@ def fof(records, box, ll, nfof_min=32):
     nh = lib().orc_fof(records.ctypes.data, PARTICLE_MIN_DTYPE.itemsize // 4, n, box, ll, nfof_min,
                        label.ctypes.data, halos.ctypes.data)
     return label, halos[:nh].copy()
+
+
+NET_CIN = (8, 16, 16, 16)
+NET_COUT = (16, 16, 16, 3)
+NET_NPARAM = sum(27 * a * b + b for a, b in zip(NET_CIN, NET_COUT))
+
+
+def _dptr(a):
+    return np.ascontiguousarray(a, np.float64).ctypes.data
+
+
+def conv3d(x, w, b):
+    """x [n,n,n,cin], w [27,cin,cout], b [cout] -> [n,n,n,cout] (periodic)."""
+    x = np.ascontiguousarray(x, np.float64)
+    n, cin = x.shape[0], x.shape[3]
+    cout = w.shape[2]
+    out = np.zeros((n, n, n, cout))
+    w = np.ascontiguousarray(w, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    lib().orc_conv3d(n, cin, cout, x.ctypes.data, w.ctypes.data, b.ctypes.data, out.ctypes.data)
+    return out
+
+
+def net_forward(params, x, target=None, round_bf16=False):
+    params = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    n = x.shape[0]
+    y = np.zeros((n, n, n, 3))
+    t = np.ascontiguousarray(target, np.float64) if target is not None else None
+    loss = lib().orc_net_forward(n, params.ctypes.data, x.ctypes.data, t.ctypes.data if t is not None else None,
+                                 y.ctypes.data, 1 if round_bf16 else 0)
+    return y, loss
+
+
+def net_gradient(params, x, target, round_bf16=False):
+    params = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(target, np.float64)
+    g = np.zeros(NET_NPARAM)
+    loss = lib().orc_net_gradient(x.shape[0], params.ctypes.data, x.ctypes.data, t.ctypes.data, g.ctypes.data,
+                                  1 if round_bf16 else 0)
+    return loss, g

@@ -1,0 +1,114 @@
+"""The learned force residual's BF16 MFMA kernels against the double-precision
+oracle net (tests/test_net_cpu.py pins the oracle to finite differences).
+
+Tolerances: the oracle rounds the input, the weights and each stored
+activation to bfloat16 exactly as the device does, so the forward output
+differs only by fp32 vs double accumulation and the occasional bfloat16
+rounding flip of an activation: within 1e-2 of the output's maximum.  The
+device also stores the layer gradients in bfloat16: each parameter block's
+gradient is within 5e-2 (relative 2-norm) of the oracle's."""
+import numpy as np
+import pytest
+
+import cola_halo_amd as ch
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+OM = 0.273
+
+
+def _blocks():
+    off = 0
+    out = []
+    for ci, co in zip(orc.NET_CIN, orc.NET_COUT):
+        out.append(("W", off, off + 27 * ci * co))
+        off += 27 * ci * co
+        out.append(("b", off, off + co))
+        off += co
+    return out
+
+
+def _setup(n, seed):
+    rng = np.random.default_rng(seed)
+    params = rng.normal(0, 0.15, orc.NET_NPARAM).astype(np.float32)
+    x = np.zeros((n, n, n, 8), np.float32)
+    x[..., 0] = rng.standard_normal((n, n, n))
+    x[..., 1] = 0.5
+    x[..., 2:] = 0.3 * rng.standard_normal((n, n, n, 6))
+    return params, x
+
+
+def test_nparam_matches_oracle():
+    assert ch.ResidualNet.nparam() == orc.NET_NPARAM
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_forward_matches_oracle(n):
+    print(f"net forward n={n}", flush=True)
+    params, x = _setup(n, n)
+    with ch.ResidualNet(n) as net:
+        net.set_params(params)
+        y = net.forward(x)
+    ref, _ = orc.net_forward(params.astype(np.float64), x.astype(np.float64), round_bf16=True)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err <= 1e-2, err
+
+
+def test_gradient_matches_oracle():
+    print("net gradient n=32", flush=True)
+    n = 32
+    params, x = _setup(n, 7)
+    rng = np.random.default_rng(8)
+    target = rng.standard_normal((n, n, n, 3)).astype(np.float32)
+    with ch.ResidualNet(n) as net:
+        net.set_params(params)
+        net.set_scales(1.0, 1.0)
+        loss, g = net.gradient(x, target)
+    ref_loss, ref = orc.net_gradient(params.astype(np.float64), x.astype(np.float64), target.astype(np.float64),
+                                     round_bf16=True)
+    assert loss == pytest.approx(ref_loss, rel=1e-2)
+    for kind, a, b in _blocks():
+        rel = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
+        assert rel <= 5e-2, (kind, a, rel)
+
+
+def test_pairs_training_and_corrected_force(eh_table):
+    """Force pairs from a 32^3 run (coarse 64^3, fine 128^3): training lowers
+    the held-out loss below 1 (the loss of predicting no correction), and an
+    attached net changes the coarse force by exactly its residual."""
+    print("net pairs + training", flush=True)
+    k, p = eh_table
+    nc, box = 32, 100.0
+    with ch.Context(nc, 64, box) as coarse, ch.Context(nc, 128, box) as fine, ch.ResidualNet(64, seed=3) as net:
+        for c in (coarse, fine):
+            c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+        for seed in (1, 2, 3):
+            def on_snapshot(a, s):
+                pass
+            fine.lpt_set_displacement(0.1, OM, seed)
+            fine.set_times(fine.times()[0], 0.1)
+            for istep in range(1, 10):
+                coarse.copy_particles_from(fine)
+                net.add_pair(coarse, fine)
+                fine.step(float(np.float32(OM)), (istep + 0.5) * 0.1, (istep + 1.0) * 0.1)
+        assert net.num_samples() == 27
+        before = net.eval(18, 9)
+        losses = net.train(18, 6, lr=2e-3)
+        after = net.eval(18, 9)
+        assert losses[-1] < losses[0]
+        assert after < before and after < 1.0, (before, after, losses)
+        # attached net: force grids change by out_scale * y
+        coarse.copy_particles_from(fine)
+        coarse.move_particles()
+        coarse.pm_calculate_forces()
+        f_plain = coarse.get_forces()
+        coarse.set_residual_net(net)
+        coarse.pm_calculate_forces()
+        f_net = coarse.get_forces()
+        coarse.pm_calculate_forces()
+        assert np.array_equal(f_net, coarse.get_forces())  # deterministic
+        assert not np.array_equal(f_plain, f_net)
+        coarse.set_residual_net(None)
+        coarse.pm_calculate_forces()
+        assert np.array_equal(f_plain, coarse.get_forces())
