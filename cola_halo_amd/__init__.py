@@ -94,6 +94,7 @@ _SIGNATURES = {
     "cola_resnet_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
     "cola_resnet_get_scales": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "cola_resnet_set_scales": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
+    "cola_resnet_fit_scale": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_forward_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                             C.c_void_p]),
@@ -331,6 +332,9 @@ class ResidualNet:
 
     def set_scales(self, in_scale, out_scale):
         _check(lib().cola_resnet_set_scales(self._h, in_scale, out_scale))
+
+    def fit_scale(self, count):
+        _check(lib().cola_resnet_fit_scale(self._h, count))
 
     def forward(self, x):
         x = np.ascontiguousarray(x, np.float32)

@@ -1494,7 +1494,14 @@ int cola_resnet_set_scales(cola_resnet* h, float in_scale, float out_scale)
   if (!h || !(out_scale > 0.0f) || !(in_scale > 0.0f)) return kErrArg;
   h->r.in_scale = in_scale;
   h->r.out_scale = out_scale;
+  h->r.scale_set = true;
   return kOk;
+}
+
+int cola_resnet_fit_scale(cola_resnet* h, int count)
+{
+  if (!h) return kErrArg;
+  return resnet_fit_scale(&h->r, count, h->stream);
 }
 
 namespace {

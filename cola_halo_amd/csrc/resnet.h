@@ -39,6 +39,7 @@ struct ResNet {
   double* d_loss = nullptr;
   float* d_a = nullptr;      // scale factor of standalone evaluations
   float out_scale = 1.0f;    // residual = out_scale * y
+  bool scale_set = false;    // out_scale fitted to training targets (or set by the caller)
   float in_scale = 1.0f;     // feature = asinh(delta) * in_scale
 
   // training set: per sample the feature grid (bf16 [n^3]), the target
@@ -61,6 +62,8 @@ int resnet_store_pair(ResNet* r, const float* coarse_delta, long long crow, cons
 // mean loss over stored samples [first, first + count); train: epochs over
 // samples [0, ntrain) in shuffled order, one Adam step per sample
 int resnet_eval(ResNet* r, int first, int count, double* loss, hipStream_t s);
+// out_scale = RMS of the targets of samples [0, count)
+int resnet_fit_scale(ResNet* r, int count, hipStream_t s);
 int resnet_train(ResNet* r, int ntrain, int epochs, float lr, unsigned seed, double* epoch_loss, hipStream_t s);
 // forward from d_x; writes d_y (and keeps h1..h3)
 int resnet_forward(ResNet* r, hipStream_t s);

@@ -522,9 +522,15 @@ int run_sample(ResNet* r, int i, bool train, float lr, double* loss, hipStream_t
   return kOk;
 }
 
+}  // namespace
+
 // out_scale = RMS of the stored targets (so a zero prediction has loss 1)
-int set_out_scale(ResNet* r, int count, hipStream_t s)
+int resnet_fit_scale(ResNet* r, int count, hipStream_t s)
 {
+  if (count < 1 || count > (int)r->s_target.size()) {
+    set_error("learned residual: sample count out of range");
+    return kErrArg;
+  }
   const long long nc = (long long)r->n * r->n * r->n;
   hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, r->d_loss);
   for (int i = 0; i < count; ++i)
@@ -534,10 +540,9 @@ int set_out_scale(ResNet* r, int count, hipStream_t s)
   COLA_HIP(hipStreamSynchronize(s));
   const double rms = std::sqrt(sq / (3.0 * (double)nc * count));
   r->out_scale = rms > 0.0 ? (float)rms : 1.0f;
+  r->scale_set = true;
   return kOk;
 }
-
-}  // namespace
 
 int resnet_eval(ResNet* r, int first, int count, double* loss, hipStream_t s)
 {
@@ -561,7 +566,7 @@ int resnet_train(ResNet* r, int ntrain, int epochs, float lr, unsigned seed, dou
     set_error("learned residual: need 1 <= ntrain <= stored samples and epochs >= 1");
     return kErrArg;
   }
-  if (r->adam_t == 0) COLA_TRY(set_out_scale(r, ntrain, s));
+  if (!r->scale_set) COLA_TRY(resnet_fit_scale(r, ntrain, s));
   std::vector<int> order(ntrain);
   for (int i = 0; i < ntrain; ++i) order[i] = i;
   std::mt19937 gen(seed);

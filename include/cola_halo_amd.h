@@ -270,6 +270,9 @@ int cola_resnet_get_params(cola_resnet* net, float* out);
 int cola_resnet_set_params(cola_resnet* net, const float* in);
 int cola_resnet_get_scales(cola_resnet* net, float* in_scale, float* out_scale);
 int cola_resnet_set_scales(cola_resnet* net, float in_scale, float out_scale);
+/* out_scale = RMS of the residual targets of stored samples [0, count), so
+   that predicting no correction has loss 1 (training fits it if unset). */
+int cola_resnet_fit_scale(cola_resnet* net, int count);
 /* Host arrays (tests): x [n^3][8] channel-last input (rounded to bfloat16),
    y [n^3][3] output; the gradient entry point also returns the mean squared
    error against target [n^3][3] (in output units) and its gradient with

@@ -75,8 +75,9 @@ def test_gradient_matches_oracle():
 
 def test_pairs_training_and_corrected_force(eh_table):
     """Force pairs from a 32^3 run (coarse 64^3, fine 128^3): training lowers
-    the held-out loss below 1 (the loss of predicting no correction), and an
-    attached net changes the coarse force by exactly its residual."""
+    the held-out loss below 1 (the loss of predicting no correction); an
+    attached net changes the coarse force, deterministically, and detaching it
+    restores the plain force bit for bit."""
     print("net pairs + training", flush=True)
     k, p = eh_table
     nc, box = 32, 100.0
@@ -84,8 +85,6 @@ def test_pairs_training_and_corrected_force(eh_table):
         for c in (coarse, fine):
             c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
         for seed in (1, 2, 3):
-            def on_snapshot(a, s):
-                pass
             fine.lpt_set_displacement(0.1, OM, seed)
             fine.set_times(fine.times()[0], 0.1)
             for istep in range(1, 10):
@@ -93,7 +92,9 @@ def test_pairs_training_and_corrected_force(eh_table):
                 net.add_pair(coarse, fine)
                 fine.step(float(np.float32(OM)), (istep + 0.5) * 0.1, (istep + 1.0) * 0.1)
         assert net.num_samples() == 27
+        net.fit_scale(18)
         before = net.eval(18, 9)
+        assert before == pytest.approx(1.0, rel=0.5)  # targets normalised by the training RMS
         losses = net.train(18, 6, lr=2e-3)
         after = net.eval(18, 9)
         assert losses[-1] < losses[0]
