@@ -241,6 +241,9 @@ int alloc_grids(cola_context* c)
   const size_t bytes = sizeof(float) * c->grid_floats;
   COLA_HIP(hipMalloc(&c->density, bytes));
   for (int a = 0; a < 3; ++a) COLA_HIP(hipMalloc(&c->fgrid[a], bytes));
+  // the row padding is never written by the deposit; keep it finite
+  COLA_HIP(hipMemset(c->density, 0, bytes));
+  for (int a = 0; a < 3; ++a) COLA_HIP(hipMemset(c->fgrid[a], 0, bytes));
   c->have_grids = true;
   return kOk;
 }
@@ -373,7 +376,6 @@ int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
   c->boxf = (float)boxsize;
   c->np = (long long)nc * nc * nc;
   c->wpar = (float)std::pow((double)ngrid / (double)nc, 3.0);
-  c->grid_floats = (long long)ngrid * ngrid * 2 * (ngrid / 2 + 1);
   int st = kOk;
   auto fail = [&](int s) {
     cola_context_destroy(c);
@@ -383,7 +385,7 @@ int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
     set_error("hipStreamCreate failed");
     return fail(kErrHip);
   }
-  st = c->pm_fft.init(ngrid);
+  st = c->pm_fft.init(ngrid, -1, -1, true);
   if (st) {
     set_error("unsupported PM grid size " + std::to_string(ngrid) + " (needs factors 2, 3, 5)");
     return fail(st);
@@ -397,7 +399,8 @@ int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
   c->pm_fft.hook_user = c;
   c->lpt_fft.hook = lpt_fft_hook;
   c->lpt_fft.hook_user = c;
-  c->cic = make_cic_geom(ngrid, c->boxf, c->wpar);
+  c->cic = make_cic_geom(ngrid, c->boxf, c->wpar, 2LL * c->pm_fft.nzp);
+  c->grid_floats = (long long)ngrid * ngrid * c->cic.row;
   // default deposit tile 8 x 8 x 32 cells, halved until it divides ngrid
   auto pick = [&](int want) {
     for (int t = want; t > 1; t /= 2)
@@ -949,25 +952,37 @@ int cola_get_forces(cola_context* c, float* out, size_t np_out)
   return kOk;
 }
 
+// Device grid (rows of cic.row floats) -> host FFTW layout (rows of 2(ng/2+1)).
+// The two padding floats of each host row are set to zero (on the device they
+// hold what the in-place transforms left there).
+static int grid_to_host(const cola_context* c, const float* d, float* out)
+{
+  const long long hrow = 2 * (c->ng / 2 + 1);
+  COLA_HIP(hipMemcpy2D(out, sizeof(float) * hrow, d, sizeof(float) * c->cic.row, sizeof(float) * hrow,
+                       (size_t)c->ng * c->ng, hipMemcpyDeviceToHost));
+  for (long long r = 0; r < (long long)c->ng * c->ng; ++r) out[r * hrow + c->ng] = out[r * hrow + c->ng + 1] = 0.0f;
+  return kOk;
+}
+
+static size_t host_grid_floats(const cola_context* c) { return (size_t)c->ng * c->ng * 2 * (c->ng / 2 + 1); }
+
 int cola_get_density(cola_context* c, float* out, size_t n_out)
 {
-  if (!valid(c) || !out || n_out < (size_t)c->grid_floats) return kErrArg;
+  if (!valid(c) || !out || n_out < host_grid_floats(c)) return kErrArg;
   if (!c->have_grids || !c->density_is_real) {
     set_error("the density grid is transformed in place by the force; call cola_pm_assign first");
     return kErrState;
   }
   COLA_HIP(hipStreamSynchronize(c->stream));
-  COLA_HIP(hipMemcpy(out, c->density, sizeof(float) * c->grid_floats, hipMemcpyDeviceToHost));
-  return kOk;
+  return grid_to_host(c, c->density, out);
 }
 
 int cola_get_force_grid(cola_context* c, int axis, float* out, size_t n_out)
 {
-  if (!valid(c) || !out || axis < 0 || axis > 2 || n_out < (size_t)c->grid_floats) return kErrArg;
+  if (!valid(c) || !out || axis < 0 || axis > 2 || n_out < host_grid_floats(c)) return kErrArg;
   if (!c->have_grids) return kErrState;
   COLA_HIP(hipStreamSynchronize(c->stream));
-  COLA_HIP(hipMemcpy(out, c->fgrid[axis], sizeof(float) * c->grid_floats, hipMemcpyDeviceToHost));
-  return kOk;
+  return grid_to_host(c, c->fgrid[axis], out);
 }
 
 double cola_last_mass_sum(const cola_context* c) { return c ? c->last_mass : 0.0; }
@@ -1070,7 +1085,7 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
   COLA_TRY(set_deposit_tiles(c, tx, ty, tz));
   if (fft_log2_tile != c->fft_log2_tile || fft_log2_tile != c->fft_log2_rows) {
     FftPlan a, b;
-    COLA_TRY(a.init(c->ng, fft_log2_tile, fft_log2_tile));
+    COLA_TRY(a.init(c->ng, fft_log2_tile, fft_log2_tile, true));
     if (b.init(c->nc, fft_log2_tile, fft_log2_tile) != kOk) {
       a.destroy();
       return kErrUnsupported;
@@ -1100,7 +1115,7 @@ int cola_set_fft_tiles(cola_context* c, int log2_cols, int log2_rows, int xcd_gr
   drop_step_graph(c);
   if (log2_cols != c->fft_log2_tile || log2_rows != c->fft_log2_rows) {
     FftPlan a;
-    const int st = a.init(c->ng, log2_cols, log2_rows);
+    const int st = a.init(c->ng, log2_cols, log2_rows, true);
     if (st != kOk) {
       set_error("FFT tile widths not supported for this grid");
       return st;
@@ -1706,8 +1721,8 @@ int cola_fft_time(int n, int reps, double* ms_out)
 {
   if (n <= 0 || n % 2 || reps < 1 || !ms_out) return kErrArg;
   FftPlan plan;
-  COLA_TRY(plan.init(n));
-  const long long floats = (long long)n * n * 2 * (n / 2 + 1);
+  COLA_TRY(plan.init(n, -1, -1, true));  // the PM layout: rows padded to 128 bytes
+  const long long floats = (long long)n * n * 2 * plan.nzp;
   float* buf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // work, pristine copy, 3 force grids
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int st = kOk;

@@ -64,6 +64,10 @@ struct FftPlan {
   float2* d_tw = nullptr;  // exp(-2 pi i k / n), k < n
   FftPassGeom grows, gy, gx;
   int xgroup = 8;  // adjacent column tiles per XCD dispatch group (measured best)
+  // Complex row stride of the half spectrum (>= n/2+1): rows padded to a
+  // multiple of 16 complex values start on 128-byte lines, so column tiles
+  // of 8 values never straddle a line; real rows are 2 nzp floats.
+  int nzp = 0;
 
   // Optional per-pass profiling hook: pass 0 z r2c, 1 y fwd, 2 x fwd,
   // 3 x inv, 4 y inv, 5 z c2r, 6 fused x; begin = 1 before the launch, 0 after.
@@ -73,10 +77,12 @@ struct FftPlan {
   // log2T_cols >= 0 fixes the tile width (log2 transforms per workgroup) of
   // the y and x column passes, log2T_rows >= 0 that of the z row passes
   // (row pairs); -1 picks the measured defaults.
-  int init(int n, int log2T_cols = -1, int log2T_rows = -1);
+  // pad_rows: complex rows of round_up(n/2+1, 16) values instead of n/2+1.
+  int init(int n, int log2T_cols = -1, int log2T_rows = -1, bool pad_rows = false);
   void destroy();
 
-  // In place on the padded layout: real -> half-spectrum [x][y][kz].
+  // In place on the padded layout (real rows of 2 nzp floats): real ->
+  // half-spectrum [x][y][kz] with complex rows of nzp values.
   int r2c(float* data, hipStream_t s) const;
   // c2c over x (optionally out of place from in_k with a k-space operator),
   // then y, then c2r over z, in place on out.

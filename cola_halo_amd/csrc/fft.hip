@@ -176,16 +176,59 @@ struct TileArgs {
   long long bs;          // kCols: batch stride (blockIdx.y)
   long long rs_in;       // kR2C: real row stride (floats); kC2R: complex row stride
   long long rs_out;      // kR2C: complex row stride; kC2R: real row stride (floats)
-  int nz;                // x pass: columns are (y, kz) pairs, nz = n/2+1
+  int nz;                // x pass: columns are (y, kz) pairs with kz < nz (the row stride)
   unsigned magNz;
+  int zero_pad = 0;      // kC2R: write zeros to the two padding floats of each real row
+  int kz_valid = 0;      // x pass: tiles lying wholly in kz >= kz_valid (row padding) return at once
   KOpParams kp;
 };
+
+// An x-pass tile of 2^log2T columns starting at c0 that holds only padding
+// columns (kz_valid <= kz < nz within one row).
+__device__ __forceinline__ bool padding_tile(const TileArgs& A, int c0, int log2T)
+{
+  if (A.kz_valid <= 0) return false;
+  const int kz = (int)((unsigned)c0 % (unsigned)A.nz);
+  return kz >= A.kz_valid && kz + (1 << log2T) <= A.nz;
+}
 
 // One workgroup: T transforms of length n.  MODE kCols: column c of the tile
 // is global column c0 + c, positions r at stride sr; LDS [r][c].  kR2C/kC2R:
 // transform c packs real rows 2(c0+c), 2(c0+c)+1; LDS [c][r] padded by one.
-template <int DIR, int MODE, KOp OP, int EMAX>
-__global__ void __launch_bounds__(kFftMaxThreads)
+// Radix sets compiled into a kernel: every radix, or only 3, 4 and 8 (the
+// passes of n = 2^a 3^b with a != 1 mod 3 or a >= 4, 768 among them).  The
+// smaller set needs fewer registers: the column pass then fits 8 waves per
+// SIMD, two 1024-thread workgroups per CU.
+constexpr int kRsAll = 0;
+constexpr int kRs348 = 1;
+
+template <int RS, class F>
+__device__ inline void radix_dispatch(int R, F&& f)
+{
+  if constexpr (RS == kRs348) {
+    switch (R) {
+      case 3: f(std::integral_constant<int, 3>{}); break;
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      default: f(std::integral_constant<int, 8>{}); break;
+    }
+  } else {
+    switch (R) {
+      case 2: f(std::integral_constant<int, 2>{}); break;
+      case 3: f(std::integral_constant<int, 3>{}); break;
+      case 4: f(std::integral_constant<int, 4>{}); break;
+      case 5: f(std::integral_constant<int, 5>{}); break;
+      default: f(std::integral_constant<int, 8>{}); break;
+    }
+  }
+}
+
+constexpr int tile_waves_per_eu(int mode, KOp op, int emax, int rs)
+{
+  return (mode == 0 && op == KOp::None && emax == 8 && rs == kRs348) ? 8 : 1;
+}
+
+template <int DIR, int MODE, KOp OP, int EMAX, int RS>
+__global__ void __launch_bounds__(kFftMaxThreads) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu(MODE, OP, EMAX, RS))))
 fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __restrict__ tw, TileArgs A)
 {
   extern __shared__ float2 smem[];
@@ -201,6 +244,7 @@ fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __
     const int tile = lb - batch * A.tpb_pad;
     if (tile >= A.tpb) return;  // padding workgroup (before any barrier)
     c0 = tile << log2T;
+    if (padding_tile(A, c0, log2T)) return;
   } else {
     c0 = blockIdx.x << log2T;
   }
@@ -380,15 +424,7 @@ fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __
     }
   };
 
-  auto with_radix = [&](int R, auto&& f) {
-    switch (R) {
-      case 2: f(std::integral_constant<int, 2>{}); break;
-      case 3: f(std::integral_constant<int, 3>{}); break;
-      case 4: f(std::integral_constant<int, 4>{}); break;
-      case 5: f(std::integral_constant<int, 5>{}); break;
-      default: f(std::integral_constant<int, 8>{}); break;
-    }
-  };
+  auto with_radix = [&](int R, auto&& f) { radix_dispatch<RS>(R, f); };
 
   // Each pass keeps its butterflies in a register tile local to its radix
   // case: inputs from HBM (first pass) or LDS, outputs to LDS or HBM (last).
@@ -428,8 +464,8 @@ fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __
       }
     }
   } else {
-    if (MODE == kC2R && tid < T && c0 + tid < A.ncols) {
-      // the two padding floats of each real row are written as zero
+    if (MODE == kC2R && A.zero_pad && tid < T && c0 + tid < A.ncols) {
+      // the two padding floats of each FFTW-layout real row are written as zero
       float* oa = static_cast<float*>(out_) + (2LL * (c0 + tid)) * A.rs_out;
       oa[n] = 0.0f;
       oa[n + 1] = 0.0f;
@@ -504,17 +540,6 @@ __device__ inline void col_load_lds(float2 (&reg)[EMAX], const FftPass& p, const
   }
 }
 
-template <class F>
-__device__ inline void radix_case(int R, F&& f)
-{
-  switch (R) {
-    case 2: f(std::integral_constant<int, 2>{}); break;
-    case 3: f(std::integral_constant<int, 3>{}); break;
-    case 4: f(std::integral_constant<int, 4>{}); break;
-    case 5: f(std::integral_constant<int, 5>{}); break;
-    default: f(std::integral_constant<int, 8>{}); break;
-  }
-}
 
 // One workgroup: T columns (y, kz) of the half spectrum after the forward z
 // and y passes.  Forward x FFT (plan A.P, radix R_LAST last), then for each
@@ -522,7 +547,7 @@ __device__ inline void radix_case(int R, F&& f)
 // radices in reverse order (A.P2, radix R_LAST first): the forward's last
 // butterflies are exactly the inverse's first, so delta(k) stays in registers
 // and never goes to memory.  Outputs: the three x-inverted force spectra.
-template <int R_LAST, int EMAX>
+template <int R_LAST, int EMAX, int RS>
 __global__ void __launch_bounds__(kFftMaxThreads)
 fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __restrict__ out1,
             float2* __restrict__ out2, const float2* __restrict__ tw, TileArgs A, FftRadices Pinv)
@@ -534,6 +559,7 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
   const int lb = xcd_group_remap(blockIdx.x, gridDim.x, A.xgroup);
   if (lb >= A.tpb) return;  // padding workgroup (before any barrier)
   const int c0 = lb << log2T;
+  if (padding_tile(A, c0, log2T)) return;
   ColCtx x{smem + n, smem, tid, nthr, log2T};
   for (int k = tid; k < n; k += nthr) smem[k] = tw[k];
   using Tile = float2[EMAX];
@@ -541,7 +567,7 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
   const int tmask = (1 << log2T) - 1;
 
   // forward pass 0 from global memory
-  radix_case(A.P.pass[0].R, [&](auto Rc) {
+  radix_dispatch<RS>(A.P.pass[0].R, [&](auto Rc) {
     constexpr int R = decltype(Rc)::value;
     const FftPass& p = A.P.pass[0];
     const int nb = p.m << log2T;
@@ -562,7 +588,7 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
   });
   __syncthreads();
   for (int pi = 1; pi < np_ - 1; ++pi) {
-    radix_case(A.P.pass[pi].R, [&](auto Rc) {
+    radix_dispatch<RS>(A.P.pass[pi].R, [&](auto Rc) {
       constexpr int R = decltype(Rc)::value;
       Tile reg;
       col_load_lds<R, EMAX>(reg, A.P.pass[pi], x);
@@ -579,29 +605,52 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
   __syncthreads();
   col_compute<R_LAST, EMAX, -1>(keep, pl, x);
 
+  // f1 / k^2 of each kept value (pm.c:339 evaluates (f1 / k^2) * d_axis
+  // left to right, so the quotient is shared by the three axes bit for bit)
+  float green[EMAX];
+#pragma unroll
+  for (int q = 0; q < EMAX / R_LAST; ++q) {
+    const int u = tid + q * nthr;
+    const int c = u & tmask, b = u >> log2T;
+    const int col = c0 + c;
+    const int iy = (int)fdiv((unsigned)col, (unsigned)A.nz, A.magNz);
+    const int J0 = wave_pm(iy, n), K = col - iy * A.nz;
+#pragma unroll
+    for (int t = 0; t < R_LAST; ++t) {
+      const int I0 = wave_pm(b + t * pl.m, n);
+      const int k2 = K * K + I0 * I0 + J0 * J0;
+      green[q * R_LAST + t] = k2 == 0 ? 0.0f : A.kp.f1 / (float)k2;
+    }
+  }
+
   float2* outs[3] = {out0, out1, out2};
+#pragma unroll 1
   for (int axis = 0; axis < 3; ++axis) {
-    KOpParams kp = A.kp;
-    kp.axis = axis;
     Tile reg;
-    // inverse pass 0 (L = 1): operator on the forward outputs b + t*m
+    // inverse pass 0 (L = 1): pm.c:322-346 operator on the forward outputs b + t*m
 #pragma unroll
     for (int q = 0; q < EMAX / R_LAST; ++q) {
       const int u = tid + q * nthr;
       const int c = u & tmask, b = u >> log2T;
       const int col = c0 + c;
       const int iy = (int)fdiv((unsigned)col, (unsigned)A.nz, A.magNz);
-      const int iz = col - iy * A.nz;
+      const int J0 = wave_pm(iy, n), K = col - iy * A.nz;
 #pragma unroll
-      for (int t = 0; t < R_LAST; ++t)
-        reg[q * R_LAST + t] = apply_kop<KOp::PmPoisson>(keep[q * R_LAST + t], b + t * pl.m, iy, iz, n, kp);
+      for (int t = 0; t < R_LAST; ++t) {
+        const int I0 = wave_pm(b + t * pl.m, n);
+        const float di = (float)(axis == 0 ? I0 : (axis == 1 ? J0 : K));
+        const float f2 = green[q * R_LAST + t] * di;
+        const float2 v = keep[q * R_LAST + t];
+        reg[q * R_LAST + t] = (I0 == 0 && J0 == 0 && K == 0) ? make_float2(0.0f, 0.0f)
+                                                              : make_float2(-f2 * v.y, f2 * v.x);
+      }
     }
     col_compute<R_LAST, EMAX, 1>(reg, Pinv.pass[0], x);
     col_store_lds<R_LAST, EMAX>(reg, Pinv.pass[0], x);
     __syncthreads();
     for (int pi = 1; pi < np_; ++pi) {
       const bool last = pi == np_ - 1;
-      radix_case(Pinv.pass[pi].R, [&](auto Rc) {
+      radix_dispatch<RS>(Pinv.pass[pi].R, [&](auto Rc) {
         constexpr int R = decltype(Rc)::value;
         const FftPass& p = Pinv.pass[pi];
         Tile r2;
@@ -643,18 +692,26 @@ size_t tile_lds_bytes(int mode, int n, int log2T)
   return sizeof(float2) * (data + n);
 }
 
-template <int DIR, int MODE, KOp OP, int E>
+template <int DIR, int MODE, KOp OP, int E, int RS>
 int launch_e(dim3 grid, int threads, size_t lds, hipStream_t s, const void* in, void* out, const float2* tw,
              const TileArgs& A)
 {
   static bool attr = false;
   if (!attr) {
-    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_tile<DIR, MODE, OP, E>),
+    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_tile<DIR, MODE, OP, E, RS>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
   }
-  hipLaunchKernelGGL((fft_tile<DIR, MODE, OP, E>), grid, dim3(threads), lds, s, in, out, tw, A);
+  hipLaunchKernelGGL((fft_tile<DIR, MODE, OP, E, RS>), grid, dim3(threads), lds, s, in, out, tw, A);
   return kOk;
+}
+
+// Every pass of radix 3, 4 or 8.
+bool radices_348(const FftRadices& r)
+{
+  for (int i = 0; i < r.npass; ++i)
+    if (r.pass[i].R != 3 && r.pass[i].R != 4 && r.pass[i].R != 8) return false;
+  return true;
 }
 
 template <int DIR, int MODE, KOp OP>
@@ -662,10 +719,13 @@ int launch_tile(const FftPassGeom& g, dim3 grid, hipStream_t s, const void* in, 
                 const TileArgs& A)
 {
   const size_t lds = tile_lds_bytes(MODE, A.P.n, g.log2T);
-  if (g.emax <= 8) return launch_e<DIR, MODE, OP, 8>(grid, g.threads, lds, s, in, out, tw, A);
-  if (g.emax <= 12) return launch_e<DIR, MODE, OP, 12>(grid, g.threads, lds, s, in, out, tw, A);
-  if (g.emax <= 16) return launch_e<DIR, MODE, OP, 16>(grid, g.threads, lds, s, in, out, tw, A);
-  return launch_e<DIR, MODE, OP, 24>(grid, g.threads, lds, s, in, out, tw, A);
+  if (g.emax <= 8) {
+    if (radices_348(A.P)) return launch_e<DIR, MODE, OP, 8, kRs348>(grid, g.threads, lds, s, in, out, tw, A);
+    return launch_e<DIR, MODE, OP, 8, kRsAll>(grid, g.threads, lds, s, in, out, tw, A);
+  }
+  if (g.emax <= 12) return launch_e<DIR, MODE, OP, 12, kRsAll>(grid, g.threads, lds, s, in, out, tw, A);
+  if (g.emax <= 16) return launch_e<DIR, MODE, OP, 16, kRsAll>(grid, g.threads, lds, s, in, out, tw, A);
+  return launch_e<DIR, MODE, OP, 24, kRsAll>(grid, g.threads, lds, s, in, out, tw, A);
 }
 
 int grid_x(long long ncols, int log2T) { return (int)((ncols + (1LL << log2T) - 1) >> log2T); }
@@ -770,9 +830,10 @@ static int cap_log2(int n)
   return l;
 }
 
-int FftPlan::init(int n_, int log2T_cols, int log2T_rows)
+int FftPlan::init(int n_, int log2T_cols, int log2T_rows, bool pad_rows)
 {
   n = n_;
+  nzp = pad_rows ? (n / 2 + 1 + 15) / 16 * 16 : n / 2 + 1;
   if (n % 2 != 0 || !fft_factor(n, &rad)) return kErrUnsupported;
   // widest power-of-two tile with n*T <= 6144 (48 KiB of LDS), no wider than
   // the columns, and a register tile of at most 24 values per thread
@@ -788,7 +849,7 @@ int FftPlan::init(int n_, int log2T_cols, int log2T_rows)
   };
   // columns: 8 adjacent complex values (64 B, 128 B per XCD-paired tiles);
   // rows: 2 row pairs per workgroup (measured fastest on MI355X at n = 768)
-  gx = pick(log2T_cols >= 0 ? log2T_cols : std::min(3, cap_log2(n)), (long long)n * (n / 2 + 1));
+  gx = pick(log2T_cols >= 0 ? log2T_cols : std::min(3, cap_log2(n)), (long long)n * nzp);
   gy = pick(log2T_cols >= 0 ? log2T_cols : std::min(3, cap_log2(n)), n / 2 + 1);
   grows = pick(log2T_rows >= 0 ? log2T_rows : 1, (long long)n * n / 2);
   if (gx.emax == 0 || gy.emax == 0 || grows.emax == 0) return kErrUnsupported;
@@ -810,7 +871,7 @@ void FftPlan::destroy()
 
 int FftPlan::r2c(float* data, hipStream_t s) const
 {
-  const long long nz = n / 2 + 1;
+  const long long nz = n / 2 + 1, np_ = nzp;
   const int npairs = n * n / 2;
   float2* cdata = reinterpret_cast<float2*>(data);
   auto mark = [&](int pass, int begin) {
@@ -821,26 +882,29 @@ int FftPlan::r2c(float* data, hipStream_t s) const
   // z: rows
   A.log2T = grows.log2T;
   A.ncols = npairs;
-  A.rs_in = 2 * nz;
-  A.rs_out = nz;
+  A.rs_in = 2 * np_;
+  A.rs_out = np_;
+  A.kz_valid = 0;
   mark(0, 1);
   COLA_TRY((launch_tile<-1, kR2C, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, data, cdata, d_tw, A)));
   mark(0, 0);
   // y: columns kz of each x plane
   A.log2T = gy.log2T;
   A.ncols = (int)nz;
-  A.sr = nz;
-  A.bs = (long long)n * nz;
+  A.sr = np_;
+  A.bs = (long long)n * np_;
   set_groups(A, grid_x(nz, gy.log2T), xgroup);
   mark(1, 1);
   COLA_TRY((launch_tile<-1, kCols, KOp::None>(gy, dim3(A.tpb_pad * n), s, cdata, cdata, d_tw, A)));
   mark(1, 0);
-  // x: columns (y, kz)
-  const long long ncols = (long long)n * nz;
+  // x: columns (y, kz), row padding included (its tiles return at once)
+  const long long ncols = (long long)n * np_;
   A.log2T = gx.log2T;
   A.ncols = (int)ncols;
   A.sr = ncols;
   A.bs = 0;
+  A.nz = (int)np_;
+  A.kz_valid = np_ > nz ? (int)nz : 0;
   set_groups(A, grid_x(ncols, gx.log2T), xgroup);
   mark(2, 1);
   COLA_TRY((launch_tile<-1, kCols, KOp::None>(gx, dim3(A.tpb_pad), s, cdata, cdata, d_tw, A)));
@@ -869,18 +933,18 @@ static FftRadices reversed(const FftRadices& r)
   return v;
 }
 
-template <int RL, int E>
+template <int RL, int E, int RS>
 static int launch_fused_e(const FftPassGeom& g, int grid, hipStream_t s, const float2* in, float2* o0, float2* o1,
                           float2* o2, const float2* tw, const TileArgs& A, const FftRadices& Pinv)
 {
   static bool attr = false;
   if (!attr) {
-    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_x_fused<RL, E>),
+    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_x_fused<RL, E, RS>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr = true;
   }
-  hipLaunchKernelGGL((fft_x_fused<RL, E>), dim3(grid), dim3(g.threads), tile_lds_bytes(kCols, A.P.n, g.log2T), s, in,
-                     o0, o1, o2, tw, A, Pinv);
+  hipLaunchKernelGGL((fft_x_fused<RL, E, RS>), dim3(grid), dim3(g.threads), tile_lds_bytes(kCols, A.P.n, g.log2T), s,
+                     in, o0, o1, o2, tw, A, Pinv);
   return kOk;
 }
 
@@ -888,8 +952,12 @@ template <int RL>
 static int launch_fused_r(const FftPassGeom& g, int grid, hipStream_t s, const float2* in, float2* o0, float2* o1,
                           float2* o2, const float2* tw, const TileArgs& A, const FftRadices& Pinv)
 {
-  if (g.emax <= 8) return launch_fused_e<RL, 8>(g, grid, s, in, o0, o1, o2, tw, A, Pinv);
-  return launch_fused_e<RL, 12>(g, grid, s, in, o0, o1, o2, tw, A, Pinv);
+  if (g.emax <= 8) {
+    if constexpr (RL == 3 || RL == 4 || RL == 8)
+      if (radices_348(A.P)) return launch_fused_e<RL, 8, kRs348>(g, grid, s, in, o0, o1, o2, tw, A, Pinv);
+    return launch_fused_e<RL, 8, kRsAll>(g, grid, s, in, o0, o1, o2, tw, A, Pinv);
+  }
+  return launch_fused_e<RL, 12, kRsAll>(g, grid, s, in, o0, o1, o2, tw, A, Pinv);
 }
 
 bool FftPlan::can_fuse() const
@@ -910,7 +978,7 @@ int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hi
     }
     return kOk;
   }
-  const long long nz = n / 2 + 1;
+  const long long nz = n / 2 + 1, np_ = nzp;
   const int npairs = n * n / 2;
   float2* cdata = reinterpret_cast<float2*>(density);
   auto mark = [&](int pass, int begin) {
@@ -919,30 +987,32 @@ int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hi
   TileArgs A;
   A.P = rad;
   A.kp = kp;
-  A.nz = (int)nz;
-  A.magNz = magic_for((int)nz);
+  A.nz = (int)np_;
+  A.magNz = magic_for((int)np_);
+  A.kz_valid = 0;
   // z and y forward
   A.log2T = grows.log2T;
   A.ncols = npairs;
-  A.rs_in = 2 * nz;
-  A.rs_out = nz;
+  A.rs_in = 2 * np_;
+  A.rs_out = np_;
   mark(0, 1);
   COLA_TRY((launch_tile<-1, kR2C, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, density, cdata, d_tw, A)));
   mark(0, 0);
   A.log2T = gy.log2T;
   A.ncols = (int)nz;
-  A.sr = nz;
-  A.bs = (long long)n * nz;
+  A.sr = np_;
+  A.bs = (long long)n * np_;
   set_groups(A, grid_x(nz, gy.log2T), xgroup);
   mark(1, 1);
   COLA_TRY((launch_tile<-1, kCols, KOp::None>(gy, dim3(A.tpb_pad * n), s, cdata, cdata, d_tw, A)));
   mark(1, 0);
-  // fused x: forward, Poisson per axis, inverse
-  const long long ncols = (long long)n * nz;
+  // fused x: forward, Poisson per axis, inverse (row padding tiles return at once)
+  const long long ncols = (long long)n * np_;
   A.log2T = gx.log2T;
   A.ncols = (int)ncols;
   A.sr = ncols;
   A.bs = 0;
+  A.kz_valid = np_ > nz ? (int)nz : 0;
   set_groups(A, grid_x(ncols, gx.log2T), xgroup);
   const FftRadices Pinv = reversed(rad);
   float2* o[3] = {reinterpret_cast<float2*>(out[0]), reinterpret_cast<float2*>(out[1]),
@@ -959,18 +1029,20 @@ int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hi
   for (int a = 0; a < 3; ++a) {
     TileArgs B;
     B.P = rad;
+    B.kz_valid = 0;
     B.log2T = gy.log2T;
     B.ncols = (int)nz;
-    B.sr = nz;
-    B.bs = (long long)n * nz;
+    B.sr = np_;
+    B.bs = (long long)n * np_;
     set_groups(B, grid_x(nz, gy.log2T), xgroup);
     mark(4, 1);
     COLA_TRY((launch_tile<1, kCols, KOp::None>(gy, dim3(B.tpb_pad * n), s, o[a], o[a], d_tw, B)));
     mark(4, 0);
     B.log2T = grows.log2T;
     B.ncols = npairs;
-    B.rs_in = nz;
-    B.rs_out = 2 * nz;
+    B.rs_in = np_;
+    B.rs_out = 2 * np_;
+    B.zero_pad = np_ == nz;
     mark(5, 1);
     COLA_TRY((launch_tile<1, kC2R, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, o[a], out[a], d_tw, B)));
     mark(5, 0);
@@ -981,7 +1053,7 @@ int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hi
 
 int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hipStream_t s) const
 {
-  const long long nz = n / 2 + 1;
+  const long long nz = n / 2 + 1, np_ = nzp;
   const int npairs = n * n / 2;
   float2* cout = reinterpret_cast<float2*>(out);
   auto mark = [&](int pass, int begin) {
@@ -990,14 +1062,15 @@ int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hi
   TileArgs A;
   A.P = rad;
   A.kp = kp;
-  A.nz = (int)nz;
-  A.magNz = magic_for((int)nz);
+  A.nz = (int)np_;
+  A.magNz = magic_for((int)np_);
   // x: columns (y, kz), k-space operator applied on load
-  const long long ncols = (long long)n * nz;
+  const long long ncols = (long long)n * np_;
   A.log2T = gx.log2T;
   A.ncols = (int)ncols;
   A.sr = ncols;
   A.bs = 0;
+  A.kz_valid = np_ > nz ? (int)nz : 0;
   set_groups(A, grid_x(ncols, gx.log2T), xgroup);
   const float2* src = in_k ? in_k : cout;
   const dim3 grid(A.tpb_pad);
@@ -1020,8 +1093,9 @@ int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hi
   // y
   A.log2T = gy.log2T;
   A.ncols = (int)nz;
-  A.sr = nz;
-  A.bs = (long long)n * nz;
+  A.sr = np_;
+  A.bs = (long long)n * np_;
+  A.kz_valid = 0;
   set_groups(A, grid_x(nz, gy.log2T), xgroup);
   mark(4, 1);
   COLA_TRY((launch_tile<1, kCols, KOp::None>(gy, dim3(A.tpb_pad * n), s, cout, cout, d_tw, A)));
@@ -1029,8 +1103,9 @@ int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hi
   // z: rows
   A.log2T = grows.log2T;
   A.ncols = npairs;
-  A.rs_in = nz;
-  A.rs_out = 2 * nz;
+  A.rs_in = np_;
+  A.rs_out = 2 * np_;
+  A.zero_pad = np_ == nz;
   mark(5, 1);
   COLA_TRY((launch_tile<1, kC2R, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, cout, out, d_tw, A)));
   mark(5, 0);

@@ -26,9 +26,9 @@ struct CicGeom {
   float wpar = 0.0f;     // pm.c:216  (Ngrid / nc)^3
   float x_right = 0.0f;  // pm.c:422-423  ghost threshold L/Ng*Ng - L/Ng
   float shift = 0.0f;    // pm.c:434  -BoxSize
-  long long row = 0;     // padded row length 2(ng/2+1)
+  long long row = 0;     // real row length in floats, >= 2(ng/2+1) (in-place r2c)
 };
-CicGeom make_cic_geom(int ng, float boxsize, float wpar);
+CicGeom make_cic_geom(int ng, float boxsize, float wpar, long long row);
 
 // Tiling of the density grid for the binned deposit (power-of-two tiles).
 struct TileGeom {

@@ -552,7 +552,7 @@ unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThrea
 
 }  // namespace
 
-CicGeom make_cic_geom(int ng, float boxsize, float wpar)
+CicGeom make_cic_geom(int ng, float boxsize, float wpar, long long row)
 {
   CicGeom g;
   g.ng = ng;
@@ -561,7 +561,7 @@ CicGeom make_cic_geom(int ng, float boxsize, float wpar)
   const float eps = boxsize / ng;
   g.x_right = boxsize / ng * (0 + ng) - eps;
   g.shift = -boxsize;
-  g.row = 2LL * (ng / 2 + 1);
+  g.row = row;
   return g;
 }
 

@@ -67,6 +67,7 @@ k_conv(const unsigned short* __restrict__ in, const unsigned short* __restrict__
        int cout, int n, unsigned short* __restrict__ out_bf, float* __restrict__ out_f,
        const unsigned short* __restrict__ hprev)
 {
+  static_assert(CIN == 8, "16-channel layers use k_conv16");
   constexpr int kTpc = 32 / CIN;                 // taps per K chunk
   constexpr int kChunks = (27 + kTpc - 1) / kTpc;
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
@@ -107,6 +108,120 @@ k_conv(const unsigned short* __restrict__ in, const unsigned short* __restrict__
         if (r < 3) out_f[(long long)r * n * n * n + cell] = v;
       } else {
         out_bf[cell * 16 + r] = f2bf(bf2f(hprev[cell * 16 + r]) > 0.0f ? v : 0.0f);
+      }
+    }
+  }
+}
+
+// 16-channel convolution with operand reuse.  A wave computes kConvTy output
+// tiles (16 cells along z x 16 channels) at rows y0 .. y0 + kConvTy - 1 of
+// one x plane, streaming the kConvTy + 2 input rows of each of the 3 x
+// offsets once.  Each input row is one 16-byte load per lane (cell z0 + r,
+// channels 8 (g % 2) .. + 7) plus one edge load; its z-shifted copies come
+// from DPP row shifts.  K is ordered so that every MFMA operand is a row, its
+// shifted copy or a lane-group select of two rows:
+//   chunk 3 (dx+1) + (dy+1), dx, dy in {-1,0,1}: taps (dx,dy,-1) | (dx,dy,+1)
+//   chunk 9 + (dx+1):                          taps (dx,-1,0) | (dx,0,0)
+//   chunk 12:                                  taps (-1,+1,0) | (0,+1,0)
+//   chunk 13:                                  taps (+1,+1,0) | none
+// (left of | for lane groups 0 and 1, right for 2 and 3).
+constexpr int kConvTy = 8;
+
+// Lane groups 0, 1 (rows 0, 1 of the wave): lane r takes lane r - 1 (cell
+// z - 1), lane 0 keeps `edge`; lane groups 2, 3: lane r takes lane r + 1
+// (cell z + 1), lane 15 keeps `edge`.
+__device__ inline unsigned dz_pair(unsigned center, unsigned edge)
+{
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)edge, (int)center, 0x111, 0x3, 0xf, false);
+  return (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)center, 0x101, 0xc, 0xf, false);
+}
+
+__device__ inline uint4 dz_pair4(uint4 c, uint4 e)
+{
+  return make_uint4(dz_pair(c.x, e.x), dz_pair(c.y, e.y), dz_pair(c.z, e.z), dz_pair(c.w, e.w));
+}
+
+__device__ inline uint4 sel4(bool lo, uint4 a, uint4 b)
+{
+  return lo ? a : b;
+}
+
+__device__ inline floatx4 mfma(uint4 a, uint4 b, floatx4 acc)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc, 0,
+                                                 0, 0);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
+         int cout, int n, unsigned short* __restrict__ out_bf, float* __restrict__ out_f,
+         const unsigned short* __restrict__ hprev)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const bool lo = g < 2;
+  uint4 b[14];
+#pragma unroll
+  for (int c = 0; c < 14; ++c) b[c] = *reinterpret_cast<const uint4*>(wpk + ((long long)c * 64 + lane) * 8);
+  const float bcol = (MODE == kBwdData || r >= cout) ? 0.0f : bias[r];
+  const int ch = 8 * (g & 1);
+  const int nzt = n / 16, nyb = n / kConvTy;
+  const long long units = (long long)n * nyb * nzt;
+  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  for (long long u = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); u < units; u += nwaves) {
+    const int zt = (int)(u % nzt);
+    const long long t2 = u / nzt;
+    const int y0 = (int)(t2 % nyb) * kConvTy, x = (int)(t2 / nyb);
+    const int z = zt * 16 + r;
+    const int ze = lo ? wrapi(zt * 16 - 1, n) : wrapi(zt * 16 + 16, n);
+    const int xs[3] = {wrapi(x - 1, n), x, wrapi(x + 1, n)};
+    floatx4 acc[kConvTy];
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    uint4 prev[3] = {zero4, zero4, zero4};
+#pragma unroll
+    for (int j = 0; j < kConvTy + 2; ++j) {  // input row y0 - 1 + j feeds output rows j - 2, j - 1, j
+      const int yy = wrapi(y0 - 1 + j, n);
+      uint4 cen[3], sh[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const long long rb = ((long long)xs[d] * n + yy) * n;
+        cen[d] = *reinterpret_cast<const uint4*>(in + (rb + z) * 16 + ch);
+        const uint4 e = *reinterpret_cast<const uint4*>(in + (rb + ze) * 16 + ch);
+        sh[d] = dz_pair4(cen[d], e);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (j >= 2) acc[j - 2] = mfma(sh[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
+        if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(sh[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
+        if (j < kConvTy) acc[j] = mfma(sh[d], b[3 * d], acc[j]);                          // dy = -1
+      }
+      if (j >= 1 && j - 1 < kConvTy) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[j - 1] = mfma(sel4(lo, prev[d], cen[d]), b[9 + d], acc[j - 1]);
+      }
+      if (j >= 2) {
+        acc[j - 2] = mfma(sel4(lo, cen[0], cen[1]), b[12], acc[j - 2]);
+        acc[j - 2] = mfma(sel4(lo, cen[2], zero4), b[13], acc[j - 2]);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) prev[d] = cen[d];
+    }
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) {
+      const long long row0 = ((long long)x * n + y0 + t) * n + zt * 16 + 4 * g;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const long long cell = row0 + i;
+        const float v = acc[t][i] + bcol;
+        if (MODE == kFwdRelu) {
+          out_bf[cell * 16 + r] = f2bf(v > 0.0f ? v : 0.0f);
+        } else if (MODE == kFwdLinear) {
+          if (r < 3) out_f[(long long)r * n * n * n + cell] = v;
+        } else {
+          out_bf[cell * 16 + r] = f2bf(bf2f(hprev[cell * 16 + r]) > 0.0f ? v : 0.0f);
+        }
       }
     }
   }
@@ -183,28 +298,49 @@ __global__ void k_wgrad_reduce(const float* __restrict__ part, int nwaves, int c
   grad[p] = s;
 }
 
-// fp32 master weights W[t][ci][co] -> forward B packs (k = tap-major over ci)
-// and data-gradient B packs (A = gradient with 16 channels co, column = ci).
+// Tap of chunk c for lane-group half h (0: lane groups 0, 1; 1: groups 2, 3)
+// in the K order of k_conv16, or -1 for none; taps t = 9 (dx+1) + 3 (dy+1) + (dz+1).
+__device__ inline int conv16_tap(int c, int h)
+{
+  if (c < 9) return 3 * c + (h ? 2 : 0);                  // (dx, dy, -1) | (dx, dy, +1)
+  if (c < 12) return 9 * (c - 9) + (h ? 4 : 1);           // (dx, -1, 0) | (dx, 0, 0)
+  if (c == 12) return h ? 16 : 7;                         // (-1, +1, 0) | (0, +1, 0)
+  return h ? -1 : 25;                                     // (+1, +1, 0) | none
+}
+
+// fp32 master weights W[t][ci][co] -> forward B packs and data-gradient B
+// packs.  cin = 8: k = tap-major over ci, 4 taps per chunk (k_conv<8>);
+// cin = 16: the K order of k_conv16.  The data gradient is the forward
+// convolution of the 16-channel output gradient with mirrored taps
+// (26 - t) and transposed channels, in the K order of k_conv16.
 __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned short* __restrict__ wf,
                        unsigned short* __restrict__ wb)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
   const int c = blockIdx.x;  // chunk
-  {
-    const int tpc = 32 / cin, chunks = (27 + tpc - 1) / tpc;
-    if (c < chunks)
+  if (cin == 8) {
+    if (c < 7)
       for (int j = 0; j < 8; ++j) {
-        const int k = 8 * g + j, tap = c * tpc + k / cin, ci = k % cin;
-        const float v = (tap < 27 && col < cout) ? w[((long long)tap * cin + ci) * cout + col] : 0.0f;
+        const int k = 8 * g + j, tap = c * 4 + k / 8, ci = k % 8;
+        const float v = (tap < 27 && col < cout) ? w[((long long)tap * 8 + ci) * cout + col] : 0.0f;
         wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
       }
-  }
-  if (wb && c < 14)
+  } else {
+    const int tap = conv16_tap(c, g >> 1);
     for (int j = 0; j < 8; ++j) {
-      const int k = 8 * g + j, tap = 2 * c + k / 16, co = k % 16;
-      const float v = (tap < 27 && co < cout && col < cin) ? w[((long long)tap * cin + col) * cout + co] : 0.0f;
+      const int ci = 8 * (g & 1) + j;
+      const float v = (tap >= 0 && col < cout) ? w[((long long)tap * 16 + ci) * cout + col] : 0.0f;
+      wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
+    }
+  }
+  if (wb) {
+    const int tap = conv16_tap(c, g >> 1);
+    for (int j = 0; j < 8; ++j) {
+      const int co = 8 * (g & 1) + j;
+      const float v = (tap >= 0 && co < cout && col < cin) ? w[((long long)(26 - tap) * cin + col) * cout + co] : 0.0f;
       wb[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
     }
+  }
 }
 
 // Input features: channel 0 = asinh(delta) * scale, channel 1 = a, rest 0.
@@ -343,6 +479,7 @@ long long param_offset(int layer)
 unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
 constexpr int kConvBlocks = 2048;
+constexpr int kConv16Blocks = 768;  // 3 waves per SIMD resident on 256 CUs
 constexpr int kWgradBlocks = 512;
 
 template <typename T>
@@ -429,16 +566,23 @@ int resnet_features(ResNet* r, const float* delta, long long row, const float* d
   return kOk;
 }
 
+unsigned conv16_blocks(int n)
+{
+  const long long units = (long long)n * (n / kConvTy) * (n / 16);
+  return (unsigned)std::min<long long>(kConv16Blocks, (units + 3) / 4);
+}
+
 int resnet_forward(ResNet* r, hipStream_t s)
 {
   const int n = r->n;
+  const unsigned nb = conv16_blocks(n);
   hipLaunchKernelGGL((k_conv<8, kFwdRelu>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_x, r->d_wf[0],
                      r->d_param + param_offset(0) + 27 * 8 * 16, 16, n, r->d_h[0], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv<16, kFwdRelu>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_h[0], r->d_wf[1],
+  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), 0, s, r->d_h[0], r->d_wf[1],
                      r->d_param + param_offset(1) + 27 * 16 * 16, 16, n, r->d_h[1], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv<16, kFwdRelu>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_h[1], r->d_wf[2],
+  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), 0, s, r->d_h[1], r->d_wf[2],
                      r->d_param + param_offset(2) + 27 * 16 * 16, 16, n, r->d_h[2], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv<16, kFwdLinear>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_h[2], r->d_wf[3],
+  hipLaunchKernelGGL((k_conv16<kFwdLinear>), dim3(nb), dim3(kThreads), 0, s, r->d_h[2], r->d_wf[3],
                      r->d_param + param_offset(3) + 27 * 16 * 3, 3, n, nullptr, r->d_y, nullptr);
   COLA_HIP(hipGetLastError());
   return kOk;
@@ -469,7 +613,7 @@ int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStr
                        kNetCin[l], kNetCout[l], r->d_grad + param_offset(l));
     if (l == 0) break;
     // gradient at the output of layer l - 1: data gradient of layer l masked by relu'(h_{l-1})
-    hipLaunchKernelGGL((k_conv<16, kBwdData>), dim3(kConvBlocks), dim3(kThreads), 0, s, G, r->d_wb[l], nullptr, 0,
+    hipLaunchKernelGGL((k_conv16<kBwdData>), dim3(conv16_blocks(n)), dim3(kThreads), 0, s, G, r->d_wb[l], nullptr, 0,
                        n, Gp, nullptr, r->d_h[l - 1]);
     std::swap(G, Gp);
   }
