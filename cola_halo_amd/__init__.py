@@ -107,6 +107,7 @@ _SIGNATURES = {
     "cola_set_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_step_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]),
     "cola_set_fft_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "cola_set_fft_tiles_xy": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cola_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cola_fft_time": (C.c_int, [C.c_int, C.c_int, C.c_void_p]),
     "cola_coarse_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -295,12 +296,13 @@ class Subsampler:
 class Tuning(C.Structure):
     """cola_tuning: the launch shapes the autotuner chose."""
     _fields_ = [("tile", C.c_int * 3), ("fft_log2_cols", C.c_int), ("fft_log2_rows", C.c_int),
-                ("xcd_group", C.c_int), ("ms_before", C.c_double), ("ms_after", C.c_double)]
+                ("xcd_group", C.c_int), ("fft_log2_xcols", C.c_int), ("ms_before", C.c_double),
+                ("ms_after", C.c_double)]
 
     def as_dict(self):
         return {"tile": list(self.tile), "fft_log2_cols": self.fft_log2_cols, "fft_log2_rows": self.fft_log2_rows,
-                "xcd_group": self.xcd_group, "ms_before": round(self.ms_before, 4),
-                "ms_after": round(self.ms_after, 4)}
+                "fft_log2_xcols": self.fft_log2_xcols, "xcd_group": self.xcd_group,
+                "ms_before": round(self.ms_before, 4), "ms_after": round(self.ms_after, 4)}
 
 
 class ResidualNet:
@@ -630,8 +632,10 @@ class Context:
     def set_step_graph(self, on=True):
         _check(lib().cola_set_step_graph(self._h, 1 if on else 0))
 
-    def set_fft_tiles(self, log2_cols=-1, log2_rows=-1, xcd_group=8):
-        _check(lib().cola_set_fft_tiles(self._h, log2_cols, log2_rows, xcd_group))
+    def set_fft_tiles(self, log2_cols=-1, log2_rows=-1, xcd_group=8, log2_xcols=-1):
+        """Launch shape of the PM transforms (log2_xcols: the x column pass
+        apart from the y pass, -1 = as log2_cols)."""
+        _check(lib().cola_set_fft_tiles_xy(self._h, log2_cols, log2_xcols, log2_rows, xcd_group))
 
     def autotune(self, reps=3):
         """Pick the fastest deposit tile / FFT launch shapes for the current

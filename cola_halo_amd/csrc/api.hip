@@ -85,6 +85,7 @@ struct cola_context {
   FftPlan pm_fft, lpt_fft;
   int fft_log2_tile = -1;  // column passes (cola_set_tiling sets both)
   int fft_log2_rows = -1;
+  int fft_log2_xtile = -1;  // x column pass when set apart from the y pass (-1: as fft_log2_tile)
   CicGeom cic;
   TileGeom tiles;
   BinBuffers bins;
@@ -1102,20 +1103,26 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
     c->lpt_fft.hook_user = c;
     c->fft_log2_rows = fft_log2_tile;
     c->fft_log2_tile = fft_log2_tile;
+    c->fft_log2_xtile = -1;
   }
   return kOk;
 }
 
 int cola_set_fft_tiles(cola_context* c, int log2_cols, int log2_rows, int xcd_group)
 {
-  if (!valid(c) || log2_cols < -1 || log2_cols > 6 || log2_rows < -1 || log2_rows > 6 || xcd_group < 1 ||
-      xcd_group > 64 || (xcd_group & (xcd_group - 1)))
+  return cola_set_fft_tiles_xy(c, log2_cols, -1, log2_rows, xcd_group);
+}
+
+int cola_set_fft_tiles_xy(cola_context* c, int log2_ycols, int log2_xcols, int log2_rows, int xcd_group)
+{
+  if (!valid(c) || log2_ycols < -1 || log2_ycols > 6 || log2_xcols < -1 || log2_xcols > 6 || log2_rows < -1 ||
+      log2_rows > 6 || xcd_group < 1 || xcd_group > 64 || (xcd_group & (xcd_group - 1)))
     return kErrArg;
   COLA_HIP(hipStreamSynchronize(c->stream));
   drop_step_graph(c);
-  if (log2_cols != c->fft_log2_tile || log2_rows != c->fft_log2_rows) {
+  if (log2_ycols != c->fft_log2_tile || log2_rows != c->fft_log2_rows || log2_xcols != c->fft_log2_xtile) {
     FftPlan a;
-    const int st = a.init(c->ng, log2_cols, log2_rows, true);
+    const int st = a.init(c->ng, log2_ycols, log2_rows, true, log2_xcols);
     if (st != kOk) {
       set_error("FFT tile widths not supported for this grid");
       return st;
@@ -1124,8 +1131,9 @@ int cola_set_fft_tiles(cola_context* c, int log2_cols, int log2_rows, int xcd_gr
     c->pm_fft = a;
     c->pm_fft.hook = fft_hook;
     c->pm_fft.hook_user = c;
-    c->fft_log2_tile = log2_cols;
+    c->fft_log2_tile = log2_ycols;
     c->fft_log2_rows = log2_rows;
+    c->fft_log2_xtile = log2_xcols;
   }
   c->pm_fft.xgroup = xcd_group;
   return kOk;
@@ -1180,12 +1188,13 @@ int cola_autotune(cola_context* c, int reps, cola_tuning* out)
   best.tile[2] = c->tiles.tz;
   best.fft_log2_cols = c->fft_log2_tile;
   best.fft_log2_rows = c->fft_log2_rows;
+  best.fft_log2_xcols = c->fft_log2_xtile;
   best.xcd_group = c->pm_fft.xgroup;
   int st = time_pm_fields(c, reps, &best.ms_before);
   best.ms_after = best.ms_before;
   auto apply = [&](const cola_tuning& t) -> int {
     int r = set_deposit_tiles(c, t.tile[0], t.tile[1], t.tile[2]);
-    if (!r) r = cola_set_fft_tiles(c, t.fft_log2_cols, t.fft_log2_rows, t.xcd_group);
+    if (!r) r = cola_set_fft_tiles_xy(c, t.fft_log2_cols, t.fft_log2_xcols, t.fft_log2_rows, t.xcd_group);
     return r;
   };
   auto trial = [&](cola_tuning t) {
@@ -1198,13 +1207,24 @@ int cola_autotune(cola_context* c, int reps, cola_tuning* out)
       best.ms_after = ms;
     }
   };
-  // coordinate descent: deposit tile, then FFT column / row tiles, then XCD group
+  // coordinate descent: deposit tile, then FFT column / row tiles, the x
+  // column tile, then XCD group
   static const int tiles[][3] = {{8, 8, 32}, {16, 8, 32}, {8, 16, 32}, {16, 16, 16}, {8, 8, 64}, {4, 8, 64}, {16, 16, 32}};
   for (const auto& tl : tiles) {
     cola_tuning t = best;
     t.tile[0] = tl[0], t.tile[1] = tl[1], t.tile[2] = tl[2];
     trial(t);
   }
+  // the x column pass (fused with the Poisson operator) apart from the y
+  // pass, before and after the y column / z row widths
+  auto sweep_x = [&]() {
+    for (int xcols = 1; xcols <= 4; ++xcols) {
+      cola_tuning t = best;
+      t.fft_log2_xcols = xcols;
+      trial(t);
+    }
+  };
+  sweep_x();
   for (int cols = 2; cols <= 4; ++cols)
     for (int rows = 0; rows <= 2; ++rows) {
       cola_tuning t = best;
@@ -1212,6 +1232,7 @@ int cola_autotune(cola_context* c, int reps, cola_tuning* out)
       t.fft_log2_rows = rows;
       trial(t);
     }
+  sweep_x();
   for (int g : {4, 8, 16, 64}) {
     cola_tuning t = best;
     t.xcd_group = g;

@@ -78,7 +78,8 @@ struct FftPlan {
   // the y and x column passes, log2T_rows >= 0 that of the z row passes
   // (row pairs); -1 picks the measured defaults.
   // pad_rows: complex rows of round_up(n/2+1, 16) values instead of n/2+1.
-  int init(int n, int log2T_cols = -1, int log2T_rows = -1, bool pad_rows = false);
+  // log2T_xcols >= 0 gives the x column pass (plain or fused) its own width.
+  int init(int n, int log2T_cols = -1, int log2T_rows = -1, bool pad_rows = false, int log2T_xcols = -1);
   void destroy();
 
   // In place on the padded layout (real rows of 2 nzp floats): real ->

@@ -222,9 +222,14 @@ __device__ inline void radix_dispatch(int R, F&& f)
   }
 }
 
+// Minimum waves per SIMD requested for the radix-{3,4,8} kernels with 8
+// values per thread: 8 for the plain column pass (61 VGPRs), 6 for the row
+// passes (<= 80 VGPRs); both without spills.
 constexpr int tile_waves_per_eu(int mode, KOp op, int emax, int rs)
 {
-  return (mode == 0 && op == KOp::None && emax == 8 && rs == kRs348) ? 8 : 1;
+  if (emax != 8 || rs != kRs348) return 1;
+  if (mode == 0) return op == KOp::None ? 8 : 1;
+  return 6;
 }
 
 template <int DIR, int MODE, KOp OP, int EMAX, int RS>
@@ -830,7 +835,7 @@ static int cap_log2(int n)
   return l;
 }
 
-int FftPlan::init(int n_, int log2T_cols, int log2T_rows, bool pad_rows)
+int FftPlan::init(int n_, int log2T_cols, int log2T_rows, bool pad_rows, int log2T_xcols)
 {
   n = n_;
   nzp = pad_rows ? (n / 2 + 1 + 15) / 16 * 16 : n / 2 + 1;
@@ -842,14 +847,15 @@ int FftPlan::init(int n_, int log2T_cols, int log2T_rows, bool pad_rows)
     while (l > 0 && (1LL << (l - 1)) >= ncols) --l;
     while (l >= 0) {
       FftPassGeom g = pass_geom(rad, l);
-      if (g.emax > 0) return g;
+      if (g.emax > 0 && tile_lds_bytes(kCols, n, l) <= 160 * 1024) return g;  // fits one CU's LDS
       --l;
     }
     return FftPassGeom{};
   };
   // columns: 8 adjacent complex values (64 B, 128 B per XCD-paired tiles);
   // rows: 2 row pairs per workgroup (measured fastest on MI355X at n = 768)
-  gx = pick(log2T_cols >= 0 ? log2T_cols : std::min(3, cap_log2(n)), (long long)n * nzp);
+  const int xcols = log2T_xcols >= 0 ? log2T_xcols : log2T_cols;
+  gx = pick(xcols >= 0 ? xcols : std::min(3, cap_log2(n)), (long long)n * nzp);
   gy = pick(log2T_cols >= 0 ? log2T_cols : std::min(3, cap_log2(n)), n / 2 + 1);
   grows = pick(log2T_rows >= 0 ? log2T_rows : 1, (long long)n * n / 2);
   if (gx.emax == 0 || gy.emax == 0 || grows.emax == 0) return kErrUnsupported;

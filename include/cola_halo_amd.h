@@ -385,6 +385,9 @@ int cola_set_fft_xcd_group(cola_context* ctx, int group);
    the y/x column passes and of the z row passes (-1 = default), and the XCD
    group.  Results do not depend on them. */
 int cola_set_fft_tiles(cola_context* ctx, int log2_cols, int log2_rows, int xcd_group);
+/* As cola_set_fft_tiles with separate widths for the y and the x column
+   passes (log2_xcols = -1: as log2_ycols). */
+int cola_set_fft_tiles_xy(cola_context* ctx, int log2_ycols, int log2_xcols, int log2_rows, int xcd_group);
 
 /* Autotuner (no reference counterpart): times the PM force fields (deposit +
    transforms, `reps` per candidate) of the current particle state for deposit
@@ -396,6 +399,7 @@ int cola_set_fft_tiles(cola_context* ctx, int log2_cols, int log2_rows, int xcd_
 typedef struct {
   int tile[3];
   int fft_log2_cols, fft_log2_rows, xcd_group;
+  int fft_log2_xcols; /* x column pass width, -1 = as fft_log2_cols (the y pass) */
   double ms_before, ms_after; /* mean PM force-field time of the start and the chosen shape */
 } cola_tuning;
 int cola_autotune(cola_context* ctx, int reps, cola_tuning* out);

@@ -30,11 +30,11 @@ def test_autotune_keeps_results(eh_table):
     assert np.array_equal(f0, f1)
 
 
-@pytest.mark.parametrize("cols,rows,group", [(2, 0, 4), (3, 1, 8), (4, 2, 64)])
-def test_fft_launch_shapes_agree_bit_for_bit(eh_table, cols, rows, group):
+@pytest.mark.parametrize("cols,rows,group,xcols", [(2, 0, 4, -1), (3, 1, 8, 2), (4, 2, 64, 1), (2, 1, 8, 3)])
+def test_fft_launch_shapes_agree_bit_for_bit(eh_table, cols, rows, group, xcols):
     k, p = eh_table
     out = []
-    for shape in (None, (cols, rows, group)):
+    for shape in (None, (cols, rows, group, xcols)):
         with ch.Context(32, 96, 100.0) as ctx:
             ctx.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
             if shape:
