@@ -96,6 +96,7 @@ _SIGNATURES = {
     "cola_resnet_set_scales": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
     "cola_resnet_fit_scale": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_forward_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cola_resnet_time_forward": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                             C.c_void_p]),
     "cola_copy_particles": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -343,6 +344,12 @@ class ResidualNet:
         y = np.zeros(x.shape[:3] + (3,), np.float32)
         _check(lib().cola_resnet_forward_host(self._h, x.ctypes.data, y.ctypes.data))
         return y
+
+    def time_forward(self, reps=20):
+        """Mean device time of one forward pass on the current input (ms)."""
+        ms = C.c_double()
+        _check(lib().cola_resnet_time_forward(self._h, reps, C.byref(ms)))
+        return ms.value
 
     def gradient(self, x, target):
         x = np.ascontiguousarray(x, np.float32)

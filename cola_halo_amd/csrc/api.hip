@@ -1574,6 +1574,28 @@ int cola_resnet_forward_host(cola_resnet* h, const float* x, float* y)
   return kOk;
 }
 
+int cola_resnet_time_forward(cola_resnet* h, int reps, double* ms)
+{
+  if (!h || reps < 1 || !ms) return kErrArg;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  COLA_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return kErrHip;
+  }
+  int st = resnet_forward(&h->r, h->stream);  // warm
+  if (!st && hipEventRecord(e0, h->stream) != hipSuccess) st = kErrHip;
+  for (int i = 0; i < reps && !st; ++i) st = resnet_forward(&h->r, h->stream);
+  float t = 0.0f;
+  if (!st && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+    st = kErrHip;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t / reps;
+  return st;
+}
+
 int cola_resnet_gradient_host(cola_resnet* h, const float* x, const float* target, double* loss, float* grad)
 {
   if (!h || !x || !target || !loss || !grad) return kErrArg;

@@ -9,9 +9,8 @@
 // computed per wave as a 16-cell (along z) x 16-channel tile with
 // v_mfma_f32_16x16x32_bf16.  Its lanes hold (g = lane / 16, r = lane % 16):
 // A rows r, k = 8g..8g+7; B k = 8g..8g+7, column r; D rows 4g..4g+3,
-// column r.  With cin = 16 one K chunk of 32 covers two taps (8 channels per
-// lane group), with cin = 8 four taps, so every A operand is one 16-byte load
-// of 8 consecutive channels of one cell.
+// column r.  Every cell operand is 8 consecutive channels of one cell (one
+// 16-byte load); the K order of each kernel is given with it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,12 +40,6 @@ __host__ __device__ inline unsigned short f2bf(float f)
 
 __host__ __device__ inline float bf2f(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
 
-__device__ inline bf16x8 load8(const unsigned short* p)
-{
-  const uint4 v = *reinterpret_cast<const uint4*>(p);
-  return __builtin_bit_cast(bf16x8, v);
-}
-
 __device__ inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 __device__ inline long long cell_at(int x, int y, int z, int t, int sign, int n)
@@ -57,69 +50,15 @@ __device__ inline long long cell_at(int x, int y, int z, int t, int sign, int n)
 
 enum ConvMode { kFwdRelu = 0, kFwdLinear = 1, kBwdData = 2 };
 
-// One wave per 16-cell x 16-channel output tile, grid-stride over tiles; the
-// B operands of all K chunks stay in registers.  CIN = channels of the A
-// tensor (8 or 16).  kBwdData reads the A tensor at cell - d_t and masks the
-// result by the derivative of the previous layer's relu.
-template <int CIN, int MODE>
-__global__ void __launch_bounds__(kThreads)
-k_conv(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
-       int cout, int n, unsigned short* __restrict__ out_bf, float* __restrict__ out_f,
-       const unsigned short* __restrict__ hprev)
-{
-  static_assert(CIN == 8, "16-channel layers use k_conv16");
-  constexpr int kTpc = 32 / CIN;                 // taps per K chunk
-  constexpr int kChunks = (27 + kTpc - 1) / kTpc;
-  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  bf16x8 b[kChunks];
-#pragma unroll
-  for (int c = 0; c < kChunks; ++c) b[c] = load8(wpk + ((long long)c * 64 + lane) * 8);
-  const float bcol = (MODE == kBwdData || r >= cout) ? 0.0f : bias[r];
-  const int tile_tap = (8 * g) / CIN;           // tap within the chunk of this lane group
-  const int ci0 = (8 * g) % CIN;                // first channel of this lane group
-  const int sign = MODE == kBwdData ? -1 : 1;
-  const long long ntiles = (long long)n * n * (n / 16);
-  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
-  const long long wave = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-  for (long long tile = wave; tile < ntiles; tile += nwaves) {
-    const int zt = (int)(tile % (n / 16));
-    const long long xy = tile / (n / 16);
-    const int y = (int)(xy % n), x = (int)(xy / n);
-    const int z = zt * 16 + r;
-    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
-      const int t = c * kTpc + tile_tap;
-      bf16x8 a;
-      if (t < 27)
-        a = load8(in + cell_at(x, y, z, t, sign, n) * CIN + ci0);
-      else
-        a = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[c], acc, 0, 0, 0);
-    }
-    const long long row0 = ((long long)x * n + y) * n + zt * 16 + 4 * g;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const long long cell = row0 + i;
-      float v = acc[i] + bcol;
-      if (MODE == kFwdRelu) {
-        out_bf[cell * 16 + r] = f2bf(v > 0.0f ? v : 0.0f);
-      } else if (MODE == kFwdLinear) {
-        if (r < 3) out_f[(long long)r * n * n * n + cell] = v;
-      } else {
-        out_bf[cell * 16 + r] = f2bf(bf2f(hprev[cell * 16 + r]) > 0.0f ? v : 0.0f);
-      }
-    }
-  }
-}
-
 // 16-channel convolution with operand reuse.  A wave computes kConvTy output
 // tiles (16 cells along z x 16 channels) at rows y0 .. y0 + kConvTy - 1 of
 // one x plane, streaming the kConvTy + 2 input rows of each of the 3 x
-// offsets once.  Each input row is one 16-byte load per lane (cell z0 + r,
-// channels 8 (g % 2) .. + 7) plus one edge load; its z-shifted copies come
-// from DPP row shifts.  K is ordered so that every MFMA operand is a row, its
-// shifted copy or a lane-group select of two rows:
+// offsets once.  Each input row is two 16-byte loads per lane: cell z0 + r
+// and its z-shifted neighbour (z - 1 for lane groups 0-1, z + 1 for 2-3),
+// channels 8 (g % 2) .. + 7 (loading the neighbour directly from L1 measured
+// faster than shifting the row across lanes with DPP).  K is ordered so that
+// every MFMA operand is a row, its shifted copy or a lane-group select of two
+// rows:
 //   chunk 3 (dx+1) + (dy+1), dx, dy in {-1,0,1}: taps (dx,dy,-1) | (dx,dy,+1)
 //   chunk 9 + (dx+1):                          taps (dx,-1,0) | (dx,0,0)
 //   chunk 12:                                  taps (-1,+1,0) | (0,+1,0)
@@ -127,29 +66,27 @@ k_conv(const unsigned short* __restrict__ in, const unsigned short* __restrict__
 // (left of | for lane groups 0 and 1, right for 2 and 3).
 constexpr int kConvTy = 8;
 
-// Lane groups 0, 1 (rows 0, 1 of the wave): lane r takes lane r - 1 (cell
-// z - 1), lane 0 keeps `edge`; lane groups 2, 3: lane r takes lane r + 1
-// (cell z + 1), lane 15 keeps `edge`.
-__device__ inline unsigned dz_pair(unsigned center, unsigned edge)
-{
-  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)edge, (int)center, 0x111, 0x3, 0xf, false);
-  return (unsigned)__builtin_amdgcn_update_dpp((int)lo, (int)center, 0x101, 0xc, 0xf, false);
-}
-
-__device__ inline uint4 dz_pair4(uint4 c, uint4 e)
-{
-  return make_uint4(dz_pair(c.x, e.x), dz_pair(c.y, e.y), dz_pair(c.z, e.z), dz_pair(c.w, e.w));
-}
-
 __device__ inline uint4 sel4(bool lo, uint4 a, uint4 b)
 {
   return lo ? a : b;
 }
 
-__device__ inline floatx4 mfma(uint4 a, uint4 b, floatx4 acc)
+// Output tile transposed: the packed weights go in as the A operand (row =
+// output channel) and the input cells as B (column = cell), both with the
+// register layout [lane g, r][j] = k 8g + j of row/column r.  Lane (g, r)
+// then holds output channels 4g..4g+3 of cell r: 8 contiguous bytes.
+__device__ inline floatx4 mfma(uint4 x, uint4 w, floatx4 acc)
 {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc, 0,
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, w), __builtin_bit_cast(bf16x8, x), acc, 0,
                                                  0, 0);
+}
+
+// Four fp32 values -> four bfloat16 (round to nearest even) in 8 bytes.
+__device__ inline uint2 pack_bf16x4(float a, float b, float c, float d)
+{
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  const bf16x2 lo = {(__bf16)a, (__bf16)b}, hi = {(__bf16)c, (__bf16)d};
+  return make_uint2(__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi));
 }
 
 template <int MODE>
@@ -163,7 +100,9 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
   uint4 b[14];
 #pragma unroll
   for (int c = 0; c < 14; ++c) b[c] = *reinterpret_cast<const uint4*>(wpk + ((long long)c * 64 + lane) * 8);
-  const float bcol = (MODE == kBwdData || r >= cout) ? 0.0f : bias[r];
+  float bch[4];  // bias of output channels 4g..4g+3
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bch[i] = (MODE == kBwdData || 4 * g + i >= cout) ? 0.0f : bias[4 * g + i];
   const int ch = 8 * (g & 1);
   const int nzt = n / 16, nyb = n / kConvTy;
   const long long units = (long long)n * nyb * nzt;
@@ -174,7 +113,7 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
     const long long t2 = u / nzt;
     const int y0 = (int)(t2 % nyb) * kConvTy, x = (int)(t2 / nyb);
     const int z = zt * 16 + r;
-    const int ze = lo ? wrapi(zt * 16 - 1, n) : wrapi(zt * 16 + 16, n);
+    const int zsh = lo ? wrapi(z - 1, n) : wrapi(z + 1, n);
     const int xs[3] = {wrapi(x - 1, n), x, wrapi(x + 1, n)};
     floatx4 acc[kConvTy];
 #pragma unroll
@@ -188,8 +127,7 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
       for (int d = 0; d < 3; ++d) {
         const long long rb = ((long long)xs[d] * n + yy) * n;
         cen[d] = *reinterpret_cast<const uint4*>(in + (rb + z) * 16 + ch);
-        const uint4 e = *reinterpret_cast<const uint4*>(in + (rb + ze) * 16 + ch);
-        sh[d] = dz_pair4(cen[d], e);
+        sh[d] = *reinterpret_cast<const uint4*>(in + (rb + zsh) * 16 + ch);
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -210,19 +148,83 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
     }
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) {
-      const long long row0 = ((long long)x * n + y0 + t) * n + zt * 16 + 4 * g;
+      const long long cell = ((long long)x * n + y0 + t) * n + zt * 16 + r;
+      float v[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const long long cell = row0 + i;
-        const float v = acc[t][i] + bcol;
-        if (MODE == kFwdRelu) {
-          out_bf[cell * 16 + r] = f2bf(v > 0.0f ? v : 0.0f);
-        } else if (MODE == kFwdLinear) {
-          if (r < 3) out_f[(long long)r * n * n * n + cell] = v;
-        } else {
-          out_bf[cell * 16 + r] = f2bf(bf2f(hprev[cell * 16 + r]) > 0.0f ? v : 0.0f);
+      for (int i = 0; i < 4; ++i) v[i] = acc[t][i] + bch[i];
+      if (MODE == kFwdRelu) {
+        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
+            pack_bf16x4(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f), fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f));
+      } else if (MODE == kFwdLinear) {
+        if (g == 0) {
+          const long long nc = (long long)n * n * n;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) out_f[i * nc + cell] = v[i];
         }
+      } else {
+        const uint2 hp = *reinterpret_cast<const uint2*>(hprev + cell * 16 + 4 * g);
+        const float m0 = bf2f((unsigned short)(hp.x & 0xffffu)), m1 = bf2f((unsigned short)(hp.x >> 16));
+        const float m2 = bf2f((unsigned short)(hp.y & 0xffffu)), m3 = bf2f((unsigned short)(hp.y >> 16));
+        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
+            pack_bf16x4(m0 > 0.0f ? v[0] : 0.0f, m1 > 0.0f ? v[1] : 0.0f, m2 > 0.0f ? v[2] : 0.0f,
+                        m3 > 0.0f ? v[3] : 0.0f);
       }
+    }
+  }
+}
+
+// 8-channel convolution (the input features) with the operand reuse of
+// k_conv16: one 16-byte load holds all 8 channels of a cell, so a K chunk of
+// 32 is one input row's three z taps, lane group g loading cell z + g - 1
+// (group 3 idle):  chunk 3 (dx+1) + (dy+1): taps (dx,dy,-1) | (dx,dy,0) |
+// (dx,dy,+1) | none.
+__global__ void __launch_bounds__(kThreads)
+k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
+        int n, unsigned short* __restrict__ out_bf)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  uint4 b[9];
+#pragma unroll
+  for (int c = 0; c < 9; ++c) b[c] = *reinterpret_cast<const uint4*>(wpk + ((long long)c * 64 + lane) * 8);
+  float bch[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bch[i] = bias[4 * g + i];
+  const int nzt = n / 16, nyb = n / kConvTy;
+  const long long units = (long long)n * nyb * nzt;
+  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
+  for (long long u = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); u < units; u += nwaves) {
+    const int zt = (int)(u % nzt);
+    const long long t2 = u / nzt;
+    const int y0 = (int)(t2 % nyb) * kConvTy, x = (int)(t2 / nyb);
+    const int z = zt * 16 + r;
+    const int zsh = wrapi(z + (g < 3 ? g - 1 : 0), n);
+    const int xs[3] = {wrapi(x - 1, n), x, wrapi(x + 1, n)};
+    floatx4 acc[kConvTy];
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int j = 0; j < kConvTy + 2; ++j) {  // input row y0 - 1 + j feeds output rows j - 2, j - 1, j
+      const int yy = wrapi(y0 - 1 + j, n);
+      uint4 op[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const long long rb = ((long long)xs[d] * n + yy) * n;
+        const uint4 c = *reinterpret_cast<const uint4*>(in + (rb + zsh) * 8);
+        op[d] = g < 3 ? c : make_uint4(0u, 0u, 0u, 0u);
+      }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (j >= 2) acc[j - 2] = mfma(op[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
+        if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(op[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
+        if (j < kConvTy) acc[j] = mfma(op[d], b[3 * d], acc[j]);                          // dy = -1
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) {
+      const long long cell = ((long long)x * n + y0 + t) * n + zt * 16 + r;
+      *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
+          pack_bf16x4(fmaxf(acc[t][0] + bch[0], 0.0f), fmaxf(acc[t][1] + bch[1], 0.0f),
+                      fmaxf(acc[t][2] + bch[2], 0.0f), fmaxf(acc[t][3] + bch[3], 0.0f));
     }
   }
 }
@@ -309,7 +311,7 @@ __device__ inline int conv16_tap(int c, int h)
 }
 
 // fp32 master weights W[t][ci][co] -> forward B packs and data-gradient B
-// packs.  cin = 8: k = tap-major over ci, 4 taps per chunk (k_conv<8>);
+// packs.  cin = 8: the K order of k_conv8;
 // cin = 16: the K order of k_conv16.  The data gradient is the forward
 // convolution of the 16-channel output gradient with mirrored taps
 // (26 - t) and transposed channels, in the K order of k_conv16.
@@ -319,10 +321,11 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned 
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
   const int c = blockIdx.x;  // chunk
   if (cin == 8) {
-    if (c < 7)
+    // k_conv8: chunk c = 3 (dx+1) + (dy+1), lane group g -> tap dz = g - 1
+    if (c < 9)
       for (int j = 0; j < 8; ++j) {
-        const int k = 8 * g + j, tap = c * 4 + k / 8, ci = k % 8;
-        const float v = (tap < 27 && col < cout) ? w[((long long)tap * 8 + ci) * cout + col] : 0.0f;
+        const int tap = 3 * c + g;
+        const float v = (g < 3 && col < cout) ? w[((long long)tap * 8 + j) * cout + col] : 0.0f;
         wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
       }
   } else {
@@ -478,7 +481,6 @@ long long param_offset(int layer)
 
 unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
-constexpr int kConvBlocks = 2048;
 constexpr int kConv16Blocks = 768;  // 3 waves per SIMD resident on 256 CUs
 constexpr int kWgradBlocks = 512;
 
@@ -576,8 +578,8 @@ int resnet_forward(ResNet* r, hipStream_t s)
 {
   const int n = r->n;
   const unsigned nb = conv16_blocks(n);
-  hipLaunchKernelGGL((k_conv<8, kFwdRelu>), dim3(kConvBlocks), dim3(kThreads), 0, s, r->d_x, r->d_wf[0],
-                     r->d_param + param_offset(0) + 27 * 8 * 16, 16, n, r->d_h[0], nullptr, nullptr);
+  hipLaunchKernelGGL(k_conv8, dim3(nb), dim3(kThreads), 0, s, r->d_x, r->d_wf[0],
+                     r->d_param + param_offset(0) + 27 * 8 * 16, n, r->d_h[0]);
   hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), 0, s, r->d_h[0], r->d_wf[1],
                      r->d_param + param_offset(1) + 27 * 16 * 16, 16, n, r->d_h[1], nullptr, nullptr);
   hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), 0, s, r->d_h[1], r->d_wf[2],

@@ -17,6 +17,7 @@ held out, 10 COLA steps from a = 0.1):
 
 Usage: python -m cola_halo_amd.learned [--nc 128 --box 250 --coarse 192
        --fine 384 --ntrain 56 --nval 8 --epochs 8] > result.json
+       python -m cola_halo_amd.learned --time-net 192   (forward timing only)
 """
 import argparse
 import json
@@ -59,6 +60,23 @@ def timed_run(ctx, seed):
     return ctx.set_snapshot(1.0, Om), dt
 
 
+def time_net(n, reps=20):
+    """Device time of the net's forward pass on an n^3 grid (random weights and
+    features): ms per pass and the MFMA rate of the executed products (the
+    K chunks of 32 as issued: 9 for the 8-channel layer, 14 for the others)."""
+    rng = np.random.default_rng(1)
+    x = np.zeros((n, n, n, 8), np.float32)
+    x[..., 0] = rng.standard_normal((n, n, n))
+    x[..., 1] = 0.5
+    with ch.ResidualNet(n) as net:
+        net.forward(x)
+        ms = net.time_forward(reps)
+    cells = n ** 3
+    flops = cells * 16 * 32 * 2 * (9 + 3 * 14)
+    return {"n": n, "forward_ms": round(ms, 4), "mfma_tflops": round(flops / (ms * 1e-3) / 1e12, 1),
+            "cells_per_s": round(cells / (ms * 1e-3), 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nc", type=int, default=128)
@@ -70,7 +88,12 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--pk-grid", type=int, default=0, help="power-spectrum grid (default: the fine grid)")
+    ap.add_argument("--time-net", type=int, default=0, metavar="N",
+                    help="only time the net's forward pass on an N^3 grid and exit")
     args = ap.parse_args()
+    if args.time_net:
+        print(json.dumps(time_net(args.time_net)))
+        return
     nc, box = args.nc, args.box
     pk_n = args.pk_grid or args.fine
     k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
@@ -101,7 +124,16 @@ def main():
                        "out_scale": net.scales()[1],
                        "loss_of_zero_correction": 1.0}
 
-    # judgement on the held-out seeds
+    # judgement on the held-out seeds: graph-captured steps for all three runs
+    # (a context of its own for the corrected run, so no graph is re-captured;
+    # one untimed warm-up run per context captures them)
+    corrected = ch.Context(nc, args.coarse, box)
+    corrected.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
+    corrected.set_residual_net(net)
+    runs = (("fine", fine), ("coarse", coarse), ("corrected", corrected))
+    for name, ctx in runs:
+        ctx.set_step_graph(True)
+        timed_run(ctx, args.ntrain + 1)
     kb = None
     ratios = {"coarse": [], "corrected": []}
     mf = {"fine": [], "coarse": [], "corrected": []}
@@ -110,14 +142,12 @@ def main():
     edges = 32 * 2.0 ** np.arange(0, 12)
     for seed in range(args.ntrain + 1, args.ntrain + args.nval + 1):
         pk = {}
-        for name, ctx, use_net in (("fine", fine, False), ("coarse", coarse, False), ("corrected", coarse, True)):
-            ctx.set_residual_net(net if use_net else None)
+        for name, ctx in runs:
             snap, dt = timed_run(ctx, seed)
             times[name] += dt
             kb, pk[name], cnt = ctx.power_spectrum(pk_n, 0.0, 1.0, 25, deconvolve=True, source="snapshot")
             halos = ctx.fof_find_halos(ll, 32)
             mf[name].append(np.histogram(halos["nfof"], bins=edges)[0])
-        coarse.set_residual_net(None)
         for name in ("coarse", "corrected"):
             ratios[name].append(pk[name] / pk["fine"])
     sel = kb > 0
@@ -134,6 +164,7 @@ def main():
     nps = nc ** 3 * NSTEPS * args.nval
     out["particle_steps_per_s"] = {name: round(nps / t, 1) for name, t in times.items()}
     out["speedup_corrected_vs_fine"] = round(times["fine"] / times["corrected"], 3)
+    corrected.close()
     net.close()
     fine.close()
     coarse.close()

@@ -279,6 +279,9 @@ int cola_resnet_fit_scale(cola_resnet* net, int count);
    respect to every parameter. */
 int cola_resnet_forward_host(cola_resnet* net, const float* x, float* y);
 int cola_resnet_gradient_host(cola_resnet* net, const float* x, const float* target, double* loss, float* grad);
+/* Mean device time (HIP events, `reps` launches) of one forward pass of the
+   four layers on the current device input, in ms (no reference counterpart). */
+int cola_resnet_time_forward(cola_resnet* net, int reps, double* ms);
 /* Device copy of the particle state of src into dst (same particle count). */
 int cola_copy_particles(cola_context* dst, cola_context* src);
 /* Training pair from two contexts holding the same particles: coarse (the
