@@ -97,6 +97,7 @@ _SIGNATURES = {
     "cola_resnet_fit_scale": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_forward_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_resnet_time_forward": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "cola_resnet_set_restriction": (C.c_int, [C.c_void_p, C.c_float]),
     "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                             C.c_void_p]),
     "cola_copy_particles": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -358,6 +359,12 @@ class ResidualNet:
         g = np.zeros(self.nparam(), np.float32)
         _check(lib().cola_resnet_gradient_host(self._h, x.ctypes.data, t.ctypes.data, C.byref(loss), g.ctypes.data))
         return loss.value, g
+
+    def set_restriction(self, side_weight):
+        """Training targets: the fine force averaged over the 3^3 fine
+        neighbours of each coarse cell with weights (w, 1 - 2w, w) per axis
+        (0 = the point value, 0.25 = full weighting, default 0.125)."""
+        _check(lib().cola_resnet_set_restriction(self._h, side_weight))
 
     def add_pair(self, coarse, fine):
         _check(lib().cola_resnet_add_pair(self._h, coarse._h, fine._h))

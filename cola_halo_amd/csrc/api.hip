@@ -1574,6 +1574,13 @@ int cola_resnet_forward_host(cola_resnet* h, const float* x, float* y)
   return kOk;
 }
 
+int cola_resnet_set_restriction(cola_resnet* h, float side_weight)
+{
+  if (!h || !(side_weight >= 0.0f && side_weight <= 0.25f)) return kErrArg;
+  h->r.restrict_side = side_weight;
+  return kOk;
+}
+
 int cola_resnet_time_forward(cola_resnet* h, int reps, double* ms)
 {
   if (!h || reps < 1 || !ms) return kErrArg;

@@ -41,6 +41,7 @@ struct ResNet {
   float out_scale = 1.0f;    // residual = out_scale * y
   bool scale_set = false;    // out_scale fitted to training targets (or set by the caller)
   float in_scale = 1.0f;     // feature = asinh(delta) * in_scale
+  float restrict_side = 0.125f;  // training targets: fine force averaged with weights (w, 1-2w, w) per axis
 
   // training set: per sample the feature grid (bf16 [n^3]), the target
   // residual (bf16 [3][n^3]) and the scale factor a

@@ -420,12 +420,32 @@ __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __re
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// Training pair: feature asinh(delta) * scale (bf16) and target = fine force
-// at the coincident fine cell (2i, 2j, 2k) minus the coarse force (bf16).
+// Fine force restricted to coarse cell (x, y, z): the average over the 3^3
+// fine neighbours of the coincident fine cell (2x, 2y, 2z) with weights
+// (ws, 1 - 2 ws, ws) per axis, periodic (ws = 0: the point value, 1/4: full
+// weighting).
+__device__ inline float restrict_fine(const float* __restrict__ f, long long frow, int nf, int x, int y, int z, float ws)
+{
+  if (ws == 0.0f) return f[((long long)(2 * x) * nf + 2 * y) * frow + 2 * z];
+  const float w[3] = {ws, 1.0f - 2.0f * ws, ws};
+  float s = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    const int fx = wrapi(2 * x + a - 1, nf);
+    for (int b = 0; b < 3; ++b) {
+      const long long row = ((long long)fx * nf + wrapi(2 * y + b - 1, nf)) * frow;
+      const float wab = w[a] * w[b];
+      for (int c = 0; c < 3; ++c) s += wab * w[c] * f[row + wrapi(2 * z + c - 1, nf)];
+    }
+  }
+  return s;
+}
+
+// Training pair: feature asinh(delta) * scale (bf16) and target = restricted
+// fine force minus the coarse force (bf16).
 __global__ void k_store_pair(const float* __restrict__ cd, long long crow, const float* __restrict__ c0,
                              const float* __restrict__ c1, const float* __restrict__ c2, const float* __restrict__ f0,
                              const float* __restrict__ f1, const float* __restrict__ f2, long long frow, int n,
-                             float scale, unsigned short* __restrict__ feat, unsigned short* __restrict__ target)
+                             float scale, float ws, unsigned short* __restrict__ feat, unsigned short* __restrict__ target)
 {
   const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
   const long long nc = (long long)n * n * n;
@@ -434,11 +454,10 @@ __global__ void k_store_pair(const float* __restrict__ cd, long long crow, const
   const long long xy = cell / n;
   const int y = (int)(xy % n), x = (int)(xy / n);
   const long long ca = xy * crow + z;
-  const long long fa = ((long long)(2 * x) * (2 * n) + 2 * y) * frow + 2 * z;
   feat[cell] = f2bf(asinhf(cd[ca]) * scale);
-  target[cell] = f2bf(f0[fa] - c0[ca]);
-  target[nc + cell] = f2bf(f1[fa] - c1[ca]);
-  target[2 * nc + cell] = f2bf(f2[fa] - c2[ca]);
+  target[cell] = f2bf(restrict_fine(f0, frow, 2 * n, x, y, z, ws) - c0[ca]);
+  target[nc + cell] = f2bf(restrict_fine(f1, frow, 2 * n, x, y, z, ws) - c1[ca]);
+  target[2 * nc + cell] = f2bf(restrict_fine(f2, frow, 2 * n, x, y, z, ws) - c2[ca]);
 }
 
 __global__ void __launch_bounds__(kThreads) k_sumsq(const unsigned short* __restrict__ t, long long n, double* out)
@@ -648,7 +667,7 @@ int resnet_store_pair(ResNet* r, const float* coarse_delta, long long crow, cons
   r->s_target.push_back(target);
   r->s_a.push_back(a);
   hipLaunchKernelGGL(k_store_pair, dim3(grid_cells(nc)), dim3(kThreads), 0, s, coarse_delta, crow, cf[0], cf[1], cf[2],
-                     ff[0], ff[1], ff[2], frow, r->n, r->in_scale, feat, target);
+                     ff[0], ff[1], ff[2], frow, r->n, r->in_scale, r->restrict_side, feat, target);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

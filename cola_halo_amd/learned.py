@@ -6,8 +6,9 @@ held out, 10 COLA steps from a = 0.1):
 
 1. force pairs: each training seed runs plain COLA on the fine grid; before
    every force evaluation the particles are copied to the coarse context and
-   the pair (coarse density, fine force at the coincident cells minus coarse
-   force, a) is stored on the device;
+   the pair (coarse density, fine force restricted to the coarse cells -
+   weights (1/8, 3/4, 1/8) per axis by default - minus coarse force, a) is
+   stored on the device;
 2. training: Adam on the device's BF16 MFMA kernels, one step per pair;
 3. judgement on the held-out seeds: plain fine, plain coarse and corrected
    coarse runs to z = 0; P(k) ratios to the fine run for k <= 1 h/Mpc (shell
@@ -88,6 +89,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--pk-grid", type=int, default=0, help="power-spectrum grid (default: the fine grid)")
+    ap.add_argument("--restriction", type=float, default=0.125, metavar="W",
+                    help="side weight w of the fine-to-coarse restriction (w, 1-2w, w): 0 point values, 0.25 full weighting")
     ap.add_argument("--time-net", type=int, default=0, metavar="N",
                     help="only time the net's forward pass on an N^3 grid and exit")
     args = ap.parse_args()
@@ -104,6 +107,7 @@ def main():
     for c in (fine, coarse):
         c.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
     net = ch.ResidualNet(args.coarse, seed=1)
+    net.set_restriction(args.restriction)
 
     t0 = time.perf_counter()
     for seed in range(1, args.ntrain + args.nval + 1):

@@ -286,9 +286,15 @@ int cola_resnet_time_forward(cola_resnet* net, int reps, double* ms);
 int cola_copy_particles(cola_context* dst, cola_context* src);
 /* Training pair from two contexts holding the same particles: coarse (the
    net's grid) and fine (twice the cells per axis).  Stores asinh(delta) of
-   the coarse density and the residual = fine force at the coincident cells
-   minus the coarse force, with the scale factor a_x of the coarse context. */
+   the coarse density and the residual = fine force restricted to the coarse
+   cells minus the coarse force (see cola_resnet_set_restriction), with the scale
+   factor a_x of the coarse context. */
 int cola_resnet_add_pair(cola_resnet* net, cola_context* coarse, cola_context* fine);
+/* How cola_resnet_add_pair restricts the fine force to a coarse cell: the
+   average over the 3^3 fine neighbours of the coincident fine cell with
+   weights (w, 1 - 2w, w) per axis, 0 <= w <= 1/4 (0: the point value, 1/4:
+   full weighting; default 1/8). */
+int cola_resnet_set_restriction(cola_resnet* net, float side_weight);
 int cola_resnet_num_samples(const cola_resnet* net);
 /* Adam, one step per stored sample, epochs over samples [0, ntrain) in a
    shuffled order; out_scale is set to the targets' RMS on the first call.
