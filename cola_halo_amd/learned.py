@@ -14,7 +14,10 @@ held out, 10 COLA steps from a = 0.1):
    coarse runs to z = 0; P(k) ratios to the fine run for k <= 1 h/Mpc (shell
    estimator on a common grid, CIC window divided out) and FoF mass functions
    (b = 0.2, nfof >= 32); particle-steps/s of the corrected coarse run against
-   the plain fine run.
+   the plain fine run;
+4. optionally (--production-nc 512 --production-box 1000): the trained net,
+   on the same cell size, corrects coarse-grid runs of a larger box, judged
+   against plain fine-grid runs of seeds 1..4 the same way.
 
 Usage: python -m cola_halo_amd.learned [--nc 128 --box 250 --coarse 192
        --fine 384 --ntrain 56 --nval 8 --epochs 8] > result.json
@@ -61,6 +64,71 @@ def timed_run(ctx, seed):
     return ctx.set_snapshot(1.0, Om), dt
 
 
+def judge(runs, seeds, nc, box, pk_n):
+    """Runs (name, context) on each seed with graph-captured steps (one untimed
+    warm-up run per context captures the graphs); P(k) ratios to the "fine"
+    run for k <= 1 h/Mpc on a pk_n^3 grid, FoF mass-function ratios (b = 0.2,
+    nfof >= 32) and particle-steps/s of the step loops."""
+    for name, ctx in runs:
+        ctx.set_step_graph(True)
+        timed_run(ctx, seeds[0])
+    kb = None
+    ratios = {name: [] for name, _ in runs if name != "fine"}
+    mf = {name: [] for name, _ in runs}
+    times = {name: 0.0 for name, _ in runs}
+    ll = float(np.float32(0.2 * np.float32(box) / nc))
+    edges = 32 * 2.0 ** np.arange(0, 12)
+    for seed in seeds:
+        pk = {}
+        for name, ctx in runs:
+            snap, dt = timed_run(ctx, seed)
+            del snap
+            times[name] += dt
+            kb, pk[name], cnt = ctx.power_spectrum(pk_n, 0.0, 1.0, 25, deconvolve=True, source="snapshot")
+            halos = ctx.fof_find_halos(ll, 32)
+            mf[name].append(np.histogram(halos["nfof"], bins=edges)[0])
+        for name in ratios:
+            ratios[name].append(pk[name] / pk["fine"])
+    sel = kb > 0
+    res = {}
+    for name in ratios:
+        r = np.mean(ratios[name], axis=0)
+        res[name] = {"pk_ratio": [round(float(v), 5) for v in r[sel]],
+                     "max_abs_pk_err_k_le_1": round(float(np.abs(r[sel] - 1).max()), 5),
+                     "mass_function_ratio": [round(float(a / b), 4) if b > 0 else None
+                                             for a, b in zip(np.sum(mf[name], axis=0), np.sum(mf["fine"], axis=0))]}
+    nps = nc ** 3 * NSTEPS * len(seeds)
+    out = {"k": [round(float(v), 4) for v in kb[sel]], "judgement": res,
+           "mass_function_bins_nfof": [int(v) for v in edges],
+           "mass_function_fine_counts": [int(v) for v in np.sum(mf["fine"], axis=0)],
+           "particle_steps_per_s": {name: round(nps / t, 1) for name, t in times.items()},
+           "speedup_corrected_vs_fine": round(times["fine"] / times["corrected"], 3)}
+    return out
+
+
+def production(net, args, k, p):
+    """The trained net (same cell size: box / coarse grid) on larger boxes:
+    corrected runs on the coarse grid against plain runs on the fine grid."""
+    nc, box = args.production_nc, args.production_box
+    ng_c = nc * args.coarse // args.nc
+    ng_f = nc * args.fine // args.nc
+    big = ch.ResidualNet(ng_c, seed=1)
+    big.set_params(net.params())
+    big.set_scales(*net.scales())
+    fine = ch.Context(nc, ng_f, box)
+    corrected = ch.Context(nc, ng_c, box)
+    for c in (fine, corrected):
+        c.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
+    corrected.set_residual_net(big)
+    res = {"nc": nc, "box": box, "coarse": ng_c, "fine": ng_f, "seeds": args.production_seeds}
+    res.update(judge((("fine", fine), ("corrected", corrected)), list(range(1, args.production_seeds + 1)), nc, box,
+                     args.production_pk_grid or ng_c))
+    corrected.close()
+    fine.close()
+    big.close()
+    return res
+
+
 def time_net(n, reps=20):
     """Device time of the net's forward pass on an n^3 grid (random weights and
     features): ms per pass and the MFMA rate of the executed products (the
@@ -93,6 +161,11 @@ def main():
                     help="side weight w of the fine-to-coarse restriction (w, 1-2w, w): 0 point values, 0.25 full weighting")
     ap.add_argument("--time-net", type=int, default=0, metavar="N",
                     help="only time the net's forward pass on an N^3 grid and exit")
+    ap.add_argument("--production-nc", type=int, default=0, metavar="NC",
+                    help="after training, also judge the net on NC^3 particles (same cell size and particle spacing)")
+    ap.add_argument("--production-box", type=float, default=1000.0)
+    ap.add_argument("--production-seeds", type=int, default=4)
+    ap.add_argument("--production-pk-grid", type=int, default=0)
     args = ap.parse_args()
     if args.time_net:
         print(json.dumps(time_net(args.time_net)))
@@ -128,50 +201,18 @@ def main():
                        "out_scale": net.scales()[1],
                        "loss_of_zero_correction": 1.0}
 
-    # judgement on the held-out seeds: graph-captured steps for all three runs
-    # (a context of its own for the corrected run, so no graph is re-captured;
-    # one untimed warm-up run per context captures them)
+    # judgement on the held-out seeds
     corrected = ch.Context(nc, args.coarse, box)
     corrected.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
     corrected.set_residual_net(net)
     runs = (("fine", fine), ("coarse", coarse), ("corrected", corrected))
-    for name, ctx in runs:
-        ctx.set_step_graph(True)
-        timed_run(ctx, args.ntrain + 1)
-    kb = None
-    ratios = {"coarse": [], "corrected": []}
-    mf = {"fine": [], "coarse": [], "corrected": []}
-    times = {"fine": 0.0, "coarse": 0.0, "corrected": 0.0}
-    ll = float(np.float32(0.2 * np.float32(box) / nc))
-    edges = 32 * 2.0 ** np.arange(0, 12)
-    for seed in range(args.ntrain + 1, args.ntrain + args.nval + 1):
-        pk = {}
-        for name, ctx in runs:
-            snap, dt = timed_run(ctx, seed)
-            times[name] += dt
-            kb, pk[name], cnt = ctx.power_spectrum(pk_n, 0.0, 1.0, 25, deconvolve=True, source="snapshot")
-            halos = ctx.fof_find_halos(ll, 32)
-            mf[name].append(np.histogram(halos["nfof"], bins=edges)[0])
-        for name in ("coarse", "corrected"):
-            ratios[name].append(pk[name] / pk["fine"])
-    sel = kb > 0
-    res = {}
-    for name in ("coarse", "corrected"):
-        r = np.mean(ratios[name], axis=0)
-        res[name] = {"pk_ratio": [round(float(v), 5) for v in r[sel]],
-                     "max_abs_pk_err_k_le_1": round(float(np.abs(r[sel] - 1).max()), 5),
-                     "mass_function_ratio": [round(float(a / b), 4) if b > 0 else None
-                                             for a, b in zip(np.sum(mf[name], axis=0), np.sum(mf["fine"], axis=0))]}
-    out["k"] = [round(float(v), 4) for v in kb[sel]]
-    out["judgement"] = res
-    out["mass_function_bins_nfof"] = [int(v) for v in edges]
-    nps = nc ** 3 * NSTEPS * args.nval
-    out["particle_steps_per_s"] = {name: round(nps / t, 1) for name, t in times.items()}
-    out["speedup_corrected_vs_fine"] = round(times["fine"] / times["corrected"], 3)
+    out.update(judge(runs, list(range(args.ntrain + 1, args.ntrain + args.nval + 1)), nc, box, pk_n))
     corrected.close()
-    net.close()
     fine.close()
     coarse.close()
+    if args.production_nc:
+        out["production"] = production(net, args, k, p)
+    net.close()
     print(json.dumps(out))
 
 
