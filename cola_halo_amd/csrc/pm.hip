@@ -92,10 +92,11 @@ struct TileSet {
 
 __device__ inline bool tiles_of(float x, float y, float z, const CicGeom& g, const TileGeom& t, TileSet& s)
 {
+  // cells are in [0, ng) and tiles are powers of two: shifts, not divisions
   return cic_with_ghost(x, y, z, g, [&](int i, int j, int k, float) {
-    s.add(s.x, s.nx, i / t.tx);
-    s.add(s.y, s.ny, j / t.ty);
-    s.add(s.z, s.nz, k / t.tz);
+    s.add(s.x, s.nx, i >> t.lx);
+    s.add(s.y, s.ny, j >> t.ly);
+    s.add(s.z, s.nz, k >> t.lz);
   });
 }
 
@@ -123,14 +124,14 @@ __global__ void __launch_bounds__(kThreads) k_wrap(DevParticles p, float L)
 // neighbours usually fall in one tile) elect a leader that applies one atomic
 // for all of them.  delta = +1 counts, -1 claims slots; returns, per lane, the
 // counter value before this lane's unit (for -1: the claimed slot + 1).
-__device__ inline int wave_atomic(int* counters, long long key, bool active, int delta)
+__device__ inline int wave_atomic(int* counters, int key, bool active, int delta)
 {
   const int lane = __lane_id();
   int result = 0;
   unsigned long long pending = __ballot(active);
   while (pending) {
     const int leader = __ffsll((long long)pending) - 1;
-    const long long lkey = __shfl(key, leader);
+    const int lkey = __shfl(key, leader);
     const unsigned long long mask = __ballot(active && key == lkey) & pending;
     const int cnt = __popcll(mask);
     int base = 0;
@@ -145,12 +146,13 @@ __device__ inline int wave_atomic(int* counters, long long key, bool active, int
   return result;
 }
 
-__device__ inline void tile_keys(const TileSet& s, const TileGeom& t, long long keys[8], int& nk)
+// Tile indices fit 32 bits (make_tile_geom rejects more than 2^31 - 1 tiles).
+__device__ inline void tile_keys(const TileSet& s, const TileGeom& t, int keys[8], int& nk)
 {
   nk = 0;
   for (int a = 0; a < s.nx; ++a)
     for (int b = 0; b < s.ny; ++b)
-      for (int c = 0; c < s.nz; ++c) keys[nk++] = ((long long)s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c];
+      for (int c = 0; c < s.nz; ++c) keys[nk++] = (s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c];
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -159,7 +161,7 @@ k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int
 {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   const bool live = i < p.np;
-  long long keys[8];
+  int keys[8];
   int nk = 0;
   if (live) {
     float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
@@ -188,7 +190,7 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
 {
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
   const bool live = i < p.np;
-  long long keys[8];
+  int keys[8];
   int nk = 0;
   float x = 0, y = 0, z = 0;
   if (live) {
@@ -203,7 +205,7 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
   for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
   for (int k = 0; k < kmax; ++k) {
     const bool act = k < nk;
-    const long long key = act ? keys[k] : 0;
+    const int key = act ? keys[k] : 0;
     // counts return to zero for the next step as the slots are claimed
     const int before = wave_atomic(counts, key, act, -1);
     if (act) {
@@ -337,13 +339,32 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
   __syncthreads();
   double s = 0.0;
   const int zmask = t.tz - 1, ymask = t.ty - 1;
-  for (int e = threadIdx.x; e < ncell; e += kThreads) {
-    const int lk = e & zmask;
-    const int lj = (e >> t.lz) & ymask;
-    const int li = e >> (t.lz + t.ly);
-    const float v = (float)(-1.0 + (double)cells[e] * kUnfix);  // pm.c:220 density starts at -1
-    density[((long long)(x0 + li) * g.ng + (y0 + lj)) * g.row + (z0 + lk)] = v;
-    s += (double)v;
+  auto cell_value = [&](int e) {
+    return (float)(-1.0 + (double)cells[e] * kUnfix);  // pm.c:220 density starts at -1
+  };
+  if (t.lz >= 2) {
+    // four consecutive z cells per thread: one 16-byte store (rows start on
+    // 16-byte boundaries and z0 is a multiple of 4)
+    for (int e = threadIdx.x << 2; e < ncell; e += kThreads << 2) {
+      const int lk = e & zmask;
+      const int lj = (e >> t.lz) & ymask;
+      const int li = e >> (t.lz + t.ly);
+      const float4 v = make_float4(cell_value(e), cell_value(e + 1), cell_value(e + 2), cell_value(e + 3));
+      *reinterpret_cast<float4*>(density + ((long long)(x0 + li) * g.ng + (y0 + lj)) * g.row + (z0 + lk)) = v;
+      s += (double)v.x;
+      s += (double)v.y;
+      s += (double)v.z;
+      s += (double)v.w;
+    }
+  } else {
+    for (int e = threadIdx.x; e < ncell; e += kThreads) {
+      const int lk = e & zmask;
+      const int lj = (e >> t.lz) & ymask;
+      const int li = e >> (t.lz + t.ly);
+      const float v = cell_value(e);
+      density[((long long)(x0 + li) * g.ng + (y0 + lj)) * g.row + (z0 + lk)] = v;
+      s += (double)v;
+    }
   }
   s = wave_sum(s);
   if (__lane_id() == 0) wsum[threadIdx.x >> 6] = s;
@@ -588,7 +609,7 @@ bool make_tile_geom(int ng, int tx, int ty, int tz, TileGeom* t)
   t->nty = ng / ty;
   t->ntz = ng / tz;
   t->ntiles = (long long)t->ntx * t->nty * t->ntz;
-  return true;
+  return t->ntiles <= 0x7fffffffLL;
 }
 
 int bin_alloc(BinBuffers* b, const TileGeom& t, long long np)
