@@ -103,6 +103,7 @@ _SIGNATURES = {
     "cola_copy_particles": (C.c_int, [C.c_void_p, C.c_void_p]),
     "cola_resnet_add_pair": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_resnet_num_samples": (C.c_int, [C.c_void_p]),
+    "cola_resnet_get_sample": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "cola_resnet_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint, C.c_void_p]),
     "cola_resnet_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "cola_set_residual_net": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -368,6 +369,15 @@ class ResidualNet:
 
     def add_pair(self, coarse, fine):
         _check(lib().cola_resnet_add_pair(self._h, coarse._h, fine._h))
+
+    def sample(self, i):
+        """Stored training pair i: (feature [n,n,n], target [3,n,n,n], a)."""
+        n = self.n
+        f = np.zeros((n, n, n), np.float32)
+        t = np.zeros((3, n, n, n), np.float32)
+        a = C.c_float()
+        _check(lib().cola_resnet_get_sample(self._h, i, f.ctypes.data, t.ctypes.data, C.byref(a)))
+        return f, t, a.value
 
     def num_samples(self):
         return lib().cola_resnet_num_samples(self._h)

@@ -1669,6 +1669,26 @@ int cola_resnet_add_pair(cola_resnet* h, cola_context* coarse, cola_context* fin
 
 int cola_resnet_num_samples(const cola_resnet* h) { return h ? (int)h->r.s_feat.size() : 0; }
 
+int cola_resnet_get_sample(cola_resnet* h, int i, float* feature, float* target, float* a)
+{
+  if (!h || i < 0 || i >= (int)h->r.s_feat.size() || !feature || !target || !a) return kErrArg;
+  const long long nc = (long long)h->r.n * h->r.n * h->r.n;
+  std::vector<unsigned short> f(nc), t(3 * nc);
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_HIP(hipMemcpy(f.data(), h->r.s_feat[i], sizeof(unsigned short) * nc, hipMemcpyDeviceToHost));
+  COLA_HIP(hipMemcpy(t.data(), h->r.s_target[i], sizeof(unsigned short) * 3 * nc, hipMemcpyDeviceToHost));
+  auto widen = [](unsigned short b) {
+    const unsigned u = (unsigned)b << 16;
+    float v;
+    std::memcpy(&v, &u, sizeof v);
+    return v;
+  };
+  for (long long c = 0; c < nc; ++c) feature[c] = widen(f[c]);
+  for (long long c = 0; c < 3 * nc; ++c) target[c] = widen(t[c]);
+  *a = h->r.s_a[i];
+  return kOk;
+}
+
 int cola_resnet_train(cola_resnet* h, int ntrain, int epochs, float lr, unsigned int seed, double* epoch_loss)
 {
   if (!h) return kErrArg;

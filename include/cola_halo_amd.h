@@ -296,6 +296,10 @@ int cola_resnet_add_pair(cola_resnet* net, cola_context* coarse, cola_context* f
    full weighting; default 1/8). */
 int cola_resnet_set_restriction(cola_resnet* net, float side_weight);
 int cola_resnet_num_samples(const cola_resnet* net);
+/* Stored sample i as fp32 host arrays (tests): feature [n^3] (bf16 asinh(delta)
+   times the input scale), target [3][n^3] (bf16 residual, C order per axis)
+   and its scale factor a. */
+int cola_resnet_get_sample(cola_resnet* net, int i, float* feature, float* target, float* a);
 /* Adam, one step per stored sample, epochs over samples [0, ntrain) in a
    shuffled order; out_scale is set to the targets' RMS on the first call.
    epoch_loss (may be NULL) receives the mean loss per epoch (1 = predicting

@@ -113,3 +113,60 @@ def test_pairs_training_and_corrected_force(eh_table):
         coarse.set_residual_net(None)
         coarse.pm_calculate_forces()
         assert np.array_equal(f_plain, coarse.get_forces())
+
+
+def _bf16(x):
+    """Round float32 values to bfloat16 (nearest even), as float32."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def _restrict(f, w):
+    """Fine grid f [2n]^3 -> coarse cells: weights (w, 1-2w, w) per axis
+    around the coincident fine cell 2i, periodic (numpy, float64)."""
+    f = f.astype(np.float64)
+    wt = (w, 1.0 - 2.0 * w, w)
+    out = 0.0
+    for a in range(3):
+        for b in range(3):
+            for c in range(3):
+                g = np.roll(f, (1 - a, 1 - b, 1 - c), axis=(0, 1, 2))
+                out = out + wt[a] * wt[b] * wt[c] * g[::2, ::2, ::2]
+    return out
+
+
+@pytest.mark.parametrize("w", [0.0, 0.125, 0.25])
+def test_pair_targets_match_restricted_forces(eh_table, w):
+    """Force-pair generator: the stored target is the fine force restricted to
+    the coarse cells (numpy restriction of the fine grids the library left
+    behind) minus the coarse force, the feature is asinh(delta) * in_scale of
+    the coarse density, both rounded to bfloat16."""
+    print(f"pair targets w={w}", flush=True)
+    k, p = eh_table
+    nc, box = 32, 100.0
+    with ch.Context(nc, 64, box) as coarse, ch.Context(nc, 128, box) as fine, ch.ResidualNet(64, seed=3) as net:
+        for c in (coarse, fine):
+            c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+        fine.lpt_set_displacement(0.1, OM, 5)
+        fine.set_times(fine.times()[0], 0.1)
+        for istep in range(1, 4):
+            fine.step(float(np.float32(OM)), (istep + 0.5) * 0.1, (istep + 1.0) * 0.1)
+        coarse.copy_particles_from(fine)
+        net.set_restriction(w)
+        net.add_pair(coarse, fine)
+        fc = [coarse.get_force_grid(a)[:, :, :64] for a in range(3)]
+        ff = [fine.get_force_grid(a)[:, :, :128] for a in range(3)]
+        coarse.pm_assign()
+        delta = coarse.get_density()[:, :, :64]
+        feat, target, a = net.sample(0)
+        in_scale = net.scales()[0]
+    for ax in range(3):
+        want = _restrict(ff[ax], w) - fc[ax].astype(np.float64)
+        scale = np.abs(want).max()
+        err = np.abs(target[ax] - _bf16(want.astype(np.float32))).max()
+        # float32 sums on the device vs float64 here can flip a bfloat16 rounding
+        assert err <= scale * 2.0 ** -8, (ax, err, scale)
+    want_f = _bf16((np.arcsinh(delta.astype(np.float64)) * in_scale).astype(np.float32))
+    assert np.abs(feat - want_f).max() <= np.abs(want_f).max() * 2.0 ** -8
+    assert 0.39 < a < 0.41  # a_x of the coarse context after three steps from 0.1
