@@ -190,8 +190,8 @@ def main():
         ensemble_bench(args)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this build has a single MI355X; "
-              "the multi-GPU design (seed sharding, slab decomposition) is described in DESIGN.md",
+        print("bench.py measures a single GPU: ensembles would shard by seed and one realization by x slabs, "
+              "neither of which this benchmark runs",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1,4 +1,4 @@
-"""Train and judge the learned coarse-to-fine force residual on one GPU.
+"""Train and assess the learned coarse-to-fine force residual on one GPU.
 
 Workflow (defaults: 128^3 particles, box 250 Mpc/h, coarse grid 192^3 =
 factor 1.5, fine grid 384^3 = factor 3, seeds 1-56 for training and 57-64
@@ -10,13 +10,13 @@ held out, 10 COLA steps from a = 0.1):
    weights (1/8, 3/4, 1/8) per axis by default - minus coarse force, a) is
    stored on the device;
 2. training: Adam on the device's BF16 MFMA kernels, one step per pair;
-3. judgement on the held-out seeds: plain fine, plain coarse and corrected
+3. assessment on the held-out seeds: plain fine, plain coarse and corrected
    coarse runs to z = 0; P(k) ratios to the fine run for k <= 1 h/Mpc (shell
    estimator on a common grid, CIC window divided out) and FoF mass functions
    (b = 0.2, nfof >= 32); particle-steps/s of the corrected coarse run against
    the plain fine run;
 4. optionally (--production-nc 512 --production-box 1000): the trained net,
-   on the same cell size, corrects coarse-grid runs of a larger box, judged
+   on the same cell size, corrects coarse-grid runs of a larger box, assessed
    against plain fine-grid runs of seeds 1..4 the same way.
 
 Usage: python -m cola_halo_amd.learned [--nc 128 --box 250 --coarse 192
@@ -64,7 +64,7 @@ def timed_run(ctx, seed):
     return ctx.set_snapshot(1.0, Om), dt
 
 
-def judge(runs, seeds, nc, box, pk_n):
+def assess(runs, seeds, nc, box, pk_n):
     """Runs (name, context) on each seed with graph-captured steps (one untimed
     warm-up run per context captures the graphs); P(k) ratios to the "fine"
     run for k <= 1 h/Mpc on a pk_n^3 grid, FoF mass-function ratios (b = 0.2,
@@ -98,7 +98,7 @@ def judge(runs, seeds, nc, box, pk_n):
                      "mass_function_ratio": [round(float(a / b), 4) if b > 0 else None
                                              for a, b in zip(np.sum(mf[name], axis=0), np.sum(mf["fine"], axis=0))]}
     nps = nc ** 3 * NSTEPS * len(seeds)
-    out = {"k": [round(float(v), 4) for v in kb[sel]], "judgement": res,
+    out = {"k": [round(float(v), 4) for v in kb[sel]], "assessment": res,
            "mass_function_bins_nfof": [int(v) for v in edges],
            "mass_function_fine_counts": [int(v) for v in np.sum(mf["fine"], axis=0)],
            "particle_steps_per_s": {name: round(nps / t, 1) for name, t in times.items()},
@@ -121,7 +121,7 @@ def production(net, args, k, p):
         c.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
     corrected.set_residual_net(big)
     res = {"nc": nc, "box": box, "coarse": ng_c, "fine": ng_f, "seeds": args.production_seeds}
-    res.update(judge((("fine", fine), ("corrected", corrected)), list(range(1, args.production_seeds + 1)), nc, box,
+    res.update(assess((("fine", fine), ("corrected", corrected)), list(range(1, args.production_seeds + 1)), nc, box,
                      args.production_pk_grid or ng_c))
     corrected.close()
     fine.close()
@@ -162,7 +162,7 @@ def main():
     ap.add_argument("--time-net", type=int, default=0, metavar="N",
                     help="only time the net's forward pass on an N^3 grid and exit")
     ap.add_argument("--production-nc", type=int, default=0, metavar="NC",
-                    help="after training, also judge the net on NC^3 particles (same cell size and particle spacing)")
+                    help="after training, also assess the net on NC^3 particles (same cell size and particle spacing)")
     ap.add_argument("--production-box", type=float, default=1000.0)
     ap.add_argument("--production-seeds", type=int, default=4)
     ap.add_argument("--production-pk-grid", type=int, default=0)
@@ -201,12 +201,12 @@ def main():
                        "out_scale": net.scales()[1],
                        "loss_of_zero_correction": 1.0}
 
-    # judgement on the held-out seeds
+    # assessment on the held-out seeds
     corrected = ch.Context(nc, args.coarse, box)
     corrected.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
     corrected.set_residual_net(net)
     runs = (("fine", fine), ("coarse", coarse), ("corrected", corrected))
-    out.update(judge(runs, list(range(args.ntrain + 1, args.ntrain + args.nval + 1)), nc, box, pk_n))
+    out.update(assess(runs, list(range(args.ntrain + 1, args.ntrain + args.nval + 1)), nc, box, pk_n))
     corrected.close()
     fine.close()
     coarse.close()
