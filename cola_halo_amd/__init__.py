@@ -661,6 +661,13 @@ class Context:
         apart from the y pass, -1 = as log2_cols)."""
         _check(lib().cola_set_fft_tiles_xy(self._h, log2_cols, log2_xcols, log2_rows, xcd_group))
 
+    def apply_tuning(self, t):
+        """Launch shapes from a tuning dict (autotune's result, e.g. of another
+        context of the same sizes)."""
+        tx, ty, tz = t["tile"]
+        _check(lib().cola_set_tiling(self._h, tx, ty, tz, t["fft_log2_cols"]))
+        self.set_fft_tiles(t["fft_log2_cols"], t["fft_log2_rows"], t["xcd_group"], t["fft_log2_xcols"])
+
     def autotune(self, reps=3):
         """Pick the fastest deposit tile / FFT launch shapes for the current
         particle state; returns the choice (results do not depend on it)."""
