@@ -65,11 +65,16 @@ def timed_run(ctx, seed):
 
 
 def assess(runs, seeds, nc, box, pk_n):
-    """Runs (name, context) on each seed with graph-captured steps (one untimed
+    """Runs (name, context) on each seed with graph-captured steps and launch
+    shapes autotuned on the first seed's initial conditions (one untimed
     warm-up run per context captures the graphs); P(k) ratios to the "fine"
     run for k <= 1 h/Mpc on a pk_n^3 grid, FoF mass-function ratios (b = 0.2,
     nfof >= 32) and particle-steps/s of the step loops."""
+    tuning = {}
     for name, ctx in runs:
+        ctx.lpt_set_displacement(1.0 / NSTEPS, OM, seeds[0])
+        ctx.set_times(ctx.times()[0], 1.0 / NSTEPS)
+        tuning[name] = ctx.autotune(reps=3)
         ctx.set_step_graph(True)
         timed_run(ctx, seeds[0])
     kb = None
@@ -102,6 +107,7 @@ def assess(runs, seeds, nc, box, pk_n):
            "mass_function_bins_nfof": [int(v) for v in edges],
            "mass_function_fine_counts": [int(v) for v in np.sum(mf["fine"], axis=0)],
            "particle_steps_per_s": {name: round(nps / t, 1) for name, t in times.items()},
+           "tuning": tuning,
            "speedup_corrected_vs_fine": round(times["fine"] / times["corrected"], 3)}
     return out
 
