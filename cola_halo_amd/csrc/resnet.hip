@@ -66,6 +66,40 @@ enum ConvMode { kFwdRelu = 0, kFwdLinear = 1, kBwdData = 2 };
 // (left of | for lane groups 0 and 1, right for 2 and 3).
 constexpr int kConvTy = 8;
 
+// A workgroup (4 waves) owns a block of 2 x 16 cells along z by 2 kConvTy
+// rows along y of one x plane; wave w computes z tile w % 2 and the
+// kConvTy rows of half w / 2.  Its input brick (3 x planes, the rows and the
+// z cells with one halo cell each side, periodic) is staged in LDS by
+// direct global-to-LDS loads (global_load_lds_dwordx4: no registers, all in
+// flight at once), and the waves read their operands from there.
+constexpr int kBlockZ = 32, kBlockY = 2 * kConvTy;
+constexpr int kBrickZ = kBlockZ + 2, kBrickY = kBlockY + 2;
+constexpr int kBrickCells = 3 * kBrickY * kBrickZ;
+
+__device__ inline int brick_cell(int dx, int yy, int zz) { return (dx * kBrickY + yy) * kBrickZ + zz; }
+
+// Stages the brick of block (x, y0, z0): QPC 16-byte quarters per cell (2
+// for 16 channels, 1 for 8).  Chunk c of the brick goes to brick[c]; a
+// wave's 64 consecutive chunks land at its uniform LDS base + 16 lane.
+template <int QPC>
+__device__ inline void load_brick(uint4* brick, const unsigned short* __restrict__ in, int n, int x, int y0, int z0)
+{
+  const int lane = threadIdx.x & 63;
+  for (int base = threadIdx.x & ~63; base < kBrickCells * QPC; base += kThreads) {
+    const int c = base + lane;
+    if (c < kBrickCells * QPC) {
+      const int q = c % QPC, cell = c / QPC;
+      const int zz = cell % kBrickZ, rest = cell / kBrickZ;
+      const int yy = rest % kBrickY, dx = rest / kBrickY;
+      const long long src =
+          ((long long)wrapi(x + dx - 1, n) * n + wrapi(y0 + yy - 1, n)) * n + wrapi(z0 + zz - 1, n);
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(in + src * (8 * QPC) + q * 8),
+                                       (__attribute__((address_space(3))) void*)(brick + base), 16, 0, 0);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // every load of this thread has landed in LDS
+}
+
 __device__ inline uint4 sel4(bool lo, uint4 a, uint4 b)
 {
   return lo ? a : b;
@@ -103,31 +137,33 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
   float bch[4];  // bias of output channels 4g..4g+3
 #pragma unroll
   for (int i = 0; i < 4; ++i) bch[i] = (MODE == kBwdData || 4 * g + i >= cout) ? 0.0f : bias[4 * g + i];
-  const int ch = 8 * (g & 1);
-  const int nzt = n / 16, nyb = n / kConvTy;
-  const long long units = (long long)n * nyb * nzt;
-  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
+  extern __shared__ uint4 brick[];  // [dx][yy][zz][2 halves of 8 channels]
+  const int half = g & 1;
+  const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
+  const int zc = 1 + 16 * wz + r;          // brick z of the lane's cell
+  const int zs = lo ? zc - 1 : zc + 1;     // its z neighbour for the dz = -1 | +1 taps
+  const int nzb = n / kBlockZ, nyb = n / kBlockY;
+  const long long units = (long long)n * nyb * nzb;
   const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-  for (long long u = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); u < units; u += nwaves) {
-    const int zt = (int)(u % nzt);
-    const long long t2 = u / nzt;
-    const int y0 = (int)(t2 % nyb) * kConvTy, x = (int)(t2 / nyb);
-    const int z = zt * 16 + r;
-    const int zsh = lo ? wrapi(z - 1, n) : wrapi(z + 1, n);
-    const int xs[3] = {wrapi(x - 1, n), x, wrapi(x + 1, n)};
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const int zb = (int)(u % nzb);
+    const long long t2 = u / nzb;
+    const int y0 = (int)(t2 % nyb) * kBlockY, x = (int)(t2 / nyb);
+    __syncthreads();  // the previous block's operand reads are done
+    load_brick<2>(brick, in, n, x, y0, zb * kBlockZ);
+    __syncthreads();
     floatx4 acc[kConvTy];
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     uint4 prev[3] = {zero4, zero4, zero4};
 #pragma unroll
-    for (int j = 0; j < kConvTy + 2; ++j) {  // input row y0 - 1 + j feeds output rows j - 2, j - 1, j
-      const int yy = wrapi(y0 - 1 + j, n);
+    for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
+      const int yy = kConvTy * wy + j;
       uint4 cen[3], sh[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const long long rb = ((long long)xs[d] * n + yy) * n;
-        cen[d] = *reinterpret_cast<const uint4*>(in + (rb + z) * 16 + ch);
-        sh[d] = *reinterpret_cast<const uint4*>(in + (rb + zsh) * 16 + ch);
+        cen[d] = brick[brick_cell(d, yy, zc) * 2 + half];
+        sh[d] = brick[brick_cell(d, yy, zs) * 2 + half];
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -148,7 +184,7 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
     }
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) {
-      const long long cell = ((long long)x * n + y0 + t) * n + zt * 16 + r;
+      const long long cell = ((long long)x * n + y0 + kConvTy * wy + t) * n + zb * kBlockZ + 16 * wz + r;
       float v[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) v[i] = acc[t][i] + bch[i];
@@ -189,27 +225,28 @@ k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict_
   float bch[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) bch[i] = bias[4 * g + i];
-  const int nzt = n / 16, nyb = n / kConvTy;
-  const long long units = (long long)n * nyb * nzt;
-  const long long nwaves = (long long)gridDim.x * (kThreads / 64);
-  for (long long u = (long long)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6); u < units; u += nwaves) {
-    const int zt = (int)(u % nzt);
-    const long long t2 = u / nzt;
-    const int y0 = (int)(t2 % nyb) * kConvTy, x = (int)(t2 / nyb);
-    const int z = zt * 16 + r;
-    const int zsh = wrapi(z + (g < 3 ? g - 1 : 0), n);
-    const int xs[3] = {wrapi(x - 1, n), x, wrapi(x + 1, n)};
+  extern __shared__ uint4 brick[];  // [dx][yy][zz], 8 channels per cell
+  const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
+  const int zs = 1 + 16 * wz + r + (g < 3 ? g - 1 : 0);  // brick z of the lane's tap
+  const int nzb = n / kBlockZ, nyb = n / kBlockY;
+  const long long units = (long long)n * nyb * nzb;
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const int zb = (int)(u % nzb);
+    const long long t2 = u / nzb;
+    const int y0 = (int)(t2 % nyb) * kBlockY, x = (int)(t2 / nyb);
+    __syncthreads();  // the previous block's operand reads are done
+    load_brick<1>(brick, in, n, x, y0, zb * kBlockZ);
+    __syncthreads();
     floatx4 acc[kConvTy];
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int j = 0; j < kConvTy + 2; ++j) {  // input row y0 - 1 + j feeds output rows j - 2, j - 1, j
-      const int yy = wrapi(y0 - 1 + j, n);
+    for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
+      const int yy = kConvTy * wy + j;
       uint4 op[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const long long rb = ((long long)xs[d] * n + yy) * n;
-        const uint4 c = *reinterpret_cast<const uint4*>(in + (rb + zsh) * 8);
+        const uint4 c = brick[brick_cell(d, yy, zs)];
         op[d] = g < 3 ? c : make_uint4(0u, 0u, 0u, 0u);
       }
 #pragma unroll
@@ -221,7 +258,7 @@ k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict_
     }
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) {
-      const long long cell = ((long long)x * n + y0 + t) * n + zt * 16 + r;
+      const long long cell = ((long long)x * n + y0 + kConvTy * wy + t) * n + zb * kBlockZ + 16 * wz + r;
       *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
           pack_bf16x4(fmaxf(acc[t][0] + bch[0], 0.0f), fmaxf(acc[t][1] + bch[1], 0.0f),
                       fmaxf(acc[t][2] + bch[2], 0.0f), fmaxf(acc[t][3] + bch[3], 0.0f));
@@ -500,7 +537,7 @@ long long param_offset(int layer)
 
 unsigned grid_cells(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
-constexpr int kConv16Blocks = 768;  // 3 waves per SIMD resident on 256 CUs
+constexpr int kConvBlocks = 512;  // two workgroups per CU on 256 CUs
 constexpr int kWgradBlocks = 512;
 
 template <typename T>
@@ -587,23 +624,28 @@ int resnet_features(ResNet* r, const float* delta, long long row, const float* d
   return kOk;
 }
 
-unsigned conv16_blocks(int n)
+// one workgroup per block of 32 z cells x 2 kConvTy rows, persistent over
+// the blocks (two workgroups resident per CU: the 16-channel brick is 57 KiB)
+unsigned conv_blocks(int n)
 {
-  const long long units = (long long)n * (n / kConvTy) * (n / 16);
-  return (unsigned)std::min<long long>(kConv16Blocks, (units + 3) / 4);
+  const long long units = (long long)n * (n / kBlockY) * (n / kBlockZ);
+  return (unsigned)std::min<long long>(kConvBlocks, units);
 }
+
+constexpr size_t kBrick16Bytes = sizeof(uint4) * 2 * kBrickCells;
+constexpr size_t kBrick8Bytes = sizeof(uint4) * kBrickCells;
 
 int resnet_forward(ResNet* r, hipStream_t s)
 {
   const int n = r->n;
-  const unsigned nb = conv16_blocks(n);
-  hipLaunchKernelGGL(k_conv8, dim3(nb), dim3(kThreads), 0, s, r->d_x, r->d_wf[0],
+  const unsigned nb = conv_blocks(n);
+  hipLaunchKernelGGL(k_conv8, dim3(nb), dim3(kThreads), kBrick8Bytes, s, r->d_x, r->d_wf[0],
                      r->d_param + param_offset(0) + 27 * 8 * 16, n, r->d_h[0]);
-  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), 0, s, r->d_h[0], r->d_wf[1],
+  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), kBrick16Bytes, s, r->d_h[0], r->d_wf[1],
                      r->d_param + param_offset(1) + 27 * 16 * 16, 16, n, r->d_h[1], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), 0, s, r->d_h[1], r->d_wf[2],
+  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), kBrick16Bytes, s, r->d_h[1], r->d_wf[2],
                      r->d_param + param_offset(2) + 27 * 16 * 16, 16, n, r->d_h[2], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv16<kFwdLinear>), dim3(nb), dim3(kThreads), 0, s, r->d_h[2], r->d_wf[3],
+  hipLaunchKernelGGL((k_conv16<kFwdLinear>), dim3(nb), dim3(kThreads), kBrick16Bytes, s, r->d_h[2], r->d_wf[3],
                      r->d_param + param_offset(3) + 27 * 16 * 3, 3, n, nullptr, r->d_y, nullptr);
   COLA_HIP(hipGetLastError());
   return kOk;
@@ -634,8 +676,8 @@ int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStr
                        kNetCin[l], kNetCout[l], r->d_grad + param_offset(l));
     if (l == 0) break;
     // gradient at the output of layer l - 1: data gradient of layer l masked by relu'(h_{l-1})
-    hipLaunchKernelGGL((k_conv16<kBwdData>), dim3(conv16_blocks(n)), dim3(kThreads), 0, s, G, r->d_wb[l], nullptr, 0,
-                       n, Gp, nullptr, r->d_h[l - 1]);
+    hipLaunchKernelGGL((k_conv16<kBwdData>), dim3(conv_blocks(n)), dim3(kThreads), kBrick16Bytes, s, G, r->d_wb[l],
+                       nullptr, 0, n, Gp, nullptr, r->d_h[l - 1]);
     std::swap(G, Gp);
   }
   COLA_HIP(hipGetLastError());
