@@ -1061,7 +1061,7 @@ int set_deposit_tiles(cola_context* c, int tx, int ty, int tz)
 {
   TileGeom t;
   if (!make_tile_geom(c->ng, tx, ty, tz, &t)) {
-    set_error("deposit tile sides must be powers of two dividing ngrid, <= 4096 cells");
+    set_error("deposit tile sides must be powers of two dividing ngrid (x side >= 2), <= 4096 cells");
     return kErrArg;
   }
   if (t.tx == c->tiles.tx && t.ty == c->tiles.ty && t.tz == c->tiles.tz && c->bins.counts) return kOk;

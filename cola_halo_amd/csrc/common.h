@@ -48,6 +48,20 @@ __host__ __device__ inline float2 cmul(float2 a, float2 b)
 __host__ __device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __host__ __device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Workgroups whose data overlap (adjacent FFT column tiles sharing
+// 128-byte lines, lattice rows of particles gathering from the same grid
+// rows) go in groups of G to the same XCD in consecutive dispatch rounds:
+// block 8G q + 8h + x (XCD x, h < G) -> logical block 8G q + G x + h.  A
+// bijection of [0, nb); the last nb mod 8G blocks keep their index.
+__device__ inline int xcd_group_remap(int b, int nb, int G)
+{
+  const int span = 8 * G;
+  if (G <= 1 || b >= nb - nb % span) return b;
+  const int q = b / span, r = b - q * span;
+  return span * q + G * (r & 7) + (r >> 3);
+}
+
 constexpr double kPiD = 3.14159265358979323846;
 
 }  // namespace cola

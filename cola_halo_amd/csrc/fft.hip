@@ -116,18 +116,6 @@ __device__ inline void dft(float2* v)
 
 __device__ inline unsigned fdiv(unsigned x, unsigned d, unsigned mag) { return d == 1 ? x : __umulhi(x, mag); }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-// L2.  Adjacent column tiles share the 128-byte lines they read, so groups of
-// G adjacent tiles go to the same XCD in consecutive dispatch rounds:
-// block 8G q + 8h + x (XCD x, h < G) -> tile 8G q + G x + h.
-__device__ inline int xcd_group_remap(int b, int nb, int G)
-{
-  const int span = 8 * G;
-  if (G <= 1 || b >= nb - nb % span) return b;
-  const int q = b / span, r = b - q * span;
-  return span * q + G * (r & 7) + (r >> 3);
-}
-
 // Signed wave number of index i: the PM convention keeps the Nyquist index
 // at +n/2 (pm.c:325,331), the LPT convention maps it to -n/2 (lpt.c:362-375).
 __device__ inline int wave_pm(int i, int n) { return i <= n / 2 ? i : i - n; }

@@ -78,26 +78,74 @@ __device__ inline bool cic_with_ghost(float x, float y, float z, const CicGeom& 
   return ok;
 }
 
-// Distinct tiles touched by a particle's cells (<= 2 per axis).
-struct TileSet {
-  int x[2], y[2], z[2];
-  int nx = 0, ny = 0, nz = 0;
-  __device__ inline void add(int* a, int& n, int v)
-  {
-    for (int i = 0; i < n; ++i)
-      if (a[i] == v) return;
-    if (n < 2) a[n++] = v;
-  }
+// Tiles touched by a particle's CIC cells, as the lowest and highest tile
+// index per axis.  Along y and z the cells are J, J+1 (wrapped), so at most
+// two tiles.  Along x the cells are iI, iI+1 within [0, ng) and, for a ghost
+// (x > x_right, so X >= ng - 2 up to rounding), the copy's cells, of which
+// only 0 lies in the grid: the particle's own valid cells are ng-2 and ng-1,
+// in one tile whenever tiles are at least 2 cells wide along x
+// (make_tile_geom), and the ghost adds tile 0.  So every axis touches at
+// most two tiles, and {lo, hi} lists them exactly.  Returns false (no tiles)
+// where cic_cells emits nothing (y or z below -1 cell).
+struct TileSpan {
+  int x0, x1, y0, y1, z0, z1;  // x0 > x1: no cell in the grid along x
 };
 
-__device__ inline bool tiles_of(float x, float y, float z, const CicGeom& g, const TileGeom& t, TileSet& s)
+__device__ inline bool tile_span(float x, float y, float z, const CicGeom& g, const TileGeom& t, TileSpan& s)
 {
-  // cells are in [0, ng) and tiles are powers of two: shifts, not divisions
-  return cic_with_ghost(x, y, z, g, [&](int i, int j, int k, float) {
-    s.add(s.x, s.nx, i >> t.lx);
-    s.add(s.y, s.ny, j >> t.ly);
-    s.add(s.z, s.nz, k >> t.lz);
-  });
+  const int ng = g.ng;
+  // the same float expressions as cic_cells, so the same cells
+  const int iI = (int)floorf(x * g.scale);
+  int J = (int)(y * g.scale);
+  int K = (int)(z * g.scale);
+  if (J < 0 || K < 0) {
+    s.x0 = 1;
+    s.x1 = 0;
+    return false;
+  }
+  if (J >= ng) J = 0;
+  if (K >= ng) K = 0;
+  const int J1 = J + 1 >= ng ? 0 : J + 1;
+  const int K1 = K + 1 >= ng ? 0 : K + 1;
+  s.y0 = min(J, J1) >> t.ly;
+  s.y1 = max(J, J1) >> t.ly;
+  s.z0 = min(K, K1) >> t.lz;
+  s.z1 = max(K, K1) >> t.lz;
+  int lo = 0x7fffffff, hi = -1;
+  auto add = [&](int i) {
+    const bool in = 0 <= i && i < ng;
+    lo = in ? min(lo, i >> t.lx) : lo;
+    hi = in ? max(hi, i >> t.lx) : hi;
+  };
+  add(iI);
+  add(iI + 1);
+  if (x > g.x_right) {
+    const int ig = (int)floorf((x + g.shift) * g.scale);
+    add(ig);
+    add(ig + 1);
+  }
+  s.x0 = lo;
+  s.x1 = hi;
+  return true;
+}
+
+// Number of (x, y, z) tile combinations of a span (0 to 8).
+__device__ inline int span_count(const TileSpan& s)
+{
+  if (s.x0 > s.x1) return 0;
+  return (1 + (s.x1 != s.x0)) * (1 + (s.y1 != s.y0)) * (1 + (s.z1 != s.z0));
+}
+
+// Tile index of combination k < span_count(s): z fastest, then y, then x.
+// Tile indices fit 32 bits (make_tile_geom rejects more than 2^31 - 1 tiles).
+__device__ inline int span_key(const TileSpan& s, const TileGeom& t, int k)
+{
+  const bool two_z = s.z1 != s.z0, two_y = s.y1 != s.y0;
+  const int c = two_z && (k & 1) ? s.z1 : s.z0;
+  const int r = two_z ? k >> 1 : k;
+  const int b = two_y && (r & 1) ? s.y1 : s.y0;
+  const int a = (two_y ? r >> 1 : r) ? s.x1 : s.x0;
+  return (a * t.nty + b) * t.ntz + c;
 }
 
 __device__ inline void wrap1(float& x, float L)
@@ -120,50 +168,53 @@ __global__ void __launch_bounds__(kThreads) k_wrap(DevParticles p, float L)
   }
 }
 
-// Wavefront-aggregated counter update: lanes holding the same key (lattice
-// neighbours usually fall in one tile) elect a leader that applies one atomic
-// for all of them.  delta = +1 counts, -1 claims slots; returns, per lane, the
-// counter value before this lane's unit (for -1: the claimed slot + 1).
-__device__ inline int wave_atomic(int* counters, int key, bool active, int delta)
+// Workgroup-aggregated counter updates.  The (tile, particle) entries of a
+// workgroup's particles (lattice neighbours, so mostly a few tiles) are
+// combined in an LDS hash table: one global atomic per distinct tile and
+// workgroup, all issued at once, instead of one per entry.  Entries that
+// find no free slot within kHashProbes probes (more distinct tiles than a
+// workgroup's particles normally touch) take a global atomic of their own.
+constexpr int kHashLog2 = 9;
+constexpr int kHashSlots = 1 << kHashLog2;
+constexpr int kHashProbes = 32;
+constexpr int kHashEmpty = -1;
+
+struct BinHash {
+  int key[kHashSlots];
+  int cnt[kHashSlots];
+};
+
+__device__ inline void hash_clear(BinHash& h)
 {
-  const int lane = __lane_id();
-  int result = 0;
-  unsigned long long pending = __ballot(active);
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const int lkey = __shfl(key, leader);
-    const unsigned long long mask = __ballot(active && key == lkey) & pending;
-    const int cnt = __popcll(mask);
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&counters[lkey], delta * cnt);
-    base = __shfl(base, leader);
-    if ((mask >> lane) & 1ULL) {
-      const int rank = __popcll(mask & ((1ULL << lane) - 1ULL));
-      result = base + delta * rank;
-    }
-    pending &= ~mask;
+  for (int s = threadIdx.x; s < kHashSlots; s += kThreads) {
+    h.key[s] = kHashEmpty;
+    h.cnt[s] = 0;
   }
-  return result;
 }
 
-// Tile indices fit 32 bits (make_tile_geom rejects more than 2^31 - 1 tiles).
-__device__ inline void tile_keys(const TileSet& s, const TileGeom& t, int keys[8], int& nk)
+// Slot holding key (inserted if absent), or -1 after kHashProbes probes.
+__device__ inline int hash_slot(BinHash& h, int key)
 {
-  nk = 0;
-  for (int a = 0; a < s.nx; ++a)
-    for (int b = 0; b < s.ny; ++b)
-      for (int c = 0; c < s.nz; ++c) keys[nk++] = (s.x[a] * t.nty + s.y[b]) * t.ntz + s.z[c];
+  unsigned s = ((unsigned)key * 2654435761u) >> (32 - kHashLog2);
+#pragma unroll 1
+  for (int probe = 0; probe < kHashProbes; ++probe) {
+    const int prev = atomicCAS(&h.key[s], kHashEmpty, key);
+    if (prev == kHashEmpty || prev == key) return (int)s;
+    s = (s + 1) & (kHashSlots - 1);
+  }
+  return -1;
 }
 
 __global__ void __launch_bounds__(kThreads)
 k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
             float L)
 {
+  __shared__ BinHash h;
+  hash_clear(h);
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const bool live = i < p.np;
-  int keys[8];
+  TileSpan ts;
   int nk = 0;
-  if (live) {
+  if (i < p.np) {
     float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
     if (wrap_first) {
       const float x0 = x, y0 = y, z0 = z;
@@ -174,42 +225,73 @@ k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int
       if (y != y0) p.x[1][i] = y;
       if (z != z0) p.x[2][i] = z;
     }
-    TileSet ts;
-    if (!tiles_of(x, y, z, g, t, ts)) atomicOr(overflow, 1);
-    tile_keys(ts, t, keys, nk);
+    if (!tile_span(x, y, z, g, t, ts)) atomicOr(overflow, 1);
+    nk = span_count(ts);
   }
-  // the slot loop is uniform across the wavefront (up to the wave's max)
-  int kmax = nk;
-  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
-  for (int k = 0; k < kmax; ++k) wave_atomic(counts, k < nk ? keys[k] : 0, k < nk, 1);
+  __syncthreads();
+  for (int k = 0; k < nk; ++k) {
+    const int key = span_key(ts, t, k);
+    const int slot = hash_slot(h, key);
+    if (slot >= 0)
+      atomicAdd(&h.cnt[slot], 1);
+    else
+      atomicAdd(&counts[key], 1);
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < kHashSlots; s += kThreads)
+    if (h.key[s] != kHashEmpty) atomicAdd(&counts[h.key[s]], h.cnt[s]);
 }
 
+// Claims the entries' slots by counting each tile's counter back down to zero
+// (ready for the next step): a workgroup takes the cnt slots below the
+// counter's value c, and its entry of rank r in the tile gets position
+// offsets[tile] + c - 1 - r.
 __global__ void __launch_bounds__(kThreads)
 k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, const int* __restrict__ offsets,
               float* __restrict__ sorted, long long capacity, int* __restrict__ overflow)
 {
+  __shared__ BinHash h;
+  __shared__ long long base[kHashSlots];
+  hash_clear(h);
   const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const bool live = i < p.np;
-  int keys[8];
+  TileSpan ts;
   int nk = 0;
   float x = 0, y = 0, z = 0;
-  if (live) {
+  if (i < p.np) {
     x = p.x[0][i];
     y = p.x[1][i];
     z = p.x[2][i];
-    TileSet ts;
-    tiles_of(x, y, z, g, t, ts);
-    tile_keys(ts, t, keys, nk);
+    tile_span(x, y, z, g, t, ts);
+    nk = span_count(ts);
   }
-  int kmax = nk;
-  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
-  for (int k = 0; k < kmax; ++k) {
-    const bool act = k < nk;
-    const int key = act ? keys[k] : 0;
-    // counts return to zero for the next step as the slots are claimed
-    const int before = wave_atomic(counts, key, act, -1);
-    if (act) {
-      const long long pos = (long long)offsets[key] + before - 1;
+  __syncthreads();
+  // slot and rank of each entry (at most 8; static indices keep them in registers)
+  int slot[8], rank[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    slot[k] = -1;
+    rank[k] = 0;
+    if (k < nk) {
+      slot[k] = hash_slot(h, span_key(ts, t, k));
+      if (slot[k] >= 0) rank[k] = atomicAdd(&h.cnt[slot[k]], 1);
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < kHashSlots; s += kThreads) {
+    const int key = h.key[s];
+    if (key != kHashEmpty) base[s] = (long long)offsets[key] + atomicSub(&counts[key], h.cnt[s]) - 1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (k < nk) {
+      long long pos;
+      if (slot[k] >= 0) {
+        pos = base[slot[k]] - rank[k];
+      } else {
+        const int key = span_key(ts, t, k);
+        pos = (long long)offsets[key] + atomicSub(&counts[key], 1) - 1;
+      }
       if (pos < capacity) {
         sorted[3 * pos + 0] = x;
         sorted[3 * pos + 1] = y;
@@ -399,6 +481,13 @@ k_sum_doubles(const double* __restrict__ v, long long n, double* ring, const Ste
 }
 
 // ---- readout (pm.c:359-412) -------------------------------------------------
+// Particles stay in lattice order (a workgroup's 256 particles are a lattice
+// row at nc = 256), and lattice neighbours gather from overlapping grid
+// rows: consecutive workgroups go in groups of 8 to one XCD, so the shared
+// grid lines are read once into that XCD's L2 (256^3 on 768^3: the fused
+// readout 2.0 -> 1.68 ms; groups of 16 and 32 measured the same).
+constexpr int kReadoutGroup = 8;
+
 // One axis of the CIC gather at one position, term order of pm.c:397-410.
 struct Gather3 {
   float f[3];
@@ -469,7 +558,7 @@ __global__ void __launch_bounds__(kThreads)
 k_readout(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
           const float* __restrict__ g2)
 {
-  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, kReadoutGroup) * kThreads + threadIdx.x;
   if (i >= p.np) return;
   const Gather3 f = particle_force(p.x[0][i], p.x[1][i], p.x[2][i], g, g0, g1, g2);
 #pragma unroll
@@ -493,7 +582,7 @@ __global__ void __launch_bounds__(kThreads)
 k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
                      const float* __restrict__ g2, const StepParams* __restrict__ prm)
 {
-  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, kReadoutGroup) * kThreads + threadIdx.x;
   if (i >= p.np) return;
   const KickFactors kf = prm->k;
   const DriftFactors df = prm->d;
@@ -595,7 +684,8 @@ static int log2_exact(int v)
 
 bool make_tile_geom(int ng, int tx, int ty, int tz, TileGeom* t)
 {
-  if (tx <= 0 || ty <= 0 || tz <= 0 || ng % tx || ng % ty || ng % tz) return false;
+  // tx >= 2: a ghost's x cells then span at most two x tiles (tile_span)
+  if (tx < 2 || ty <= 0 || tz <= 0 || ng % tx || ng % ty || ng % tz) return false;
   if ((long long)tx * ty * tz > 4096) return false;
   const int lx = log2_exact(tx), ly = log2_exact(ty), lz = log2_exact(tz);
   if (lx < 0 || ly < 0 || lz < 0) return false;

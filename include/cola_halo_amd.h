@@ -387,8 +387,8 @@ int cola_write_subsample(cola_subsampler* s, const char* filename, const cola_pa
    C order, int n.  Returns 10021 when the file cannot be opened. */
 int cola_write_coarse_grid(const char* filename, float boxsize, int n, const float* delta);
 
-/* Deposit tile shape (cells per tile along x, y, z; each must divide ngrid,
-   product <= 4096) and FFT tile width (log2 of the transforms per
+/* Deposit tile shape (cells per tile along x, y, z: powers of two dividing
+   ngrid, x side >= 2, product <= 4096) and FFT tile width (log2 of the transforms per
    workgroup, -1 = automatic).  Results do not depend on either; the
    autotuner sweeps them for speed. */
 int cola_set_tiling(cola_context* ctx, int tx, int ty, int tz, int fft_log2_tile);

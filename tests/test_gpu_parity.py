@@ -138,6 +138,42 @@ def test_deposit_clustered(eh_table):
     assert np.abs(dens[:, :, :ng] - ref[:, :, :ng]).max() <= 1e-5 * np.abs(ref[:, :, :ng]).max()
 
 
+@pytest.mark.parametrize("clustered", [False, True])
+def test_deposit_tiny_tiles_overflow_workgroup_table(eh_table, clustered):
+    """2x2x2-cell tiles: uniformly placed particles give a workgroup of 256
+    particles ~870 distinct (tile, particle) keys, more than its 512-slot LDS
+    table holds, so many entries take the per-entry global-atomic path; the
+    clustered case repeats keys heavily instead.  The grid must match the
+    default tiling bit for bit and the oracle within float rounding."""
+    nc, ng, box = 16, 48, 64.0
+    _say(f"deposit tiny tiles clustered={clustered}")
+    parts = _random_particles(nc, box, 21, edge_cases=not clustered)
+    if clustered:
+        rng = np.random.default_rng(4)
+        centers = rng.uniform(0, box, (64, 3))
+        parts["x"] = ((centers[rng.integers(0, 64, len(parts))] + rng.normal(0, 1.0, (len(parts), 3))) % box
+                      ).astype(np.float32)
+    wpar = float(np.float32((ng / nc) ** 3))
+    grids = []
+    with _context(nc, ng, box, eh_table) as ctx:
+        for tile in ((8, 8, 16), (2, 2, 2), (2, 1, 1)):
+            ctx.set_tiling(*tile)
+            ctx.set_particles(parts, 0.5, 0.55)
+            try:
+                ctx.pm_assign()
+            except ch.ColaError as e:
+                # dense clusters: see test_deposit_clustered
+                assert clustered and e.code == ch.ERR_MASS, e
+            grids.append(ctx.get_density())
+        with pytest.raises(ch.ColaError):
+            ctx.set_tiling(1, 2, 2)  # x tiles must be at least 2 cells wide
+    for g in grids[1:]:
+        assert np.array_equal(g, grids[0])
+    ref = np.zeros(orc.padded_real_shape(ng), np.float32)
+    orc.lib().orc_pm_deposit(ng, box, wpar, parts.ctypes.data, len(parts), ref.ctypes.data)
+    assert np.abs(grids[0][:, :, :ng] - ref[:, :, :ng]).max() <= 1e-5 * np.abs(ref[:, :, :ng]).max()
+
+
 # ---------------------------------------------------------------- force
 @pytest.mark.parametrize("nc,ng,box", [(16, 48, 64.0), (32, 96, 100.0), (24, 36, 80.0)])
 def test_force_matches_oracle(eh_table, nc, ng, box):
