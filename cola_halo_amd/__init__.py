@@ -96,6 +96,7 @@ _SIGNATURES = {
     "cola_resnet_set_scales": (C.c_int, [C.c_void_p, C.c_float, C.c_float]),
     "cola_resnet_fit_scale": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_forward_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "cola_resnet_forward_features_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_resnet_time_forward": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "cola_resnet_set_restriction": (C.c_int, [C.c_void_p, C.c_float]),
     "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
@@ -347,8 +348,18 @@ class ResidualNet:
         _check(lib().cola_resnet_forward_host(self._h, x.ctypes.data, y.ctypes.data))
         return y
 
+    def forward_features(self, x):
+        """The inference path on x [n,n,n,8]: channel 0 the feature, channel 1
+        the scale factor a in every cell, channels 2-7 zero; the first layer
+        runs on the feature channel with a folded into its bias."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.zeros(x.shape[:3] + (3,), np.float32)
+        _check(lib().cola_resnet_forward_features_host(self._h, x.ctypes.data, y.ctypes.data))
+        return y
+
     def time_forward(self, reps=20):
-        """Mean device time of one forward pass on the current input (ms)."""
+        """Mean device time of one inference forward pass on the feature grid
+        of the last host input (ms)."""
         ms = C.c_double()
         _check(lib().cola_resnet_time_forward(self._h, reps, C.byref(ms)))
         return ms.value

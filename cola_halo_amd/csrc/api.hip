@@ -305,14 +305,11 @@ int pm_fields(cola_context* c)
   KOpParams kp;
   kp.f1 = -1.0 / std::pow((double)c->ng, 3.0) / scale;  // pm.c:317
   float* const out[3] = {c->fgrid[0], c->fgrid[1], c->fgrid[2]};
-  if (c->net) {
-    // the net reads the real-space density before the r2c overwrites it
-    COLA_TRY(resnet_features(c->net, c->density, c->cic.row, &c->d_prm->a, c->stream));
-    COLA_TRY(resnet_forward(c->net, c->stream));
-  }
+  // the net reads the real-space density before the r2c overwrites it
+  if (c->net) COLA_TRY(resnet_features(c->net, c->density, c->cic.row, c->stream));
   c->density_is_real = false;
   COLA_TRY(c->pm_fft.forces(c->density, out, kp, c->stream));
-  if (c->net) COLA_TRY(resnet_add_residual(c->net, out, c->cic.row, c->stream));
+  if (c->net) COLA_TRY(resnet_correct_forces(c->net, &c->d_prm->a, out, c->cic.row, c->stream));
   return kOk;
 }
 
@@ -1554,6 +1551,23 @@ int upload_input(cola_resnet* h, const float* x)
     xb[i] = (unsigned short)(u >> 16);
   }
   COLA_HIP(hipMemcpy(h->r.d_x, xb.data(), xb.size() * 2, hipMemcpyHostToDevice));
+  // the feature path's inputs: channel 0 as the feature grid, channel 1 of
+  // the first cell as a
+  std::vector<unsigned short> f(nc);
+  for (long long i = 0; i < nc; ++i) f[i] = xb[i * 8];
+  COLA_HIP(hipMemcpy(h->r.d_f1, f.data(), f.size() * 2, hipMemcpyHostToDevice));
+  COLA_HIP(hipMemcpy(h->r.d_a, &x[1], sizeof(float), hipMemcpyHostToDevice));
+  return kOk;
+}
+
+int download_output(cola_resnet* h, float* y)
+{
+  const long long nc = (long long)h->r.n * h->r.n * h->r.n;
+  std::vector<float> yd(3 * nc);
+  COLA_HIP(hipMemcpyAsync(yd.data(), h->r.d_y, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost, h->stream));
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  for (long long cell = 0; cell < nc; ++cell)
+    for (int a = 0; a < 3; ++a) y[cell * 3 + a] = yd[a * nc + cell];
   return kOk;
 }
 
@@ -1562,16 +1576,19 @@ int upload_input(cola_resnet* h, const float* x)
 int cola_resnet_forward_host(cola_resnet* h, const float* x, float* y)
 {
   if (!h || !x || !y) return kErrArg;
-  const long long nc = (long long)h->r.n * h->r.n * h->r.n;
   COLA_HIP(hipStreamSynchronize(h->stream));
   COLA_TRY(upload_input(h, x));
   COLA_TRY(resnet_forward(&h->r, h->stream));
-  std::vector<float> yd(3 * nc);
-  COLA_HIP(hipMemcpyAsync(yd.data(), h->r.d_y, sizeof(float) * 3 * nc, hipMemcpyDeviceToHost, h->stream));
+  return download_output(h, y);
+}
+
+int cola_resnet_forward_features_host(cola_resnet* h, const float* x, float* y)
+{
+  if (!h || !x || !y) return kErrArg;
   COLA_HIP(hipStreamSynchronize(h->stream));
-  for (long long cell = 0; cell < nc; ++cell)
-    for (int a = 0; a < 3; ++a) y[cell * 3 + a] = yd[a * nc + cell];
-  return kOk;
+  COLA_TRY(upload_input(h, x));
+  COLA_TRY(resnet_forward_features(&h->r, h->r.d_f1, h->r.d_a, 0.0f, h->stream));
+  return download_output(h, y);
 }
 
 int cola_resnet_set_restriction(cola_resnet* h, float side_weight)
@@ -1590,9 +1607,11 @@ int cola_resnet_time_forward(cola_resnet* h, int reps, double* ms)
     (void)hipEventDestroy(e0);
     return kErrHip;
   }
-  int st = resnet_forward(&h->r, h->stream);  // warm
+  // the inference path: one-channel first layer from the feature grid
+  auto fwd = [&]() { return resnet_forward_features(&h->r, h->r.d_f1, h->r.d_a, 0.0f, h->stream); };
+  int st = fwd();  // warm
   if (!st && hipEventRecord(e0, h->stream) != hipSuccess) st = kErrHip;
-  for (int i = 0; i < reps && !st; ++i) st = resnet_forward(&h->r, h->stream);
+  for (int i = 0; i < reps && !st; ++i) st = fwd();
   float t = 0.0f;
   if (!st && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
               hipEventElapsedTime(&t, e0, e1) != hipSuccess))

@@ -394,23 +394,24 @@ __device__ inline double wave_sum(double v)
 // reference's serial float sum differs from it by float rounding only).
 constexpr float kFix = 4294967296.0f;  // 2^32
 constexpr double kUnfix = 1.0 / 4294967296.0;
+constexpr int kDepositThreads = 256;
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kDepositThreads)
 k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const float* __restrict__ sorted,
                 float* __restrict__ density, double* __restrict__ tile_sums)
 {
   extern __shared__ unsigned long long cells[];
-  __shared__ double wsum[kThreads / 64];
+  __shared__ double wsum[kDepositThreads / 64];
   const long long tile = blockIdx.x;
   const int tz = (int)(tile % t.ntz);
   const int ty = (int)((tile / t.ntz) % t.nty);
   const int tx = (int)(tile / ((long long)t.ntz * t.nty));
   const int x0 = tx << t.lx, y0 = ty << t.ly, z0 = tz << t.lz;
   const int ncell = 1 << (t.lx + t.ly + t.lz);
-  for (int e = threadIdx.x; e < ncell; e += kThreads) cells[e] = 0ULL;
+  for (int e = threadIdx.x; e < ncell; e += kDepositThreads) cells[e] = 0ULL;
   __syncthreads();
   const int beg = offsets[tile], end = offsets[tile + 1];
-  for (int k = beg + threadIdx.x; k < end; k += kThreads) {
+  for (int k = beg + threadIdx.x; k < end; k += kDepositThreads) {
     const float x = sorted[3LL * k + 0], y = sorted[3LL * k + 1], z = sorted[3LL * k + 2];
     cic_with_ghost(x, y, z, g, [&](int i, int j, int kk, float w) {
       const unsigned li = (unsigned)(i - x0), lj = (unsigned)(j - y0), lk = (unsigned)(kk - z0);
@@ -427,7 +428,7 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
   if (t.lz >= 2) {
     // four consecutive z cells per thread: one 16-byte store (rows start on
     // 16-byte boundaries and z0 is a multiple of 4)
-    for (int e = threadIdx.x << 2; e < ncell; e += kThreads << 2) {
+    for (int e = threadIdx.x << 2; e < ncell; e += kDepositThreads << 2) {
       const int lk = e & zmask;
       const int lj = (e >> t.lz) & ymask;
       const int li = e >> (t.lz + t.ly);
@@ -439,7 +440,7 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
       s += (double)v.w;
     }
   } else {
-    for (int e = threadIdx.x; e < ncell; e += kThreads) {
+    for (int e = threadIdx.x; e < ncell; e += kDepositThreads) {
       const int lk = e & zmask;
       const int lj = (e >> t.lz) & ymask;
       const int li = e >> (t.lz + t.ly);
@@ -453,7 +454,7 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
   __syncthreads();
   if (threadIdx.x == 0) {
     double tot = 0.0;
-    for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
+    for (int w = 0; w < kDepositThreads / 64; ++w) tot += wsum[w];
     tile_sums[tile] = tot;
   }
 }
@@ -771,7 +772,7 @@ int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& 
                          const StepParams* prm, hipStream_t s)
 {
   const size_t lds = sizeof(unsigned long long) * t.tx * t.ty * t.tz;
-  hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)t.ntiles), dim3(kThreads), lds, s, g, t, b.offsets, b.sorted,
+  hipLaunchKernelGGL(k_deposit_tiles, dim3((unsigned)t.ntiles), dim3(kDepositThreads), lds, s, g, t, b.offsets, b.sorted,
                      density, b.tile_sums);
   const long long per = (long long)kThreads * 16;
   hipLaunchKernelGGL(k_sum_doubles, dim3((unsigned)((t.ntiles + per - 1) / per)), dim3(kThreads), 0, s, b.tile_sums,

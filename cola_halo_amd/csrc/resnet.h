@@ -30,6 +30,7 @@ struct ResNet {
   unsigned short* d_wb[kNetLayers] = {nullptr, nullptr, nullptr, nullptr};
   // activations: input [n^3][8], h1..h3 [n^3][16] (bf16), output y [3][n^3] fp32
   unsigned short* d_x = nullptr;
+  unsigned short* d_f1 = nullptr;  // feature grid [n^3] (bf16) of the inference path
   unsigned short* d_h[3] = {nullptr, nullptr, nullptr};
   float* d_y = nullptr;
   // gradients at layer outputs [n^3][16] (bf16), two buffers
@@ -53,9 +54,8 @@ int resnet_init(ResNet* r, int n, unsigned seed);
 void resnet_free(ResNet* r);
 // fp32 master parameters -> bf16 MFMA packs
 int resnet_pack(ResNet* r, hipStream_t s);
-// input features from a density grid (rows of `row` floats) and the scale
-// factor *d_a (device memory, so a captured step reads each step's value)
-int resnet_features(ResNet* r, const float* delta, long long row, const float* d_a, hipStream_t s);
+// feature grid d_f1 from a density grid (rows of `row` floats)
+int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s);
 // stores a training pair: features of the coarse density, target = fine force
 // at the coincident points (fine cell 2i) minus coarse force, and a
 int resnet_store_pair(ResNet* r, const float* coarse_delta, long long crow, const float* const coarse_f[3],
@@ -66,12 +66,17 @@ int resnet_eval(ResNet* r, int first, int count, double* loss, hipStream_t s);
 // out_scale = RMS of the targets of samples [0, count)
 int resnet_fit_scale(ResNet* r, int count, hipStream_t s);
 int resnet_train(ResNet* r, int ntrain, int epochs, float lr, unsigned seed, double* epoch_loss, hipStream_t s);
-// forward from d_x; writes d_y (and keeps h1..h3)
+// forward from the 8-channel input d_x; writes d_y (and keeps h1..h3)
 int resnet_forward(ResNet* r, hipStream_t s);
+// the same net from a feature grid and the scale factor (read from device
+// memory at pa when pa is given, so a captured step reads each step's value;
+// else a): writes d_y
+int resnet_forward_features(ResNet* r, const unsigned short* feat, const float* pa, float a, hipStream_t s);
+// inference on d_f1 with the output added to the force grids (rows of `row`
+// floats): out[a] = out[a] + out_scale * y[a]
+int resnet_correct_forces(ResNet* r, const float* pa, float* const out[3], long long row, hipStream_t s);
 // loss against target (bf16 [3][n^3] or fp32 [3][n^3]) and gradient into d_grad
 int resnet_backward(ResNet* r, const unsigned short* target_bf16, const float* target_f32, hipStream_t s);
 int resnet_adam(ResNet* r, float lr, hipStream_t s);
-// out[a] (rows of `row` floats) += out_scale * y[a]
-int resnet_add_residual(const ResNet* r, float* const out[3], long long row, hipStream_t s);
 
 }  // namespace cola

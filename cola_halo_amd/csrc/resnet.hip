@@ -48,7 +48,20 @@ __device__ inline long long cell_at(int x, int y, int z, int t, int sign, int n)
   return ((long long)wrapi(x + sign * dx, n) * n + wrapi(y + sign * dy, n)) * n + wrapi(z + sign * dz, n);
 }
 
-enum ConvMode { kFwdRelu = 0, kFwdLinear = 1, kBwdData = 2 };
+enum ConvMode { kFwdRelu = 0, kFwdLinear = 1, kBwdData = 2, kFwdAdd = 3 };
+
+// Outputs of a convolution by mode: bf16 activations (kFwdRelu; kBwdData
+// masked by relu'(hprev)), the fp32 net output y [3][n^3] (kFwdLinear), or
+// scale * y added in place to three grids with rows of `row` floats
+// (kFwdAdd: the corrected PM force, out[a] = out[a] + out_scale * y[a]).
+struct ConvOut {
+  unsigned short* bf = nullptr;
+  float* f = nullptr;
+  const unsigned short* hprev = nullptr;
+  float* add[3] = {nullptr, nullptr, nullptr};
+  long long row = 0;
+  float scale = 0.0f;
+};
 
 // 16-channel convolution with operand reuse.  A wave computes kConvTy output
 // tiles (16 cells along z x 16 channels) at rows y0 .. y0 + kConvTy - 1 of
@@ -65,39 +78,64 @@ enum ConvMode { kFwdRelu = 0, kFwdLinear = 1, kBwdData = 2 };
 //   chunk 13:                                  taps (+1,+1,0) | none
 // (left of | for lane groups 0 and 1, right for 2 and 3).
 constexpr int kConvTy = 8;
+// Chunk of the first layer's forward pack holding k_conv1's weights (the
+// 8-channel k_conv8 order uses chunks 0-8).
+constexpr int kConv1Chunk = 9;
 
-// A workgroup (4 waves) owns a block of 2 x 16 cells along z by 2 kConvTy
-// rows along y of one x plane; wave w computes z tile w % 2 and the
-// kConvTy rows of half w / 2.  Its input brick (3 x planes, the rows and the
-// z cells with one halo cell each side, periodic) is staged in LDS by
-// direct global-to-LDS loads (global_load_lds_dwordx4: no registers, all in
-// flight at once), and the waves read their operands from there.
+// A workgroup (4 waves) owns a column of blocks of 2 x 16 cells along z by
+// 2 kConvTy rows along y and marches along x through `march` consecutive x
+// planes (a unit); wave w computes z tile w % 2 and the kConvTy rows of half
+// w / 2 of each plane.  Its input planes (the rows and the z cells with one
+// halo cell each side, periodic) live in a ring of 4 LDS slots filled by
+// direct global-to-LDS loads (global_load_lds_dwordx4: no registers):
+// output plane x reads the slots of planes x - 1, x, x + 1 while plane x + 2
+// streams into the fourth, so each input plane is loaded once per unit
+// instead of three times.
 constexpr int kBlockZ = 32, kBlockY = 2 * kConvTy;
 constexpr int kBrickZ = kBlockZ + 2, kBrickY = kBlockY + 2;
-constexpr int kBrickCells = 3 * kBrickY * kBrickZ;
+constexpr int kPlaneCells = kBrickY * kBrickZ;
+constexpr int kRingSlots = 4;
 
-__device__ inline int brick_cell(int dx, int yy, int zz) { return (dx * kBrickY + yy) * kBrickZ + zz; }
+// Ring cell of plane p (p >= -1) at brick row yy and brick z zz.
+__device__ inline int ring_cell(int p, int yy, int zz) { return (((p + kRingSlots) & (kRingSlots - 1)) * kBrickY + yy) * kBrickZ + zz; }
 
-// Stages the brick of block (x, y0, z0): QPC 16-byte quarters per cell (2
-// for 16 channels, 1 for 8).  Chunk c of the brick goes to brick[c]; a
-// wave's 64 consecutive chunks land at its uniform LDS base + 16 lane.
+// Starts the loads of input plane p (periodic) of the column (y0, z0) into
+// its ring slot: QPC 16-byte quarters per cell (2 for 16 channels, 1 for 8).
+// Chunk c of the plane goes to its slot's chunk c; a wave's 64 consecutive
+// chunks land at its uniform LDS base + 16 lane.  The caller waits.
 template <int QPC>
-__device__ inline void load_brick(uint4* brick, const unsigned short* __restrict__ in, int n, int x, int y0, int z0)
+__device__ inline void load_plane(uint4* ring, const unsigned short* __restrict__ in, int n, int p, int y0, int z0)
 {
   const int lane = threadIdx.x & 63;
-  for (int base = threadIdx.x & ~63; base < kBrickCells * QPC; base += kThreads) {
+  uint4* slot = ring + ring_cell(p, 0, 0) * QPC;
+  const long long xoff = (long long)wrapi(p, n) * n;
+  for (int base = threadIdx.x & ~63; base < kPlaneCells * QPC; base += kThreads) {
     const int c = base + lane;
-    if (c < kBrickCells * QPC) {
+    if (c < kPlaneCells * QPC) {
       const int q = c % QPC, cell = c / QPC;
-      const int zz = cell % kBrickZ, rest = cell / kBrickZ;
-      const int yy = rest % kBrickY, dx = rest / kBrickY;
-      const long long src =
-          ((long long)wrapi(x + dx - 1, n) * n + wrapi(y0 + yy - 1, n)) * n + wrapi(z0 + zz - 1, n);
+      const int zz = cell % kBrickZ, yy = cell / kBrickZ;
+      const long long src = (xoff + wrapi(y0 + yy - 1, n)) * n + wrapi(z0 + zz - 1, n);
       __builtin_amdgcn_global_load_lds(static_cast<const void*>(in + src * (8 * QPC) + q * 8),
-                                       (__attribute__((address_space(3))) void*)(brick + base), 16, 0, 0);
+                                       (__attribute__((address_space(3))) void*)(slot + base), 16, 0, 0);
     }
   }
-  __builtin_amdgcn_s_waitcnt(0);  // every load of this thread has landed in LDS
+}
+
+// Unit u of a march: column (x-segment, y block, z block).
+struct ConvUnit {
+  int x0, x1, y0, z0;
+};
+
+__device__ inline ConvUnit conv_unit(long long u, int n, int march)
+{
+  const int nzb = n / kBlockZ, nyb = n / kBlockY;
+  ConvUnit c;
+  c.z0 = (int)(u % nzb) * kBlockZ;
+  const long long t2 = u / nzb;
+  c.y0 = (int)(t2 % nyb) * kBlockY;
+  c.x0 = (int)(t2 / nyb) * march;
+  c.x1 = min(c.x0 + march, n);
+  return c;
 }
 
 __device__ inline uint4 sel4(bool lo, uint4 a, uint4 b)
@@ -126,8 +164,7 @@ __device__ inline uint2 pack_bf16x4(float a, float b, float c, float d)
 template <int MODE>
 __global__ void __launch_bounds__(kThreads)
 k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
-         int cout, int n, unsigned short* __restrict__ out_bf, float* __restrict__ out_f,
-         const unsigned short* __restrict__ hprev)
+         int cout, int n, int march, ConvOut o)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const bool lo = g < 2;
@@ -137,73 +174,84 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
   float bch[4];  // bias of output channels 4g..4g+3
 #pragma unroll
   for (int i = 0; i < 4; ++i) bch[i] = (MODE == kBwdData || 4 * g + i >= cout) ? 0.0f : bias[4 * g + i];
-  extern __shared__ uint4 brick[];  // [dx][yy][zz][2 halves of 8 channels]
+  extern __shared__ uint4 ring[];  // [slot][yy][zz][2 halves of 8 channels]
   const int half = g & 1;
   const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
   const int zc = 1 + 16 * wz + r;          // brick z of the lane's cell
   const int zs = lo ? zc - 1 : zc + 1;     // its z neighbour for the dz = -1 | +1 taps
-  const int nzb = n / kBlockZ, nyb = n / kBlockY;
-  const long long units = (long long)n * nyb * nzb;
+  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
   const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
-    const int zb = (int)(u % nzb);
-    const long long t2 = u / nzb;
-    const int y0 = (int)(t2 % nyb) * kBlockY, x = (int)(t2 / nyb);
-    __syncthreads();  // the previous block's operand reads are done
-    load_brick<2>(brick, in, n, x, y0, zb * kBlockZ);
+    const ConvUnit cu = conv_unit(u, n, march);
+    __syncthreads();  // the previous unit's operand reads are done
+    for (int p = cu.x0 - 1; p <= cu.x0 + 1; ++p) load_plane<2>(ring, in, n, p, cu.y0, cu.z0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    floatx4 acc[kConvTy];
+    for (int x = cu.x0; x < cu.x1; ++x) {
+      if (x + 2 <= cu.x1) load_plane<2>(ring, in, n, x + 2, cu.y0, cu.z0);  // lands during this plane
+      floatx4 acc[kConvTy];
 #pragma unroll
-    for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    uint4 prev[3] = {zero4, zero4, zero4};
+      for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      uint4 prev[3] = {zero4, zero4, zero4};
 #pragma unroll
-    for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
-      const int yy = kConvTy * wy + j;
-      uint4 cen[3], sh[3];
+      for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
+        const int yy = kConvTy * wy + j;
+        uint4 cen[3], sh[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        cen[d] = brick[brick_cell(d, yy, zc) * 2 + half];
-        sh[d] = brick[brick_cell(d, yy, zs) * 2 + half];
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        if (j >= 2) acc[j - 2] = mfma(sh[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
-        if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(sh[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
-        if (j < kConvTy) acc[j] = mfma(sh[d], b[3 * d], acc[j]);                          // dy = -1
-      }
-      if (j >= 1 && j - 1 < kConvTy) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) acc[j - 1] = mfma(sel4(lo, prev[d], cen[d]), b[9 + d], acc[j - 1]);
-      }
-      if (j >= 2) {
-        acc[j - 2] = mfma(sel4(lo, cen[0], cen[1]), b[12], acc[j - 2]);
-        acc[j - 2] = mfma(sel4(lo, cen[2], zero4), b[13], acc[j - 2]);
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) prev[d] = cen[d];
-    }
-#pragma unroll
-    for (int t = 0; t < kConvTy; ++t) {
-      const long long cell = ((long long)x * n + y0 + kConvTy * wy + t) * n + zb * kBlockZ + 16 * wz + r;
-      float v[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = acc[t][i] + bch[i];
-      if (MODE == kFwdRelu) {
-        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
-            pack_bf16x4(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f), fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f));
-      } else if (MODE == kFwdLinear) {
-        if (g == 0) {
-          const long long nc = (long long)n * n * n;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) out_f[i * nc + cell] = v[i];
+        for (int d = 0; d < 3; ++d) {
+          cen[d] = ring[ring_cell(x - 1 + d, yy, zc) * 2 + half];
+          sh[d] = ring[ring_cell(x - 1 + d, yy, zs) * 2 + half];
         }
-      } else {
-        const uint2 hp = *reinterpret_cast<const uint2*>(hprev + cell * 16 + 4 * g);
-        const float m0 = bf2f((unsigned short)(hp.x & 0xffffu)), m1 = bf2f((unsigned short)(hp.x >> 16));
-        const float m2 = bf2f((unsigned short)(hp.y & 0xffffu)), m3 = bf2f((unsigned short)(hp.y >> 16));
-        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
-            pack_bf16x4(m0 > 0.0f ? v[0] : 0.0f, m1 > 0.0f ? v[1] : 0.0f, m2 > 0.0f ? v[2] : 0.0f,
-                        m3 > 0.0f ? v[3] : 0.0f);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (j >= 2) acc[j - 2] = mfma(sh[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
+          if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(sh[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
+          if (j < kConvTy) acc[j] = mfma(sh[d], b[3 * d], acc[j]);                          // dy = -1
+        }
+        if (j >= 1 && j - 1 < kConvTy) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) acc[j - 1] = mfma(sel4(lo, prev[d], cen[d]), b[9 + d], acc[j - 1]);
+        }
+        if (j >= 2) {
+          acc[j - 2] = mfma(sel4(lo, cen[0], cen[1]), b[12], acc[j - 2]);
+          acc[j - 2] = mfma(sel4(lo, cen[2], zero4), b[13], acc[j - 2]);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) prev[d] = cen[d];
+      }
+      // plane x + 2 (and the previous plane's stores) complete; every wave is
+      // past its reads of plane x - 1, whose slot the next prefetch takes
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) {
+        const long long cell = ((long long)x * n + cu.y0 + kConvTy * wy + t) * n + cu.z0 + 16 * wz + r;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = acc[t][i] + bch[i];
+        if (MODE == kFwdRelu) {
+          *reinterpret_cast<uint2*>(o.bf + cell * 16 + 4 * g) =
+              pack_bf16x4(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f), fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f));
+        } else if (MODE == kFwdLinear) {
+          if (g == 0) {
+            const long long nc = (long long)n * n * n;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) o.f[i * nc + cell] = v[i];
+          }
+        } else if (MODE == kFwdAdd) {
+          if (g == 0) {
+            const long long at = (cell / n) * o.row + cell % n;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) o.add[i][at] = o.add[i][at] + o.scale * v[i];
+          }
+        } else {
+          const uint2 hp = *reinterpret_cast<const uint2*>(o.hprev + cell * 16 + 4 * g);
+          const float m0 = bf2f((unsigned short)(hp.x & 0xffffu)), m1 = bf2f((unsigned short)(hp.x >> 16));
+          const float m2 = bf2f((unsigned short)(hp.y & 0xffffu)), m3 = bf2f((unsigned short)(hp.y >> 16));
+          *reinterpret_cast<uint2*>(o.bf + cell * 16 + 4 * g) =
+              pack_bf16x4(m0 > 0.0f ? v[0] : 0.0f, m1 > 0.0f ? v[1] : 0.0f, m2 > 0.0f ? v[2] : 0.0f,
+                          m3 > 0.0f ? v[3] : 0.0f);
+        }
       }
     }
   }
@@ -216,7 +264,7 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
 // (dx,dy,+1) | none.
 __global__ void __launch_bounds__(kThreads)
 k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
-        int n, unsigned short* __restrict__ out_bf)
+        int n, int march, unsigned short* __restrict__ out_bf)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   uint4 b[9];
@@ -225,43 +273,144 @@ k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict_
   float bch[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) bch[i] = bias[4 * g + i];
-  extern __shared__ uint4 brick[];  // [dx][yy][zz], 8 channels per cell
+  extern __shared__ uint4 ring[];  // [slot][yy][zz], 8 channels per cell
   const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
   const int zs = 1 + 16 * wz + r + (g < 3 ? g - 1 : 0);  // brick z of the lane's tap
-  const int nzb = n / kBlockZ, nyb = n / kBlockY;
-  const long long units = (long long)n * nyb * nzb;
+  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
-    const int zb = (int)(u % nzb);
-    const long long t2 = u / nzb;
-    const int y0 = (int)(t2 % nyb) * kBlockY, x = (int)(t2 / nyb);
-    __syncthreads();  // the previous block's operand reads are done
-    load_brick<1>(brick, in, n, x, y0, zb * kBlockZ);
+    const ConvUnit cu = conv_unit(u, n, march);
+    __syncthreads();  // the previous unit's operand reads are done
+    for (int p = cu.x0 - 1; p <= cu.x0 + 1; ++p) load_plane<1>(ring, in, n, p, cu.y0, cu.z0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    floatx4 acc[kConvTy];
+    for (int x = cu.x0; x < cu.x1; ++x) {
+      if (x + 2 <= cu.x1) load_plane<1>(ring, in, n, x + 2, cu.y0, cu.z0);  // lands during this plane
+      floatx4 acc[kConvTy];
 #pragma unroll
-    for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
-      const int yy = kConvTy * wy + j;
-      uint4 op[3];
+      for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
+        const int yy = kConvTy * wy + j;
+        uint4 op[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const uint4 c = brick[brick_cell(d, yy, zs)];
-        op[d] = g < 3 ? c : make_uint4(0u, 0u, 0u, 0u);
+        for (int d = 0; d < 3; ++d) {
+          const uint4 c = ring[ring_cell(x - 1 + d, yy, zs)];
+          op[d] = g < 3 ? c : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (j >= 2) acc[j - 2] = mfma(op[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
+          if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(op[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
+          if (j < kConvTy) acc[j] = mfma(op[d], b[3 * d], acc[j]);                          // dy = -1
+        }
       }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        if (j >= 2) acc[j - 2] = mfma(op[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
-        if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(op[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
-        if (j < kConvTy) acc[j] = mfma(op[d], b[3 * d], acc[j]);                          // dy = -1
+      for (int t = 0; t < kConvTy; ++t) {
+        const long long cell = ((long long)x * n + cu.y0 + kConvTy * wy + t) * n + cu.z0 + 16 * wz + r;
+        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
+            pack_bf16x4(fmaxf(acc[t][0] + bch[0], 0.0f), fmaxf(acc[t][1] + bch[1], 0.0f),
+                        fmaxf(acc[t][2] + bch[2], 0.0f), fmaxf(acc[t][3] + bch[3], 0.0f));
       }
     }
+  }
+}
+
+// First layer from the feature channel alone.  The first layer's input is 8
+// channels: the feature, a (the same in every cell) and six zeros.  A
+// constant channel of a periodic convolution adds a * sum_t W[t][1][co] to
+// every cell, so it joins the bias, and the feature's 27 taps fill one K
+// chunk of 32 (taps 27-31 zero): one MFMA per tile instead of nine, and 2
+// bytes per input cell instead of 16.  Lane group g gathers taps 8g .. 8g+7
+// of its cell from a ring of bf16 feature planes (the march of k_conv16);
+// the planes stream through registers, the loads of plane x + 2 in flight
+// during plane x's products.  a comes from device memory (pa) when given.
+__global__ void __launch_bounds__(kThreads)
+k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__ wpk, const float* __restrict__ w,
+        const float* __restrict__ bias, const float* __restrict__ pa, float a_host, int n, int march,
+        unsigned short* __restrict__ out_bf)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const uint4 b = *reinterpret_cast<const uint4*>(wpk + ((long long)kConv1Chunk * 64 + lane) * 8);
+  // bias + a * sum_t W[t][1][co], with a and W rounded to bf16 as the
+  // 8-channel layer sees them (the products are exact in fp32)
+  const float abf = bf2f(f2bf(pa ? *pa : a_host));
+  float bch[4];
 #pragma unroll
-    for (int t = 0; t < kConvTy; ++t) {
-      const long long cell = ((long long)x * n + y0 + kConvTy * wy + t) * n + zb * kBlockZ + 16 * wz + r;
-      *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
-          pack_bf16x4(fmaxf(acc[t][0] + bch[0], 0.0f), fmaxf(acc[t][1] + bch[1], 0.0f),
-                      fmaxf(acc[t][2] + bch[2], 0.0f), fmaxf(acc[t][3] + bch[3], 0.0f));
+  for (int i = 0; i < 4; ++i) {
+    float sa = 0.0f;
+    for (int t = 0; t < 27; ++t) sa += abf * bf2f(f2bf(w[(t * 8 + 1) * 16 + 4 * g + i]));
+    bch[i] = bias[4 * g + i] + sa;
+  }
+  __shared__ unsigned short ring[kRingSlots * kPlaneCells];
+  const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
+  const int zc = 1 + 16 * wz + r;
+  // per tap slot j of this lane: plane offset dx, and the ring offset of (dy, dz)
+  int tdx[8], toff[8];
+  bool tval[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int t = 8 * g + j;
+    tval[j] = t < 27;
+    tdx[j] = tval[j] ? t / 9 - 1 : 0;
+    toff[j] = tval[j] ? ((t / 3) % 3 - 1) * kBrickZ + (t % 3 - 1) : 0;
+  }
+  constexpr int kPer = (kPlaneCells + kThreads - 1) / kThreads;  // cells per thread and plane
+  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const ConvUnit cu = conv_unit(u, n, march);
+    auto fetch = [&](int p, unsigned short (&v)[kPer]) {
+      const long long xoff = (long long)wrapi(p, n) * n;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int c = threadIdx.x + k * kThreads;
+        if (c < kPlaneCells) {
+          const int zz = c % kBrickZ, yy = c / kBrickZ;
+          v[k] = f[(xoff + wrapi(cu.y0 + yy - 1, n)) * n + wrapi(cu.z0 + zz - 1, n)];
+        }
+      }
+    };
+    auto put = [&](int p, const unsigned short (&v)[kPer]) {
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int c = threadIdx.x + k * kThreads;
+        if (c < kPlaneCells) ring[ring_cell(p, 0, 0) + c] = v[k];
+      }
+    };
+    __syncthreads();  // the previous unit's reads are done
+    for (int p = cu.x0 - 1; p <= cu.x0 + 1; ++p) {
+      unsigned short v[kPer];
+      fetch(p, v);
+      put(p, v);
+    }
+    __syncthreads();
+    for (int x = cu.x0; x < cu.x1; ++x) {
+      const bool pre = x + 2 <= cu.x1;
+      unsigned short nxt[kPer];
+      if (pre) fetch(x + 2, nxt);
+      int base[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) base[j] = ring_cell(x + tdx[j], kConvTy * wy + 1, zc) + toff[j];
+      floatx4 acc[kConvTy];
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) {
+        unsigned short v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = tval[j] ? ring[base[j] + t * kBrickZ] : (unsigned short)0;
+        const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
+                                    v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
+        acc[t] = mfma(op, b, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+      }
+      if (pre) put(x + 2, nxt);  // slot of plane x - 2, free since the last barrier
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) {
+        const long long cell = ((long long)x * n + cu.y0 + kConvTy * wy + t) * n + cu.z0 + 16 * wz + r;
+        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
+            pack_bf16x4(fmaxf(acc[t][0] + bch[0], 0.0f), fmaxf(acc[t][1] + bch[1], 0.0f),
+                        fmaxf(acc[t][2] + bch[2], 0.0f), fmaxf(acc[t][3] + bch[3], 0.0f));
+      }
     }
   }
 }
@@ -365,6 +514,13 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned 
         const float v = (g < 3 && col < cout) ? w[((long long)tap * 8 + j) * cout + col] : 0.0f;
         wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
       }
+    // k_conv1: the feature channel's taps t = 8g + j in one chunk
+    if (c == kConv1Chunk)
+      for (int j = 0; j < 8; ++j) {
+        const int tap = 8 * g + j;
+        const float v = (tap < 27 && col < cout) ? w[((long long)tap * 8) * cout + col] : 0.0f;
+        wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
+      }
   } else {
     const int tap = conv16_tap(c, g >> 1);
     for (int j = 0; j < 8; ++j) {
@@ -383,20 +539,21 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned 
   }
 }
 
-// Input features: channel 0 = asinh(delta) * scale, channel 1 = a, rest 0.
-__global__ void k_features(const float* __restrict__ delta, long long row, int n, const float* __restrict__ pa,
-                           float scale, unsigned short* __restrict__ x)
+// Feature of each cell: asinh(delta) * scale in bf16 (the first layer's
+// channel 0; channel 1, a, enters k_conv1 through its bias).
+__global__ void k_features(const float* __restrict__ delta, long long row, int n, float scale,
+                           unsigned short* __restrict__ f)
 {
   const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
   const long long nc = (long long)n * n * n;
   if (cell >= nc) return;
   const long long line = cell / n;
-  const float d = delta[line * row + cell % n];
-  unsigned short v[8] = {f2bf(asinhf(d) * scale), f2bf(*pa), 0, 0, 0, 0, 0, 0};
-  *reinterpret_cast<uint4*>(x + cell * 8) = *reinterpret_cast<uint4*>(v);
+  f[cell] = f2bf(asinhf(delta[line * row + cell % n]) * scale);
 }
 
-// Features of a stored sample (bf16 asinh(delta) * scale) and its a.
+// 8-channel first-layer input of a stored sample (the weight gradient's
+// operand): channel 0 its feature (bf16 asinh(delta) * scale), channel 1 its
+// a (bf16), rest 0.
 __global__ void k_features_stored(const unsigned short* __restrict__ f, int n, float a, unsigned short* __restrict__ x)
 {
   const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
@@ -516,18 +673,6 @@ __global__ void __launch_bounds__(kThreads) k_sumsq(const unsigned short* __rest
   }
 }
 
-__global__ void k_add_residual(const float* __restrict__ y, int n, float scale, float* __restrict__ o0,
-                               float* __restrict__ o1, float* __restrict__ o2, long long row)
-{
-  const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const long long nc = (long long)n * n * n;
-  if (cell >= nc) return;
-  const long long at = (cell / n) * row + cell % n;
-  o0[at] += scale * y[cell];
-  o1[at] += scale * y[nc + cell];
-  o2[at] += scale * y[2 * nc + cell];
-}
-
 long long param_offset(int layer)
 {
   long long off = 0;
@@ -554,8 +699,11 @@ int dev_alloc(T** p, long long count)
 
 }  // namespace
 
+int conv_lds_attributes();
+
 int resnet_init(ResNet* r, int n, unsigned seed)
 {
+  COLA_TRY(conv_lds_attributes());
   if (n < 32 || n % 32 != 0) {
     set_error("learned residual: the grid must be a multiple of 32 cells per axis");
     return kErrArg;
@@ -571,6 +719,7 @@ int resnet_init(ResNet* r, int n, unsigned seed)
     COLA_TRY(dev_alloc(&r->d_wb[l], 14LL * 64 * 8));
   }
   COLA_TRY(dev_alloc(&r->d_x, nc * 8));
+  COLA_TRY(dev_alloc(&r->d_f1, nc));
   for (int l = 0; l < 3; ++l) COLA_TRY(dev_alloc(&r->d_h[l], nc * 16));
   COLA_TRY(dev_alloc(&r->d_y, 3 * nc));
   COLA_TRY(dev_alloc(&r->d_loss, 1));
@@ -597,7 +746,7 @@ void resnet_free(ResNet* r)
   };
   fr(r->d_param), fr(r->d_m), fr(r->d_v), fr(r->d_grad);
   for (int l = 0; l < kNetLayers; ++l) fr(r->d_wf[l]), fr(r->d_wb[l]);
-  fr(r->d_x), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a);
+  fr(r->d_x), fr(r->d_f1), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a);
   for (int l = 0; l < 3; ++l) fr(r->d_h[l]);
   for (int l = 0; l < 2; ++l) fr(r->d_g[l]);
   for (auto p : r->s_feat) fr(p);
@@ -615,40 +764,120 @@ int resnet_pack(ResNet* r, hipStream_t s)
   return kOk;
 }
 
-int resnet_features(ResNet* r, const float* delta, long long row, const float* d_a, hipStream_t s)
+int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s)
 {
   const long long nc = (long long)r->n * r->n * r->n;
-  hipLaunchKernelGGL(k_features, dim3(grid_cells(nc)), dim3(kThreads), 0, s, delta, row, r->n, d_a, r->in_scale,
-                     r->d_x);
+  hipLaunchKernelGGL(k_features, dim3(grid_cells(nc)), dim3(kThreads), 0, s, delta, row, r->n, r->in_scale, r->d_f1);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
 
-// one workgroup per block of 32 z cells x 2 kConvTy rows, persistent over
-// the blocks (two workgroups resident per CU: the 16-channel brick is 57 KiB)
-unsigned conv_blocks(int n)
+// Persistent workgroups (two resident per CU: the 16-channel ring is 77 KiB)
+// over units of `march` x planes of one (y, z) block column.  The march is
+// the longest that still gives every workgroup about four units (load
+// balance); its two halo planes are the only planes loaded twice.
+struct ConvLaunch {
+  unsigned blocks;
+  int march;
+};
+
+ConvLaunch conv_launch(int n)
 {
-  const long long units = (long long)n * (n / kBlockY) * (n / kBlockZ);
-  return (unsigned)std::min<long long>(kConvBlocks, units);
+  const long long cols = (long long)(n / kBlockY) * (n / kBlockZ);
+  const long long want = 4LL * kConvBlocks;
+  const long long nseg = std::min<long long>(n, std::max<long long>(1, (want + cols - 1) / cols));
+  ConvLaunch c;
+  c.march = (int)((n + nseg - 1) / nseg);
+  const long long units = (long long)((n + c.march - 1) / c.march) * cols;
+  c.blocks = (unsigned)std::min<long long>(kConvBlocks, units);
+  return c;
 }
 
-constexpr size_t kBrick16Bytes = sizeof(uint4) * 2 * kBrickCells;
-constexpr size_t kBrick8Bytes = sizeof(uint4) * kBrickCells;
+constexpr size_t kRing16Bytes = sizeof(uint4) * 2 * kRingSlots * kPlaneCells;
+constexpr size_t kRing8Bytes = sizeof(uint4) * kRingSlots * kPlaneCells;
+
+// Dynamic LDS above 64 KiB must be allowed per kernel once.
+int conv_lds_attributes()
+{
+  static bool done = false;
+  if (done) return kOk;
+  const void* k16[] = {reinterpret_cast<const void*>(&k_conv16<kFwdRelu>),
+                       reinterpret_cast<const void*>(&k_conv16<kFwdLinear>),
+                       reinterpret_cast<const void*>(&k_conv16<kBwdData>),
+                       reinterpret_cast<const void*>(&k_conv16<kFwdAdd>)};
+  for (const void* k : k16) COLA_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRing16Bytes));
+  COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv8), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kRing8Bytes));
+  done = true;
+  return kOk;
+}
+
+namespace {
+
+// layers 2 and 3 (h1 -> h2 -> h3), then layer 4 into `o` by mode
+int conv_layers_2_to_4(ResNet* r, int mode4, const ConvOut& o4, hipStream_t s)
+{
+  const int n = r->n;
+  const ConvLaunch cl = conv_launch(n);
+  ConvOut o;
+  o.bf = r->d_h[1];
+  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[0], r->d_wf[1],
+                     r->d_param + param_offset(1) + 27 * 16 * 16, 16, n, cl.march, o);
+  o.bf = r->d_h[2];
+  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[1], r->d_wf[2],
+                     r->d_param + param_offset(2) + 27 * 16 * 16, 16, n, cl.march, o);
+  const float* b4 = r->d_param + param_offset(3) + 27 * 16 * 3;
+  if (mode4 == kFwdAdd)
+    hipLaunchKernelGGL((k_conv16<kFwdAdd>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[2], r->d_wf[3],
+                       b4, 3, n, cl.march, o4);
+  else
+    hipLaunchKernelGGL((k_conv16<kFwdLinear>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[2],
+                       r->d_wf[3], b4, 3, n, cl.march, o4);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int conv_layer_1(ResNet* r, const unsigned short* feat, const float* pa, float a, hipStream_t s)
+{
+  const ConvLaunch cl = conv_launch(r->n);
+  hipLaunchKernelGGL(k_conv1, dim3(cl.blocks), dim3(kThreads), 0, s, feat, r->d_wf[0], r->d_param + param_offset(0),
+                     r->d_param + param_offset(0) + 27 * 8 * 16, pa, a, r->n, cl.march, r->d_h[0]);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace
 
 int resnet_forward(ResNet* r, hipStream_t s)
 {
   const int n = r->n;
-  const unsigned nb = conv_blocks(n);
-  hipLaunchKernelGGL(k_conv8, dim3(nb), dim3(kThreads), kBrick8Bytes, s, r->d_x, r->d_wf[0],
-                     r->d_param + param_offset(0) + 27 * 8 * 16, n, r->d_h[0]);
-  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), kBrick16Bytes, s, r->d_h[0], r->d_wf[1],
-                     r->d_param + param_offset(1) + 27 * 16 * 16, 16, n, r->d_h[1], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(nb), dim3(kThreads), kBrick16Bytes, s, r->d_h[1], r->d_wf[2],
-                     r->d_param + param_offset(2) + 27 * 16 * 16, 16, n, r->d_h[2], nullptr, nullptr);
-  hipLaunchKernelGGL((k_conv16<kFwdLinear>), dim3(nb), dim3(kThreads), kBrick16Bytes, s, r->d_h[2], r->d_wf[3],
-                     r->d_param + param_offset(3) + 27 * 16 * 3, 3, n, nullptr, r->d_y, nullptr);
-  COLA_HIP(hipGetLastError());
-  return kOk;
+  COLA_TRY(conv_lds_attributes());
+  const ConvLaunch cl = conv_launch(n);
+  hipLaunchKernelGGL(k_conv8, dim3(cl.blocks), dim3(kThreads), kRing8Bytes, s, r->d_x, r->d_wf[0],
+                     r->d_param + param_offset(0) + 27 * 8 * 16, n, cl.march, r->d_h[0]);
+  ConvOut o;
+  o.f = r->d_y;
+  return conv_layers_2_to_4(r, kFwdLinear, o, s);
+}
+
+int resnet_forward_features(ResNet* r, const unsigned short* feat, const float* pa, float a, hipStream_t s)
+{
+  COLA_TRY(conv_lds_attributes());
+  COLA_TRY(conv_layer_1(r, feat, pa, a, s));
+  ConvOut o;
+  o.f = r->d_y;
+  return conv_layers_2_to_4(r, kFwdLinear, o, s);
+}
+
+int resnet_correct_forces(ResNet* r, const float* pa, float* const out[3], long long row, hipStream_t s)
+{
+  COLA_TRY(conv_lds_attributes());
+  COLA_TRY(conv_layer_1(r, r->d_f1, pa, 0.0f, s));
+  ConvOut o;
+  for (int a = 0; a < 3; ++a) o.add[a] = out[a];
+  o.row = row;
+  o.scale = r->out_scale;
+  return conv_layers_2_to_4(r, kFwdAdd, o, s);
 }
 
 int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStream_t s)
@@ -660,6 +889,7 @@ int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStr
     COLA_TRY(dev_alloc(&r->d_g[1], nc * 16));
     COLA_TRY(dev_alloc(&r->d_wpart, (long long)r->wpart_waves * (27 * 256 + 16)));
   }
+  COLA_TRY(conv_lds_attributes());
   hipLaunchKernelGGL(k_zero1, dim3(1), dim3(64), 0, s, r->d_loss);
   hipLaunchKernelGGL(k_loss, dim3(std::min<long long>(4096, grid_cells(nc))), dim3(kThreads), 0, s, r->d_y, tb, tf,
                      n, 1.0f / r->out_scale, r->d_g[0], r->d_loss);
@@ -676,8 +906,12 @@ int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStr
                        kNetCin[l], kNetCout[l], r->d_grad + param_offset(l));
     if (l == 0) break;
     // gradient at the output of layer l - 1: data gradient of layer l masked by relu'(h_{l-1})
-    hipLaunchKernelGGL((k_conv16<kBwdData>), dim3(conv_blocks(n)), dim3(kThreads), kBrick16Bytes, s, G, r->d_wb[l],
-                       nullptr, 0, n, Gp, nullptr, r->d_h[l - 1]);
+    const ConvLaunch cl = conv_launch(n);
+    ConvOut o;
+    o.bf = Gp;
+    o.hprev = r->d_h[l - 1];
+    hipLaunchKernelGGL((k_conv16<kBwdData>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, G, r->d_wb[l],
+                       nullptr, 0, n, cl.march, o);
     std::swap(G, Gp);
   }
   COLA_HIP(hipGetLastError());
@@ -719,9 +953,10 @@ namespace {
 int run_sample(ResNet* r, int i, bool train, float lr, double* loss, hipStream_t s)
 {
   const long long nc = (long long)r->n * r->n * r->n;
+  // the 8-channel input is the first layer's weight-gradient operand
   hipLaunchKernelGGL(k_features_stored, dim3(grid_cells(nc)), dim3(kThreads), 0, s, r->s_feat[i], r->n, r->s_a[i],
                      r->d_x);
-  COLA_TRY(resnet_forward(r, s));
+  COLA_TRY(resnet_forward_features(r, r->s_feat[i], nullptr, r->s_a[i], s));
   COLA_TRY(resnet_backward(r, r->s_target[i], nullptr, s));
   if (train) COLA_TRY(resnet_adam(r, lr, s));
   COLA_HIP(hipMemcpyAsync(loss, r->d_loss, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -790,13 +1025,6 @@ int resnet_train(ResNet* r, int ntrain, int epochs, float lr, unsigned seed, dou
   return kOk;
 }
 
-int resnet_add_residual(const ResNet* r, float* const out[3], long long row, hipStream_t s)
-{
-  const long long nc = (long long)r->n * r->n * r->n;
-  hipLaunchKernelGGL(k_add_residual, dim3(grid_cells(nc)), dim3(kThreads), 0, s, r->d_y, r->n, r->out_scale, out[0],
-                     out[1], out[2], row);
-  COLA_HIP(hipGetLastError());
-  return kOk;
-}
+
 
 }  // namespace cola

@@ -279,8 +279,15 @@ int cola_resnet_fit_scale(cola_resnet* net, int count);
    respect to every parameter. */
 int cola_resnet_forward_host(cola_resnet* net, const float* x, float* y);
 int cola_resnet_gradient_host(cola_resnet* net, const float* x, const float* target, double* loss, float* grad);
-/* Mean device time (HIP events, `reps` launches) of one forward pass of the
-   four layers on the current device input, in ms (no reference counterpart). */
+/* The inference path on the same host input (no reference counterpart):
+   channel 0 of x is the feature and channel 1 of the first cell the scale
+   factor a (x must hold a in every cell of channel 1 and zeros in channels
+   2-7); the first layer runs on the feature channel with a folded into its
+   bias.  Equals cola_resnet_forward_host up to fp32 summation order. */
+int cola_resnet_forward_features_host(cola_resnet* net, const float* x, float* y);
+/* Mean device time (HIP events, `reps` launches) of one inference forward
+   pass (four layers from the feature grid of the last host input), in ms (no
+   reference counterpart). */
 int cola_resnet_time_forward(cola_resnet* net, int reps, double* ms);
 /* Device copy of the particle state of src into dst (same particle count). */
 int cola_copy_particles(cola_context* dst, cola_context* src);

@@ -55,6 +55,29 @@ def test_forward_matches_oracle(n):
     assert err <= 1e-2, err
 
 
+@pytest.mark.parametrize("n", [32, 64])
+def test_feature_path_matches_general_path(n):
+    """The inference path (first layer on the feature channel, a folded into
+    its bias) against the 8-channel path and the oracle on an input of that
+    form (channel 1 = a everywhere, channels 2-7 zero).  Only the fp32
+    summation order differs, so outputs agree far inside the oracle
+    tolerance; a wrong tap, channel or bias term would not."""
+    print(f"net feature path n={n}", flush=True)
+    params, x = _setup(n, 100 + n)
+    x[..., 1] = 0.37
+    x[..., 2:] = 0.0
+    with ch.ResidualNet(n) as net:
+        net.set_params(params)
+        y8 = net.forward(x)
+        y1 = net.forward_features(x)
+        ms = net.time_forward(2)
+    assert ms > 0.0
+    scale = np.abs(y8).max()
+    assert np.abs(y1 - y8).max() <= 2e-3 * scale, np.abs(y1 - y8).max() / scale
+    ref, _ = orc.net_forward(params.astype(np.float64), x.astype(np.float64), round_bf16=True)
+    assert np.abs(y1 - ref).max() <= 1e-2 * np.abs(ref).max()
+
+
 def test_gradient_matches_oracle():
     print("net gradient n=32", flush=True)
     n = 32
@@ -104,12 +127,25 @@ def test_pairs_training_and_corrected_force(eh_table):
         coarse.move_particles()
         coarse.pm_calculate_forces()
         f_plain = coarse.get_forces()
+        g_plain = np.stack([coarse.get_force_grid(ax)[:, :, :64] for ax in range(3)], axis=-1)
+        coarse.pm_assign()
+        dens = coarse.get_density()[:, :, :64]
+        a = coarse.times()[0]
         coarse.set_residual_net(net)
         coarse.pm_calculate_forces()
         f_net = coarse.get_forces()
+        g_net = np.stack([coarse.get_force_grid(ax)[:, :, :64] for ax in range(3)], axis=-1)
         coarse.pm_calculate_forces()
         assert np.array_equal(f_net, coarse.get_forces())  # deterministic
         assert not np.array_equal(f_plain, f_net)
+        # the grid correction is out_scale * net(feature of the density, a)
+        in_scale, out_scale = net.scales()
+        x = np.zeros((64, 64, 64, 8), np.float32)
+        x[..., 0] = np.arcsinh(dens.astype(np.float64)) * in_scale
+        x[..., 1] = a
+        y = net.forward(x)
+        corr = (g_net.astype(np.float64) - g_plain) / out_scale
+        assert np.abs(corr - y).max() <= 1e-2 * np.abs(y).max(), np.abs(corr - y).max() / np.abs(y).max()
         coarse.set_residual_net(None)
         coarse.pm_calculate_forces()
         assert np.array_equal(f_plain, coarse.get_forces())
