@@ -189,6 +189,16 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
     __syncthreads();
     for (int x = cu.x0; x < cu.x1; ++x) {
       if (x + 2 <= cu.x1) load_plane<2>(ring, in, n, x + 2, cu.y0, cu.z0);  // lands during this plane
+      // kFwdAdd: the force values this lane updates, loaded while the products run
+      float fold[kConvTy][3];
+      if (MODE == kFwdAdd && g == 0) {
+#pragma unroll
+        for (int t = 0; t < kConvTy; ++t) {
+          const long long at = ((long long)x * n + cu.y0 + kConvTy * wy + t) * o.row + cu.z0 + 16 * wz + r;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) fold[t][i] = o.add[i][at];
+        }
+      }
       floatx4 acc[kConvTy];
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -225,7 +235,8 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) {
-        const long long cell = ((long long)x * n + cu.y0 + kConvTy * wy + t) * n + cu.z0 + 16 * wz + r;
+        const long long line = (long long)x * n + cu.y0 + kConvTy * wy + t;
+        const long long cell = line * n + cu.z0 + 16 * wz + r;
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = acc[t][i] + bch[i];
@@ -240,9 +251,9 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
           }
         } else if (MODE == kFwdAdd) {
           if (g == 0) {
-            const long long at = (cell / n) * o.row + cell % n;
+            const long long at = line * o.row + cu.z0 + 16 * wz + r;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) o.add[i][at] = o.add[i][at] + o.scale * v[i];
+            for (int i = 0; i < 3; ++i) o.add[i][at] = fold[t][i] + o.scale * v[i];
           }
         } else {
           const uint2 hp = *reinterpret_cast<const uint2*>(o.hprev + cell * 16 + 4 * g);
@@ -541,14 +552,12 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned 
 
 // Feature of each cell: asinh(delta) * scale in bf16 (the first layer's
 // channel 0; channel 1, a, enters k_conv1 through its bias).
+// One workgroup per grid line (x, y).
 __global__ void k_features(const float* __restrict__ delta, long long row, int n, float scale,
                            unsigned short* __restrict__ f)
 {
-  const long long cell = (long long)blockIdx.x * kThreads + threadIdx.x;
-  const long long nc = (long long)n * n * n;
-  if (cell >= nc) return;
-  const long long line = cell / n;
-  f[cell] = f2bf(asinhf(delta[line * row + cell % n]) * scale);
+  const long long line = blockIdx.x;
+  for (int z = threadIdx.x; z < n; z += kThreads) f[line * n + z] = f2bf(asinhf(delta[line * row + z]) * scale);
 }
 
 // 8-channel first-layer input of a stored sample (the weight gradient's
@@ -766,8 +775,8 @@ int resnet_pack(ResNet* r, hipStream_t s)
 
 int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s)
 {
-  const long long nc = (long long)r->n * r->n * r->n;
-  hipLaunchKernelGGL(k_features, dim3(grid_cells(nc)), dim3(kThreads), 0, s, delta, row, r->n, r->in_scale, r->d_f1);
+  hipLaunchKernelGGL(k_features, dim3((unsigned)((long long)r->n * r->n)), dim3(kThreads), 0, s, delta, row, r->n,
+                     r->in_scale, r->d_f1);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
