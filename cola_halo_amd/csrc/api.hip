@@ -305,11 +305,15 @@ int pm_fields(cola_context* c)
   KOpParams kp;
   kp.f1 = -1.0 / std::pow((double)c->ng, 3.0) / scale;  // pm.c:317
   float* const out[3] = {c->fgrid[0], c->fgrid[1], c->fgrid[2]};
-  // the net reads the real-space density before the r2c overwrites it
-  if (c->net) COLA_TRY(resnet_features(c->net, c->density, c->cic.row, c->stream));
+  // the net's features come from the real-space density before the r2c
+  // overwrites it; its last layer adds its output to the force grids
+  if (c->net) {
+    COLA_TRY(resnet_features(c->net, c->density, c->cic.row, c->stream));
+    COLA_TRY(resnet_body(c->net, &c->d_prm->a, c->stream));
+  }
   c->density_is_real = false;
   COLA_TRY(c->pm_fft.forces(c->density, out, kp, c->stream));
-  if (c->net) COLA_TRY(resnet_correct_forces(c->net, &c->d_prm->a, out, c->cic.row, c->stream));
+  if (c->net) COLA_TRY(resnet_head_add(c->net, out, c->cic.row, c->stream));
   return kOk;
 }
 

@@ -30,7 +30,7 @@ struct ResNet {
   unsigned short* d_wb[kNetLayers] = {nullptr, nullptr, nullptr, nullptr};
   // activations: input [n^3][8], h1..h3 [n^3][16] (bf16), output y [3][n^3] fp32
   unsigned short* d_x = nullptr;
-  unsigned short* d_f1 = nullptr;  // feature grid [n^3] (bf16) of the inference path
+  unsigned short* d_f1 = nullptr;  // feature grid [n^3] (bf16) of host inputs (timing, tests)
   unsigned short* d_h[3] = {nullptr, nullptr, nullptr};
   float* d_y = nullptr;
   // gradients at layer outputs [n^3][16] (bf16), two buffers
@@ -54,8 +54,6 @@ int resnet_init(ResNet* r, int n, unsigned seed);
 void resnet_free(ResNet* r);
 // fp32 master parameters -> bf16 MFMA packs
 int resnet_pack(ResNet* r, hipStream_t s);
-// feature grid d_f1 from a density grid (rows of `row` floats)
-int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s);
 // stores a training pair: features of the coarse density, target = fine force
 // at the coincident points (fine cell 2i) minus coarse force, and a
 int resnet_store_pair(ResNet* r, const float* coarse_delta, long long crow, const float* const coarse_f[3],
@@ -72,9 +70,13 @@ int resnet_forward(ResNet* r, hipStream_t s);
 // memory at pa when pa is given, so a captured step reads each step's value;
 // else a): writes d_y
 int resnet_forward_features(ResNet* r, const unsigned short* feat, const float* pa, float a, hipStream_t s);
-// inference on d_f1 with the output added to the force grids (rows of `row`
-// floats): out[a] = out[a] + out_scale * y[a]
-int resnet_correct_forces(ResNet* r, const float* pa, float* const out[3], long long row, hipStream_t s);
+// inference in parts around the PM force FFTs: the feature grid d_f1 from
+// the real-space density grid (rows of `row` floats) before the r2c, layers
+// 1-3 (a read from pa), and layer 4 added to the force grids:
+// out[a] = out[a] + out_scale * y[a]
+int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s);
+int resnet_body(ResNet* r, const float* pa, hipStream_t s);
+int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s);
 // loss against target (bf16 [3][n^3] or fp32 [3][n^3]) and gradient into d_grad
 int resnet_backward(ResNet* r, const unsigned short* target_bf16, const float* target_f32, hipStream_t s);
 int resnet_adam(ResNet* r, float lr, hipStream_t s);

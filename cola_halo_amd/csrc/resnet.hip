@@ -40,6 +40,9 @@ __host__ __device__ inline unsigned short f2bf(float f)
 
 __host__ __device__ inline float bf2f(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
 
+// The net's input feature of a density contrast: asinh(delta) * scale in bf16.
+__device__ inline unsigned short feature_of(float delta, float scale) { return f2bf(asinhf(delta) * scale); }
+
 __device__ inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 __device__ inline long long cell_at(int x, int y, int z, int t, int sign, int n)
@@ -337,7 +340,9 @@ k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict_
 // of its cell from a ring of bf16 feature planes (the march of k_conv16);
 // the planes stream through registers, the loads of plane x + 2 in flight
 // during plane x's products.  a comes from device memory (pa) when given.
-__global__ void __launch_bounds__(kThreads)
+constexpr int kConv1Waves = 3;  // waves per SIMD (= workgroups per CU) requested for k_conv1 (2: 1.5 ms slower at 768^3)
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kConv1Waves)))
 k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__ wpk, const float* __restrict__ w,
         const float* __restrict__ bias, const float* __restrict__ pa, float a_host, int n, int march,
         unsigned short* __restrict__ out_bf)
@@ -550,14 +555,26 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned 
   }
 }
 
-// Feature of each cell: asinh(delta) * scale in bf16 (the first layer's
-// channel 0; channel 1, a, enters k_conv1 through its bias).
-// One workgroup per grid line (x, y).
-__global__ void k_features(const float* __restrict__ delta, long long row, int n, float scale,
-                           unsigned short* __restrict__ f)
+// Feature grid from a density grid: one workgroup per grid line (x, y),
+// each thread's loads issued before its transcendental work.
+__global__ void __launch_bounds__(kThreads)
+k_features(const float* __restrict__ delta, long long row, int n, float scale, unsigned short* __restrict__ f)
 {
+  constexpr int kPer = 4;
   const long long line = blockIdx.x;
-  for (int z = threadIdx.x; z < n; z += kThreads) f[line * n + z] = f2bf(asinhf(delta[line * row + z]) * scale);
+  for (int z0 = 0; z0 < n; z0 += kPer * kThreads) {
+    float d[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int z = z0 + threadIdx.x + k * kThreads;
+      d[k] = z < n ? delta[line * row + z] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int z = z0 + threadIdx.x + k * kThreads;
+      if (z < n) f[line * n + z] = feature_of(d[k], scale);
+    }
+  }
 }
 
 // 8-channel first-layer input of a stored sample (the weight gradient's
@@ -657,7 +674,7 @@ __global__ void k_store_pair(const float* __restrict__ cd, long long crow, const
   const long long xy = cell / n;
   const int y = (int)(xy % n), x = (int)(xy / n);
   const long long ca = xy * crow + z;
-  feat[cell] = f2bf(asinhf(cd[ca]) * scale);
+  feat[cell] = feature_of(cd[ca], scale);
   target[cell] = f2bf(restrict_fine(f0, frow, 2 * n, x, y, z, ws) - c0[ca]);
   target[nc + cell] = f2bf(restrict_fine(f1, frow, 2 * n, x, y, z, ws) - c1[ca]);
   target[2 * nc + cell] = f2bf(restrict_fine(f2, frow, 2 * n, x, y, z, ws) - c2[ca]);
@@ -773,13 +790,6 @@ int resnet_pack(ResNet* r, hipStream_t s)
   return kOk;
 }
 
-int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_features, dim3((unsigned)((long long)r->n * r->n)), dim3(kThreads), 0, s, delta, row, r->n,
-                     r->in_scale, r->d_f1);
-  COLA_HIP(hipGetLastError());
-  return kOk;
-}
 
 // Persistent workgroups (two resident per CU: the 16-channel ring is 77 KiB)
 // over units of `march` x planes of one (y, z) block column.  The march is
@@ -823,8 +833,8 @@ int conv_lds_attributes()
 
 namespace {
 
-// layers 2 and 3 (h1 -> h2 -> h3), then layer 4 into `o` by mode
-int conv_layers_2_to_4(ResNet* r, int mode4, const ConvOut& o4, hipStream_t s)
+// layers 2 and 3 (h1 -> h2 -> h3)
+int conv_layers_2_3(ResNet* r, hipStream_t s)
 {
   const int n = r->n;
   const ConvLaunch cl = conv_launch(n);
@@ -835,24 +845,42 @@ int conv_layers_2_to_4(ResNet* r, int mode4, const ConvOut& o4, hipStream_t s)
   o.bf = r->d_h[2];
   hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[1], r->d_wf[2],
                      r->d_param + param_offset(2) + 27 * 16 * 16, 16, n, cl.march, o);
-  const float* b4 = r->d_param + param_offset(3) + 27 * 16 * 3;
-  if (mode4 == kFwdAdd)
-    hipLaunchKernelGGL((k_conv16<kFwdAdd>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[2], r->d_wf[3],
-                       b4, 3, n, cl.march, o4);
-  else
-    hipLaunchKernelGGL((k_conv16<kFwdLinear>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[2],
-                       r->d_wf[3], b4, 3, n, cl.march, o4);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
 
+// layer 4 (h3 -> output) into `o` by mode
+int conv_layer_4(ResNet* r, int mode, const ConvOut& o, hipStream_t s)
+{
+  const int n = r->n;
+  const ConvLaunch cl = conv_launch(n);
+  const float* b4 = r->d_param + param_offset(3) + 27 * 16 * 3;
+  if (mode == kFwdAdd)
+    hipLaunchKernelGGL((k_conv16<kFwdAdd>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[2], r->d_wf[3],
+                       b4, 3, n, cl.march, o);
+  else
+    hipLaunchKernelGGL((k_conv16<kFwdLinear>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[2],
+                       r->d_wf[3], b4, 3, n, cl.march, o);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+// First layer from a bf16 feature grid.
 int conv_layer_1(ResNet* r, const unsigned short* feat, const float* pa, float a, hipStream_t s)
 {
   const ConvLaunch cl = conv_launch(r->n);
-  hipLaunchKernelGGL(k_conv1, dim3(cl.blocks), dim3(kThreads), 0, s, feat, r->d_wf[0], r->d_param + param_offset(0),
+  const long long units = (long long)((r->n + cl.march - 1) / cl.march) * (r->n / kBlockY) * (r->n / kBlockZ);
+  const unsigned blocks = (unsigned)std::min<long long>(256LL * kConv1Waves, units);
+  hipLaunchKernelGGL(k_conv1, dim3(blocks), dim3(kThreads), 0, s, feat, r->d_wf[0], r->d_param + param_offset(0),
                      r->d_param + param_offset(0) + 27 * 8 * 16, pa, a, r->n, cl.march, r->d_h[0]);
   COLA_HIP(hipGetLastError());
   return kOk;
+}
+
+int conv_layers_2_to_4(ResNet* r, int mode4, const ConvOut& o4, hipStream_t s)
+{
+  COLA_TRY(conv_layers_2_3(r, s));
+  return conv_layer_4(r, mode4, o4, s);
 }
 
 }  // namespace
@@ -878,15 +906,28 @@ int resnet_forward_features(ResNet* r, const unsigned short* feat, const float* 
   return conv_layers_2_to_4(r, kFwdLinear, o, s);
 }
 
-int resnet_correct_forces(ResNet* r, const float* pa, float* const out[3], long long row, hipStream_t s)
+int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_features, dim3((unsigned)((long long)r->n * r->n)), dim3(kThreads), 0, s, delta, row, r->n,
+                     r->in_scale, r->d_f1);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int resnet_body(ResNet* r, const float* pa, hipStream_t s)
 {
   COLA_TRY(conv_lds_attributes());
   COLA_TRY(conv_layer_1(r, r->d_f1, pa, 0.0f, s));
+  return conv_layers_2_3(r, s);
+}
+
+int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s)
+{
   ConvOut o;
   for (int a = 0; a < 3; ++a) o.add[a] = out[a];
   o.row = row;
   o.scale = r->out_scale;
-  return conv_layers_2_to_4(r, kFwdAdd, o, s);
+  return conv_layer_4(r, kFwdAdd, o, s);
 }
 
 int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStream_t s)
