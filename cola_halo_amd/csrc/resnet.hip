@@ -40,8 +40,29 @@ __host__ __device__ inline unsigned short f2bf(float f)
 
 __host__ __device__ inline float bf2f(unsigned short h) { return __builtin_bit_cast(float, (unsigned)h << 16); }
 
+// asinh(x) to a few float ulps, cheaper than the libm-grade asinhf: the odd
+// Taylor series to x^7 for |x| < 1/8 (truncation below 1e-9 relative), else
+// ln(|x| + sqrt(x^2 + 1)) from the hardware log2 and square root (argument
+// >= 1.007, no cancellation), ln(2|x|) beyond 1e9.  The feature is rounded
+// to bfloat16 (8 bits) afterwards.
+__device__ inline float feature_asinh(float x)
+{
+  constexpr float kLn2 = 0.693147180559945f;
+  const float ax = fabsf(x);
+  float r;
+  if (ax < 0.125f) {
+    const float x2 = ax * ax;
+    r = ax * (1.0f + x2 * (-1.0f / 6.0f + x2 * (3.0f / 40.0f + x2 * (-15.0f / 336.0f))));
+  } else if (ax < 1e9f) {
+    r = __builtin_amdgcn_logf(ax + __builtin_amdgcn_sqrtf(ax * ax + 1.0f)) * kLn2;
+  } else {
+    r = (__builtin_amdgcn_logf(ax) + 1.0f) * kLn2;
+  }
+  return copysignf(r, x);
+}
+
 // The net's input feature of a density contrast: asinh(delta) * scale in bf16.
-__device__ inline unsigned short feature_of(float delta, float scale) { return f2bf(asinhf(delta) * scale); }
+__device__ inline unsigned short feature_of(float delta, float scale) { return f2bf(feature_asinh(delta) * scale); }
 
 __device__ inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
