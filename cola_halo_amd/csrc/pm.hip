@@ -205,6 +205,37 @@ __device__ inline int hash_slot(BinHash& h, int key)
   return -1;
 }
 
+// Runs of equal keys in adjacent lanes of a wavefront (lattice neighbours
+// mostly share a tile): the run's first lane (head) applies one atomic for
+// the whole run, so the LDS table sees one atomic per run instead of one per
+// lane (same-address atomics of a wavefront serialise).  For every active
+// lane: its run's head lane; for a head: the run's length.  Wave-uniform
+// control flow required (ballots).
+struct LaneRun {
+  int head, len;
+  bool is_head;
+};
+
+__device__ inline LaneRun lane_run(int key, bool active)
+{
+  const int lane = __lane_id();
+  const int prev_key = __shfl_up(key, 1);
+  const unsigned long long act = __ballot(active);
+  const bool prev_act = lane > 0 && ((act >> (lane - 1)) & 1ULL);
+  LaneRun r;
+  r.is_head = active && (!prev_act || prev_key != key);
+  const unsigned long long heads = __ballot(r.is_head);
+  // the run ends before the next head or inactive lane
+  const unsigned long long stop = heads | ~act;
+  const unsigned long long after = lane == 63 ? 0ULL : stop >> (lane + 1);
+  const int next = after ? lane + __ffsll((long long)after) : 64;
+  r.len = next - lane;
+  // head: the highest head at or below this lane
+  const unsigned long long upto = heads & (lane == 63 ? ~0ULL : (2ULL << lane) - 1ULL);
+  r.head = upto ? 63 - __clzll((long long)upto) : lane;
+  return r;
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
             float L)
@@ -229,13 +260,20 @@ k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int
     nk = span_count(ts);
   }
   __syncthreads();
-  for (int k = 0; k < nk; ++k) {
-    const int key = span_key(ts, t, k);
-    const int slot = hash_slot(h, key);
-    if (slot >= 0)
-      atomicAdd(&h.cnt[slot], 1);
-    else
-      atomicAdd(&counts[key], 1);
+  // the slot loop is uniform across the wavefront (up to the wave's max)
+  int kmax = nk;
+  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  for (int k = 0; k < kmax; ++k) {
+    const bool act = k < nk;
+    const int key = act ? span_key(ts, t, k) : 0;
+    const LaneRun run = lane_run(key, act);
+    if (run.is_head) {
+      const int slot = hash_slot(h, key);
+      if (slot >= 0)
+        atomicAdd(&h.cnt[slot], run.len);
+      else
+        atomicAdd(&counts[key], run.len);
+    }
   }
   __syncthreads();
   for (int s = threadIdx.x; s < kHashSlots; s += kThreads)
@@ -265,15 +303,31 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
     nk = span_count(ts);
   }
   __syncthreads();
-  // slot and rank of each entry (at most 8; static indices keep them in registers)
+  // slot and rank of each entry (at most 8; static indices keep them in
+  // registers).  Fallback entries (slot -1) claim their global slot here:
+  // rank is then the counter value before their unit.
+  int kmax = nk;
+  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  const int lane = __lane_id();
   int slot[8], rank[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     slot[k] = -1;
     rank[k] = 0;
-    if (k < nk) {
-      slot[k] = hash_slot(h, span_key(ts, t, k));
-      if (slot[k] >= 0) rank[k] = atomicAdd(&h.cnt[slot[k]], 1);
+    if (k < kmax) {
+      const bool act = k < nk;
+      const int key = act ? span_key(ts, t, k) : 0;
+      const LaneRun run = lane_run(key, act);
+      int sl = -1, b = 0;
+      if (run.is_head) {
+        sl = hash_slot(h, key);
+        b = sl >= 0 ? atomicAdd(&h.cnt[sl], run.len) : atomicSub(&counts[key], run.len);
+      }
+      sl = __shfl(sl, run.head);
+      b = __shfl(b, run.head);
+      const int m = lane - run.head;  // position in the run
+      slot[k] = sl;
+      rank[k] = sl >= 0 ? b + m : b - m;
     }
   }
   __syncthreads();
@@ -286,12 +340,10 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
   for (int k = 0; k < 8; ++k) {
     if (k < nk) {
       long long pos;
-      if (slot[k] >= 0) {
+      if (slot[k] >= 0)
         pos = base[slot[k]] - rank[k];
-      } else {
-        const int key = span_key(ts, t, k);
-        pos = (long long)offsets[key] + atomicSub(&counts[key], 1) - 1;
-      }
+      else
+        pos = (long long)offsets[span_key(ts, t, k)] + rank[k] - 1;
       if (pos < capacity) {
         sorted[3 * pos + 0] = x;
         sorted[3 * pos + 1] = y;
