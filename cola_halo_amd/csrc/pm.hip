@@ -236,43 +236,66 @@ __device__ inline LaneRun lane_run(int key, bool active)
   return r;
 }
 
+// Particles per thread in the binning kernels: a workgroup bins kThreads *
+// kBinPer lattice-ordered particles (several lattice rows share tiles), so
+// its table flush and barriers are paid once per 1024 particles and each
+// thread has kBinPer independent loads in flight.  Particle q * kThreads +
+// lane of a workgroup's chunk keeps adjacent lanes on adjacent particles.
+constexpr int kBinPer = 4;
+
 __global__ void __launch_bounds__(kThreads)
 k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
             float L)
 {
   __shared__ BinHash h;
+  float px[kBinPer], py[kBinPer], pz[kBinPer];
+  bool live[kBinPer];
+#pragma unroll
+  for (int q = 0; q < kBinPer; ++q) {
+    const long long i = ((long long)blockIdx.x * kBinPer + q) * kThreads + threadIdx.x;
+    live[q] = i < p.np;
+    px[q] = live[q] ? p.x[0][i] : 0.0f;
+    py[q] = live[q] ? p.x[1][i] : 0.0f;
+    pz[q] = live[q] ? p.x[2][i] : 0.0f;
+  }
   hash_clear(h);
-  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  TileSpan ts;
-  int nk = 0;
-  if (i < p.np) {
-    float x = p.x[0][i], y = p.x[1][i], z = p.x[2][i];
-    if (wrap_first) {
-      const float x0 = x, y0 = y, z0 = z;
-      wrap1(x, L);
-      wrap1(y, L);
-      wrap1(z, L);
-      if (x != x0) p.x[0][i] = x;
-      if (y != y0) p.x[1][i] = y;
-      if (z != z0) p.x[2][i] = z;
+  TileSpan ts[kBinPer];
+  int nk[kBinPer];
+#pragma unroll
+  for (int q = 0; q < kBinPer; ++q) {
+    nk[q] = 0;
+    if (live[q]) {
+      float x = px[q], y = py[q], z = pz[q];
+      if (wrap_first) {
+        const long long i = ((long long)blockIdx.x * kBinPer + q) * kThreads + threadIdx.x;
+        wrap1(x, L);
+        wrap1(y, L);
+        wrap1(z, L);
+        if (x != px[q]) p.x[0][i] = x;
+        if (y != py[q]) p.x[1][i] = y;
+        if (z != pz[q]) p.x[2][i] = z;
+      }
+      if (!tile_span(x, y, z, g, t, ts[q])) atomicOr(overflow, 1);
+      nk[q] = span_count(ts[q]);
     }
-    if (!tile_span(x, y, z, g, t, ts)) atomicOr(overflow, 1);
-    nk = span_count(ts);
   }
   __syncthreads();
-  // the slot loop is uniform across the wavefront (up to the wave's max)
-  int kmax = nk;
-  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
-  for (int k = 0; k < kmax; ++k) {
-    const bool act = k < nk;
-    const int key = act ? span_key(ts, t, k) : 0;
-    const LaneRun run = lane_run(key, act);
-    if (run.is_head) {
-      const int slot = hash_slot(h, key);
-      if (slot >= 0)
-        atomicAdd(&h.cnt[slot], run.len);
-      else
-        atomicAdd(&counts[key], run.len);
+#pragma unroll
+  for (int q = 0; q < kBinPer; ++q) {
+    // the slot loop is uniform across the wavefront (up to the wave's max)
+    int kmax = nk[q];
+    for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+    for (int k = 0; k < kmax; ++k) {
+      const bool act = k < nk[q];
+      const int key = act ? span_key(ts[q], t, k) : 0;
+      const LaneRun run = lane_run(key, act);
+      if (run.is_head) {
+        const int slot = hash_slot(h, key);
+        if (slot >= 0)
+          atomicAdd(&h.cnt[slot], run.len);
+        else
+          atomicAdd(&counts[key], run.len);
+      }
     }
   }
   __syncthreads();
@@ -283,51 +306,61 @@ k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int
 // Claims the entries' slots by counting each tile's counter back down to zero
 // (ready for the next step): a workgroup takes the cnt slots below the
 // counter's value c, and its entry of rank r in the tile gets position
-// offsets[tile] + c - 1 - r.
+// offsets[tile] + c - 1 - r.  Entries that found no table slot claim theirs
+// one by one from the counter.
 __global__ void __launch_bounds__(kThreads)
 k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, const int* __restrict__ offsets,
               float* __restrict__ sorted, long long capacity, int* __restrict__ overflow)
 {
   __shared__ BinHash h;
   __shared__ long long base[kHashSlots];
+  float px[kBinPer], py[kBinPer], pz[kBinPer];
+  bool live[kBinPer];
+#pragma unroll
+  for (int q = 0; q < kBinPer; ++q) {
+    const long long i = ((long long)blockIdx.x * kBinPer + q) * kThreads + threadIdx.x;
+    live[q] = i < p.np;
+    px[q] = live[q] ? p.x[0][i] : 0.0f;
+    py[q] = live[q] ? p.x[1][i] : 0.0f;
+    pz[q] = live[q] ? p.x[2][i] : 0.0f;
+  }
   hash_clear(h);
-  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
-  TileSpan ts;
-  int nk = 0;
-  float x = 0, y = 0, z = 0;
-  if (i < p.np) {
-    x = p.x[0][i];
-    y = p.x[1][i];
-    z = p.x[2][i];
-    tile_span(x, y, z, g, t, ts);
-    nk = span_count(ts);
+  TileSpan ts[kBinPer];
+  int nk[kBinPer];
+#pragma unroll
+  for (int q = 0; q < kBinPer; ++q) {
+    nk[q] = 0;
+    if (live[q]) {
+      tile_span(px[q], py[q], pz[q], g, t, ts[q]);
+      nk[q] = span_count(ts[q]);
+    }
   }
   __syncthreads();
-  // slot and rank of each entry (at most 8; static indices keep them in
-  // registers).  Fallback entries (slot -1) claim their global slot here:
-  // rank is then the counter value before their unit.
-  int kmax = nk;
-  for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  // per entry: table slot << 16 | rank in the workgroup's share of the tile
+  // (ranks < kThreads * kBinPer * 8 = 8192), or -1 without a slot.  Static
+  // indices keep the codes in registers.
   const int lane = __lane_id();
-  int slot[8], rank[8];
+  int code[kBinPer][8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    slot[k] = -1;
-    rank[k] = 0;
-    if (k < kmax) {
-      const bool act = k < nk;
-      const int key = act ? span_key(ts, t, k) : 0;
-      const LaneRun run = lane_run(key, act);
-      int sl = -1, b = 0;
-      if (run.is_head) {
-        sl = hash_slot(h, key);
-        b = sl >= 0 ? atomicAdd(&h.cnt[sl], run.len) : atomicSub(&counts[key], run.len);
+  for (int q = 0; q < kBinPer; ++q) {
+    int kmax = nk[q];
+    for (int off = 32; off > 0; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      code[q][k] = -1;
+      if (k < kmax) {
+        const bool act = k < nk[q];
+        const int key = act ? span_key(ts[q], t, k) : 0;
+        const LaneRun run = lane_run(key, act);
+        int sl = -1, b = 0;
+        if (run.is_head) {
+          sl = hash_slot(h, key);
+          if (sl >= 0) b = atomicAdd(&h.cnt[sl], run.len);
+        }
+        sl = __shfl(sl, run.head);
+        b = __shfl(b, run.head);
+        if (sl >= 0) code[q][k] = sl << 16 | (b + lane - run.head);
       }
-      sl = __shfl(sl, run.head);
-      b = __shfl(b, run.head);
-      const int m = lane - run.head;  // position in the run
-      slot[k] = sl;
-      rank[k] = sl >= 0 ? b + m : b - m;
     }
   }
   __syncthreads();
@@ -337,19 +370,24 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    if (k < nk) {
-      long long pos;
-      if (slot[k] >= 0)
-        pos = base[slot[k]] - rank[k];
-      else
-        pos = (long long)offsets[span_key(ts, t, k)] + rank[k] - 1;
-      if (pos < capacity) {
-        sorted[3 * pos + 0] = x;
-        sorted[3 * pos + 1] = y;
-        sorted[3 * pos + 2] = z;
-      } else {
-        atomicOr(overflow, 2);
+  for (int q = 0; q < kBinPer; ++q) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k < nk[q]) {
+        long long pos;
+        if (code[q][k] >= 0) {
+          pos = base[code[q][k] >> 16] - (code[q][k] & 0xffff);
+        } else {
+          const int key = span_key(ts[q], t, k);
+          pos = (long long)offsets[key] + atomicSub(&counts[key], 1) - 1;
+        }
+        if (pos < capacity) {
+          sorted[3 * pos + 0] = px[q];
+          sorted[3 * pos + 1] = py[q];
+          sorted[3 * pos + 2] = pz[q];
+        } else {
+          atomicOr(overflow, 2);
+        }
       }
     }
   }
@@ -712,6 +750,7 @@ __global__ void k_zero_mass(double* ring, const StepParams* __restrict__ prm)
 }
 
 unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+unsigned bin_blocks(long long n) { return (unsigned)((n + kThreads * kBinPer - 1) / (kThreads * kBinPer)); }
 
 }  // namespace
 
@@ -794,7 +833,7 @@ int launch_bin_count(const DevParticles& p, const CicGeom& g, const TileGeom& t,
                      float boxsize, hipStream_t s)
 {
   if (p.np == 0) return kOk;
-  hipLaunchKernelGGL(k_bin_count, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, t, b.counts, b.overflow,
+  hipLaunchKernelGGL(k_bin_count, dim3(bin_blocks(p.np)), dim3(kThreads), 0, s, p, g, t, b.counts, b.overflow,
                      wrap_first ? 1 : 0, boxsize);
   COLA_HIP(hipGetLastError());
   return kOk;
@@ -814,7 +853,7 @@ int launch_bin_scan(const TileGeom& t, BinBuffers& b, hipStream_t s)
 int launch_bin_scatter(const DevParticles& p, const CicGeom& g, const TileGeom& t, BinBuffers& b, hipStream_t s)
 {
   if (p.np == 0) return kOk;
-  hipLaunchKernelGGL(k_bin_scatter, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, t, b.counts, b.offsets,
+  hipLaunchKernelGGL(k_bin_scatter, dim3(bin_blocks(p.np)), dim3(kThreads), 0, s, p, g, t, b.counts, b.offsets,
                      b.sorted, b.capacity, b.overflow);
   COLA_HIP(hipGetLastError());
   return kOk;
