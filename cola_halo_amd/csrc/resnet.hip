@@ -227,15 +227,25 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
       uint4 prev[3] = {zero4, zero4, zero4};
+      // operands of input row j + 1 are read from LDS while row j's products
+      // run (one row of software pipelining hides the LDS latency)
+      uint4 cen[3], sh[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cen[d] = ring[ring_cell(x - 1 + d, kConvTy * wy, zc) * 2 + half];
+        sh[d] = ring[ring_cell(x - 1 + d, kConvTy * wy, zs) * 2 + half];
+      }
 #pragma unroll
       for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
-        const int yy = kConvTy * wy + j;
-        uint4 cen[3], sh[3];
+        uint4 ncen[3], nsh[3];
+        if (j + 1 < kConvTy + 2) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          cen[d] = ring[ring_cell(x - 1 + d, yy, zc) * 2 + half];
-          sh[d] = ring[ring_cell(x - 1 + d, yy, zs) * 2 + half];
+          for (int d = 0; d < 3; ++d) {
+            ncen[d] = ring[ring_cell(x - 1 + d, kConvTy * wy + j + 1, zc) * 2 + half];
+            nsh[d] = ring[ring_cell(x - 1 + d, kConvTy * wy + j + 1, zs) * 2 + half];
+          }
         }
+        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of this row's products
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           if (j >= 2) acc[j - 2] = mfma(sh[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
@@ -251,7 +261,13 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
           acc[j - 2] = mfma(sel4(lo, cen[2], zero4), b[13], acc[j - 2]);
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) prev[d] = cen[d];
+        for (int d = 0; d < 3; ++d) {
+          prev[d] = cen[d];
+          if (j + 1 < kConvTy + 2) {
+            cen[d] = ncen[d];
+            sh[d] = nsh[d];
+          }
+        }
       }
       // plane x + 2 (and the previous plane's stores) complete; every wave is
       // past its reads of plane x - 1, whose slot the next prefetch takes
