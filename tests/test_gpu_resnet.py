@@ -206,3 +206,34 @@ def test_pair_targets_match_restricted_forces(eh_table, w):
     want_f = _bf16((np.arcsinh(delta.astype(np.float64)) * in_scale).astype(np.float32))
     assert np.abs(feat - want_f).max() <= np.abs(want_f).max() * 2.0 ** -8
     assert 0.39 < a < 0.41  # a_x of the coarse context after three steps from 0.1
+
+
+def test_corrected_graph_steps_equal_eager_every_replay(eh_table):
+    """Corrected coarse steps (net attached: feature pass, first layers,
+    force FFTs, last layer added into the force grids) captured as a graph
+    and replayed every step, with synchronising host reads in between, equal
+    the eager steps bit for bit; the scale factor reaches the net through
+    the step's device struct, so every replay sees its own a."""
+    print("net graph replay 32^3 / 64^3", flush=True)
+    k, p = eh_table
+    nc, ng, box = 32, 64, 100.0
+    with ch.ResidualNet(ng, seed=5) as net, ch.Context(nc, ng, box) as a, ch.Context(nc, ng, box) as b:
+        net.set_scales(1.0, 0.5)
+        for c in (a, b):
+            c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+            c.set_residual_net(net)
+            c.lpt_set_displacement(0.1, OM, 21)
+            c.set_times(c.times()[0], 0.1)
+        b.set_step_graph(True)
+        omf = float(np.float32(OM))
+        for istep in range(1, 10):
+            avel1, apos1 = (istep + 0.5) * 0.1, (istep + 1.0) * 0.1
+            a.step(omf, avel1, apos1)
+            b.step(omf, avel1, apos1)
+            pa, pb = a.get_particles(), b.get_particles()
+            assert np.array_equal(pa, pb), istep
+        b.set_step_graph(False)
+        for c in (a, b):
+            c.move_particles()
+            c.pm_calculate_forces()
+        assert np.array_equal(a.get_forces(), b.get_forces())
