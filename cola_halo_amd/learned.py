@@ -22,6 +22,7 @@ held out, 10 COLA steps from a = 0.1):
 Usage: python -m cola_halo_amd.learned [--nc 128 --box 250 --coarse 192
        --fine 384 --ntrain 56 --nval 8 --epochs 8] > result.json
        python -m cola_halo_amd.learned --time-net 192   (forward timing only)
+       python -m cola_halo_amd.learned --time-step 512 768 1000   (corrected steps only)
 """
 import argparse
 import json
@@ -152,6 +153,26 @@ def time_net(n, reps=20):
             "cells_per_s": round(cells / (ms * 1e-3), 1)}
 
 
+def time_step(nc, ng, box, steps=5):
+    """Wall time of corrected coarse steps (eager) with a randomly initialised
+    net attached: nc^3 particles on an ng^3 grid of side box, 2LPT at a = 0.1
+    (seed 1), `steps` COLA steps; for kernel traces of the corrected step."""
+    k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
+    Om = float(np.float32(OM))
+    with ch.ResidualNet(ng, seed=1) as net, ch.Context(nc, ng, box) as ctx:
+        ctx.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
+        ctx.set_residual_net(net)
+        ctx.lpt_set_displacement(1.0 / NSTEPS, OM, 1)
+        ctx.set_times(ctx.times()[0], 1.0 / NSTEPS)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for istep in range(1, steps + 1):
+            ctx.step(Om, (istep + 0.5) / NSTEPS, (istep + 1.0) / NSTEPS)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+    return {"nc": nc, "ngrid": ng, "box": box, "steps": steps, "ms_per_step": round(ms, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nc", type=int, default=128)
@@ -167,6 +188,8 @@ def main():
                     help="side weight w of the fine-to-coarse restriction (w, 1-2w, w): 0 point values, 0.25 full weighting")
     ap.add_argument("--time-net", type=int, default=0, metavar="N",
                     help="only time the net's forward pass on an N^3 grid and exit")
+    ap.add_argument("--time-step", type=float, nargs=3, default=None, metavar=("NC", "NG", "BOX"),
+                    help="only time five corrected steps (random net) of NC^3 particles on an NG^3 grid and exit")
     ap.add_argument("--production-nc", type=int, default=0, metavar="NC",
                     help="after training, also assess the net on NC^3 particles (same cell size and particle spacing)")
     ap.add_argument("--production-box", type=float, default=1000.0)
@@ -175,6 +198,10 @@ def main():
     args = ap.parse_args()
     if args.time_net:
         print(json.dumps(time_net(args.time_net)))
+        return
+    if args.time_step:
+        nc, ng, box = args.time_step
+        print(json.dumps(time_step(int(nc), int(ng), box)))
         return
     nc, box = args.nc, args.box
     pk_n = args.pk_grid or args.fine
