@@ -713,16 +713,19 @@ def run_cola_batch(ctxs, omega_m, a_final, nsteps, seeds, a_out_final=True):
     for istep in range(1, nsteps + 1):
         avel0, apos0 = (istep - 0.5) * da, istep * da
         avel1, apos1 = (istep + 0.5) * da, (istep + 1.0) * da
-        if not (avel0 <= aout <= avel1):
+        # main.c:176-203, as run_cola: a snapshot inside [avel0, apos0] is taken
+        # from the forces; one in (apos0, avel1] after the kick to avel1
+        snap_now = avel0 <= aout <= apos0
+        snap_mid = (not snap_now) and apos0 < aout <= avel1
+        if not (snap_now or snap_mid):
             step_batch(ctxs, Om, [avel1] * len(ctxs), [apos1] * len(ctxs))
             continue
-        snaps = []
         for ctx in ctxs:
             ctx.move_particles()
             ctx.pm_calculate_forces()
-        for ctx in ctxs:
-            snaps.append(ctx.set_snapshot(aout, Om))
-        return snaps
+            if snap_mid:
+                ctx.kick(Om, avel1)
+        return [ctx.set_snapshot(aout, Om) for ctx in ctxs]
     return []
 
 

@@ -96,6 +96,7 @@ struct cola_context {
   StepParams* d_prm = nullptr;  // the values of the step in flight
   bool graph_mode = false;
   ResNet* net = nullptr;  // learned force residual added to the force grids (not owned)
+  cola_resnet* net_handle = nullptr;  // the handle `net` belongs to
   hipGraphExec_t step_graph = nullptr;  // captured fused step
   int mass_slot = 0;     // next slot
   int mass_pending = 0;  // slots written since the last check
@@ -134,6 +135,14 @@ struct cola_context {
   long long phase_n[kNumPhases] = {0};
   int open_phase = -1;
   hipEvent_t open_event = nullptr;
+};
+
+// A net's activation buffers serve one context at a time: `attached` is the
+// context whose steps run the net on the context's stream.
+struct cola_resnet {
+  ResNet r;
+  hipStream_t stream = nullptr;
+  cola_context* attached = nullptr;
 };
 
 namespace {
@@ -439,6 +448,7 @@ int cola_context_destroy(cola_context* c)
 {
   if (!c) return kOk;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->net_handle) c->net_handle->attached = nullptr;
   for (const EventPair& ep : c->events) {
     (void)hipEventDestroy(ep.a);
     (void)hipEventDestroy(ep.b);
@@ -1084,6 +1094,9 @@ extern "C" {
 int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
 {
   if (!valid(c)) return kErrArg;
+  // the captured step graph holds the old plans' twiddle tables and tile shapes
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  drop_step_graph(c);
   COLA_TRY(set_deposit_tiles(c, tx, ty, tz));
   if (fft_log2_tile != c->fft_log2_tile || fft_log2_tile != c->fft_log2_rows) {
     FftPlan a, b;
@@ -1463,10 +1476,18 @@ int cola_copy_particles(cola_context* dst, cola_context* src)
 
 }  // extern "C"
 
-struct cola_resnet {
-  ResNet r;
-  hipStream_t stream = nullptr;
-};
+namespace {
+
+// Work on the net's own stream (host I/O, training, timing) starts only after
+// the attached context's queued steps, which use the same activation buffers.
+int net_quiesce(cola_resnet* h)
+{
+  if (h->attached) COLA_HIP(hipStreamSynchronize(h->attached->stream));
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1493,6 +1514,12 @@ int cola_resnet_create(int n, unsigned int seed, cola_resnet** out)
 int cola_resnet_destroy(cola_resnet* h)
 {
   if (!h) return kOk;
+  if (cola_context* c = h->attached) {  // detach: the context's graph and steps must not see the net again
+    (void)hipStreamSynchronize(c->stream);
+    drop_step_graph(c);
+    c->net = nullptr;
+    c->net_handle = nullptr;
+  }
   (void)hipStreamSynchronize(h->stream);
   resnet_free(&h->r);
   (void)hipStreamDestroy(h->stream);
@@ -1503,7 +1530,7 @@ int cola_resnet_destroy(cola_resnet* h)
 int cola_resnet_get_params(cola_resnet* h, float* out)
 {
   if (!h || !out) return kErrArg;
-  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(net_quiesce(h));
   COLA_HIP(hipMemcpy(out, h->r.d_param, sizeof(float) * kNetParams, hipMemcpyDeviceToHost));
   return kOk;
 }
@@ -1511,7 +1538,7 @@ int cola_resnet_get_params(cola_resnet* h, float* out)
 int cola_resnet_set_params(cola_resnet* h, const float* in)
 {
   if (!h || !in) return kErrArg;
-  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(net_quiesce(h));
   COLA_HIP(hipMemcpy(h->r.d_param, in, sizeof(float) * kNetParams, hipMemcpyHostToDevice));
   COLA_TRY(resnet_pack(&h->r, h->stream));
   COLA_HIP(hipStreamSynchronize(h->stream));
@@ -1529,6 +1556,7 @@ int cola_resnet_get_scales(cola_resnet* h, float* in_scale, float* out_scale)
 int cola_resnet_set_scales(cola_resnet* h, float in_scale, float out_scale)
 {
   if (!h || !(out_scale > 0.0f) || !(in_scale > 0.0f)) return kErrArg;
+  COLA_TRY(net_quiesce(h));
   h->r.in_scale = in_scale;
   h->r.out_scale = out_scale;
   h->r.scale_set = true;
@@ -1538,6 +1566,7 @@ int cola_resnet_set_scales(cola_resnet* h, float in_scale, float out_scale)
 int cola_resnet_fit_scale(cola_resnet* h, int count)
 {
   if (!h) return kErrArg;
+  COLA_TRY(net_quiesce(h));
   return resnet_fit_scale(&h->r, count, h->stream);
 }
 
@@ -1580,7 +1609,7 @@ int download_output(cola_resnet* h, float* y)
 int cola_resnet_forward_host(cola_resnet* h, const float* x, float* y)
 {
   if (!h || !x || !y) return kErrArg;
-  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(net_quiesce(h));
   COLA_TRY(upload_input(h, x));
   COLA_TRY(resnet_forward(&h->r, h->stream));
   return download_output(h, y);
@@ -1589,7 +1618,7 @@ int cola_resnet_forward_host(cola_resnet* h, const float* x, float* y)
 int cola_resnet_forward_features_host(cola_resnet* h, const float* x, float* y)
 {
   if (!h || !x || !y) return kErrArg;
-  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(net_quiesce(h));
   COLA_TRY(upload_input(h, x));
   COLA_TRY(resnet_forward_features(&h->r, h->r.d_f1, h->r.d_a, 0.0f, h->stream));
   return download_output(h, y);
@@ -1605,6 +1634,7 @@ int cola_resnet_set_restriction(cola_resnet* h, float side_weight)
 int cola_resnet_time_forward(cola_resnet* h, int reps, double* ms)
 {
   if (!h || reps < 1 || !ms) return kErrArg;
+  COLA_TRY(net_quiesce(h));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   COLA_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1630,7 +1660,7 @@ int cola_resnet_gradient_host(cola_resnet* h, const float* x, const float* targe
 {
   if (!h || !x || !target || !loss || !grad) return kErrArg;
   const long long nc = (long long)h->r.n * h->r.n * h->r.n;
-  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(net_quiesce(h));
   COLA_TRY(upload_input(h, x));
   std::vector<float> t(3 * nc);
   for (long long cell = 0; cell < nc; ++cell)
@@ -1660,6 +1690,7 @@ int cola_resnet_add_pair(cola_resnet* h, cola_context* coarse, cola_context* fin
   }
   COLA_TRY(require_particles(coarse));
   COLA_TRY(require_particles(fine));
+  COLA_TRY(net_quiesce(h));
   ResNet* const saved = coarse->net;
   coarse->net = nullptr;  // the pair holds the plain coarse force
   int st = pm_assign(coarse, true);
@@ -1697,7 +1728,7 @@ int cola_resnet_get_sample(cola_resnet* h, int i, float* feature, float* target,
   if (!h || i < 0 || i >= (int)h->r.s_feat.size() || !feature || !target || !a) return kErrArg;
   const long long nc = (long long)h->r.n * h->r.n * h->r.n;
   std::vector<unsigned short> f(nc), t(3 * nc);
-  COLA_HIP(hipStreamSynchronize(h->stream));
+  COLA_TRY(net_quiesce(h));
   COLA_HIP(hipMemcpy(f.data(), h->r.s_feat[i], sizeof(unsigned short) * nc, hipMemcpyDeviceToHost));
   COLA_HIP(hipMemcpy(t.data(), h->r.s_target[i], sizeof(unsigned short) * 3 * nc, hipMemcpyDeviceToHost));
   auto widen = [](unsigned short b) {
@@ -1715,12 +1746,14 @@ int cola_resnet_get_sample(cola_resnet* h, int i, float* feature, float* target,
 int cola_resnet_train(cola_resnet* h, int ntrain, int epochs, float lr, unsigned int seed, double* epoch_loss)
 {
   if (!h) return kErrArg;
+  COLA_TRY(net_quiesce(h));
   return resnet_train(&h->r, ntrain, epochs, lr, seed, epoch_loss, h->stream);
 }
 
 int cola_resnet_eval(cola_resnet* h, int first, int count, double* loss)
 {
   if (!h || !loss) return kErrArg;
+  COLA_TRY(net_quiesce(h));
   return resnet_eval(&h->r, first, count, loss, h->stream);
 }
 
@@ -1731,10 +1764,18 @@ int cola_set_residual_net(cola_context* c, cola_resnet* h)
     set_error("residual net grid differs from the context's PM grid");
     return kErrArg;
   }
+  if (h && h->attached && h->attached != c) {
+    set_error("residual net is attached to another context (each context needs its own net: the activation "
+              "buffers belong to the net)");
+    return kErrState;
+  }
   COLA_HIP(hipStreamSynchronize(c->stream));
   if (h) COLA_HIP(hipStreamSynchronize(h->stream));
   drop_step_graph(c);
+  if (c->net_handle && c->net_handle != h) c->net_handle->attached = nullptr;
   c->net = h ? &h->r : nullptr;
+  c->net_handle = h;
+  if (h) h->attached = c;
   return kOk;
 }
 

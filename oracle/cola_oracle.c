@@ -663,6 +663,91 @@ void orc_fft_c2r_3d(int N, float* data)
   free(tw);
 }
 
+/* lpt.c:358-503 for one rank: from the ZA displacement spectra cdisp[3]
+   (complex [i][j][kz], kz <= N/2) to the real ZA displacement (in cdisp,
+   unnormalised c2r) and the unnormalised real 2LPT displacement in
+   cdigrad[0..2] (lpt.c:433 aliases cdisp2 to cdigrad). */
+static void lpt_stages(int N, double Box, float* cdisp[3], float* cdigrad[6])
+{
+  const long nz = N / 2 + 1;
+  double kvec[3];
+  /* lpt.c:358-398: ZA displacement gradients i k_j dis_i */
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++)
+      for (int k = 0; k <= N / 2; k++) {
+        long coord = ((long)i * N + j) * nz + k;
+        if (i < N / 2)
+          kvec[0] = i * 2 * M_PI / Box;
+        else
+          kvec[0] = -(N - i) * 2 * M_PI / Box;
+        if (j < N / 2)
+          kvec[1] = j * 2 * M_PI / Box;
+        else
+          kvec[1] = -(N - j) * 2 * M_PI / Box;
+        if (k < N / 2)
+          kvec[2] = k * 2 * M_PI / Box;
+        else
+          kvec[2] = -(N - k) * 2 * M_PI / Box;
+        static const int src[6] = {0, 0, 0, 1, 1, 2};
+        static const int dir[6] = {0, 1, 2, 1, 2, 2};
+        for (int g = 0; g < 6; g++) {
+          const float* c = cdisp[src[g]] + 2 * coord;
+          cdigrad[g][2 * coord] = -c[1] * kvec[dir[g]];
+          cdigrad[g][2 * coord + 1] = c[0] * kvec[dir[g]];
+        }
+      }
+  for (int g = 0; g < 6; g++) orc_fft_c2r_3d(N, cdigrad[g]);
+
+  /* lpt.c:409-422: second-order source into digrad[3] */
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++)
+      for (int k = 0; k < N; k++) {
+        long coord = ((long)i * N + j) * (2 * nz) + k;
+        float* d0 = cdigrad[0];
+        float* d1 = cdigrad[1];
+        float* d2 = cdigrad[2];
+        float* d3 = cdigrad[3];
+        float* d4 = cdigrad[4];
+        float* d5 = cdigrad[5];
+        d3[coord] = d0[coord] * (d3[coord] + d5[coord]) + d3[coord] * d5[coord] - d1[coord] * d1[coord] -
+                    d2[coord] * d2[coord] - d4[coord] * d4[coord];
+      }
+  orc_fft_r2c_3d(N, cdigrad[3]);
+
+  /* lpt.c:433-494: 2nd-order displacement = source k/(i k^2); cdisp2 aliases cdigrad[0..2] */
+  float* cdisp2[3] = {cdigrad[0], cdigrad[1], cdigrad[2]};
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < N; j++)
+      for (int k = 0; k <= N / 2; k++) {
+        long coord = ((long)i * N + j) * nz + k;
+        if (i < N / 2)
+          kvec[0] = i * 2 * M_PI / Box;
+        else
+          kvec[0] = -(N - i) * 2 * M_PI / Box;
+        if (j < N / 2)
+          kvec[1] = j * 2 * M_PI / Box;
+        else
+          kvec[1] = -(N - j) * 2 * M_PI / Box;
+        if (k < N / 2)
+          kvec[2] = k * 2 * M_PI / Box;
+        else
+          kvec[2] = -(N - k) * 2 * M_PI / Box;
+        double kmag2 = kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2];
+        const float s0 = cdigrad[3][2 * coord], s1 = cdigrad[3][2 * coord + 1];
+        for (int a = 0; a < 3; a++) {
+          if (kmag2 > 0.0) {
+            cdisp2[a][2 * coord] = s1 * kvec[a] / kmag2;
+            cdisp2[a][2 * coord + 1] = -s0 * kvec[a] / kmag2;
+          } else
+            cdisp2[a][2 * coord] = cdisp2[a][2 * coord + 1] = 0.0;
+        }
+      }
+  for (int a = 0; a < 3; a++) {
+    orc_fft_c2r_3d(N, cdisp[a]);
+    orc_fft_c2r_3d(N, cdisp2[a]);
+  }
+}
+
 /* ======================================================================
  * lpt.c:145-578 (one rank: Local_x_start = 0, Local_nx = Nmesh)
  * ====================================================================== */
@@ -751,86 +836,13 @@ int orc_lpt_set_displacement(int N, double InitTime, double omega_m, int Seed, d
   free(seedtable);
   if (cdisp_out)
     for (int a = 0; a < 3; a++) memcpy(cdisp_out + a * 2 * csize, cdisp[a], sizeof(float) * 2 * csize);
-
-  /* lpt.c:358-398: ZA displacement gradients i k_j dis_i */
-  for (int i = 0; i < N; i++)
-    for (int j = 0; j < N; j++)
-      for (int k = 0; k <= N / 2; k++) {
-        long coord = ((long)i * N + j) * nz + k;
-        if (i < N / 2)
-          kvec[0] = i * 2 * M_PI / Box;
-        else
-          kvec[0] = -(N - i) * 2 * M_PI / Box;
-        if (j < N / 2)
-          kvec[1] = j * 2 * M_PI / Box;
-        else
-          kvec[1] = -(N - j) * 2 * M_PI / Box;
-        if (k < N / 2)
-          kvec[2] = k * 2 * M_PI / Box;
-        else
-          kvec[2] = -(N - k) * 2 * M_PI / Box;
-        static const int src[6] = {0, 0, 0, 1, 1, 2};
-        static const int dir[6] = {0, 1, 2, 1, 2, 2};
-        for (int g = 0; g < 6; g++) {
-          const float* c = cdisp[src[g]] + 2 * coord;
-          cdigrad[g][2 * coord] = -c[1] * kvec[dir[g]];
-          cdigrad[g][2 * coord + 1] = c[0] * kvec[dir[g]];
-        }
-      }
-  for (int g = 0; g < 6; g++) orc_fft_c2r_3d(N, cdigrad[g]);
-
-  /* lpt.c:409-422: second-order source into digrad[3] */
-  for (int i = 0; i < N; i++)
-    for (int j = 0; j < N; j++)
-      for (int k = 0; k < N; k++) {
-        long coord = ((long)i * N + j) * (2 * nz) + k;
-        float* d0 = cdigrad[0];
-        float* d1 = cdigrad[1];
-        float* d2 = cdigrad[2];
-        float* d3 = cdigrad[3];
-        float* d4 = cdigrad[4];
-        float* d5 = cdigrad[5];
-        d3[coord] = d0[coord] * (d3[coord] + d5[coord]) + d3[coord] * d5[coord] - d1[coord] * d1[coord] -
-                    d2[coord] * d2[coord] - d4[coord] * d4[coord];
-      }
-  orc_fft_r2c_3d(N, cdigrad[3]);
-
-  /* lpt.c:433-494: 2nd-order displacement = source k/(i k^2); cdisp2 aliases cdigrad[0..2] */
-  float* cdisp2[3] = {cdigrad[0], cdigrad[1], cdigrad[2]};
-  for (int i = 0; i < N; i++)
-    for (int j = 0; j < N; j++)
-      for (int k = 0; k <= N / 2; k++) {
-        long coord = ((long)i * N + j) * nz + k;
-        if (i < N / 2)
-          kvec[0] = i * 2 * M_PI / Box;
-        else
-          kvec[0] = -(N - i) * 2 * M_PI / Box;
-        if (j < N / 2)
-          kvec[1] = j * 2 * M_PI / Box;
-        else
-          kvec[1] = -(N - j) * 2 * M_PI / Box;
-        if (k < N / 2)
-          kvec[2] = k * 2 * M_PI / Box;
-        else
-          kvec[2] = -(N - k) * 2 * M_PI / Box;
-        double kmag2 = kvec[0] * kvec[0] + kvec[1] * kvec[1] + kvec[2] * kvec[2];
-        const float s0 = cdigrad[3][2 * coord], s1 = cdigrad[3][2 * coord + 1];
-        for (int a = 0; a < 3; a++) {
-          if (kmag2 > 0.0) {
-            cdisp2[a][2 * coord] = s1 * kvec[a] / kmag2;
-            cdisp2[a][2 * coord + 1] = -s0 * kvec[a] / kmag2;
-          } else
-            cdisp2[a][2 * coord] = cdisp2[a][2 * coord + 1] = 0.0;
-        }
-      }
-  for (int a = 0; a < 3; a++) {
-    orc_fft_c2r_3d(N, cdisp[a]);
-    orc_fft_c2r_3d(N, cdisp2[a]);
-  }
+  lpt_stages(N, Box, cdisp, cdigrad);
   if (disp_out)
     for (int a = 0; a < 3; a++) memcpy(disp_out + a * 2 * csize, cdisp[a], sizeof(float) * 2 * csize);
   if (disp2_out)
-    for (int a = 0; a < 3; a++) memcpy(disp2_out + a * 2 * csize, cdisp2[a], sizeof(float) * 2 * csize);
+    for (int a = 0; a < 3; a++) memcpy(disp2_out + a * 2 * csize, cdigrad[a], sizeof(float) * 2 * csize);
+  float* cdisp2[3] = {cdigrad[0], cdigrad[1], cdigrad[2]};
+#undef CD
 
   /* lpt.c:507-575: particles on the lattice */
   if (particles) {
@@ -864,7 +876,31 @@ int orc_lpt_set_displacement(int N, double InitTime, double omega_m, int Seed, d
       }
     }
   }
-#undef CD
+  for (int a = 0; a < 3; a++) free(cdisp[a]);
+  for (int a = 0; a < 6; a++) free(cdigrad[a]);
+  return 0;
+}
+
+/* The 2LPT stages alone (lpt.c:358-503) on a caller's ZA displacement
+   spectra cdisp_in (3 complex N x N x (N/2+1) grids, interleaved floats):
+   disp_out and disp2_out get the real ZA and 2LPT displacement grids in
+   the padded layout, disp2 already times 1/N^3 as lpt.c:542 applies it. */
+int orc_lpt_from_modes(int N, double Box, const float* cdisp_in, float* disp_out, float* disp2_out)
+{
+  const long csize = (long)N * N * (N / 2 + 1);
+  float* cdisp[3];
+  float* cdigrad[6];
+  for (int a = 0; a < 3; a++) {
+    cdisp[a] = malloc(sizeof(float) * 2 * csize);
+    memcpy(cdisp[a], cdisp_in + a * 2 * csize, sizeof(float) * 2 * csize);
+  }
+  for (int a = 0; a < 6; a++) cdigrad[a] = calloc(2 * csize, sizeof(float));
+  lpt_stages(N, Box, cdisp, cdigrad);
+  const double nmesh3_inv = 1.0 / pow((double)N, 3.0);
+  for (int a = 0; a < 3; a++) {
+    memcpy(disp_out + a * 2 * csize, cdisp[a], sizeof(float) * 2 * csize);
+    for (long c = 0; c < 2 * csize; c++) disp2_out[a * 2 * csize + c] = nmesh3_inv * cdigrad[a][c];
+  }
   for (int a = 0; a < 3; a++) free(cdisp[a]);
   for (int a = 0; a < 6; a++) free(cdigrad[a]);
   return 0;
@@ -1051,6 +1087,23 @@ static void readout(int Ngrid, float BoxSize, const Pos3* P, int np, int axes, c
     }
   }
 #undef RED
+}
+
+/* The CIC readout alone (pm.c:359-412 with the ghost copies of pm.c:416-532)
+   of one caller grid (padded real layout) at the particles: out[i] = the
+   particle's gather plus its ghost's gather. */
+void orc_pm_readout(int Ngrid, float BoxSize, const OrcParticle* p, int np, const float* grid, float* out)
+{
+  Pos3* pos = malloc(sizeof(Pos3) * 2 * (size_t)np + sizeof(Pos3));
+  int* index = malloc(sizeof(int) * (size_t)np + sizeof(int));
+  float* fall = malloc(sizeof(float) * 3 * (2 * (size_t)np + 1));
+  const int ng = make_ghosts(Ngrid, BoxSize, p, np, pos, index);
+  readout(Ngrid, BoxSize, pos, np + ng, 0, grid, fall);
+  for (int i = 0; i < np; i++) out[i] = fall[3 * i];
+  for (int n = 0; n < ng; n++) out[index[n]] += fall[3 * (np + n)];
+  free(pos);
+  free(index);
+  free(fall);
 }
 
 /* pm.c:565-596.  force is np x 3; density_out (padded real, before the FFT)

@@ -80,12 +80,19 @@ void orc_fft_c2r_3d(int n, float* data);   /* complex in, padded real out, in pl
 int orc_lpt_set_displacement(int nc, double a_init, double omega_m, int seed,
                              double box, OrcParticle* particles,
                              float* cdisp_out, float* disp_out, float* disp2_out);
+/* lpt.c:358-503 on given ZA displacement spectra (3 complex nc x nc x (nc/2+1)
+   grids): real ZA displacement (unnormalised c2r, as lpt.c uses it) and 2LPT
+   displacement times 1/nc^3, padded real layout. */
+int orc_lpt_from_modes(int nc, double box, const float* cdisp_in, float* disp_out, float* disp2_out);
 
 /* ---- pm.c / move.c ---- */
 void orc_wrap(OrcParticle* p, int np, float boxsize);
 void orc_pm_deposit(int ngrid, float boxsize, float wpar, const OrcParticle* p,
                     int np, float* density);
 double orc_density_sum(int ngrid, const float* density);
+/* pm.c:359-412 + ghosts (pm.c:416-532): CIC readout of one padded real grid
+   at np particles into out[np]. */
+void orc_pm_readout(int ngrid, float boxsize, const OrcParticle* p, int np, const float* grid, float* out);
 int orc_pm_calculate_forces(int ngrid, float boxsize, float wpar, long long np_total,
                             const OrcParticle* p, int np, float* force,
                             float* density_out, float* force_grid_out, double* mass_sum);

@@ -30,7 +30,7 @@ powerspectrum= "data/linear_matterpower.dat"
 """
 
 CASES = {
-    "reference_param_lua": (open(os.path.join(HERE, "param_reference.lua")).read(), 1),
+    "full_param_file": (open(os.path.join(HERE, "param_full.lua")).read(), 1),
     "minimal": (BASE, 1),
     "no_spaces": ("nc=32\nboxsize=50\nntimestep=5\noutput_redshifts={0}\nomega_m=0.3\nh=0.7\nsigma8=0.8\n"
                   "powerspectrum=\"pk.dat\"\n", 1),

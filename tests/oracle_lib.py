@@ -54,6 +54,8 @@ def load():
         "orc_fft_r2c_3d": (None, [i, vp]),
         "orc_fft_c2r_3d": (None, [i, vp]),
         "orc_lpt_set_displacement": (i, [i, d, d, i, d, vp, vp, vp, vp]),
+        "orc_lpt_from_modes": (i, [i, d, vp, vp, vp]),
+        "orc_pm_readout": (None, [i, f, vp, i, vp, vp]),
         "orc_wrap": (None, [vp, i, f]),
         "orc_pm_deposit": (None, [i, f, f, vp, i, vp]),
         "orc_density_sum": (d, [i, vp]),
@@ -119,6 +121,29 @@ def lpt(nc, a_init, omega_m, seed, box, want_grids=False):
     if want_grids:
         return parts, cdisp.reshape(3, -1), disp.reshape(3, -1), disp2.reshape(3, -1)
     return parts
+
+
+def lpt_from_modes(n, box, cdisp):
+    """2LPT stages on ZA spectra cdisp (complex [3, n, n, n/2+1]).  Returns
+    (disp, disp2) as real [3, n, n, n]: the ZA displacement and the 2LPT
+    displacement with lpt.c's 1/n^3 applied."""
+    c = np.ascontiguousarray(cdisp, np.complex64)
+    assert c.shape == (3, n, n, n // 2 + 1)
+    disp = np.zeros((3,) + padded_real_shape(n), np.float32)
+    disp2 = np.zeros((3,) + padded_real_shape(n), np.float32)
+    assert lib().orc_lpt_from_modes(n, box, c.ctypes.data, disp.ctypes.data, disp2.ctypes.data) == 0
+    return disp[..., :n], disp2[..., :n]
+
+
+def pm_readout(ngrid, box, parts, grid):
+    """CIC readout (with ghosts) of a real grid [ngrid, ngrid, ngrid] or its
+    padded layout at the particles."""
+    g = np.zeros(padded_real_shape(ngrid), np.float32)
+    g[..., :ngrid] = np.asarray(grid)[..., :ngrid]
+    parts = np.ascontiguousarray(parts)
+    out = np.zeros(len(parts), np.float32)
+    lib().orc_pm_readout(ngrid, box, parts.ctypes.data, len(parts), g.ctypes.data, out.ctypes.data)
+    return out
 
 
 def pm_forces(ngrid, box, wpar, parts, want_grids=False):

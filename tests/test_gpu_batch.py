@@ -62,3 +62,24 @@ def test_batch_equals_single_runs(eh_table, graph):
         with _ctx(nc, ng, box, eh_table) as c:
             (a, ref), = ch.run_cola(c, OM, 1.0, 10, seed, [0.0])
         assert np.array_equal(snap, ref), (nc, seed)
+
+
+def test_batch_snapshot_after_kick_matches_single_run(eh_table):
+    """With 49 steps to a = 1, 49 * (1/49) < float32(1.0) in double, so the
+    output time falls between apos0 and avel1 of the last step: main.c kicks
+    to avel1 before the snapshot (main.c:195-203).  Batched runs must do the
+    same as run_cola."""
+    print("batch 49 steps", flush=True)
+    da = 1.0 / 49
+    assert 49 * da < float(np.float32(1.0)) <= 49.5 * da
+    shapes = [(16, 48, 50.0, 8), (24, 72, 75.0, 9)]
+    ctxs = [_ctx(nc, ng, box, eh_table) for nc, ng, box, _ in shapes]
+    try:
+        snaps = ch.run_cola_batch(ctxs, OM, 1.0, 49, [s[3] for s in shapes])
+    finally:
+        for c in ctxs:
+            c.close()
+    for (nc, ng, box, seed), snap in zip(shapes, snaps):
+        with _ctx(nc, ng, box, eh_table) as c:
+            (a, ref), = ch.run_cola(c, OM, 1.0, 49, seed, [0.0])
+        assert np.array_equal(snap, ref), (nc, seed)

@@ -217,13 +217,20 @@ def test_corrected_graph_steps_equal_eager_every_replay(eh_table):
     print("net graph replay 32^3 / 64^3", flush=True)
     k, p = eh_table
     nc, ng, box = 32, 64, 100.0
-    with ch.ResidualNet(ng, seed=5) as net, ch.Context(nc, ng, box) as a, ch.Context(nc, ng, box) as b:
-        net.set_scales(1.0, 0.5)
-        for c in (a, b):
+    # one net per context: a net's activation buffers serve the context it
+    # is attached to; the two nets hold the same parameters
+    with ch.ResidualNet(ng, seed=5) as net_a, ch.ResidualNet(ng, seed=5) as net_b, \
+            ch.Context(nc, ng, box) as a, ch.Context(nc, ng, box) as b:
+        net_b.set_params(net_a.params())
+        for c, net in ((a, net_a), (b, net_b)):
+            net.set_scales(1.0, 0.5)
             c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
             c.set_residual_net(net)
             c.lpt_set_displacement(0.1, OM, 21)
             c.set_times(c.times()[0], 0.1)
+        with pytest.raises(ch.ColaError) as e:  # a second context may not share net_a
+            b.set_residual_net(net_a)
+        assert e.value.code == 9004
         b.set_step_graph(True)
         omf = float(np.float32(OM))
         for istep in range(1, 10):
@@ -237,3 +244,31 @@ def test_corrected_graph_steps_equal_eager_every_replay(eh_table):
             c.move_particles()
             c.pm_calculate_forces()
         assert np.array_equal(a.get_forces(), b.get_forces())
+
+
+def test_destroying_an_attached_net_detaches_it(eh_table):
+    """cola_resnet_destroy on a net still attached to a context detaches it
+    and drops the context's captured step graph: from the same state, the
+    context's next graph steps equal those of a context that never had a
+    net."""
+    print("net destroy while attached 16^3 / 32^3", flush=True)
+    k, p = eh_table
+    nc, ng, box = 16, 32, 50.0
+    omf = float(np.float32(OM))
+    with ch.Context(nc, ng, box) as a, ch.Context(nc, ng, box) as b:
+        for c in (a, b):
+            c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+            c.lpt_set_displacement(0.1, OM, 4)
+            c.set_times(c.times()[0], 0.1)
+            c.set_step_graph(True)
+        start = b.get_particles()
+        net = ch.ResidualNet(ng, seed=2)
+        net.set_scales(1.0, 0.5)
+        a.set_residual_net(net)
+        a.step(omf, 0.15, 0.2)  # captures a step graph with the net in it
+        net.close()
+        a.set_particles(start, 0.1, 0.1)
+        for istep in range(1, 4):
+            for c in (a, b):
+                c.step(omf, (istep + 0.5) * 0.1, (istep + 1.0) * 0.1)
+        assert np.array_equal(a.get_particles(), b.get_particles())

@@ -373,3 +373,234 @@ def test_kick_drift_keep_lpt_trajectory():
     expect = x0 + p["dx1"] * dD + p["dx2"] * d2
     assert np.abs(p["x"] - expect).max() < 1e-5
     assert (ax.value, av.value) == (np.float32(0.5), np.float32(0.45))
+
+
+# ------------------------------------------------------------------ CIC readout (pm.c:359-412)
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_readout_interpolates_a_linear_ramp_exactly(axis):
+    """A field linear in the cell index along one axis, g = 1 + 0.5 i_axis,
+    is read out as 1 + 0.5 X_axis (X = x Ng / L) wherever the stencil does
+    not cross the periodic seam of that axis; along the other axes the field
+    is constant, so particles in the last x layer (read through their ghost
+    copies, pm.c:416-532) get the same exact value.  Swapped D/T weights or a
+    wrong stencil index fail this."""
+    ng, box = 12, 30.0
+    rng = np.random.default_rng(10 + axis)
+    n = 400
+    p = np.zeros(n, orc.PARTICLE_DTYPE)
+    p["x"] = rng.uniform(0, box, (n, 3)).astype(np.float32)
+    # stay off the seam of the ramp axis: cell coordinate in [0, ng - 1)
+    p["x"][:, axis] = rng.uniform(0, box * (ng - 1) / ng * 0.999, n).astype(np.float32)
+    if axis != 0:
+        p["x"][:40, 0] = rng.uniform(box - box / ng, box, 40).astype(np.float32)  # ghost layer
+    idx = np.indices((ng, ng, ng))[axis].astype(np.float32)
+    out = orc.pm_readout(ng, box, p, 1.0 + 0.5 * idx)
+    X = p["x"][:, axis].astype(np.float64) * ng / box
+    assert np.abs(out - (1.0 + 0.5 * X)).max() < 1e-5 * (1.0 + 0.5 * ng)
+
+
+def test_readout_is_adjoint_of_deposit():
+    """The readout and the deposit use the same CIC weights (pm.c:229-276 and
+    pm.c:384-410, ghost copies included): for any grid g,
+    sum_p readout_p(g) = sum_c g_c (delta_c + 1) / WPAR."""
+    ng, box, wpar = 10, 20.0, 8.0
+    rng = np.random.default_rng(3)
+    n = 500
+    p = np.zeros(n, orc.PARTICLE_DTYPE)
+    p["x"] = rng.uniform(0, box, (n, 3)).astype(np.float32)
+    p["x"][:50, 0] = rng.uniform(box - box / ng, box, 50).astype(np.float32)
+    g = rng.standard_normal((ng, ng, ng)).astype(np.float32)
+    lhs = orc.pm_readout(ng, box, p, g).astype(np.float64).sum()
+    d = np.zeros(orc.padded_real_shape(ng), np.float32)
+    orc.lib().orc_pm_deposit(ng, box, wpar, p.ctypes.data, n, d.ctypes.data)
+    w = (d[..., :ng].astype(np.float64) + 1.0) / wpar
+    rhs = (g.astype(np.float64) * w).sum()
+    assert lhs == pytest.approx(rhs, abs=1e-4 * np.abs(g).max() * n ** 0.5)
+    # a single particle: its 8 weights sum to one
+    one = orc.pm_readout(ng, box, p[:1], np.ones((ng, ng, ng), np.float32))
+    assert one[0] == pytest.approx(1.0, abs=1e-6)
+
+
+# ------------------------------------------------------------------ 2LPT stages (lpt.c:358-503)
+def test_lpt_two_mode_second_order_closed_form():
+    """ZA displacement psi = A k1 sin(k1.x + al) + B k2 sin(k2.x + be)
+    (curl-free, two non-parallel modes with off-diagonal gradients and
+    phases, so the spectra have real and imaginary parts).  The 2LPT source
+    of lpt.c:409-422 (sum of the principal 2x2 minors of d_ij = d_j psi_i)
+    is A B |k1 x k2|^2 cos(k1.x + al) cos(k2.x + be), and the 2LPT
+    displacement of lpt.c:433-503 (div psi2 = source, curl-free) is
+    (A B |k1 x k2|^2 / 2) sum_{K = k1 +- k2} K sin(K.x + al +- be) / |K|^2."""
+    n, box = 16, 50.0
+    kf = 2 * np.pi / box
+    n1, n2 = np.array([1, 2, 0]), np.array([0, 1, -1])
+    k1, k2 = kf * n1, kf * n2
+    A, B, al, be = 2.0, 3.0, 0.3, 1.1
+    x = np.indices((n, n, n)).astype(np.float64) * (box / n)
+    th1 = np.tensordot(k1, x, 1) + al
+    th2 = np.tensordot(k2, x, 1) + be
+    psi = A * k1[:, None, None, None] * np.sin(th1) + B * k2[:, None, None, None] * np.sin(th2)
+    cdisp = np.fft.rfftn(psi, axes=(1, 2, 3)) / n ** 3  # lpt.c's c2r is unnormalised
+    disp, disp2 = orc.lpt_from_modes(n, box, cdisp)
+    assert np.abs(disp - psi).max() < 1e-5 * np.abs(psi).max()
+    c2 = np.sum(np.cross(k1, k2) ** 2)
+    expect = np.zeros_like(psi)
+    for K, ph in ((k1 + k2, al + be), (k1 - k2, al - be)):
+        expect += (A * B * c2 / 2) * K[:, None, None, None] * np.sin(np.tensordot(K, x, 1) + ph) / (K @ K)
+    scale = np.abs(expect).max()
+    assert scale > 0
+    assert np.abs(disp2 - expect).max() < 2e-5 * scale
+
+
+def test_lpt_single_mode_has_no_second_order():
+    """A single plane wave is one-dimensional: every 2x2 minor of d_ij
+    vanishes, so the 2LPT displacement is zero (to float rounding)."""
+    n, box = 16, 40.0
+    k = 2 * np.pi / box * np.array([1, -2, 3])
+    x = np.indices((n, n, n)).astype(np.float64) * (box / n)
+    psi = 1.5 * k[:, None, None, None] * np.sin(np.tensordot(k, x, 1))
+    cdisp = np.fft.rfftn(psi, axes=(1, 2, 3)) / n ** 3
+    _, disp2 = orc.lpt_from_modes(n, box, cdisp)
+    assert np.abs(disp2).max() < 1e-6 * np.abs(psi).max() * np.abs(k).max()
+
+
+# ------------------------------------------------------------------ white-noise modes (lpt.c:172-351)
+def _draw_pairs(seed, count):
+    """(phase, amplitude) uniform pairs of one column's RANLUX stream with
+    the redraw of a zero amplitude (lpt.c:228-233)."""
+    r = orc.Ranlxd()
+    orc.lib().orc_ranlxd_set(C.byref(r), int(seed), 202)
+    out = []
+    for _ in range(count):
+        u1 = orc.lib().orc_ranlxd_uniform(C.byref(r))
+        u2 = orc.lib().orc_ranlxd_uniform(C.byref(r))
+        while u2 == 0.0:
+            u2 = orc.lib().orc_ranlxd_uniform(C.byref(r))
+        out.append((u1, u2))
+    return out
+
+
+@pytest.mark.parametrize("n,box,seed", [(8, 40.0, 100), (16, 64.0, 7)])
+def test_lpt_modes_divergence_and_curl(n, box, seed):
+    """The ZA displacement modes are curl-free (k x psi_k = 0) and their
+    divergence is the white-noise density: i k . psi_k = -delta_k with
+    delta_k = (2pi/L)^1.5 sqrt(P(k) (-ln u2)) exp(2 pi i u1), (u1, u2) the
+    draw pair of mode kz of column (i, j) seeded from the seed table
+    (lpt.c:219-284).  Checked on every mode the reference sets from its own
+    column's draws: 0 < kz < N/2, and on the kz = 0 plane the columns it owns
+    (the mirrored half is pinned by the Hermitian test)."""
+    k, p = ch.eh98_table()
+    orc.set_power_table(k, p)
+    _, cdisp, _, _ = orc.lpt(n, 0.1, 0.273, seed, box, want_grids=True)
+    c = cdisp.view(np.complex64).reshape(3, n, n, n // 2 + 1).astype(np.complex128)
+    table = orc.seedtable(n, seed)
+    fac = (2 * np.pi / box) ** 1.5
+    w = np.where(np.arange(n) < n // 2, np.arange(n), np.arange(n) - n) * 2 * np.pi / box
+    checked = 0
+    for i in range(n):
+        for j in range(n):
+            draws = _draw_pairs(table[i, j], n // 2)
+            for kz in range(n // 2):
+                if i == n // 2 or j == n // 2 or (i == 0 and j == 0 and kz == 0):
+                    continue
+                if kz == 0 and not (i < n // 2 and (i > 0 or j < n // 2)):
+                    continue
+                kv = np.array([w[i], w[j], kz * 2 * np.pi / box])
+                psi = c[:, i, j, kz]
+                kk = np.sqrt(kv @ kv)
+                u1, u2 = draws[kz]
+                delta = fac * np.sqrt(orc.lib().orc_powerspec(kk) * -np.log(u2)) * np.exp(2j * np.pi * u1)
+                assert 1j * (kv @ psi) == pytest.approx(-delta, rel=2e-6, abs=0), (i, j, kz)
+                curl = np.cross(kv, psi)
+                assert np.abs(curl).max() <= 2e-6 * kk * np.abs(psi).max(), (i, j, kz)
+                checked += 1
+    assert checked > n * n * (n // 2 - 1) * 0.5
+
+
+@pytest.mark.parametrize("n1,n2", [(8, 16), (16, 64)])
+def test_seedtable_shells_are_resolution_independent(n1, n2):
+    """lpt.c:172-207 fills the table in shells of growing max(|i|, |j|) (the
+    corner blocks i, N-1-i), so the low-|k| columns keep their seeds when the
+    mesh grows: the n1 table is the n2 table's corner blocks; and every cell
+    of the table is written exactly once from the first N^2 draws."""
+    t1 = orc.seedtable(n1, 5)
+    t2 = orc.seedtable(n2, 5)
+    h = n1 // 2
+    idx = np.r_[np.arange(h), np.arange(n2 - h, n2)]
+    assert np.array_equal(t2[np.ix_(idx, idx)], t1)
+    u = ch.ranlxd1_uniforms(5, n2 * n2)
+    draws = np.sort((0x7fffffff * u).astype(np.uint32))
+    assert np.array_equal(np.sort(t2.ravel()), draws)
+
+
+# ------------------------------------------------------------------ snapshot velocities (cola.c:297-416)
+def _growth_gold():
+    return {c["a"]: c for c in _json("cosmo_golden.json")["growth"]}
+
+
+def test_noncola_initial_velocity_closed_form():
+    """set_noncola_initial (cola.c:297-338): v = 100/a (dx1 dD/dy + dx2
+    dD2/dy) km/s with dD/dy = Q dD/da from the exact growth integral
+    (mpmath fixture) and dD2/dy = growthD2v; positions pass through."""
+    gold = _growth_gold()
+    rng = np.random.default_rng(8)
+    n = 50
+    p = np.zeros(n, orc.PARTICLE_DTYPE)
+    p["x"] = rng.uniform(0, 10, (n, 3))
+    p["dx1"] = rng.normal(0, 1, (n, 3))
+    p["dx2"] = rng.normal(0, 0.2, (n, 3))
+    for a in (0.3, 0.5):
+        out = np.zeros(n, orc.PARTICLE_MIN_DTYPE)
+        orc.lib().orc_set_noncola_initial(p.ctypes.data, n, OM32, a, out.ctypes.data)
+        dv2 = orc.lib().orc_growthD2v(a, OM32)
+        want = 100.0 / a * (p["dx1"] * gold[a]["QdDda"] + p["dx2"] * dv2)
+        assert np.abs(out["v"] - want).max() < 3e-5 * np.abs(want).max()
+        assert np.array_equal(out["x"], p["x"])
+
+
+def test_snapshot_on_the_lpt_trajectory_closed_form():
+    """cola_set_snapshot (cola.c:341-416) with zero COLA velocity and the
+    force equal to the 2LPT force (so the kick adds nothing): the snapshot
+    sits on the 2LPT trajectory, x(aout) = x(a) + dx1 (D(aout) - D(a)) +
+    dx2 (D2(aout) - D2(a)), with the 2LPT velocity 100/aout (dx1 Q dD/da +
+    dx2 growthD2v) km/s; D and Q dD/da from the exact growth integral."""
+    gold = _growth_gold()
+    rng = np.random.default_rng(9)
+    n = 50
+    p = np.zeros(n, orc.PARTICLE_DTYPE)
+    p["x"] = rng.uniform(0, 10, (n, 3))
+    p["dx1"] = rng.normal(0, 1, (n, 3))
+    p["dx2"] = rng.normal(0, 0.2, (n, 3))
+    a_x, a_v, aout = 0.3, 0.35, 0.5
+    f = ch.step_factors(OM32, a_x, a_v, 0.45, 0.5)
+    force = np.ascontiguousarray((-(p["dx1"] * f[1] + p["dx2"] * f[2]) / (1.5 * OM32)).astype(np.float32))
+    out = np.zeros(n, orc.PARTICLE_MIN_DTYPE)
+    orc.lib().orc_set_snapshot(aout, p.ctypes.data, force.ctypes.data, n, OM32, a_x, a_v, out.ctypes.data)
+    d2 = orc.lib().orc_growthD2(aout, OM32) - orc.lib().orc_growthD2(a_x, OM32)
+    want_x = p["x"] + p["dx1"] * (gold[aout]["D"] - gold[a_x]["D"]) + p["dx2"] * d2
+    assert np.abs(out["x"] - want_x).max() < 2e-5 * 10
+    want_v = 100.0 / aout * (p["dx1"] * gold[aout]["QdDda"] + p["dx2"] * orc.lib().orc_growthD2v(aout, OM32))
+    assert np.abs(out["v"] - want_v).max() < 5e-5 * np.abs(want_v).max()
+    assert np.array_equal(out["id"], p["id"])
+
+
+def test_snapshot_kick_and_drift_of_the_cola_velocity():
+    """With no LPT displacement, the snapshot is the COLA velocity kicked by
+    the force over Sphi(a_v, aout; a) and drifted over Sq(a, aout; a_v)
+    (cola.c:376-410), velocity in km/s: v_out = 100/aout (v - 1.5 Om f Sphi),
+    x_out = x + v Sq, with Sq, Sphi from the mpmath fixtures."""
+    case = [c for c in _json("cosmo_golden.json")["time_integrals"] if c["ai"] == 0.15][0]
+    ai, af, aref = case["ai"], case["af"], case["aref"]
+    rng = np.random.default_rng(12)
+    n = 40
+    p = np.zeros(n, orc.PARTICLE_DTYPE)
+    p["x"] = rng.uniform(0, 10, (n, 3))
+    p["v"] = rng.normal(0, 1, (n, 3))
+    force = np.ascontiguousarray(rng.normal(0, 1, (n, 3)).astype(np.float32))
+    out = np.zeros(n, orc.PARTICLE_MIN_DTYPE)
+    # the kick integrates Sphi(AI = a_v, AF = aout; A = a_x)
+    orc.lib().orc_set_snapshot(af, p.ctypes.data, force.ctypes.data, n, OM32, aref, ai, out.ctypes.data)
+    want_v = 100.0 / af * (p["v"] - 1.5 * OM32 * force * case["Sphi"])
+    assert np.abs(out["v"] - want_v).max() < 3e-5 * np.abs(want_v).max()
+    # the drift integrates Sq(A = a_x, AF = aout; AC = a_v)
+    orc.lib().orc_set_snapshot(af, p.ctypes.data, force.ctypes.data, n, OM32, ai, aref, out.ctypes.data)
+    assert np.abs(out["x"] - (p["x"] + p["v"] * case["Sq"])).max() < 2e-5 * 10

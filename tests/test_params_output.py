@@ -53,7 +53,7 @@ def test_fixture_is_current_with_reference_binary(tmp_path):
     spec = importlib.util.spec_from_file_location("gen", os.path.join(HERE, "golden", "gen_params_golden.py"))
     gen = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(gen)
-    for case in ("reference_param_lua", "array_blank_before_comma", "confirm_missing_power", "crlf_line_ends"):
+    for case in ("full_param_file", "array_blank_before_comma", "confirm_missing_power", "crlf_line_ends"):
         got = gen.run_reference(GOLDEN[case]["text"], GOLDEN[case]["confirm"])
         assert got["rc"] == GOLDEN[case]["rc"]
         assert got["params"] == GOLDEN[case]["params"]
