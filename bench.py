@@ -1,20 +1,31 @@
-"""Benchmark: particle-steps/s of plain COLA on one MI355X.
+"""Benchmark: particle-steps/s on one MI355X of the learned-corrected coarse
+COLA run, gated on its P(k) agreeing with the plain fine-grid run.
 
-Workload (the configuration the metric is quoted on): 256^3 particles,
-768^3 PM grid (pm_nc_factor 3), box 500 Mpc/h, 2LPT ICs from seed 1 at
-a = 0.1, linear-in-a steps of da = 0.1 to a = 1, EH98 no-wiggle P(k)
-(Om 0.273, Ob 0.046, h 0.705, ns 0.96, sigma8 0.8).
+Default workload (the production comparison): 512^3 particles, box 1000
+Mpc/h, 2LPT ICs from seed 1 at a = 0.1, ten linear-in-a steps to a = 1,
+EH98 no-wiggle P(k) (Om 0.273, Ob 0.046, h 0.705, ns 0.96, sigma8 0.8).
+  * corrected run: 768^3 PM grid (factor 1.5) + the learned force residual
+    (4-layer 3-D conv net, BF16 MFMA, weights in cola_halo_amd/nets/,
+    trained on 128^3-particle force pairs in a 250 Mpc/h box: same cell
+    size and particle spacing);
+  * plain fine run: 1536^3 PM grid (factor 3), no net.
+A step is one COLA time step of main.c:162-206 (wrap, PM force, fused
+readout/kick/drift); the tenth step of each 10-step run is the final force
+at a = 1.  particle-steps = nc^3 x force evaluations.  Inputs are resident
+in HBM when a timed region starts; each run's K steps are timed on their
+own (synchronised on both sides) and cycle the 10-step schedule from the
+saved initial conditions.
 
-A step is one COLA time step of main.c:162-206: wrap, PM force, kick, drift
-(fused readout/kick/drift); the tenth step of each 10-step run is the final
-force evaluation at a = 1 (the snapshot itself is outside the timed region,
-as in the reference's COLA timer category).  particle-steps = nc^3 x force
-evaluations.  Inputs are resident in HBM when the timed region starts.
-
-Prints one JSON line.  The roofline object is for the kernel with the most
-device time in the timed region, from HIP events on the context's stream;
-cpu_baseline times the CPU oracle (C restatement, OpenMP) on a bounded
-sample: one step of the same workload from the same initial conditions.
+`value` is the corrected run's particle-steps/s when the P(k) gate holds:
+after a complete untimed 10-step run of each, |P_corrected / P_fine - 1| <=
+1% for every shell with k <= 1 h/Mpc (768^3 shell estimator, CIC window
+divided out); it is 0 when the gate fails.  `speedup_vs_fine` is the ratio
+of the two runs' throughputs.  `roofline` is for the kernel with the most
+device time per corrected step (HIP events on the context's stream during
+the timed region); `cpu_baseline` times the CPU restatement (oracle, C +
+OpenMP) on one plain step of the coarse grid.  `--workload plain` times
+plain COLA at 256^3 / 768^3 (box 500) instead, `--ensemble` the 48
+realizations of uneven sizes.
 """
 import argparse
 import json
@@ -28,35 +39,57 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 METRIC = "particle-steps/s on 1 MI355X at P(k) err ≤1% (k≤1 h/Mpc) vs plain fine grid"
 
-NC, NG, BOX, SEED, NSTEPS = 256, 768, 500.0, 1, 10
+NSTEPS = 10
 OM = 0.273
+PROD = dict(nc=512, coarse=768, fine=1536, box=1000.0, seed=1)
+PLAIN = dict(nc=256, ng=768, box=500.0, seed=1)
+NET_WEIGHTS = os.path.join(ROOT, "cola_halo_amd", "nets", "eh98_cell1.30.net")
+PK_KMAX, PK_BINS, PK_TOL = 1.0, 25, 0.01
 
 
-def kernel_bytes(phase, nc, ng):
-    """Algorithmic HBM bytes of one launch: each input array read once and
-    each output array written once (FP32), per the canonical per-phase model."""
+# launches of one kernel inside one profiling phase (the hidden layers 2 and 3
+# are one phase of two k_conv16 launches)
+LAUNCHES_PER_PHASE = {"net_conv_hidden": 2}
+
+
+def kernel_cost(phase, nc, ng):
+    """(algorithmic HBM bytes, MFMA flops, kernel) of one launch: each input
+    array read once and each output array written once (FP32 PM arrays,
+    bfloat16 activations), per the canonical per-phase model."""
     n_p = nc ** 3
     n_g = ng ** 3
     n_c = ng * ng * (ng // 2 + 1)
+    conv16 = 27 * 16 * 16 * 2 * n_g  # useful flops of one 16 -> 16 channel 3^3 layer
     table = {
-        "fft_z_r2c": 4 * n_g + 8 * n_c,
-        "fft_y_fwd": 16 * n_c,
-        "fft_x_fwd": 16 * n_c,
-        "fft_x_inv": 16 * n_c,   # reads delta_k, writes F_k (Poisson fused on load)
-        "fft_x_fused": 32 * n_c,  # reads delta_k once, writes the three F_k
-        "fft_y_inv": 16 * n_c,
-        "fft_z_c2r": 8 * n_c + 4 * n_g,
-        "deposit": 4 * n_g + 12 * n_p,
-        "readout_kick_drift": 12 * n_g + 72 * n_p,
-        "bin_count": 12 * n_p,
-        "bin_scatter": 24 * n_p,
+        "fft_z_r2c": (4 * n_g + 8 * n_c, 0, "fft_tile (z r2c rows)"),
+        "fft_y_fwd": (16 * n_c, 0, "fft_tile (y columns)"),
+        "fft_x_fwd": (16 * n_c, 0, "fft_tile (x columns)"),
+        "fft_x_inv": (16 * n_c, 0, "fft_tile (x columns)"),
+        "fft_x_fused": (32 * n_c, 0, "fft_x_fused"),  # reads delta_k once, writes the three F_k
+        "fft_y_inv": (16 * n_c, 0, "fft_tile (y columns)"),
+        "fft_z_c2r": (8 * n_c + 4 * n_g, 0, "fft_tile (z c2r rows)"),
+        "deposit": (4 * n_g + 12 * n_p, 0, "k_deposit_tiles"),
+        "readout_kick_drift": (12 * n_g + 72 * n_p, 0, "k_readout_kick_drift"),
+        "bin_count": (12 * n_p, 0, "k_bin_count"),
+        "bin_scatter": (24 * n_p, 0, "k_bin_scatter"),
+        # net: bfloat16 activations of 16 channels = 32 bytes per cell
+        "net_features": (4 * n_g + 2 * n_g, 0, "k_features"),
+        "net_conv1": (2 * n_g + 32 * n_g, 27 * 16 * 2 * n_g, "k_conv1"),
+        "net_conv_hidden": (64 * n_g, conv16, "k_conv16<kFwdRelu>"),
+        "net_conv_out_add": (32 * n_g + 24 * n_g, 27 * 16 * 3 * 2 * n_g, "k_conv16<kFwdAdd>"),
     }
     return table.get(phase)
 
 
-def measured_traffic(kernel, nc, ng):
+def kernel_bytes(phase, nc, ng):
+    c = kernel_cost(phase, nc, ng)
+    return c[0] if c else None
+
+
+def measured_traffic(kernel, nc, ng, net=False):
     """HBM bytes per launch of `kernel` from the calibrated rocprofv3 counter
     table (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
     counter reads for a copy kernel of known size), or None when the table
@@ -68,17 +101,69 @@ def measured_traffic(kernel, nc, ng):
     except (OSError, ValueError):
         return None
     for entry in table.get("workloads", []):
-        if entry.get("nc") == nc and entry.get("ngrid") == ng:
+        if entry.get("nc") == nc and entry.get("ngrid") == ng and entry.get("net", False) == net:
             k = entry.get("kernels", {}).get(kernel)
             if k is not None:
                 return int(k["fetch_bytes"] + k["write_bytes"])
     return None
 
 
+def roofline(stats, nc, ng, net=False):
+    """Roofline of the phase with the most device time: algorithmic bytes
+    (and MFMA flops) per launch over the mean launch time; the binding roof
+    is the larger of bytes / 8 TB/s and flops / 2.5 PFLOP/s (dense BF16)."""
+    dom, (dom_ms, dom_n) = max(stats.items(), key=lambda kv: kv[1][0])
+    cost = kernel_cost(dom, nc, ng)
+    if cost is None or dom_n == 0:
+        return None
+    nbytes, flops, kname = cost
+    launches = dom_n * LAUNCHES_PER_PHASE.get(dom, 1)
+    avg_s = dom_ms / launches / 1e3
+    t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
+    t_mfma = flops / (MFMA_BF16_PEAK_TFLOPS * 1e12)
+    out = {"bound": "hbm" if t_hbm >= t_mfma else "mfma"}
+    if out["bound"] == "hbm":
+        achieved = nbytes / avg_s / 1e9
+        out.update({"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4)})
+    else:
+        achieved = flops / avg_s / 1e12
+        out.update({"achieved": round(achieved, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4)})
+    out.update({"traffic": measured_traffic(dom, nc, ng, net), "kernel": kname, "phase": dom,
+                "avg_launch_ms": round(avg_s * 1e3, 4), "bytes_per_launch": nbytes})
+    if flops:
+        out["mfma_tflops"] = round(flops / avg_s / 1e12, 1)
+        out["mfma_frac"] = round(flops / avg_s / 1e12 / MFMA_BF16_PEAK_TFLOPS, 4)
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where the environment sets the job's CPU share."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
 def cpu_baseline(ctx_parts, nc, ng, box, threads):
-    """Oracle on the host: one PM force + kick + drift of the workload."""
+    """CPU restatement of the reference's path (the oracle: C + OpenMP) on
+    the host: one plain COLA step (wrap, PM force, kick, drift)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     import ctypes as C
 
     import oracle_lib as orc
@@ -97,8 +182,214 @@ def cpu_baseline(ctx_parts, nc, ng, box, threads):
     lib.orc_drift(parts.ctypes.data, len(parts), Om, C.byref(ax), C.byref(av), 0.2)
     dt = time.perf_counter() - t0
     return {"value": nc ** 3 / dt, "unit": "particle-steps/s", "cores": int(threads), "kind": "port",
-            "sample": f"1 step (wrap + PM force on {ng}^3 + kick + drift) of the {nc}^3 workload, "
-                      f"{dt:.1f} s, oracle status {st}"}
+            "cpu": cpu_model(),
+            "sample": f"CPU restatement (the oracle, C + OpenMP on {threads} threads), not the reference binary: "
+                      f"1 plain step (wrap + PM force on {ng}^3 + kick + drift) of {nc}^3 particles from the 2LPT "
+                      f"state, {dt:.1f} s, oracle status {st}"}
+
+
+def timed_steps(ctx, nsteps, Om):
+    """nsteps COLA steps cycling the 10-step schedule from the saved initial
+    conditions (the tenth step of a cycle is the force at a = 1); returns the
+    number of force evaluations."""
+    da = 1.0 / NSTEPS
+    forces = 0
+    for s in range(nsteps):
+        istep = s % NSTEPS + 1
+        if istep == 1 and s > 0:
+            ctx.state_restore()
+        if istep < NSTEPS:
+            ctx.step(Om, (istep + 0.5) * da, (istep + 1.0) * da)
+        else:
+            ctx.move_particles()
+            ctx.pm_calculate_forces()
+        forces += 1
+    return forces
+
+
+def time_run(ctx, steps, warmup, Om, profile=True):
+    """Warm-up, back to the initial conditions, then `steps` timed steps with
+    per-phase HIP events; returns (seconds, forces, phase stats)."""
+    timed_steps(ctx, warmup, Om)
+    ctx.synchronize()
+    ctx.state_restore()
+    ctx.synchronize()
+    ctx.set_profiling(profile)
+    ctx.reset_phase_stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    forces = timed_steps(ctx, steps, Om)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    stats = ctx.phase_stats()
+    ctx.set_profiling(False)
+    return dt, forces, stats
+
+
+def final_state(ctx, Om):
+    """An untimed complete 10-step run from the saved initial conditions and
+    its snapshot at a = 1."""
+    ctx.state_restore()
+    timed_steps(ctx, NSTEPS, Om)
+    ctx.synchronize()
+    return ctx.set_snapshot(1.0, Om)
+
+
+def tune_mid_run(ctx, Om, steps=5):
+    """Launch shapes tuned on the state after `steps` steps (clustered enough
+    that the deposit sees late-time contention), then back to the ICs."""
+    ctx.state_restore()
+    timed_steps(ctx, steps, Om)
+    ctx.synchronize()
+    t = ctx.autotune(reps=3)
+    ctx.state_restore()
+    ctx.synchronize()
+    return t
+
+
+def production_bench(args):
+    import cola_halo_amd as ch
+
+    P = PROD
+    nc, box = P["nc"], P["box"]
+    Om = float(np.float32(OM))
+    da = 1.0 / NSTEPS
+    k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
+    weights = args.weights or NET_WEIGHTS
+    net = ch.ResidualNet.from_file(P["coarse"], weights)
+    runs = {"corrected": ch.Context(nc, P["coarse"], box), "fine": ch.Context(nc, P["fine"], box)}
+    runs["corrected"].set_residual_net(net)
+    t_ic = {}
+    tuning = {}
+    for name, ctx in runs.items():
+        ctx.power_init_table(k, p, da, 0.8, OM, 1.0 - OM)
+        t = time.perf_counter()
+        ctx.lpt_set_displacement(da, OM, P["seed"])
+        ctx.set_times(ctx.times()[0], da)  # main.c:145
+        ctx.synchronize()
+        t_ic[name] = round(time.perf_counter() - t, 3)
+        ctx.state_save()
+        if not args.no_autotune:
+            tuning[name] = tune_mid_run(ctx, Om)
+    res = {}
+    for name, ctx in runs.items():
+        dt, forces, stats = time_run(ctx, args.steps, args.warmup, Om)
+        ng = P["coarse"] if name == "corrected" else P["fine"]
+        res[name] = {"seconds": dt, "forces": forces, "stats": stats, "ng": ng,
+                     "value": nc ** 3 * forces / dt, "ms_per_step": dt / args.steps * 1e3}
+    # P(k) gate and mass functions on complete untimed runs
+    ll = float(np.float32(0.2 * np.float32(box) / nc))  # main.c:353
+    edges = 32 * 2.0 ** np.arange(0, 9)
+    pk, mf = {}, {}
+    for name, ctx in runs.items():
+        final_state(ctx, Om)
+        kb, pk[name], nmodes = ctx.power_spectrum(P["coarse"], 0.0, PK_KMAX, PK_BINS, deconvolve=True,
+                                                  source="snapshot")
+        mf[name] = np.histogram(ctx.fof_find_halos(ll, 32)["nfof"], bins=edges)[0]
+    sel = kb > 0
+    ratio = pk["corrected"][sel] / pk["fine"][sel]
+    err = float(np.abs(ratio - 1.0).max())
+    gate = err <= PK_TOL
+    cpu = None
+    if not args.no_cpu_baseline:
+        ctx = runs["corrected"]
+        ctx.set_residual_net(None)
+        ctx.state_restore()
+        parts = ctx.get_particles()
+        cpu = cpu_baseline(parts, nc, P["coarse"], box, cpu_threads())
+    for ctx in runs.values():
+        ctx.close()
+    net.close()
+    c, f = res["corrected"], res["fine"]
+    phases = {k2: round(v[0] / max(c["forces"], 1), 3) for k2, v in c["stats"].items() if v[1] > 0}
+    phases_fine = {k2: round(v[0] / max(f["forces"], 1), 3) for k2, v in f["stats"].items() if v[1] > 0}
+    n_c = P["coarse"] ** 2 * (P["coarse"] // 2 + 1)
+    out = {
+        "metric": METRIC,
+        "value": round(c["value"], 1) if gate else 0.0,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(c["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (PM, particles); bf16 activations with fp32 MFMA accumulation (net)",
+        "data": "synthetic (2LPT ICs of seed 1; net weights trained by this project on its own force pairs)",
+        "config": {"workload": f"production comparison: {nc}^3 particles, box {box:g} Mpc/h, 2LPT seed {P['seed']} "
+                               f"a 0.1->1 in {NSTEPS} steps, EH98 no-wiggle P(k) sigma8 0.8; corrected "
+                               f"{P['coarse']}^3 PM + learned residual vs plain {P['fine']}^3 PM",
+                   "nc": nc, "coarse_grid": P["coarse"], "fine_grid": P["fine"], "boxsize": box,
+                   "seed": P["seed"], "net_weights": os.path.relpath(weights, ROOT), "parallelism": "single GPU"},
+        "roofline": roofline(c["stats"], nc, P["coarse"], net=True),
+        "pk_gate": {"max_abs_err_k_le_1": round(err, 5), "tolerance": PK_TOL, "pass": bool(gate),
+                    "k": [round(float(v), 4) for v in kb[sel]],
+                    "ratio_corrected_to_fine": [round(float(v), 5) for v in ratio]},
+        "mass_function_ratio_corrected_to_fine": {
+            "nfof_bins": [int(v) for v in edges],
+            "ratio": [round(float(a / b), 4) if b > 0 else None for a, b in zip(mf["corrected"], mf["fine"])],
+            "fine_counts": [int(v) for v in mf["fine"]]},
+        "fine": {"value": round(f["value"], 1), "ms_per_step": round(f["ms_per_step"], 3),
+                 "roofline": roofline(f["stats"], nc, P["fine"]), "phase_ms_per_step": phases_fine},
+        "speedup_vs_fine": round(c["value"] / f["value"], 4),
+        "step_canonical_frac_fine": round((156 * nc ** 3 + 32 * P["fine"] ** 3 + 64 * P["fine"] ** 2 * (P["fine"] // 2 + 1))
+                                          / (f["ms_per_step"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "step_canonical_frac_coarse_pm": round((156 * nc ** 3 + 32 * P["coarse"] ** 3 + 64 * n_c)
+                                               / (c["ms_per_step"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "phase_ms_per_step": phases,
+        "ic_seconds": t_ic,
+        "tuning": tuning,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def plain_bench(args):
+    """Plain COLA (no net) of one configuration: nc^3 particles on an ng^3
+    grid, seed 1, 10 steps from a = 0.1."""
+    import cola_halo_amd as ch
+
+    nc, ng, box = args.nc, args.ng, args.box
+    da = 1.0 / NSTEPS
+    Om = float(np.float32(OM))
+    k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
+    ctx = ch.Context(nc, ng, box)
+    if args.tile or args.fft_log2_tile != -2:
+        tx, ty, tz = (int(v) for v in args.tile.split(",")) if args.tile else (8, 8, 32)
+        ctx.set_tiling(tx, ty, tz, args.fft_log2_tile if args.fft_log2_tile != -2 else -1)
+    if args.xcd_group:
+        ctx.set_fft_xcd_group(args.xcd_group)
+    ctx.power_init_table(k, p, da, 0.8, OM, 1.0 - OM)
+    t_ic = time.perf_counter()
+    ctx.lpt_set_displacement(da, OM, PLAIN["seed"])
+    ctx.set_times(ctx.times()[0], da)  # main.c:145
+    ctx.synchronize()
+    t_ic = time.perf_counter() - t_ic
+    ctx.state_save()
+    tuning = None
+    if not args.no_autotune and not (args.tile or args.fft_log2_tile != -2 or args.xcd_group):
+        tuning = tune_mid_run(ctx, Om)
+    elapsed, forces, stats = time_run(ctx, args.steps, args.warmup, Om)
+    value = nc ** 3 * forces / elapsed
+    n_c = ng * ng * (ng // 2 + 1)
+    b_step = 156 * nc ** 3 + 32 * ng ** 3 + 64 * n_c
+    step_frac = b_step / (elapsed / forces) / 1e9 / HBM_PEAK_GBS
+    cpu = None
+    if not args.no_cpu_baseline:
+        ctx.state_restore()
+        cpu = cpu_baseline(ctx.get_particles(), nc, ng, box, cpu_threads())
+    ctx.close()
+    phases = {k2: round(v[0] / max(forces, 1), 3) for k2, v in stats.items() if v[1] > 0}
+    print(json.dumps({
+        "metric": "plain COLA particle-steps/s on 1 MI355X", "value": round(value, 1), "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain COLA {nc}^3 particles, {ng}^3 PM grid, box {box:g} Mpc/h, 2LPT seed "
+                               f"{PLAIN['seed']}, a 0.1->1 in {NSTEPS} steps, EH98 no-wiggle P(k) sigma8 0.8",
+                   "nc": nc, "ngrid": ng, "boxsize": box, "seed": PLAIN["seed"], "parallelism": "single GPU"},
+        "roofline": roofline(stats, nc, ng), "step_canonical_frac": round(step_frac, 4),
+        "phase_ms_per_step": phases, "ic_seconds": round(t_ic, 3), "tuning": tuning, "cpu_baseline": cpu}))
 
 
 ENSEMBLE = [(96, 288, 187.5)] * 16 + [(128, 384, 250.0)] * 16 + [(160, 480, 312.5)] * 16
@@ -173,22 +464,30 @@ def ensemble_bench(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", choices=("production", "plain"), default="production",
+                    help="production: corrected 768^3 vs plain 1536^3 at 512^3 particles (the metric); "
+                         "plain: plain COLA of --nc/--ng/--box")
+    ap.add_argument("--weights", default="", help="net-weights file (default: the committed net)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--nc", type=int, default=NC)
-    ap.add_argument("--ng", type=int, default=NG)
-    ap.add_argument("--box", type=float, default=BOX)
-    ap.add_argument("--tile", type=str, default="", help="deposit tile tx,ty,tz")
+    ap.add_argument("--nc", type=int, default=PLAIN["nc"])
+    ap.add_argument("--ng", type=int, default=PLAIN["ng"])
+    ap.add_argument("--box", type=float, default=PLAIN["box"])
+    ap.add_argument("--tile", type=str, default="", help="deposit tile tx,ty,tz (plain workload)")
     ap.add_argument("--fft-log2-tile", type=int, default=-2, help="FFT transforms per workgroup (log2)")
     ap.add_argument("--xcd-group", type=int, default=0, help="FFT column tiles per XCD group")
     ap.add_argument("--ensemble", action="store_true",
                     help="time the 48-realization ensemble (sequential vs batched graph steps) instead")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default launch shapes instead of tuning them on the initial conditions")
+                    help="keep the default launch shapes instead of tuning them")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box here has one GPU.  Ensembles would shard by seed "
+              "and one realization by x slabs across GPUs; neither runs in this benchmark.", file=sys.stderr)
+        sys.exit(2)
     if args.copy_probe:
         import cola_halo_amd as ch
 
@@ -197,143 +496,10 @@ def main():
         return
     if args.ensemble:
         ensemble_bench(args)
-        return
-    if args.gpus != 1:
-        print("bench.py measures a single GPU: ensembles would shard by seed and one realization by x slabs, "
-              "neither of which this benchmark runs",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import cola_halo_amd as ch
-
-    nc, ng, box = args.nc, args.ng, args.box
-    da = 1.0 / NSTEPS
-    a_init = da
-    Om = float(np.float32(OM))
-    k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
-    ctx = ch.Context(nc, ng, box)
-    if args.tile or args.fft_log2_tile != -2:
-        tx, ty, tz = (int(v) for v in args.tile.split(",")) if args.tile else (8, 8, 32)
-        ctx.set_tiling(tx, ty, tz, args.fft_log2_tile if args.fft_log2_tile != -2 else -1)
-    if args.xcd_group:
-        ctx.set_fft_xcd_group(args.xcd_group)
-    ctx.power_init_table(k, p, a_init, 0.8, OM, 1.0 - OM)
-    t_ic = time.perf_counter()
-    ctx.lpt_set_displacement(a_init, OM, SEED)
-    ctx.set_times(a_init, da)  # main.c:145
-    ctx.synchronize()
-    t_ic = time.perf_counter() - t_ic
-    ctx.state_save()
-    # launch-shape tuning on the initial state, outside the timed region (the
-    # reference plans its FFTs with FFTW_MEASURE at pm_init, pm.c:139-147)
-    tuning = None
-    if not args.no_autotune and not (args.tile or args.fft_log2_tile != -2 or args.xcd_group):
-        tuning = ctx.autotune(reps=3)
-        ctx.state_restore()
-        ctx.synchronize()
-
-    def run_steps(nsteps):
-        forces = 0
-        for s in range(nsteps):
-            istep = s % NSTEPS + 1
-            if istep == 1 and s > 0:
-                ctx.state_restore()
-            if istep < NSTEPS:
-                ctx.step(Om, (istep + 0.5) * da, (istep + 1.0) * da)
-            else:
-                ctx.move_particles()
-                ctx.pm_calculate_forces()
-            forces += 1
-        return forces
-
-    # warmup, then back to the initial conditions
-    run_steps(args.warmup)
-    ctx.synchronize()
-    ctx.state_restore()
-    ctx.synchronize()
-
-    ctx.set_profiling(True)
-    ctx.reset_phase_stats()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    forces = run_steps(args.steps)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    stats = ctx.phase_stats()
-    ctx.set_profiling(False)
-    elapsed = t1 - t0
-    value = nc ** 3 * forces / elapsed
-
-    # roofline of the kernel with the most device time
-    dom, (dom_ms, dom_n) = max(stats.items(), key=lambda kv: kv[1][0])
-    nbytes = kernel_bytes(dom, nc, ng)
-    roof = None
-    if nbytes is not None and dom_n > 0:
-        avg_s = dom_ms / dom_n / 1e3
-        achieved = nbytes / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, nc, ng), "kernel": dom,
-                "avg_launch_ms": round(dom_ms / dom_n, 4), "bytes_per_launch": nbytes}
-    # whole step against the canonical bytes per step (156 Np + 32 Ng + 64 Nc)
-    n_c = ng * ng * (ng // 2 + 1)
-    b_step = 156 * nc ** 3 + 32 * ng ** 3 + 64 * n_c
-    step_frac = b_step / (elapsed / forces) / 1e9 / HBM_PEAK_GBS
-
-    # output side on the final state (a = 1), outside the timed region
-    outputs = None
-    if args.steps % NSTEPS == 0:
-        ctx.synchronize()
-        t = time.perf_counter()
-        ctx.set_snapshot(1.0, Om)
-        t_snap = time.perf_counter() - t
-        ll = float(np.float32(0.2 * np.float32(box) / nc))  # main.c:353, linking factor 0.2
-        t = time.perf_counter()
-        halos = ctx.fof_find_halos(ll, 32)
-        t_fof = time.perf_counter() - t
-        t = time.perf_counter()
-        ctx.coarse_grid(nc)
-        t_grid = time.perf_counter() - t
-        t = time.perf_counter()
-        ctx.power_spectrum(ng, 0.0, 1.0, 50, deconvolve=True)
-        t_pk = time.perf_counter() - t
-        outputs = {"snapshot_to_host_ms": round(t_snap * 1e3, 2), "fof_ms": round(t_fof * 1e3, 2),
-                   "halos": int(len(halos)), f"coarse_grid_{nc}_ms": round(t_grid * 1e3, 2),
-                   f"power_spectrum_{ng}_ms": round(t_pk * 1e3, 2)}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        ctx.state_restore()
-        parts = ctx.get_particles()
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(parts, nc, ng, box, threads)
-    ctx.close()
-
-    phases = {k2: round(v[0] / max(forces, 1), 3) for k2, v in stats.items() if v[1] > 0}
-    out = {
-        "metric": METRIC,
-        "value": round(value, 1),
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"plain COLA {nc}^3 particles, {ng}^3 PM grid, box {box:g} Mpc/h, 2LPT seed {SEED}, "
-                               f"a 0.1->1 in {NSTEPS} steps, EH98 no-wiggle P(k) sigma8 0.8",
-                   "nc": nc, "ngrid": ng, "boxsize": box, "seed": SEED, "parallelism": "single GPU"},
-        "roofline": roof,
-        "step_canonical_frac": round(step_frac, 4),
-        "phase_ms_per_step": phases,
-        "ic_seconds": round(t_ic, 3),
-        "tuning": tuning,
-        "outputs": outputs,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
+    elif args.workload == "plain":
+        plain_bench(args)
+    else:
+        production_bench(args)
 
 
 if __name__ == "__main__":

@@ -55,6 +55,7 @@ _SIGNATURES = {
     "cola_growth_factor": (C.c_double, [C.c_void_p, C.c_double, C.c_double]),
     "cola_power_sigma8": (C.c_double, [C.c_void_p]),
     "cola_lpt_set_displacement": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double]),
+    "cola_lpt_modes": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_size_t]),
     "cola_lpt_get_local_nx": (C.c_int, [C.c_void_p]),
     "cola_move_particles": (C.c_int, [C.c_void_p]),
     "cola_pm_calculate_forces": (C.c_int, [C.c_void_p]),
@@ -108,6 +109,8 @@ _SIGNATURES = {
     "cola_resnet_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_uint, C.c_void_p]),
     "cola_resnet_eval": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "cola_set_residual_net": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "cola_resnet_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "cola_resnet_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "cola_set_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_step_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]),
     "cola_set_fft_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
@@ -403,6 +406,25 @@ class ResidualNet:
         _check(lib().cola_resnet_eval(self._h, first, count, C.byref(loss)))
         return loss.value
 
+    def save(self, filename):
+        """Write the parameters and scales to a net-weights file."""
+        _check(lib().cola_resnet_save(self._h, os.fsencode(filename)))
+
+    def load(self, filename):
+        """Parameters, scales and restriction weight from a net-weights file."""
+        _check(lib().cola_resnet_load(self._h, os.fsencode(filename)))
+
+    @classmethod
+    def from_file(cls, n, filename):
+        """A net on an n^3 grid with the weights of a net-weights file."""
+        net = cls(n)
+        try:
+            net.load(filename)
+        except Exception:
+            net.close()
+            raise
+        return net
+
     def close(self):
         if self._h:
             lib().cola_resnet_destroy(self._h)
@@ -434,7 +456,9 @@ class Params(C.Structure):
                 + [f for name in _PARAM_STRINGS
                    for f in ((name, C.c_char * PARAM_MAX_STRING), ("strlen_" + name, C.c_int))]
                 + [("fof_linking_factor", C.c_double), ("subsample_factor", C.c_double),
-                   ("cgrid_nc", C.c_int), ("write_longid", C.c_int)])
+                   ("cgrid_nc", C.c_int), ("write_longid", C.c_int), ("pm_ngrid", C.c_int),
+                   ("residual_net_filename", C.c_char * PARAM_MAX_STRING),
+                   ("strlen_residual_net_filename", C.c_int)])
 
     def as_dict(self):
         """Plain values; an absent string key is None."""
@@ -444,7 +468,7 @@ class Params(C.Structure):
                 continue
             if name == "zout":
                 out["zout"] = [self.zout[i] for i in range(self.n_zout)]
-            elif name in _PARAM_STRINGS:
+            elif name in _PARAM_STRINGS or name == "residual_net_filename":
                 n = getattr(self, "strlen_" + name)
                 out[name] = None if n == 0 else getattr(self, name).decode()
             else:
@@ -525,6 +549,15 @@ class Context:
     def lpt_set_displacement(self, a_init, omega_m, seed, boxsize=None):
         _check(lib().cola_lpt_set_displacement(self._h, a_init, omega_m, int(seed),
                                                self.boxsize if boxsize is None else boxsize))
+
+    def lpt_modes(self, seed, boxsize=None):
+        """The device's white-noise ZA displacement spectra of seed
+        (lpt.c:172-351): complex64 [3, nc, nc, nc/2+1]."""
+        n = self.nc
+        out = np.zeros((3, n, n, n // 2 + 1), np.complex64)
+        _check(lib().cola_lpt_modes(self._h, int(seed), self.boxsize if boxsize is None else boxsize,
+                                    out.ctypes.data, out.size * 2))
+        return out
 
     def move_particles(self):
         _check(lib().cola_move_particles(self._h))

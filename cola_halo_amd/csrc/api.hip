@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -52,12 +54,17 @@ enum Phase : int {
   kPhLptModes,
   kPhLptFft,
   kPhLptOther,
+  kPhNetFeatures,
+  kPhNetConv1,
+  kPhNetHidden,
+  kPhNetHead,
   kNumPhases
 };
 const char* const kPhaseNames[kNumPhases] = {
     "wrap",      "bin_count",  "bin_scan",   "bin_scatter", "deposit",   "fft_z_r2c",     "fft_y_fwd",
     "fft_x_fwd", "fft_x_inv",  "fft_y_inv",  "fft_z_c2r",   "fft_x_fused", "readout",   "readout_kick_drift", "kick",
-    "drift",     "snapshot",   "lpt_modes",  "lpt_fft",     "lpt_other"};
+    "drift",     "snapshot",   "lpt_modes",  "lpt_fft",     "lpt_other", "net_features", "net_conv1",
+    "net_conv_hidden", "net_conv_out_add"};
 
 struct EventPair {
   int phase;
@@ -317,12 +324,23 @@ int pm_fields(cola_context* c)
   // the net's features come from the real-space density before the r2c
   // overwrites it; its last layer adds its output to the force grids
   if (c->net) {
+    phase_begin(c, kPhNetFeatures);
     COLA_TRY(resnet_features(c->net, c->density, c->cic.row, c->stream));
-    COLA_TRY(resnet_body(c->net, &c->d_prm->a, c->stream));
+    phase_end(c, kPhNetFeatures);
+    phase_begin(c, kPhNetConv1);
+    COLA_TRY(resnet_layer1(c->net, &c->d_prm->a, c->stream));
+    phase_end(c, kPhNetConv1);
+    phase_begin(c, kPhNetHidden);
+    COLA_TRY(resnet_layers23(c->net, c->stream));
+    phase_end(c, kPhNetHidden);
   }
   c->density_is_real = false;
   COLA_TRY(c->pm_fft.forces(c->density, out, kp, c->stream));
-  if (c->net) COLA_TRY(resnet_head_add(c->net, out, c->cic.row, c->stream));
+  if (c->net) {
+    phase_begin(c, kPhNetHead);
+    COLA_TRY(resnet_head_add(c->net, out, c->cic.row, c->stream));
+    phase_end(c, kPhNetHead);
+  }
   return kOk;
 }
 
@@ -604,6 +622,44 @@ double cola_growth_factor(const cola_context* c, double astart, double aend)
 
 double cola_power_sigma8(const cola_context* c) { return c ? c->sigma8_input : 0.0; }
 
+}  // extern "C"
+
+namespace {
+
+// lpt.c:172-351: the seed table on the host, then the white-noise ZA
+// displacement spectra of every column on the device into cd[0..2]
+// (complex [i][j][kz], N x N x (N/2+1) each).
+int fill_modes(cola_context* c, int seed, double box, float2* const cd[3])
+{
+  const int N = c->nc;
+  std::vector<unsigned int> table((size_t)N * N);
+  lpt_seedtable(N, seed, table.data());
+  unsigned int* d_table = nullptr;
+  COLA_HIP(hipMalloc(&d_table, sizeof(unsigned int) * table.size()));
+  int st = hipMemcpyAsync(d_table, table.data(), sizeof(unsigned int) * table.size(), hipMemcpyHostToDevice,
+                          c->stream) == hipSuccess
+               ? kOk
+               : kErrHip;
+  if (!st) {
+    DevPower pw;
+    pw.logk = c->d_logk;
+    pw.logD = c->d_logD;
+    pw.n = (int)c->power.logk.size();
+    pw.norm = c->power.norm;
+    const double fac = std::pow(2 * kPiD / box, 1.5);  // lpt.c:166
+    phase_begin(c, kPhLptModes);
+    st = launch_mode_fill(N, d_table, pw, box, fac, cd[0], cd[1], cd[2], c->stream);
+    phase_end(c, kPhLptModes);
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !st) st = kErrHip;
+  (void)hipFree(d_table);
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cola_lpt_set_displacement(cola_context* c, double InitTime, double omega_m, int seed, double box)
 {
   if (!valid(c)) return kErrArg;
@@ -622,30 +678,13 @@ int cola_lpt_set_displacement(cola_context* c, double InitTime, double omega_m, 
   for (int g = 0; g < 6; ++g) cg[g] = work + (3 + g) * csize;
   auto rg = [](float2* p) { return reinterpret_cast<float*>(p); };
 
-  std::vector<unsigned int> table((size_t)N * N);
-  lpt_seedtable(N, seed, table.data());
-  unsigned int* d_table = nullptr;
   int st = kOk;
   auto done = [&](int s) {
     (void)hipStreamSynchronize(c->stream);
-    if (d_table) (void)hipFree(d_table);
     (void)hipFree(work);
     return s;
   };
-  if (hipMalloc(&d_table, sizeof(unsigned int) * table.size()) != hipSuccess) return done(kErrAlloc);
-  if (hipMemcpyAsync(d_table, table.data(), sizeof(unsigned int) * table.size(), hipMemcpyHostToDevice, c->stream) !=
-      hipSuccess)
-    return done(kErrHip);
-
-  DevPower pw;
-  pw.logk = c->d_logk;
-  pw.logD = c->d_logD;
-  pw.n = (int)c->power.logk.size();
-  pw.norm = c->power.norm;
-  const double fac = std::pow(2 * kPiD / box, 1.5);  // lpt.c:166
-  phase_begin(c, kPhLptModes);
-  st = launch_mode_fill(N, d_table, pw, box, fac, cd[0], cd[1], cd[2], c->stream);
-  phase_end(c, kPhLptModes);
+  st = fill_modes(c, seed, box, cd);
   if (st) return done(st);
 
   // lpt.c:358-403: i k_j dis_i for (ij) = 00 01 02 11 12 22, c2r each
@@ -696,6 +735,28 @@ int cola_lpt_set_displacement(cola_context* c, double InitTime, double omega_m, 
   c->have_particles = true;
   c->have_force = false;
   return done(kOk);
+}
+
+int cola_lpt_modes(cola_context* c, int seed, double box, float* out, size_t n_out)
+{
+  if (!valid(c) || !out || !(box > 0)) return kErrArg;
+  if (c->power.empty() || !c->d_logk) {
+    set_error("cola_lpt_modes: call cola_power_init first");
+    return kErrState;
+  }
+  const int N = c->nc;
+  const long long csize = (long long)N * N * (N / 2 + 1);
+  if (n_out < (size_t)(6 * csize)) {
+    set_error("mode buffer too small: needs 6 nc^2 (nc/2+1) floats");
+    return kErrArg;
+  }
+  float2* work = nullptr;
+  COLA_HIP(hipMalloc(&work, sizeof(float2) * csize * 3));
+  float2* const cd[3] = {work, work + csize, work + 2 * csize};
+  int st = fill_modes(c, seed, box, cd);
+  if (!st && hipMemcpy(out, work, sizeof(float2) * csize * 3, hipMemcpyDeviceToHost) != hipSuccess) st = kErrHip;
+  (void)hipFree(work);
+  return st;
 }
 
 int cola_lpt_get_local_nx(const cola_context* c) { return c ? c->nc : 0; }
@@ -1755,6 +1816,67 @@ int cola_resnet_eval(cola_resnet* h, int first, int count, double* loss)
   if (!h || !loss) return kErrArg;
   COLA_TRY(net_quiesce(h));
   return resnet_eval(&h->r, first, count, loss, h->stream);
+}
+
+namespace {
+
+constexpr char kNetMagic[8] = {'C', 'O', 'L', 'A', 'N', 'E', 'T', '1'};
+
+}  // namespace
+
+int cola_resnet_save(cola_resnet* h, const char* filename)
+{
+  if (!h || !filename) return kErrArg;
+  std::vector<float> prm(kNetParams);
+  COLA_TRY(net_quiesce(h));
+  COLA_HIP(hipMemcpy(prm.data(), h->r.d_param, sizeof(float) * kNetParams, hipMemcpyDeviceToHost));
+  FILE* f = std::fopen(filename, "wb");
+  if (!f) {
+    set_error(std::string("cannot write net-weights file ") + filename);
+    return 9000;
+  }
+  const int32_t head[2] = {kNetParams, 0};
+  const float scales[4] = {h->r.in_scale, h->r.out_scale, h->r.restrict_side, 0.0f};
+  const int32_t tail = kNetParams;
+  bool ok = std::fwrite(kNetMagic, 1, 8, f) == 8 && std::fwrite(head, 4, 2, f) == 2 &&
+            std::fwrite(scales, 4, 4, f) == 4 && std::fwrite(prm.data(), 4, kNetParams, f) == (size_t)kNetParams &&
+            std::fwrite(&tail, 4, 1, f) == 1;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) {
+    set_error(std::string("short write of net-weights file ") + filename);
+    return 9000;
+  }
+  return kOk;
+}
+
+int cola_resnet_load(cola_resnet* h, const char* filename)
+{
+  if (!h || !filename) return kErrArg;
+  FILE* f = std::fopen(filename, "rb");
+  if (!f) {
+    set_error(std::string("cannot open net-weights file ") + filename);
+    return kErrNetFile;
+  }
+  char magic[8];
+  int32_t head[2] = {0, 0}, tail = 0;
+  float scales[4];
+  std::vector<float> prm(kNetParams);
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kNetMagic, 8) == 0 &&
+            std::fread(head, 4, 2, f) == 2 && head[0] == kNetParams && std::fread(scales, 4, 4, f) == 4 &&
+            std::fread(prm.data(), 4, kNetParams, f) == (size_t)kNetParams && std::fread(&tail, 4, 1, f) == 1 &&
+            tail == kNetParams && std::fgetc(f) == EOF;
+  std::fclose(f);
+  if (ok) ok = scales[0] > 0.0f && scales[1] > 0.0f && scales[2] >= 0.0f && scales[2] <= 0.25f;
+  for (int i = 0; ok && i < kNetParams; ++i) ok = std::isfinite(prm[i]);
+  if (!ok) {
+    set_error(std::string("not a net-weights file of this net (") + std::to_string(kNetParams) +
+              " parameters): " + filename);
+    return kErrNetFile;
+  }
+  COLA_TRY(cola_resnet_set_params(h, prm.data()));
+  COLA_TRY(cola_resnet_set_scales(h, scales[0], scales[1]));
+  h->r.restrict_side = scales[2];
+  return kOk;
 }
 
 int cola_set_residual_net(cola_context* c, cola_resnet* h)

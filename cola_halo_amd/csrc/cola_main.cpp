@@ -8,7 +8,10 @@
 // same outputs at each requested redshift: FoF halo catalogue
 // (fof%05d%c.txt), GADGET-1 snapshot (snp%05d%c.0), random subsample
 // (sub%05d%c.b), coarse density grid (grid%05d%c.b), and the optional 2LPT
-// initial condition (init_%05d.0).  A fatal error prints the message and
+// initial condition (init_%05d.0).  Two keys of this build extend the file:
+// pm_ngrid (the PM grid when nc * pm_nc_factor cannot express it) and
+// residual_net (a net-weights file: the learned force residual is added to
+// every force).  A fatal error prints the message and
 // exits with the reference's abort code (as MPI_Abort does, the shell sees its
 // low 8 bits).  Steps with no output in them run as one fused call
 // (cola_step).
@@ -132,6 +135,8 @@ void print_parameters(const cola_params& p, const char* filename)
   else
     msg(kNormal, "No coarse density grid output (coarse_grid)\n");
   if (p.strlen_init_filename) msg(kNormal, "inital= \"%s\"\n", p.init_filename);
+  if (p.pm_ngrid > 0) msg(kNormal, "pm_ngrid = %d\n", p.pm_ngrid);
+  if (p.strlen_residual_net_filename) msg(kNormal, "residual_net = \"%s\"\n", p.residual_net_filename);
 }
 
 struct Run {
@@ -208,9 +213,20 @@ int main(int argc, char* argv[])
   Timer timer;
   const int nc_factor = p.pm_nc_factor;
   const double OmegaM = p.omega_m, OmegaLambda = 1.0 - OmegaM;
+  // pm_ngrid (this build's key) sets the PM grid directly: factors such as
+  // 1.5 that pm_nc_factor (an int, parameters.h:6) cannot express
+  const int ngrid = p.pm_ngrid > 0 ? p.pm_ngrid : nc_factor * p.nc;
   cola_context* ctx = nullptr;
-  check(cola_context_create(p.nc, nc_factor * p.nc, p.boxsize, &ctx));
+  check(cola_context_create(p.nc, ngrid, p.boxsize, &ctx));
   timer.ctx = ctx;
+  cola_resnet* net = nullptr;
+  if (p.strlen_residual_net_filename) {
+    // learned force residual on the PM grid (no reference counterpart)
+    check(cola_resnet_create(ngrid, 1, &net));
+    check(cola_resnet_load(net, p.residual_net_filename));
+    check(cola_set_residual_net(ctx, net));
+    msg(kNormal, "learned force residual %s on the %d^3 PM grid\n", p.residual_net_filename, ngrid);
+  }
   const int nsteps = p.ntimestep;
   const double a_final = p.a_final;
   const double da = a_final / nsteps;  // main.c:70-72
@@ -283,5 +299,6 @@ int main(int argc, char* argv[])
   }
   check(cola_subsampler_destroy(sub));
   check(cola_context_destroy(ctx));
+  if (net) check(cola_resnet_destroy(net));
   return 0;
 }

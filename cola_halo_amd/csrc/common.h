@@ -23,6 +23,7 @@ enum Status : int {
   kErrUnsupported = 9002,
   kErrHip = 9003,
   kErrState = 9004,
+  kErrNetFile = 9005,
 };
 
 // Return a status from the enclosing function if a HIP call fails.

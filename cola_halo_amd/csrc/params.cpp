@@ -249,6 +249,10 @@ int cola_read_parameters(const char* filename, cola_params* param)
       st = get_string(p, param->init_filename, &param->strlen_init_filename);
     else if (key == "write_longid")
       st = get_bool(p, &param->write_longid);
+    else if (key == "pm_ngrid")  // extension: explicit PM grid (non-integer factors)
+      param->pm_ngrid = get_int(p);
+    else if (key == "residual_net")  // extension: learned force residual weights
+      st = get_string(p, param->residual_net_filename, &param->strlen_residual_net_filename);
     else
       st = -1;
     if (st == -1) {
@@ -284,6 +288,8 @@ int cola_confirm_parameters(const cola_params* p)
       {p->strlen_subsample_filename == 1, 10021, "Error: subsample filename is empty."},
       {p->strlen_cgrid_filename == 1, 10022, "coarse_grid filename is empty."},
       {p->strlen_init_filename == 1, 10023, "init filename is empty."},
+      {p->strlen_residual_net_filename == 1, 10024, "residual_net filename is empty."},
+      {p->pm_ngrid < 0 || p->pm_ngrid % 2 != 0, 10025, "pm_ngrid must be a positive even number."},
   };
   for (const Check& c : checks) {
     if (c.bad) {

@@ -76,6 +76,9 @@ int resnet_forward_features(ResNet* r, const unsigned short* feat, const float* 
 // out[a] = out[a] + out_scale * y[a]
 int resnet_features(ResNet* r, const float* delta, long long row, hipStream_t s);
 int resnet_body(ResNet* r, const float* pa, hipStream_t s);
+// resnet_body in its two launches groups: layer 1 (from d_f1), layers 2-3
+int resnet_layer1(ResNet* r, const float* pa, hipStream_t s);
+int resnet_layers23(ResNet* r, hipStream_t s);
 int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s);
 // loss against target (bf16 [3][n^3] or fp32 [3][n^3]) and gradient into d_grad
 int resnet_backward(ResNet* r, const unsigned short* target_bf16, const float* target_f32, hipStream_t s);

@@ -958,6 +958,18 @@ int resnet_body(ResNet* r, const float* pa, hipStream_t s)
   return conv_layers_2_3(r, s);
 }
 
+int resnet_layer1(ResNet* r, const float* pa, hipStream_t s)
+{
+  COLA_TRY(conv_lds_attributes());
+  return conv_layer_1(r, r->d_f1, pa, 0.0f, s);
+}
+
+int resnet_layers23(ResNet* r, hipStream_t s)
+{
+  COLA_TRY(conv_lds_attributes());
+  return conv_layers_2_3(r, s);
+}
+
 int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s)
 {
   ConvOut o;

@@ -20,7 +20,8 @@ held out, 10 COLA steps from a = 0.1):
    against plain fine-grid runs of seeds 1..4 the same way.
 
 Usage: python -m cola_halo_amd.learned [--nc 128 --box 250 --coarse 192
-       --fine 384 --ntrain 56 --nval 8 --epochs 8] > result.json
+       --fine 384 --ntrain 56 --nval 8 --epochs 8] [--save-weights FILE] > result.json
+       python -m cola_halo_amd.learned --weights FILE ...   (no training: assess a saved net)
        python -m cola_halo_amd.learned --time-net 192   (forward timing only)
        python -m cola_halo_amd.learned --time-step 512 768 1000   (corrected steps only)
 """
@@ -195,6 +196,10 @@ def main():
     ap.add_argument("--production-box", type=float, default=1000.0)
     ap.add_argument("--production-seeds", type=int, default=4)
     ap.add_argument("--production-pk-grid", type=int, default=0)
+    ap.add_argument("--save-weights", default="", metavar="FILE",
+                    help="write the trained net to a net-weights file")
+    ap.add_argument("--weights", default="", metavar="FILE",
+                    help="skip pair generation and training: assess the net of a net-weights file")
     args = ap.parse_args()
     if args.time_net:
         print(json.dumps(time_net(args.time_net)))
@@ -212,27 +217,35 @@ def main():
     coarse = ch.Context(nc, args.coarse, box)
     for c in (fine, coarse):
         c.power_init_table(k, p, 1.0 / NSTEPS, 0.8, OM, 1.0 - OM)
-    net = ch.ResidualNet(args.coarse, seed=1)
-    net.set_restriction(args.restriction)
+    if args.weights:
+        net = ch.ResidualNet.from_file(args.coarse, args.weights)
+        out["weights"] = args.weights
+    else:
+        net = ch.ResidualNet(args.coarse, seed=1)
+        net.set_restriction(args.restriction)
 
-    t0 = time.perf_counter()
-    for seed in range(1, args.ntrain + args.nval + 1):
-        pairs_from_run(fine, coarse, net, seed)
-    out["pairs"] = {"count": net.num_samples(), "seconds": round(time.perf_counter() - t0, 2)}
+        t0 = time.perf_counter()
+        for seed in range(1, args.ntrain + args.nval + 1):
+            pairs_from_run(fine, coarse, net, seed)
+        out["pairs"] = {"count": net.num_samples(), "seconds": round(time.perf_counter() - t0, 2)}
 
-    ntr = args.ntrain * NSTEPS
-    nva = args.nval * NSTEPS
-    net.fit_scale(ntr)
-    val_before = net.eval(ntr, nva)
-    t0 = time.perf_counter()
-    losses = []
-    for e in range(args.epochs):
-        lr = args.lr if e < args.epochs // 2 else args.lr * 0.3
-        losses.append(float(net.train(ntr, 1, lr=lr, seed=e + 1)[0]))
-    out["training"] = {"epoch_loss": [round(v, 5) for v in losses], "seconds": round(time.perf_counter() - t0, 2),
-                       "val_loss": round(net.eval(ntr, nva), 5), "val_loss_untrained": round(val_before, 5),
-                       "out_scale": net.scales()[1],
-                       "loss_of_zero_correction": 1.0}
+        ntr = args.ntrain * NSTEPS
+        nva = args.nval * NSTEPS
+        net.fit_scale(ntr)
+        val_before = net.eval(ntr, nva)
+        t0 = time.perf_counter()
+        losses = []
+        for e in range(args.epochs):
+            lr = args.lr if e < args.epochs // 2 else args.lr * 0.3
+            losses.append(float(net.train(ntr, 1, lr=lr, seed=e + 1)[0]))
+        out["training"] = {"epoch_loss": [round(v, 5) for v in losses],
+                           "seconds": round(time.perf_counter() - t0, 2),
+                           "val_loss": round(net.eval(ntr, nva), 5), "val_loss_untrained": round(val_before, 5),
+                           "out_scale": net.scales()[1],
+                           "loss_of_zero_correction": 1.0}
+        if args.save_weights:
+            net.save(args.save_weights)
+            out["saved_weights"] = args.save_weights
 
     # assessment on the held-out seeds
     corrected = ch.Context(nc, args.coarse, box)

@@ -49,7 +49,8 @@ enum {
   COLA_ERR_ARG = 9001,          /* invalid argument (no reference counterpart) */
   COLA_ERR_UNSUPPORTED = 9002,  /* grid size with a prime factor > 7 */
   COLA_ERR_HIP = 9003,          /* HIP runtime error */
-  COLA_ERR_STATE = 9004         /* call out of order (e.g. force before ICs) */
+  COLA_ERR_STATE = 9004,        /* call out of order (e.g. force before ICs) */
+  COLA_ERR_NET_FILE = 9005      /* unreadable or malformed net-weights file */
 };
 
 /* Last error message of the calling thread ("" if none). */
@@ -110,6 +111,12 @@ int cola_step_factors(float omega_m, float a_x, float a_v, float avel1, float ap
 /* lpt.c:145-578 lpt_set_displacement (lpt.h:7, main.c:127): 2LPT initial
    conditions for seed on the device; sets a_x = a_init, a_v = 0. */
 int cola_lpt_set_displacement(cola_context* ctx, double a_init, double omega_m, int seed, double boxsize);
+/* lpt.c:172-351 (the cdisp fill of lpt_set_displacement, lpt.h:7): the
+   white-noise ZA displacement spectra of seed as the device builds them,
+   copied into out as 3 complex nc x nc x (nc/2+1) grids ([axis][i][j][kz],
+   re/im interleaved; n_out >= 6 nc^2 (nc/2+1) floats).  Needs
+   cola_power_init; leaves the particles untouched. */
+int cola_lpt_modes(cola_context* ctx, int seed, double boxsize, float* out, size_t n_out);
 /* lpt.c:611-614 lpt_get_local_nx (one rank: nc) */
 int cola_lpt_get_local_nx(const cola_context* ctx);
 
@@ -314,8 +321,20 @@ int cola_resnet_get_sample(cola_resnet* net, int i, float* feature, float* targe
 int cola_resnet_train(cola_resnet* net, int ntrain, int epochs, float lr, unsigned int seed, double* epoch_loss);
 int cola_resnet_eval(cola_resnet* net, int first, int count, double* loss);
 /* Attach (net) or detach (NULL) the net: every force of the context then adds
-   out_scale * net(delta, a) to the three force grids before the readout. */
+   out_scale * net(delta, a) to the three force grids before the readout.  A
+   net serves one context at a time (its activation buffers are its own):
+   attaching a net that another context holds returns COLA_ERR_STATE;
+   destroying an attached net detaches it first. */
 int cola_set_residual_net(cola_context* ctx, cola_resnet* net);
+/* Net-weights file: the trained parameters and scales, independent of the
+   grid size (the net is a convolution over cells).  Little-endian layout:
+   char magic[8] "COLANET1", int32 nparam, int32 0, float in_scale,
+   float out_scale, float restriction side weight, float 0,
+   float params[nparam] (the cola_resnet_get_params order), int32 nparam.
+   save returns 9000 when the file cannot be written (as write.c:23-24);
+   load returns COLA_ERR_NET_FILE for a missing, short or foreign file. */
+int cola_resnet_save(cola_resnet* net, const char* filename);
+int cola_resnet_load(cola_resnet* net, const char* filename);
 
 /* ---- Driver-side files (host only) -------------------------------------- */
 
@@ -353,16 +372,27 @@ typedef struct {
   double subsample_factor;
   int cgrid_nc;
   int write_longid;
+  /* Keys of this build with no reference counterpart (files without them
+     read as before): pm_ngrid = PM cells per side, overriding
+     nc * pm_nc_factor (e.g. 768 for nc = 512, a factor of 1.5; 0 = unset);
+     residual_net = a net-weights file (cola_resnet_save) whose learned force
+     residual every force evaluation adds. */
+  int pm_ngrid;
+  char residual_net_filename[COLA_PARAM_MAX_STRING];
+  int strlen_residual_net_filename;
 } cola_params;
 
 /* read_param.c:190-327 read_parameter_file (one rank; no broadcast): same
    syntax ("name = value", "--" comments, {a, b} arrays, "strings",
    t/T/1/f/F/0 booleans, numbers that must start with a digit), defaults and
    abort codes: 1001 unreadable file, 1002 bad boolean, 1003 bad string or
-   array, 1004 bad array element, 1007 unknown name. */
+   array, 1004 bad array element, 1007 unknown name.  Also reads the two
+   extension keys pm_ngrid (integer) and residual_net (string). */
 int cola_read_parameters(const char* filename, cola_params* out);
 /* confirm_param.c:8-131 confirm_parameters: returns its abort code (11001-11008,
-   11018-11020, 10021-10023) for a missing or empty required value, else 0. */
+   11018-11020, 10021-10023) for a missing or empty required value, else 0;
+   for the extension keys 10024 (empty residual_net) and 10025 (pm_ngrid
+   negative or odd). */
 int cola_confirm_parameters(const cola_params* p);
 
 /* write.c:14-107 write_snapshot for one rank: a GADGET-1 file with 4-byte

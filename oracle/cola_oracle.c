@@ -748,22 +748,13 @@ static void lpt_stages(int N, double Box, float* cdisp[3], float* cdigrad[6])
   }
 }
 
-/* ======================================================================
- * lpt.c:145-578 (one rank: Local_x_start = 0, Local_nx = Nmesh)
- * ====================================================================== */
-int orc_lpt_set_displacement(int N, double InitTime, double omega_m, int Seed, double Box,
-                             OrcParticle* particles, float* cdisp_out, float* disp_out, float* disp2_out)
+/* lpt.c:172-351 for one rank: the white-noise ZA displacement spectra
+   cdisp[3] (complex [i][j][kz], interleaved floats, zeroed by the caller). */
+static void lpt_mode_fill(int N, double Box, int Seed, float* cdisp[3])
 {
-  const double Omega = omega_m;
   const long nz = N / 2 + 1;
-  const long csize = (long)N * N * nz; /* complex elements per grid */
-  float* cdisp[3];
-  float* cdigrad[6];
-  for (int a = 0; a < 3; a++) cdisp[a] = calloc(2 * csize, sizeof(float));
-  for (int a = 0; a < 6; a++) cdigrad[a] = calloc(2 * csize, sizeof(float));
   const double fac = pow(2 * M_PI / Box, 1.5);
   const int Nsample = N;
-
   unsigned int* seedtable = malloc(sizeof(unsigned int) * N * N);
   orc_seedtable(N, Seed, seedtable);
 
@@ -834,6 +825,23 @@ int orc_lpt_set_displacement(int N, double InitTime, double omega_m, int Seed, d
     }
   }
   free(seedtable);
+#undef CD
+}
+
+/* ======================================================================
+ * lpt.c:145-578 (one rank: Local_x_start = 0, Local_nx = Nmesh)
+ * ====================================================================== */
+int orc_lpt_set_displacement(int N, double InitTime, double omega_m, int Seed, double Box,
+                             OrcParticle* particles, float* cdisp_out, float* disp_out, float* disp2_out)
+{
+  const double Omega = omega_m;
+  const long nz = N / 2 + 1;
+  const long csize = (long)N * N * nz; /* complex elements per grid */
+  float* cdisp[3];
+  float* cdigrad[6];
+  for (int a = 0; a < 3; a++) cdisp[a] = calloc(2 * csize, sizeof(float));
+  for (int a = 0; a < 6; a++) cdigrad[a] = calloc(2 * csize, sizeof(float));
+  lpt_mode_fill(N, Box, Seed, cdisp);
   if (cdisp_out)
     for (int a = 0; a < 3; a++) memcpy(cdisp_out + a * 2 * csize, cdisp[a], sizeof(float) * 2 * csize);
   lpt_stages(N, Box, cdisp, cdigrad);
@@ -842,7 +850,6 @@ int orc_lpt_set_displacement(int N, double InitTime, double omega_m, int Seed, d
   if (disp2_out)
     for (int a = 0; a < 3; a++) memcpy(disp2_out + a * 2 * csize, cdigrad[a], sizeof(float) * 2 * csize);
   float* cdisp2[3] = {cdigrad[0], cdigrad[1], cdigrad[2]};
-#undef CD
 
   /* lpt.c:507-575: particles on the lattice */
   if (particles) {
@@ -878,6 +885,20 @@ int orc_lpt_set_displacement(int N, double InitTime, double omega_m, int Seed, d
   }
   for (int a = 0; a < 3; a++) free(cdisp[a]);
   for (int a = 0; a < 6; a++) free(cdigrad[a]);
+  return 0;
+}
+
+/* lpt.c:172-351 alone: the ZA displacement spectra of Seed (3 complex
+   N x N x (N/2+1) grids, interleaved floats) into cdisp_out. */
+int orc_lpt_modes(int N, double Box, int Seed, float* cdisp_out)
+{
+  const long csize = (long)N * N * (N / 2 + 1);
+  float* cdisp[3];
+  for (int a = 0; a < 3; a++) {
+    cdisp[a] = cdisp_out + a * 2 * csize;
+    memset(cdisp[a], 0, sizeof(float) * 2 * csize);
+  }
+  lpt_mode_fill(N, Box, Seed, cdisp);
   return 0;
 }
 

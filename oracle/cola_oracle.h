@@ -80,6 +80,9 @@ void orc_fft_c2r_3d(int n, float* data);   /* complex in, padded real out, in pl
 int orc_lpt_set_displacement(int nc, double a_init, double omega_m, int seed,
                              double box, OrcParticle* particles,
                              float* cdisp_out, float* disp_out, float* disp2_out);
+/* lpt.c:172-351 alone: the ZA displacement spectra of seed (3 complex
+   nc x nc x (nc/2+1) grids, interleaved floats). */
+int orc_lpt_modes(int nc, double box, int seed, float* cdisp_out);
 /* lpt.c:358-503 on given ZA displacement spectra (3 complex nc x nc x (nc/2+1)
    grids): real ZA displacement (unnormalised c2r, as lpt.c uses it) and 2LPT
    displacement times 1/nc^3, padded real layout. */

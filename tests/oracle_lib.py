@@ -55,6 +55,7 @@ def load():
         "orc_fft_c2r_3d": (None, [i, vp]),
         "orc_lpt_set_displacement": (i, [i, d, d, i, d, vp, vp, vp, vp]),
         "orc_lpt_from_modes": (i, [i, d, vp, vp, vp]),
+        "orc_lpt_modes": (i, [i, d, i, vp]),
         "orc_pm_readout": (None, [i, f, vp, i, vp, vp]),
         "orc_wrap": (None, [vp, i, f]),
         "orc_pm_deposit": (None, [i, f, f, vp, i, vp]),
@@ -121,6 +122,13 @@ def lpt(nc, a_init, omega_m, seed, box, want_grids=False):
     if want_grids:
         return parts, cdisp.reshape(3, -1), disp.reshape(3, -1), disp2.reshape(3, -1)
     return parts
+
+
+def lpt_modes(n, box, seed):
+    """The white-noise ZA displacement spectra of seed: complex64 [3, n, n, n/2+1]."""
+    c = np.zeros((3, n, n, n // 2 + 1), np.complex64)
+    assert lib().orc_lpt_modes(n, box, seed, c.ctypes.data) == 0
+    return c
 
 
 def lpt_from_modes(n, box, cdisp):

@@ -285,6 +285,47 @@ def test_lpt_matches_oracle(eh_table, nc, box, seed):
     assert np.all(gpu["v"] == 0)
 
 
+@pytest.mark.parametrize("nc,box,seed", [(64, 100.0, 100), (256, 500.0, 1)])
+def test_lpt_modes_match_oracle_elementwise(eh_table, nc, box, seed):
+    """The white-noise ZA displacement spectra (lpt.c:172-351) read straight
+    from the device, element by element against the oracle's cdisp: the
+    device evaluates sin, cos, log, pow and sqrt in double as the reference
+    does and rounds once to float, so a value can differ only where the two
+    libm's double results straddle a float rounding boundary (1 ulp)."""
+    _say(f"lpt modes nc={nc} seed={seed}")
+    k, p = eh_table
+    orc.set_power_table(k, p)
+    with _context(nc, 3 * nc, box, eh_table) as ctx:
+        gpu = ctx.lpt_modes(seed)
+    ref = orc.lpt_modes(nc, box, seed)
+    g = gpu.view(np.float32).ravel()
+    r = ref.view(np.float32).ravel()
+    differ = g != r
+    n_diff = int(differ.sum())
+    ulps = np.abs(g.view(np.int32)[differ].astype(np.int64) - r.view(np.int32)[differ].astype(np.int64))
+    _say(f"  {n_diff} of {g.size} floats differ, max {ulps.max() if n_diff else 0} ulp")
+    assert n_diff == 0 or ulps.max() <= 1
+    assert n_diff <= 1e-6 * g.size
+
+
+def test_noncola_initial_matches_oracle(eh_table):
+    """set_noncola_initial (cola.c:297-338, the driver's initial= output at
+    main.c:131-137) on the 2LPT state: bit-exact against the oracle given
+    the same particles."""
+    nc, box, seed = 32, 100.0, 3
+    _say("noncola initial")
+    Om = float(np.float32(OM))
+    with _context(nc, 3 * nc, box, eh_table) as ctx:
+        ctx.lpt_set_displacement(0.1, OM, seed)
+        parts = ctx.get_particles()
+        a_x = ctx.times()[0]
+        snap = ctx.set_noncola_initial(Om)
+    ref = np.zeros(len(parts), orc.PARTICLE_MIN_DTYPE)
+    orc.lib().orc_set_noncola_initial(parts.ctypes.data, len(parts), Om, a_x, ref.ctypes.data)
+    assert np.array_equal(snap, ref)
+    assert np.abs(snap["v"]).max() > 0
+
+
 # ---------------------------------------------------------------- full run
 @pytest.mark.parametrize("fused", [False, True])
 def test_full_run_matches_oracle(eh_table, fused):

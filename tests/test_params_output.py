@@ -189,3 +189,26 @@ def test_subsample_selection_and_layout(tmp_path):
         assert struct.unpack_from("<i", raw, 28 + 24 * cnt)[0] == cnt
         assert len(raw) == 32 + 24 * cnt
     sub.close()
+
+
+def test_extension_keys(tmp_path):
+    """pm_ngrid and residual_net (keys of this build, unknown to the
+    reference reader) parse; files without them read as before (0 / absent);
+    an empty residual_net or an odd pm_ngrid fails the confirmation."""
+    path = tmp_path / "param.lua"
+    base = open(os.path.join(HERE, "golden", "param_full.lua")).read()
+    path.write_text(base + 'pm_ngrid = 72\nresidual_net = "nets/x.net"  -- learned residual\n')
+    p = ch.read_parameters(str(path))
+    d = p.as_dict()
+    assert d["pm_ngrid"] == 72 and d["residual_net_filename"] == "nets/x.net"
+    assert d["nc"] == 48 and d["pm_nc_factor"] == 2
+    ch.confirm_parameters(p)
+    path.write_text(base)
+    d = ch.read_parameters(str(path)).as_dict()
+    assert d["pm_ngrid"] == 0 and d["residual_net_filename"] is None
+    for extra, code in (('residual_net = ""\n', 10024), ("pm_ngrid = 73\n", 10025)):
+        path.write_text(base + extra)
+        q = ch.read_parameters(str(path))
+        with pytest.raises(ch.ColaError) as e:
+            ch.confirm_parameters(q)
+        assert e.value.code == code
