@@ -1,0 +1,194 @@
+"""The five workloads at their own sizes on the GPU.
+
+  * C1 (64^3 particles, 192^3 grid, L 100, seed 100): the whole run against
+    the oracle run live: every particle within 1e-3 of the mean
+    interparticle spacing at a = 1, velocities within 1e-3 of their scale,
+    P(k) within 0.1% for k <= k_Nyq / 2 (k_Nyq = pi nc / L).
+  * C2 (256^3 / 768^3, L 500, seed 1) and the three C3 sizes: the same
+    bounds against the oracle's runs stored by tests/golden/
+    gen_fullsize_golden.py (a fixed sample of 20000 particle IDs and the
+    P(k) of all particles; the oracle takes minutes at these sizes).
+  * C3: the 48 uneven realizations (16 x 96^3/288^3 L 187.5, 16 x
+    128^3/384^3 L 250, 16 x 160^3/480^3 L 312.5, seeds 1-48) batched with
+    graph-captured steps equal the realizations run one by one, bit for bit.
+  * C4 (128^3, coarse 192^3 / fine 384^3, L 250): with the committed net, the
+    corrected coarse run is closer to the fine run in P(k) than the plain
+    coarse run, and within 1% for k <= 1 h/Mpc (held-out seeds).
+  * C5 (512^3, L 1000): corrected 768^3 against plain 1536^3 (a padded grid
+    of 3.7e9 floats): P(k) within 1% for k <= 1 h/Mpc, and the fine grid
+    conserves mass (every cell written: the deposit's sum is -N_g + WPAR N_p).
+P(k) estimators: the library's shell estimator (CIC on an n^3 grid, window
+divided out), which tests/test_gpu_analysis.py pins to the oracle's.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cola_halo_amd as ch
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+OM = 0.273
+OMF = float(np.float32(OM))
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NET = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cola_halo_amd", "nets",
+                   "eh98_cell1.30.net")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _ctx(nc, ng, box, table):
+    k, p = table
+    ctx = ch.Context(nc, ng, box)
+    ctx.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+    return ctx
+
+
+def _periodic_dx(a, b, box):
+    d = np.abs(a.astype(np.float64) - b.astype(np.float64)) % box
+    return np.minimum(d, box - d)
+
+
+def _check_against_oracle(snap_ids, gpu_rec, ref_x, ref_v, box, nc, pk_gpu, pk_ref, label):
+    spacing = box / nc
+    dx = _periodic_dx(gpu_rec["x"], ref_x, box).max()
+    vscale = np.abs(ref_v).max()
+    dv = np.abs(gpu_rec["v"].astype(np.float64) - ref_v).max()
+    rel = np.abs(pk_gpu / pk_ref - 1.0).max()
+    _say(f"  {label}: max dx {dx / spacing:.2e} spacing, max dv {dv / vscale:.2e} of scale, P(k) {rel:.2e}")
+    assert np.array_equal(gpu_rec["id"], snap_ids)
+    assert dx <= 1e-3 * spacing
+    assert dv <= 1e-3 * vscale
+    assert rel <= 1e-3
+
+
+def test_c1_full_run_matches_oracle(eh_table):
+    nc, ng, box, seed = 64, 192, 100.0, 100
+    _say("C1 64^3 / 192^3 against the oracle run")
+    k, p = eh_table
+    orc.set_power_table(k, p)
+    kmax = 0.5 * np.pi * nc / box
+    with _ctx(nc, ng, box, eh_table) as ctx:
+        (a, snap), = ch.run_cola(ctx, OM, 1.0, 10, seed, [0.0])
+        _, pk_gpu, nm = ctx.power_spectrum(128, 0.0, kmax, 16, deconvolve=True, source="snapshot")
+    ref = np.zeros(nc ** 3, orc.PARTICLE_MIN_DTYPE)
+    a_out = np.zeros(1, np.float32)
+    nw = orc.C.c_int(0)
+    zz = np.zeros(1)
+    assert orc.lib().orc_run(nc, ng, box, OM, 1.0, 10, seed, zz.ctypes.data, 1, ref.ctypes.data, a_out.ctypes.data,
+                             orc.C.byref(nw)) == 0
+    delta, _ = orc.coarse_grid(ref, 128, box)
+    _, pk_ref, nm_ref = orc.power_spectrum(delta, box, 0.0, kmax, 16, flags=1)
+    assert np.array_equal(nm, nm_ref)
+    _check_against_oracle(ref["id"], snap, ref["x"], ref["v"], box, nc, pk_gpu[nm > 0], pk_ref[nm > 0], "C1")
+
+
+@pytest.mark.parametrize("name", ["c2", "c3_96", "c3_128", "c3_160"])
+def test_full_size_run_matches_oracle_fixture(eh_table, name):
+    g = np.load(os.path.join(GOLD, f"fullsize_{name}.npz"))
+    nc, ng, box, seed = int(g["nc"]), int(g["ngrid"]), float(g["box"]), int(g["seed"])
+    _say(f"{name}: {nc}^3 / {ng}^3 L {box:g} seed {seed} against the stored oracle run")
+    with _ctx(nc, ng, box, eh_table) as ctx:
+        (a, snap), = ch.run_cola(ctx, OM, 1.0, 10, seed, [0.0])
+        assert a == pytest.approx(float(g["a"]))
+        kb, pk, nm = ctx.power_spectrum(int(g["pk_grid"]), 0.0, float(g["kmax"]), len(g["pk"]), deconvolve=True,
+                                        source="snapshot")
+    assert np.array_equal(nm, g["nmodes"])
+    sel = nm > 0
+    ids = g["ids"]
+    _check_against_oracle(ids, snap[ids - 1], g["x"], g["v"], box, nc, pk[sel], g["pk"][sel], name)
+
+
+ENSEMBLE = [(96, 288, 187.5)] * 16 + [(128, 384, 250.0)] * 16 + [(160, 480, 312.5)] * 16
+
+
+def test_c3_ensemble_batched_equals_single_runs(eh_table):
+    """48 realizations of three sizes, steps batched across contexts with
+    graph capture, equal the realizations run alone (main.c:117-121: seed =
+    random_seed + i), bit for bit."""
+    _say("C3 48 realizations batched")
+    ctxs = [_ctx(nc, ng, box, eh_table) for nc, ng, box in ENSEMBLE]
+    try:
+        for c in ctxs:
+            c.set_step_graph(True)
+        snaps = ch.run_cola_batch(ctxs, OM, 1.0, 10, list(range(1, 49)))
+    finally:
+        for c in ctxs:
+            c.close()
+    _say("C3 single runs")
+    singles = {}
+    for i, (nc, ng, box) in enumerate(ENSEMBLE):
+        c = singles.get(ng)
+        if c is None:
+            c = singles[ng] = _ctx(nc, ng, box, eh_table)
+        (a, ref), = ch.run_cola(c, OM, 1.0, 10, i + 1, [0.0])
+        assert np.array_equal(snaps[i], ref), i
+    for c in singles.values():
+        c.close()
+
+
+def _pk_ratio(runs, seeds, nc, box, pk_n, eh_table):
+    """Mean over seeds of P_name / P_fine for k <= 1 h/Mpc (25 shells)."""
+    ratios = {name: [] for name in runs if name != "fine"}
+    kb = None
+    for seed in seeds:
+        pk = {}
+        for name, ctx in runs.items():
+            (a, snap), = ch.run_cola(ctx, OM, 1.0, 10, seed, [0.0])
+            kb, pk[name], _ = ctx.power_spectrum(pk_n, 0.0, 1.0, 25, deconvolve=True, source="snapshot")
+        for name in ratios:
+            ratios[name].append(pk[name] / pk["fine"])
+    sel = kb > 0
+    return {name: np.mean(r, axis=0)[sel] for name, r in ratios.items()}
+
+
+def test_c4_trained_net_gains_over_plain_coarse(eh_table):
+    """128^3 particles in 250 Mpc/h, held-out seeds 57 and 58 (the net trained
+    on seeds 1-56): the corrected 192^3 run's P(k) error against the 384^3
+    run is below the plain 192^3 run's and within 1%."""
+    _say("C4 corrected vs plain coarse vs fine")
+    nc, box = 128, 250.0
+    net = ch.ResidualNet.from_file(192, NET)
+    runs = {"fine": _ctx(nc, 384, box, eh_table), "coarse": _ctx(nc, 192, box, eh_table),
+            "corrected": _ctx(nc, 192, box, eh_table)}
+    runs["corrected"].set_residual_net(net)
+    try:
+        r = _pk_ratio(runs, [57, 58], nc, box, 384, eh_table)
+    finally:
+        for c in runs.values():
+            c.close()
+        net.close()
+    err_c = np.abs(r["coarse"] - 1).max()
+    err_n = np.abs(r["corrected"] - 1).max()
+    _say(f"  max |P/P_fine - 1|, k <= 1: plain coarse {err_c:.4f}, corrected {err_n:.4f}")
+    assert err_n < err_c
+    assert err_n <= 0.01
+
+
+def test_c5_production_gate_and_fine_grid_mass(eh_table):
+    """512^3 particles, L 1000, seed 1: the corrected 768^3 run within 1% of
+    the plain 1536^3 run in P(k) for k <= 1 h/Mpc; every force of the fine
+    run passes the mass check on its 1536^3 grid (3.7e9 padded floats)."""
+    _say("C5 corrected 768^3 vs plain 1536^3")
+    nc, box = 512, 1000.0
+    net = ch.ResidualNet.from_file(768, NET)
+    runs = {"fine": _ctx(nc, 1536, box, eh_table), "corrected": _ctx(nc, 768, box, eh_table)}
+    runs["corrected"].set_residual_net(net)
+    try:
+        r = _pk_ratio(runs, [1], nc, box, 768, eh_table)
+        fine = runs["fine"]
+        # the last force's mass sum over all 1536^3 cells: -N_g + WPAR N_p = 0
+        # up to float rounding of the cells (pm.c:535-562 allows 1e-7 WPAR N_p)
+        msum = fine.last_mass_sum()
+        assert abs(msum) <= 1e-7 * 27.0 * nc ** 3, msum
+    finally:
+        for c in runs.values():
+            c.close()
+        net.close()
+    err = np.abs(r["corrected"] - 1).max()
+    _say(f"  max |P_corrected / P_fine - 1|, k <= 1: {err:.4f}")
+    assert err <= 0.01

@@ -121,3 +121,73 @@ def test_driver_abort_codes(tmp_path):
     param.write_text(PARAM.format(pk="/nonexistent/pk.dat", d=tmp_path))
     r = subprocess.run([DRIVER, str(param)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 3000 & 0xFF
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_matches_oracle_run(tmp_path):
+    """The driver's z = 0 GADGET positions against the CPU oracle's run of the
+    same parameters (main.c schedule, the reference's P(k) file): within
+    1e-3 of the mean interparticle spacing, velocities within 1e-3 of their
+    scale (the GADGET file holds v / sqrt(a), write.c:74)."""
+    import oracle_lib as orc
+
+    print("driver vs oracle", flush=True)
+    nc, box, seed = 32, 64.0, 7
+    param = tmp_path / "param.lua"
+    text = PARAM.format(pk=PK_FILE, d=tmp_path).replace("nrealization= 2", "nrealization= 1")
+    for key in ("fof", "subsample", "coarse_grid", "initial"):
+        text = "\n".join(l for l in text.splitlines() if not l.startswith(key + "=")) + "\n"
+    param.write_text(text.replace("output_redshifts= {1.0, 0.0}", "output_redshifts= {0.0}"))
+    r = subprocess.run([DRIVER, str(param)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    t, pos, vel, ids = _gadget(tmp_path / f"snp{seed:05d}a.0")
+    assert orc.lib().orc_power_read_file(PK_FILE.encode()) == 0
+    ref = np.zeros(nc ** 3, orc.PARTICLE_MIN_DTYPE)
+    a_out = np.zeros(1, np.float32)
+    nw = orc.C.c_int(0)
+    zz = np.zeros(1)
+    assert orc.lib().orc_run(nc, 3 * nc, box, 0.273, 1.0, 10, seed, zz.ctypes.data, 1, ref.ctypes.data,
+                             a_out.ctypes.data, orc.C.byref(nw)) == 0
+    assert np.array_equal(ids, ref["id"].astype(np.uint32))
+    d = np.abs(pos.astype(np.float64) - ref["x"]) % box
+    assert np.minimum(d, box - d).max() <= 1e-3 * box / nc
+    v = vel.astype(np.float64) * np.sqrt(np.float64(t))
+    assert np.abs(v - ref["v"]).max() <= 1e-3 * np.abs(ref["v"]).max()
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+def test_driver_corrected_run(tmp_path):
+    """pm_ngrid and residual_net: the driver runs 64^3 particles on a 96^3
+    grid (factor 1.5, cell 1.30 Mpc/h) with the committed net and writes the
+    same snapshot as the library run with the net attached, which differs
+    from the plain run on the same grid."""
+    print("driver corrected run", flush=True)
+    net_file = os.path.join(os.path.dirname(HERE), "cola_halo_amd", "nets", "eh98_cell1.30.net")
+    nc, ng, box, seed = 64, 96, 125.0, 3
+    param = tmp_path / "param.lua"
+    param.write_text(f"""nc = {nc}
+boxsize = {box}
+random_seed = {seed}
+ntimestep = 10
+output_redshifts = {{0.0}}
+omega_m = 0.273
+h = 0.705
+sigma8 = 0.812
+powerspectrum = "{PK_FILE}"
+snapshot = "{tmp_path}/snp"
+pm_ngrid = {ng}            -- factor 1.5
+residual_net = "{net_file}"
+""")
+    r = subprocess.run([DRIVER, str(param)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "learned force residual" in r.stdout
+    t, pos, vel, ids = _gadget(tmp_path / f"snp{seed:05d}a.0")
+    with ch.Context(nc, ng, box) as ctx, ch.ResidualNet.from_file(ng, net_file) as net:
+        ctx.power_init(PK_FILE, 0.1, 0.812, 0.273, 0.727)
+        ctx.set_residual_net(net)
+        (a, snap), = ch.run_cola(ctx, 0.273, 1.0, 10, seed, [0.0])
+        ctx.set_residual_net(None)
+        (a, plain), = ch.run_cola(ctx, 0.273, 1.0, 10, seed, [0.0])
+    assert np.array_equal(pos, snap["x"])
+    assert not np.array_equal(pos, plain["x"])
+    assert np.abs(pos - plain["x"]).max() < 0.5 * box / nc
