@@ -128,6 +128,8 @@ _SIGNATURES = {
     "cola_get_phase_stats": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
     "cola_reset_phase_stats": (C.c_int, [C.c_void_p]),
     "cola_set_tiling": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cola_set_fft_schedule": (C.c_int, [C.c_void_p, C.c_int]),
+    "cola_fft_radices": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "cola_set_fft_xcd_group": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_eh98_nowiggle_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
@@ -304,11 +306,12 @@ class Tuning(C.Structure):
     """cola_tuning: the launch shapes the autotuner chose."""
     _fields_ = [("tile", C.c_int * 3), ("fft_log2_cols", C.c_int), ("fft_log2_rows", C.c_int),
                 ("xcd_group", C.c_int), ("fft_log2_xcols", C.c_int), ("ms_before", C.c_double),
-                ("ms_after", C.c_double)]
+                ("ms_after", C.c_double), ("fft_schedule", C.c_int), ("fft_radices", C.c_int * 8)]
 
     def as_dict(self):
         return {"tile": list(self.tile), "fft_log2_cols": self.fft_log2_cols, "fft_log2_rows": self.fft_log2_rows,
                 "fft_log2_xcols": self.fft_log2_xcols, "xcd_group": self.xcd_group,
+                "fft_schedule": self.fft_schedule, "fft_radices": [r for r in self.fft_radices if r],
                 "ms_before": round(self.ms_before, 4), "ms_after": round(self.ms_after, 4)}
 
 
@@ -710,7 +713,12 @@ class Context:
         context of the same sizes)."""
         tx, ty, tz = t["tile"]
         _check(lib().cola_set_tiling(self._h, tx, ty, tz, t["fft_log2_cols"]))
+        self.set_fft_schedule(t.get("fft_schedule", 0))
         self.set_fft_tiles(t["fft_log2_cols"], t["fft_log2_rows"], t["xcd_group"], t["fft_log2_xcols"])
+
+    def set_fft_schedule(self, schedule):
+        """Radix schedule of the PM transforms (0-4, see fft_radices)."""
+        _check(lib().cola_set_fft_schedule(self._h, int(schedule)))
 
     def autotune(self, reps=3):
         """Pick the fastest deposit tile / FFT launch shapes for the current
@@ -721,6 +729,13 @@ class Context:
 
     def set_fft_xcd_group(self, group):
         _check(lib().cola_set_fft_xcd_group(self._h, group))
+
+
+def fft_radices(n, schedule=0):
+    """Radix passes of an n-point transform under a radix schedule (host only)."""
+    r = np.zeros(8, np.int32)
+    k = lib().cola_fft_radices(int(n), int(schedule), r.ctypes.data, 8)
+    return [int(v) for v in r[:max(k, 0)]]
 
 
 def step_batch(ctxs, omega_m, avel1, apos1):

@@ -93,6 +93,7 @@ struct cola_context {
   int fft_log2_tile = -1;  // column passes (cola_set_tiling sets both)
   int fft_log2_rows = -1;
   int fft_log2_xtile = -1;  // x column pass when set apart from the y pass (-1: as fft_log2_tile)
+  int fft_schedule = 0;     // radix schedule of the PM transforms (fft_factor)
   CicGeom cic;
   TileGeom tiles;
   BinBuffers bins;
@@ -1161,7 +1162,7 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
   COLA_TRY(set_deposit_tiles(c, tx, ty, tz));
   if (fft_log2_tile != c->fft_log2_tile || fft_log2_tile != c->fft_log2_rows) {
     FftPlan a, b;
-    COLA_TRY(a.init(c->ng, fft_log2_tile, fft_log2_tile, true));
+    COLA_TRY(a.init(c->ng, fft_log2_tile, fft_log2_tile, true, -1, c->fft_schedule));
     if (b.init(c->nc, fft_log2_tile, fft_log2_tile) != kOk) {
       a.destroy();
       return kErrUnsupported;
@@ -1183,6 +1184,37 @@ int cola_set_tiling(cola_context* c, int tx, int ty, int tz, int fft_log2_tile)
   return kOk;
 }
 
+int cola_set_fft_schedule(cola_context* c, int schedule)
+{
+  if (!valid(c) || schedule < 0 || schedule >= kFftSchedules) return kErrArg;
+  if (schedule == c->fft_schedule) return kOk;
+  FftPlan a;
+  const int st = a.init(c->ng, c->fft_log2_tile, c->fft_log2_rows, true, c->fft_log2_xtile, schedule);
+  if (st != kOk) {
+    set_error("FFT radix schedule not available for this grid");
+    return st;
+  }
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  drop_step_graph(c);
+  a.xgroup = c->pm_fft.xgroup;
+  c->pm_fft.destroy();
+  c->pm_fft = a;
+  c->pm_fft.hook = fft_hook;
+  c->pm_fft.hook_user = c;
+  c->fft_schedule = schedule;
+  return kOk;
+}
+
+int cola_fft_radices(int n, int schedule, int* radices, int max)
+{
+  if (!radices || max < 1) return -kErrArg;
+  FftRadices r;
+  if (n < 2 || n % 2 || !fft_factor(n, &r, schedule)) return 0;
+  const int np = std::min(r.npass, max);
+  for (int i = 0; i < np; ++i) radices[i] = r.pass[i].R;
+  return r.npass;
+}
+
 int cola_set_fft_tiles(cola_context* c, int log2_cols, int log2_rows, int xcd_group)
 {
   return cola_set_fft_tiles_xy(c, log2_cols, -1, log2_rows, xcd_group);
@@ -1197,7 +1229,7 @@ int cola_set_fft_tiles_xy(cola_context* c, int log2_ycols, int log2_xcols, int l
   drop_step_graph(c);
   if (log2_ycols != c->fft_log2_tile || log2_rows != c->fft_log2_rows || log2_xcols != c->fft_log2_xtile) {
     FftPlan a;
-    const int st = a.init(c->ng, log2_ycols, log2_rows, true, log2_xcols);
+    const int st = a.init(c->ng, log2_ycols, log2_rows, true, log2_xcols, c->fft_schedule);
     if (st != kOk) {
       set_error("FFT tile widths not supported for this grid");
       return st;
@@ -1265,10 +1297,12 @@ int cola_autotune(cola_context* c, int reps, cola_tuning* out)
   best.fft_log2_rows = c->fft_log2_rows;
   best.fft_log2_xcols = c->fft_log2_xtile;
   best.xcd_group = c->pm_fft.xgroup;
+  best.fft_schedule = c->fft_schedule;
   int st = time_pm_fields(c, reps, &best.ms_before);
   best.ms_after = best.ms_before;
   auto apply = [&](const cola_tuning& t) -> int {
     int r = set_deposit_tiles(c, t.tile[0], t.tile[1], t.tile[2]);
+    if (!r) r = cola_set_fft_schedule(c, t.fft_schedule);
     if (!r) r = cola_set_fft_tiles_xy(c, t.fft_log2_cols, t.fft_log2_xcols, t.fft_log2_rows, t.xcd_group);
     return r;
   };
@@ -1313,7 +1347,19 @@ int cola_autotune(cola_context* c, int reps, cola_tuning* out)
     t.xcd_group = g;
     trial(t);
   }
+  // the radix schedule of the transforms (FFTW_MEASURE picks among its plans
+  // the same way, pm.c:139-147), then the x column width again for it
+  for (int sch = 0; sch < kFftSchedules; ++sch) {
+    cola_tuning t = best;
+    t.fft_schedule = sch;
+    trial(t);
+  }
+  sweep_x();
   if (!st) st = apply(best);
+  if (!st) {
+    for (int i = 0; i < 8; ++i) best.fft_radices[i] = 0;
+    (void)cola_fft_radices(c->ng, best.fft_schedule, best.fft_radices, 8);
+  }
   c->profiling = prof;
   *out = best;
   return st;

@@ -60,6 +60,7 @@ struct FftPassGeom {
 // Per-size plan: radix schedule, twiddles and tile shapes of the three passes.
 struct FftPlan {
   int n = 0;
+  int schedule = 0;
   FftRadices rad;
   float2* d_tw = nullptr;  // exp(-2 pi i k / n), k < n
   FftPassGeom grows, gy, gx;
@@ -79,7 +80,9 @@ struct FftPlan {
   // (row pairs); -1 picks the measured defaults.
   // pad_rows: complex rows of round_up(n/2+1, 16) values instead of n/2+1.
   // log2T_xcols >= 0 gives the x column pass (plain or fused) its own width.
-  int init(int n, int log2T_cols = -1, int log2T_rows = -1, bool pad_rows = false, int log2T_xcols = -1);
+  // schedule: the radix schedule (fft_factor).
+  int init(int n, int log2T_cols = -1, int log2T_rows = -1, bool pad_rows = false, int log2T_xcols = -1,
+           int schedule = 0);
   void destroy();
 
   // In place on the padded layout (real rows of 2 nzp floats): real ->
@@ -97,8 +100,17 @@ struct FftPlan {
   bool can_fuse() const;
 };
 
-// Radix schedule for n = 2^a 3^b 5^c; returns false otherwise.
-bool fft_factor(int n, FftRadices* r);
+// Radix schedules of n = 2^a 3^b 5^c (the order and grouping of the radix
+// passes; every schedule computes the same DFT, rounding differs):
+//   0  powers of two as radix 8 (a leftover 2 as 4 x 4) first, then 3s, 5s
+//   1  3s and 5s first, then the powers of two as in 0
+//   2  powers of two as radix 4 (an odd power adds one radix 8), then 3s, 5s
+//   3  the powers of two of 0 in reverse order (smallest first), then 3s, 5s
+//   4  as 0 with its last power-of-two radix moved to the second pass
+constexpr int kFftSchedules = 5;
+// Radix schedule `schedule` for n; false if n has another prime factor or
+// the schedule needs more than 8 passes.
+bool fft_factor(int n, FftRadices* r, int schedule = 0);
 // Register slots one thread needs for tile width 2^log2T with the given
 // workgroup size (0 if the tile exceeds kFftMaxTile or 24 slots).
 int fft_emax(const FftRadices& r, int log2T, int threads);

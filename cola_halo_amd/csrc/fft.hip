@@ -732,39 +732,55 @@ void set_groups(TileArgs& A, int tpb, int group)
 
 }  // namespace
 
-bool fft_factor(int n, FftRadices* r)
+bool fft_factor(int n, FftRadices* r, int schedule)
 {
   *r = FftRadices{};
   r->n = n;
+  if (schedule < 0 || schedule >= kFftSchedules) return false;
   int m = n;
   int twos = 0;
   while (m % 2 == 0) {
     m /= 2;
     ++twos;
   }
-  // powers of two as radix-8 passes; a leftover 2 becomes 4 x 4 instead of 8 x 2
-  std::vector<int> rad;
-  int n8 = twos / 3, n4 = 0, n2 = 0;
-  if (twos % 3 == 2) n4 = 1;
-  if (twos % 3 == 1) {
-    if (n8 > 0) {
-      --n8;
-      n4 = 2;
-    } else {
-      n2 = 1;
+  std::vector<int> pow2;  // radices of the power of two
+  if (schedule == 2) {
+    // radix 4 throughout; an odd power of two takes one radix 8 (or a 2 for n = 2)
+    int left = twos;
+    if (left % 2 == 1) {
+      pow2.push_back(left >= 3 ? 8 : 2);
+      left -= left >= 3 ? 3 : 1;
     }
+    for (; left > 0; left -= 2) pow2.push_back(4);
+  } else {
+    // radix 8 as long as it goes; a leftover 2 becomes 4 x 4 instead of 8 x 2
+    int n8 = twos / 3, n4 = 0, n2 = 0;
+    if (twos % 3 == 2) n4 = 1;
+    if (twos % 3 == 1) {
+      if (n8 > 0) {
+        --n8;
+        n4 = 2;
+      } else {
+        n2 = 1;
+      }
+    }
+    for (int i = 0; i < n8; ++i) pow2.push_back(8);
+    for (int i = 0; i < n4; ++i) pow2.push_back(4);
+    for (int i = 0; i < n2; ++i) pow2.push_back(2);
+    if (schedule == 3) std::reverse(pow2.begin(), pow2.end());
+    if (schedule == 4 && pow2.size() >= 3) std::rotate(pow2.begin() + 1, pow2.end() - 1, pow2.end());
   }
-  for (int i = 0; i < n8; ++i) rad.push_back(8);
-  for (int i = 0; i < n4; ++i) rad.push_back(4);
-  for (int i = 0; i < n2; ++i) rad.push_back(2);
+  std::vector<int> odd;
   while (m % 3 == 0) {
-    rad.push_back(3);
+    odd.push_back(3);
     m /= 3;
   }
   while (m % 5 == 0) {
-    rad.push_back(5);
+    odd.push_back(5);
     m /= 5;
   }
+  std::vector<int> rad = schedule == 1 ? odd : pow2;
+  for (int v : schedule == 1 ? pow2 : odd) rad.push_back(v);
   if (m != 1 || rad.size() > 8 || n < 2) return false;
   r->npass = (int)rad.size();
   int L = 1;
@@ -823,11 +839,12 @@ static int cap_log2(int n)
   return l;
 }
 
-int FftPlan::init(int n_, int log2T_cols, int log2T_rows, bool pad_rows, int log2T_xcols)
+int FftPlan::init(int n_, int log2T_cols, int log2T_rows, bool pad_rows, int log2T_xcols, int schedule_)
 {
   n = n_;
+  schedule = schedule_;
   nzp = pad_rows ? (n / 2 + 1 + 15) / 16 * 16 : n / 2 + 1;
-  if (n % 2 != 0 || !fft_factor(n, &rad)) return kErrUnsupported;
+  if (n % 2 != 0 || !fft_factor(n, &rad, schedule)) return kErrUnsupported;
   // widest power-of-two tile with n*T <= 6144 (48 KiB of LDS), no wider than
   // the columns, and a register tile of at most 24 values per thread
   auto pick = [&](int want, long long ncols) -> FftPassGeom {

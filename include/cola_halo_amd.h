@@ -439,18 +439,35 @@ int cola_set_fft_tiles(cola_context* ctx, int log2_cols, int log2_rows, int xcd_
    passes (log2_xcols = -1: as log2_ycols). */
 int cola_set_fft_tiles_xy(cola_context* ctx, int log2_ycols, int log2_xcols, int log2_rows, int xcd_group);
 
+/* Radix schedule of the PM transforms (the order and grouping of the
+   Stockham radix passes): 0 radix 8 first (default), 1 radices 3 and 5
+   first, 2 radix 4, 3 the powers of two smallest first, 4 the last
+   power-of-two radix moved to the second pass.  Every schedule computes the
+   same DFT; the float rounding differs (agreement ~1e-6 of the maximum),
+   as between the plans FFTW_MEASURE may pick (pm.c:139-147). */
+int cola_set_fft_schedule(cola_context* ctx, int schedule);
+/* Host-only: the radices of schedule `schedule` for an n-point transform into
+   radices[0 .. max-1]; returns the number of passes, 0 if n or the schedule
+   is not supported. */
+int cola_fft_radices(int n, int schedule, int* radices, int max);
+
 /* Autotuner (no reference counterpart): times the PM force fields (deposit +
    transforms, `reps` per candidate) of the current particle state for deposit
-   tile shapes, FFT column / row tile widths and XCD groups by coordinate
-   descent, keeps the fastest and reports it.  No result depends on the choice:
-   the deposit sums in order-independent fixed point and every transform
-   computes each column with the same arithmetic whatever the tile width.
+   tile shapes, FFT column / row tile widths, XCD groups and the radix
+   schedule by coordinate descent, keeps the fastest and reports it.  The
+   launch shapes do not change any result (the deposit sums in
+   order-independent fixed point and every transform computes each column
+   with the same arithmetic whatever the tile width); the radix schedule
+   changes the transforms' rounding only (see cola_set_fft_schedule), like
+   FFTW_MEASURE's choice of plan in the reference (pm.c:139-147).
    Wraps the positions first (move.c:33-41), as every step does. */
 typedef struct {
   int tile[3];
   int fft_log2_cols, fft_log2_rows, xcd_group;
   int fft_log2_xcols; /* x column pass width, -1 = as fft_log2_cols (the y pass) */
   double ms_before, ms_after; /* mean PM force-field time of the start and the chosen shape */
+  int fft_schedule;    /* radix schedule of the PM transforms (cola_set_fft_schedule) */
+  int fft_radices[8];  /* its radix passes, first to last (0 after the last) */
 } cola_tuning;
 int cola_autotune(cola_context* ctx, int reps, cola_tuning* out);
 

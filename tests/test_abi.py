@@ -1,6 +1,8 @@
 """The C ABI: the shared library loads without a GPU and exports every
 function include/cola_halo_amd.h declares; the Python binding covers them."""
 import ctypes as C
+
+import numpy as np
 import re
 
 import cola_halo_amd as ch
@@ -42,3 +44,20 @@ def test_error_path_without_gpu_or_args():
     h = C.c_void_p()
     assert ch.lib().cola_context_create(15, 48, 100.0, C.byref(h)) == ch.ERR_ARG  # odd nc
     assert b"even" in ch.lib().cola_last_error()
+
+
+def test_fft_radix_schedules():
+    """Every schedule factors n into supported radices whose product is n."""
+    for n in (96, 192, 768, 1536, 160, 250, 24):
+        seen = set()
+        for sch in range(5):
+            r = ch.fft_radices(n, sch)
+            assert r and int(np.prod(r)) == n and set(r) <= {2, 3, 4, 5, 8}, (n, sch, r)
+            seen.add(tuple(r))
+        assert len(seen) >= (2 if n in (96, 192, 768, 1536) else 1)
+    assert ch.fft_radices(768, 0) == [8, 8, 4, 3]
+    assert ch.fft_radices(768, 1) == [3, 8, 8, 4]
+    assert ch.fft_radices(768, 2) == [4, 4, 4, 4, 3]
+    assert ch.fft_radices(768, 3) == [4, 8, 8, 3]
+    assert ch.fft_radices(768, 4) == [8, 4, 8, 3]
+    assert ch.fft_radices(7 * 16, 0) == [] and ch.fft_radices(768, 5) == []
