@@ -23,7 +23,8 @@ divided out); it is 0 when the gate fails.  `speedup_vs_fine` is the ratio
 of the two runs' throughputs.  `roofline` is for the kernel with the most
 device time per corrected step (HIP events on the context's stream during
 the timed region); `cpu_baseline` times the CPU restatement (oracle, C +
-OpenMP) on one plain step of the coarse grid.  `--workload plain` times
+OpenMP) on one plain step of 128^3 particles on a 384^3 grid in 250 Mpc/h
+(the fine run's cell size and particle spacing).  `--workload plain` times
 plain COLA at 256^3 / 768^3 (box 500) instead, `--ensemble` the 48
 realizations of uneven sizes.
 """
@@ -50,9 +51,13 @@ NET_WEIGHTS = os.path.join(ROOT, "cola_halo_amd", "nets", "eh98_cell1.30.net")
 PK_KMAX, PK_BINS, PK_TOL = 1.0, 25, 0.01
 
 
-# launches of one kernel inside one profiling phase (the hidden layers 2 and 3
-# are one phase of two k_conv16 launches)
-LAUNCHES_PER_PHASE = {"net_conv_hidden": 2}
+# launches of one kernel inside one profiling phase: the hidden layers are one
+# phase of two k_conv16 launches (layers 2 and 3), or of one (layer 3) when
+# layers 1-2 run fused (phase net_conv12)
+def launches_per_phase(phase, stats):
+    if phase == "net_conv_hidden":
+        return 1 if stats.get("net_conv12", (0.0, 0))[1] > 0 else 2
+    return 1
 
 
 def kernel_cost(phase, nc, ng):
@@ -79,6 +84,8 @@ def kernel_cost(phase, nc, ng):
         "net_features": (4 * n_g + 2 * n_g, 0, "k_features"),
         "net_conv1": (2 * n_g + 32 * n_g, 27 * 16 * 2 * n_g, "k_conv1"),
         "net_conv_hidden": (64 * n_g, conv16, "k_conv16<kFwdRelu>"),
+        # features + layers 1-2 fused: fp32 density in, bf16 h2 out
+        "net_conv12": (4 * n_g + 32 * n_g, 27 * 16 * 2 * n_g + conv16, "k_conv12"),
         "net_conv_out_add": (32 * n_g + 24 * n_g, 27 * 16 * 3 * 2 * n_g, "k_conv16<kFwdAdd>"),
     }
     return table.get(phase)
@@ -117,7 +124,7 @@ def roofline(stats, nc, ng, net=False):
     if cost is None or dom_n == 0:
         return None
     nbytes, flops, kname = cost
-    launches = dom_n * LAUNCHES_PER_PHASE.get(dom, 1)
+    launches = dom_n * launches_per_phase(dom, stats)
     avg_s = dom_ms / launches / 1e3
     t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
     t_mfma = flops / (MFMA_BF16_PEAK_TFLOPS * 1e12)
@@ -159,6 +166,26 @@ def cpu_threads():
     return n
 
 
+# The CPU baseline's bounded sample: one plain step of 128^3 particles on a
+# 384^3 grid in 250 Mpc/h, the cell size (0.651 Mpc/h) and particle spacing
+# (1.95 Mpc/h) of the plain fine production run and of the plain 256^3 / 768^3
+# run, at a size the host finishes in seconds (particle-steps/s of a PM step
+# at fixed cell size depends on the size only through the FFT's log N).
+CPU_SAMPLE = (128, 384, 250.0)
+
+
+def sample_particles(nc, ng, box, seed=1):
+    """2LPT particles (seed 1, a = 0.1) of nc^3 in a box of side box, made by
+    the library on the GPU (the CPU baseline times only the step)."""
+    import cola_halo_amd as ch
+
+    k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
+    with ch.Context(nc, ng, box) as ctx:
+        ctx.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+        ctx.lpt_set_displacement(0.1, OM, seed)
+        return ctx.get_particles()
+
+
 def cpu_baseline(ctx_parts, nc, ng, box, threads):
     """CPU restatement of the reference's path (the oracle: C + OpenMP) on
     the host: one plain COLA step (wrap, PM force, kick, drift)."""
@@ -183,9 +210,11 @@ def cpu_baseline(ctx_parts, nc, ng, box, threads):
     dt = time.perf_counter() - t0
     return {"value": nc ** 3 / dt, "unit": "particle-steps/s", "cores": int(threads), "kind": "port",
             "cpu": cpu_model(),
-            "sample": f"CPU restatement (the oracle, C + OpenMP on {threads} threads), not the reference binary: "
-                      f"1 plain step (wrap + PM force on {ng}^3 + kick + drift) of {nc}^3 particles from the 2LPT "
-                      f"state, {dt:.1f} s, oracle status {st}"}
+            "host_cores": os.cpu_count(),
+            "sample": f"CPU restatement (the oracle, C + OpenMP on {threads} threads of {os.cpu_count()} host "
+                      f"cores), not the reference binary (FFTW/GSL/MPI are absent): 1 plain step (wrap + PM force "
+                      f"on {ng}^3 + kick + drift) of {nc}^3 particles in a {box:g} Mpc/h box from the 2LPT state "
+                      f"of seed 1, {dt:.1f} s, oracle status {st}"}
 
 
 def timed_steps(ctx, nsteps, Om):
@@ -292,11 +321,8 @@ def production_bench(args):
     gate = err <= PK_TOL
     cpu = None
     if not args.no_cpu_baseline:
-        ctx = runs["corrected"]
-        ctx.set_residual_net(None)
-        ctx.state_restore()
-        parts = ctx.get_particles()
-        cpu = cpu_baseline(parts, nc, P["coarse"], box, cpu_threads())
+        s_nc, s_ng, s_box = CPU_SAMPLE
+        cpu = cpu_baseline(sample_particles(s_nc, s_ng, s_box), s_nc, s_ng, s_box, cpu_threads())
     for ctx in runs.values():
         ctx.close()
     net.close()
@@ -377,8 +403,8 @@ def plain_bench(args):
     step_frac = b_step / (elapsed / forces) / 1e9 / HBM_PEAK_GBS
     cpu = None
     if not args.no_cpu_baseline:
-        ctx.state_restore()
-        cpu = cpu_baseline(ctx.get_particles(), nc, ng, box, cpu_threads())
+        s_nc, s_ng, s_box = CPU_SAMPLE
+        cpu = cpu_baseline(sample_particles(s_nc, s_ng, s_box), s_nc, s_ng, s_box, cpu_threads())
     ctx.close()
     phases = {k2: round(v[0] / max(forces, 1), 3) for k2, v in stats.items() if v[1] > 0}
     print(json.dumps({

@@ -100,6 +100,7 @@ _SIGNATURES = {
     "cola_resnet_forward_features_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_resnet_time_forward": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "cola_resnet_set_restriction": (C.c_int, [C.c_void_p, C.c_float]),
+    "cola_resnet_set_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                             C.c_void_p]),
     "cola_copy_particles": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -383,6 +384,12 @@ class ResidualNet:
         neighbours of each coarse cell with weights (w, 1 - 2w, w) per axis
         (0 = the point value, 0.25 = full weighting, default 0.125)."""
         _check(lib().cola_resnet_set_restriction(self._h, side_weight))
+
+    def set_fusion(self, on):
+        """Inference launch shape in a context's force: features and layers
+        1-2 in one pass (True, the default) or three launches (False); the
+        forces are the same bit for bit."""
+        _check(lib().cola_resnet_set_fusion(self._h, int(bool(on))))
 
     def add_pair(self, coarse, fine):
         _check(lib().cola_resnet_add_pair(self._h, coarse._h, fine._h))

@@ -58,13 +58,14 @@ enum Phase : int {
   kPhNetConv1,
   kPhNetHidden,
   kPhNetHead,
+  kPhNetConv12,
   kNumPhases
 };
 const char* const kPhaseNames[kNumPhases] = {
     "wrap",      "bin_count",  "bin_scan",   "bin_scatter", "deposit",   "fft_z_r2c",     "fft_y_fwd",
     "fft_x_fwd", "fft_x_inv",  "fft_y_inv",  "fft_z_c2r",   "fft_x_fused", "readout",   "readout_kick_drift", "kick",
     "drift",     "snapshot",   "lpt_modes",  "lpt_fft",     "lpt_other", "net_features", "net_conv1",
-    "net_conv_hidden", "net_conv_out_add"};
+    "net_conv_hidden", "net_conv_out_add", "net_conv12"};
 
 struct EventPair {
   int phase;
@@ -324,7 +325,14 @@ int pm_fields(cola_context* c)
   float* const out[3] = {c->fgrid[0], c->fgrid[1], c->fgrid[2]};
   // the net's features come from the real-space density before the r2c
   // overwrites it; its last layer adds its output to the force grids
-  if (c->net) {
+  if (c->net && c->net->fuse12) {
+    phase_begin(c, kPhNetConv12);
+    COLA_TRY(resnet_layers12(c->net, c->density, c->cic.row, &c->d_prm->a, c->stream));
+    phase_end(c, kPhNetConv12);
+    phase_begin(c, kPhNetHidden);
+    COLA_TRY(resnet_layer3(c->net, c->stream));
+    phase_end(c, kPhNetHidden);
+  } else if (c->net) {
     phase_begin(c, kPhNetFeatures);
     COLA_TRY(resnet_features(c->net, c->density, c->cic.row, c->stream));
     phase_end(c, kPhNetFeatures);
@@ -1922,6 +1930,18 @@ int cola_resnet_load(cola_resnet* h, const char* filename)
   COLA_TRY(cola_resnet_set_params(h, prm.data()));
   COLA_TRY(cola_resnet_set_scales(h, scales[0], scales[1]));
   h->r.restrict_side = scales[2];
+  return kOk;
+}
+
+int cola_resnet_set_fusion(cola_resnet* h, int on)
+{
+  if (!h || (on != 0 && on != 1)) return kErrArg;
+  if (h->attached) {
+    // a captured step holds the launches of the other path
+    COLA_HIP(hipStreamSynchronize(h->attached->stream));
+    drop_step_graph(h->attached);
+  }
+  h->r.fuse12 = on != 0;
   return kOk;
 }
 

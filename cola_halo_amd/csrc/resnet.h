@@ -43,6 +43,7 @@ struct ResNet {
   bool scale_set = false;    // out_scale fitted to training targets (or set by the caller)
   float in_scale = 1.0f;     // feature = asinh(delta) * in_scale
   float restrict_side = 0.125f;  // training targets: fine force averaged with weights (w, 1-2w, w) per axis
+  bool fuse12 = true;            // inference: features and layers 1-2 in one launch (resnet_layers12)
 
   // training set: per sample the feature grid (bf16 [n^3]), the target
   // residual (bf16 [3][n^3]) and the scale factor a
@@ -80,6 +81,12 @@ int resnet_body(ResNet* r, const float* pa, hipStream_t s);
 int resnet_layer1(ResNet* r, const float* pa, hipStream_t s);
 int resnet_layers23(ResNet* r, hipStream_t s);
 int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s);
+// The inference path with the first two layers fused (fuse12): h2 straight
+// from the real-space density grid (features, layer 1 and layer 2 in one
+// launch, a read from pa), then layer 3; equal to resnet_features,
+// resnet_layer1 and layer 2 bit for bit.
+int resnet_layers12(ResNet* r, const float* delta, long long row, const float* pa, hipStream_t s);
+int resnet_layer3(ResNet* r, hipStream_t s);
 // loss against target (bf16 [3][n^3] or fp32 [3][n^3]) and gradient into d_grad
 int resnet_backward(ResNet* r, const unsigned short* target_bf16, const float* target_f32, hipStream_t s);
 int resnet_adam(ResNet* r, float lr, hipStream_t s);

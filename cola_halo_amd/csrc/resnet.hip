@@ -185,6 +185,63 @@ __device__ inline uint2 pack_bf16x4(float a, float b, float c, float d)
   return make_uint2(__builtin_bit_cast(unsigned, lo), __builtin_bit_cast(unsigned, hi));
 }
 
+// Products of one output plane x in the K order of k_conv16: the kConvTy
+// output rows of wave row half wy, lanes at brick z zc (centre) and zs (z
+// neighbour), from the 16-channel input planes x - 1, x, x + 1 whose ring
+// slots start at cells sb[0], sb[1], sb[2] (slots of kPlaneCells cells, two
+// uint4 halves per cell).  Operands of input row j + 1 are read from LDS
+// while row j's products run (one row of software pipelining hides the LDS
+// latency).
+__device__ inline void conv16_plane(const uint4* __restrict__ ring, const int (&sb)[3], int wy, int zc, int zs,
+                                    int half, bool lo, const uint4 (&b)[14], floatx4 (&acc)[kConvTy])
+{
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+  for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  uint4 prev[3] = {zero4, zero4, zero4};
+  uint4 cen[3], sh[3];
+  const int row0 = kConvTy * wy;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    cen[d] = ring[(sb[d] + row0 * kBrickZ + zc) * 2 + half];
+    sh[d] = ring[(sb[d] + row0 * kBrickZ + zs) * 2 + half];
+  }
+#pragma unroll
+  for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
+    uint4 ncen[3], nsh[3];
+    if (j + 1 < kConvTy + 2) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        ncen[d] = ring[(sb[d] + (row0 + j + 1) * kBrickZ + zc) * 2 + half];
+        nsh[d] = ring[(sb[d] + (row0 + j + 1) * kBrickZ + zs) * 2 + half];
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of this row's products
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (j >= 2) acc[j - 2] = mfma(sh[d], b[3 * d + 2], acc[j - 2]);                     // dy = +1
+      if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(sh[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
+      if (j < kConvTy) acc[j] = mfma(sh[d], b[3 * d], acc[j]);                            // dy = -1
+    }
+    if (j >= 1 && j - 1 < kConvTy) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[j - 1] = mfma(sel4(lo, prev[d], cen[d]), b[9 + d], acc[j - 1]);
+    }
+    if (j >= 2) {
+      acc[j - 2] = mfma(sel4(lo, cen[0], cen[1]), b[12], acc[j - 2]);
+      acc[j - 2] = mfma(sel4(lo, cen[2], zero4), b[13], acc[j - 2]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      prev[d] = cen[d];
+      if (j + 1 < kConvTy + 2) {
+        cen[d] = ncen[d];
+        sh[d] = nsh[d];
+      }
+    }
+  }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(kThreads)
 k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
@@ -204,7 +261,6 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
   const int zc = 1 + 16 * wz + r;          // brick z of the lane's cell
   const int zs = lo ? zc - 1 : zc + 1;     // its z neighbour for the dz = -1 | +1 taps
   const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
-  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     const ConvUnit cu = conv_unit(u, n, march);
     __syncthreads();  // the previous unit's operand reads are done
@@ -224,51 +280,10 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
         }
       }
       floatx4 acc[kConvTy];
+      int sb[3];
 #pragma unroll
-      for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-      uint4 prev[3] = {zero4, zero4, zero4};
-      // operands of input row j + 1 are read from LDS while row j's products
-      // run (one row of software pipelining hides the LDS latency)
-      uint4 cen[3], sh[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        cen[d] = ring[ring_cell(x - 1 + d, kConvTy * wy, zc) * 2 + half];
-        sh[d] = ring[ring_cell(x - 1 + d, kConvTy * wy, zs) * 2 + half];
-      }
-#pragma unroll
-      for (int j = 0; j < kConvTy + 2; ++j) {  // input row j of the half feeds output rows j - 2, j - 1, j
-        uint4 ncen[3], nsh[3];
-        if (j + 1 < kConvTy + 2) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) {
-            ncen[d] = ring[ring_cell(x - 1 + d, kConvTy * wy + j + 1, zc) * 2 + half];
-            nsh[d] = ring[ring_cell(x - 1 + d, kConvTy * wy + j + 1, zs) * 2 + half];
-          }
-        }
-        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of this row's products
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          if (j >= 2) acc[j - 2] = mfma(sh[d], b[3 * d + 2], acc[j - 2]);                 // dy = +1
-          if (j >= 1 && j - 1 < kConvTy) acc[j - 1] = mfma(sh[d], b[3 * d + 1], acc[j - 1]);  // dy = 0
-          if (j < kConvTy) acc[j] = mfma(sh[d], b[3 * d], acc[j]);                          // dy = -1
-        }
-        if (j >= 1 && j - 1 < kConvTy) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d) acc[j - 1] = mfma(sel4(lo, prev[d], cen[d]), b[9 + d], acc[j - 1]);
-        }
-        if (j >= 2) {
-          acc[j - 2] = mfma(sel4(lo, cen[0], cen[1]), b[12], acc[j - 2]);
-          acc[j - 2] = mfma(sel4(lo, cen[2], zero4), b[13], acc[j - 2]);
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          prev[d] = cen[d];
-          if (j + 1 < kConvTy + 2) {
-            cen[d] = ncen[d];
-            sh[d] = nsh[d];
-          }
-        }
-      }
+      for (int d = 0; d < 3; ++d) sb[d] = ring_cell(x - 1 + d, 0, 0);
+      conv16_plane(ring, sb, wy, zc, zs, half, lo, b, acc);
       // plane x + 2 (and the previous plane's stores) complete; every wave is
       // past its reads of plane x - 1, whose slot the next prefetch takes
       __builtin_amdgcn_s_waitcnt(0);
@@ -377,6 +392,21 @@ k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict_
 // of its cell from a ring of bf16 feature planes (the march of k_conv16);
 // the planes stream through registers, the loads of plane x + 2 in flight
 // during plane x's products.  a comes from device memory (pa) when given.
+// Bias of the first layer's output channels 4g..4g+3 with its constant a
+// channel folded in: bias + a * sum_t W[t][1][co], a and W rounded to bf16
+// as the 8-channel layer sees them (the products are exact in fp32).
+__device__ inline void conv1_bias(const float* __restrict__ w, const float* __restrict__ bias, float a, int g,
+                                  float (&bch)[4])
+{
+  const float abf = bf2f(f2bf(a));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float sa = 0.0f;
+    for (int t = 0; t < 27; ++t) sa += abf * bf2f(f2bf(w[(t * 8 + 1) * 16 + 4 * g + i]));
+    bch[i] = bias[4 * g + i] + sa;
+  }
+}
+
 constexpr int kConv1Waves = 3;  // waves per SIMD (= workgroups per CU) requested for k_conv1 (2: 1.5 ms slower at 768^3)
 
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kConv1Waves)))
@@ -386,16 +416,8 @@ k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const uint4 b = *reinterpret_cast<const uint4*>(wpk + ((long long)kConv1Chunk * 64 + lane) * 8);
-  // bias + a * sum_t W[t][1][co], with a and W rounded to bf16 as the
-  // 8-channel layer sees them (the products are exact in fp32)
-  const float abf = bf2f(f2bf(pa ? *pa : a_host));
   float bch[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float sa = 0.0f;
-    for (int t = 0; t < 27; ++t) sa += abf * bf2f(f2bf(w[(t * 8 + 1) * 16 + 4 * g + i]));
-    bch[i] = bias[4 * g + i] + sa;
-  }
+  conv1_bias(w, bias, pa ? *pa : a_host, g, bch);
   __shared__ unsigned short ring[kRingSlots * kPlaneCells];
   const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
   const int zc = 1 + 16 * wz + r;
@@ -463,6 +485,154 @@ k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__
         *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
             pack_bf16x4(fmaxf(acc[t][0] + bch[0], 0.0f), fmaxf(acc[t][1] + bch[1], 0.0f),
                         fmaxf(acc[t][2] + bch[2], 0.0f), fmaxf(acc[t][3] + bch[3], 0.0f));
+      }
+    }
+  }
+}
+
+// Features, layer 1 and layer 2 in one pass (the inference path): h2 from
+// the real-space density grid (rows of `row` floats) without writing the
+// feature grid or h1 to memory.  The march of k_conv16 with h1 made on the
+// chip: a unit keeps feature planes (bf16 asinh(delta) * scale on the block
+// with two halo cells per side, kFeatY x kFeatZ, 4 ring slots) and h1 planes
+// (the kBrickY x kBrickZ brick, 3 ring slots) in LDS.  At step x the density
+// of feature plane x + 3 is loaded into registers while layer 2 runs on h1
+// planes x - 1 .. x + 1; after a barrier h1 plane x + 2 takes the slot of
+// plane x - 1 (layer 1 as in k_conv1: one K chunk of the 27 feature taps per
+// 16-cell tile of the brick, a folded into the bias).  The arithmetic is
+// that of k_features, k_conv1 and k_conv16<kFwdRelu>, so h2 equals theirs
+// bit for bit.  64,512 bytes of LDS: two workgroups per CU.
+constexpr int kFeatY = kBlockY + 4, kFeatZ = kBlockZ + 4;
+constexpr int kFeatCells = kFeatY * kFeatZ;
+constexpr int kFeatSlots = 4;
+constexpr int kH1Slots = 3;
+constexpr int kH1Tiles = (kPlaneCells + 15) / 16;
+constexpr int kFeatPer = (kFeatCells + kThreads - 1) / kThreads;
+constexpr size_t kConv12Bytes =
+    sizeof(uint4) * 2 * kH1Slots * kPlaneCells + sizeof(unsigned short) * kFeatSlots * kFeatCells;
+
+__global__ void __launch_bounds__(kThreads)
+k_conv12(const float* __restrict__ delta, long long row, float in_scale, const unsigned short* __restrict__ wpk1,
+         const float* __restrict__ w1, const float* __restrict__ b1, const float* __restrict__ pa, float a_host,
+         const unsigned short* __restrict__ wpk2, const float* __restrict__ b2, int n, int march,
+         unsigned short* __restrict__ out_bf)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const bool lo = g < 2;
+  const int wave = threadIdx.x >> 6;
+  uint4 b[14];
+#pragma unroll
+  for (int c = 0; c < 14; ++c) b[c] = *reinterpret_cast<const uint4*>(wpk2 + ((long long)c * 64 + lane) * 8);
+  const uint4 bw1 = *reinterpret_cast<const uint4*>(wpk1 + ((long long)kConv1Chunk * 64 + lane) * 8);
+  float bch1[4], bch2[4];
+  conv1_bias(w1, b1, pa ? *pa : a_host, g, bch1);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bch2[i] = b2[4 * g + i];
+  extern __shared__ uint4 smem[];
+  uint4* h1 = smem;  // [slot][yy][zz][2 halves of 8 channels]
+  unsigned short* feat = reinterpret_cast<unsigned short*>(smem + 2 * kH1Slots * kPlaneCells);  // [slot][yy][zz]
+  const int half = g & 1;
+  const int wz = wave & 1, wy = wave >> 1;
+  const int zc = 1 + 16 * wz + r;
+  const int zs = lo ? zc - 1 : zc + 1;
+  // layer-1 taps t = 8g + j of this lane: plane offset dx and feature-brick offset of (dy, dz)
+  int tdx[8], toff[8];
+  bool tval[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int t = 8 * g + j;
+    tval[j] = t < 27;
+    tdx[j] = tval[j] ? t / 9 - 1 : 0;
+    toff[j] = tval[j] ? ((t / 3) % 3 - 1) * kFeatZ + (t % 3 - 1) : 0;
+  }
+  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const ConvUnit cu = conv_unit(u, n, march);
+    // density of feature plane q (periodic) into registers; its features into ring slot q & 3
+    auto fetch = [&](int q, float (&v)[kFeatPer]) {
+      const long long xoff = (long long)wrapi(q, n) * n;
+#pragma unroll
+      for (int k = 0; k < kFeatPer; ++k) {
+        const int c = threadIdx.x + k * kThreads;
+        if (c < kFeatCells) {
+          const int yy = c / kFeatZ, zz = c - yy * kFeatZ;
+          v[k] = delta[(xoff + wrapi(cu.y0 + yy - 2, n)) * row + wrapi(cu.z0 + zz - 2, n)];
+        }
+      }
+    };
+    auto put = [&](int q, const float (&v)[kFeatPer]) {
+      unsigned short* slot = feat + (q & (kFeatSlots - 1)) * kFeatCells;
+#pragma unroll
+      for (int k = 0; k < kFeatPer; ++k) {
+        const int c = threadIdx.x + k * kThreads;
+        if (c < kFeatCells) slot[c] = feature_of(v[k], in_scale);
+      }
+    };
+    // h1 plane p from feature planes p - 1 .. p + 1 into h1 ring slot p mod 3
+    auto layer1 = [&](int p) {
+      int fb[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) fb[j] = ((p + tdx[j]) & (kFeatSlots - 1)) * kFeatCells + toff[j];
+      uint2* dst = reinterpret_cast<uint2*>(h1 + 2 * ((p + kH1Slots) % kH1Slots) * kPlaneCells);
+      for (int tt = wave; tt < kH1Tiles; tt += kThreads / 64) {
+        const int cell = tt * 16 + r;
+        const int c = min(cell, kPlaneCells - 1);
+        const int yy = c / kBrickZ, zz = c - yy * kBrickZ;
+        const int centre = (yy + 1) * kFeatZ + zz + 1;
+        unsigned short v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = tval[j] ? feat[fb[j] + centre] : (unsigned short)0;
+        const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
+                                    v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
+        const floatx4 acc = mfma(op, bw1, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+        if (cell < kPlaneCells)
+          dst[c * 4 + g] = pack_bf16x4(fmaxf(acc[0] + bch1[0], 0.0f), fmaxf(acc[1] + bch1[1], 0.0f),
+                                       fmaxf(acc[2] + bch1[2], 0.0f), fmaxf(acc[3] + bch1[3], 0.0f));
+      }
+    };
+    __syncthreads();  // the previous unit's reads are done
+    {
+      float v[kFeatPer];
+      for (int q = cu.x0 - 2; q <= cu.x0 + 1; ++q) {
+        fetch(q, v);
+        put(q, v);
+      }
+    }
+    __syncthreads();
+    layer1(cu.x0 - 1);
+    layer1(cu.x0);
+    __syncthreads();  // feature plane x0 - 2 is free
+    {
+      float v[kFeatPer];
+      fetch(cu.x0 + 2, v);
+      put(cu.x0 + 2, v);
+    }
+    __syncthreads();
+    layer1(cu.x0 + 1);
+    __syncthreads();
+    for (int x = cu.x0; x < cu.x1; ++x) {
+      const bool next = x + 1 < cu.x1;  // h1 plane x + 2 feeds output plane x + 1
+      float v[kFeatPer];
+      if (next) fetch(x + 3, v);
+      floatx4 acc[kConvTy];
+      int sb[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) sb[d] = ((x - 1 + d + kH1Slots) % kH1Slots) * kPlaneCells;
+      conv16_plane(h1, sb, wy, zc, zs, half, lo, b, acc);
+      // feature plane x + 3 takes the slot of plane x - 1, last read by layer 1
+      // of h1 plane x (before the previous barrier)
+      if (next) put(x + 3, v);
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) {
+        const long long cell = ((long long)x * n + cu.y0 + kConvTy * wy + t) * n + cu.z0 + 16 * wz + r;
+        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
+            pack_bf16x4(fmaxf(acc[t][0] + bch2[0], 0.0f), fmaxf(acc[t][1] + bch2[1], 0.0f),
+                        fmaxf(acc[t][2] + bch2[2], 0.0f), fmaxf(acc[t][3] + bch2[3], 0.0f));
+      }
+      __syncthreads();  // layer-2 reads of h1 plane x - 1 done, feature plane x + 3 stored
+      if (next) {
+        layer1(x + 2);
+        __syncthreads();
       }
     }
   }
@@ -968,6 +1138,28 @@ int resnet_layers23(ResNet* r, hipStream_t s)
 {
   COLA_TRY(conv_lds_attributes());
   return conv_layers_2_3(r, s);
+}
+
+int resnet_layers12(ResNet* r, const float* delta, long long row, const float* pa, hipStream_t s)
+{
+  const ConvLaunch cl = conv_launch(r->n);
+  hipLaunchKernelGGL(k_conv12, dim3(cl.blocks), dim3(kThreads), kConv12Bytes, s, delta, row, r->in_scale, r->d_wf[0],
+                     r->d_param + param_offset(0), r->d_param + param_offset(0) + 27 * 8 * 16, pa, 0.0f, r->d_wf[1],
+                     r->d_param + param_offset(1) + 27 * 16 * 16, r->n, cl.march, r->d_h[1]);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int resnet_layer3(ResNet* r, hipStream_t s)
+{
+  COLA_TRY(conv_lds_attributes());
+  const ConvLaunch cl = conv_launch(r->n);
+  ConvOut o;
+  o.bf = r->d_h[2];
+  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[1], r->d_wf[2],
+                     r->d_param + param_offset(2) + 27 * 16 * 16, 16, r->n, cl.march, o);
+  COLA_HIP(hipGetLastError());
+  return kOk;
 }
 
 int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s)

@@ -326,6 +326,11 @@ int cola_resnet_eval(cola_resnet* net, int first, int count, double* loss);
    attaching a net that another context holds returns COLA_ERR_STATE;
    destroying an attached net detaches it first. */
 int cola_set_residual_net(cola_context* ctx, cola_resnet* net);
+/* Launch shape of the inference path in a context's force (no reference
+   counterpart): on = 1 (default) makes the features and the first two layers
+   in one pass over the density grid, on = 0 runs them as three launches
+   through the feature and h1 grids.  Both give the same forces bit for bit. */
+int cola_resnet_set_fusion(cola_resnet* net, int on);
 /* Net-weights file: the trained parameters and scales, independent of the
    grid size (the net is a convolution over cells).  Little-endian layout:
    char magic[8] "COLANET1", int32 nparam, int32 0, float in_scale,

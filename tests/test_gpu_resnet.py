@@ -272,3 +272,36 @@ def test_destroying_an_attached_net_detaches_it(eh_table):
             for c in (a, b):
                 c.step(omf, (istep + 0.5) * 0.1, (istep + 1.0) * 0.1)
         assert np.array_equal(a.get_particles(), b.get_particles())
+
+
+@pytest.mark.parametrize("nc,ng,box", [(32, 64, 100.0), (256, 768, 500.0)])
+def test_fused_first_layers_equal_separate_launches(eh_table, nc, ng, box):
+    """The inference path's fused pass (features, layer 1 and layer 2 from
+    the density grid in one launch) gives the corrected forces of the three
+    separate launches bit for bit: at 64^3 every workgroup marches one x
+    plane, at 768^3 hundreds of planes (ring slots reused many times)."""
+    print(f"fused layers 1-2, {nc}^3 / {ng}^3", flush=True)
+    k, p = eh_table
+    with ch.ResidualNet(ng, seed=9) as net, ch.Context(nc, ng, box) as c:
+        net.set_scales(1.3, 0.7)
+        c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+        c.lpt_set_displacement(0.1, OM, 4)
+        c.set_times(c.times()[0], 0.1)
+        omf = float(np.float32(OM))
+        for istep in range(1, 4):
+            c.step(omf, (istep + 0.5) * 0.1, (istep + 1.0) * 0.1)
+        c.move_particles()
+        c.pm_calculate_forces()
+        plain = c.get_forces()
+        c.set_residual_net(net)
+        c.pm_calculate_forces()
+        fused = c.get_forces()
+        net.set_fusion(False)
+        c.pm_calculate_forces()
+        separate = c.get_forces()
+        net.set_fusion(True)
+        c.pm_calculate_forces()
+        again = c.get_forces()
+    assert not np.array_equal(plain, fused)
+    assert np.array_equal(fused, separate)
+    assert np.array_equal(fused, again)
