@@ -83,10 +83,10 @@ def kernel_cost(phase, nc, ng):
         # net: bfloat16 activations of 16 channels = 32 bytes per cell
         "net_features": (4 * n_g + 2 * n_g, 0, "k_features"),
         "net_conv1": (2 * n_g + 32 * n_g, 27 * 16 * 2 * n_g, "k_conv1"),
-        "net_conv_hidden": (64 * n_g, conv16, "k_conv16<kFwdRelu>"),
+        "net_conv_hidden": (64 * n_g, conv16, "k_conv16r<kFwdRelu>"),
         # features + layers 1-2 fused: fp32 density in, bf16 h2 out
-        "net_conv12": (4 * n_g + 32 * n_g, 27 * 16 * 2 * n_g + conv16, "k_conv12"),
-        "net_conv_out_add": (32 * n_g + 24 * n_g, 27 * 16 * 3 * 2 * n_g, "k_conv16<kFwdAdd>"),
+        "net_conv12": (4 * n_g + 32 * n_g, 27 * 16 * 2 * n_g + conv16, "k_conv12r"),
+        "net_conv_out_add": (32 * n_g + 24 * n_g, 27 * 16 * 3 * 2 * n_g, "k_conv_head_r<kFwdAdd>"),
     }
     return table.get(phase)
 

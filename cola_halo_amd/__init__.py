@@ -101,8 +101,10 @@ _SIGNATURES = {
     "cola_resnet_time_forward": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "cola_resnet_set_restriction": (C.c_int, [C.c_void_p, C.c_float]),
     "cola_resnet_set_fusion": (C.c_int, [C.c_void_p, C.c_int]),
+    "cola_resnet_set_plane_order": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                             C.c_void_p]),
+    "cola_resnet_adam_step": (C.c_int, [C.c_void_p, C.c_float]),
     "cola_copy_particles": (C.c_int, [C.c_void_p, C.c_void_p]),
     "cola_resnet_add_pair": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_resnet_num_samples": (C.c_int, [C.c_void_p]),
@@ -379,6 +381,10 @@ class ResidualNet:
         _check(lib().cola_resnet_gradient_host(self._h, x.ctypes.data, t.ctypes.data, C.byref(loss), g.ctypes.data))
         return loss.value, g
 
+    def adam_step(self, lr):
+        """One Adam step on the gradient of the last gradient() call."""
+        _check(lib().cola_resnet_adam_step(self._h, lr))
+
     def set_restriction(self, side_weight):
         """Training targets: the fine force averaged over the 3^3 fine
         neighbours of each coarse cell with weights (w, 1 - 2w, w) per axis
@@ -390,6 +396,12 @@ class ResidualNet:
         1-2 in one pass (True, the default) or three launches (False); the
         forces are the same bit for bit."""
         _check(lib().cola_resnet_set_fusion(self._h, int(bool(on))))
+
+    def set_plane_order(self, by_input_planes):
+        """Loop order of the hidden 16 -> 16 layers: by input planes (True,
+        the default: k_conv16r) or by output planes (False: k_conv16);
+        results agree to float rounding."""
+        _check(lib().cola_resnet_set_plane_order(self._h, int(bool(by_input_planes))))
 
     def add_pair(self, coarse, fine):
         _check(lib().cola_resnet_add_pair(self._h, coarse._h, fine._h))

@@ -1795,6 +1795,15 @@ int cola_resnet_gradient_host(cola_resnet* h, const float* x, const float* targe
   return st;
 }
 
+int cola_resnet_adam_step(cola_resnet* h, float lr)
+{
+  if (!h || !(lr > 0.0f)) return kErrArg;
+  COLA_TRY(net_quiesce(h));
+  COLA_TRY(resnet_adam(&h->r, lr, h->stream));
+  COLA_HIP(hipStreamSynchronize(h->stream));
+  return kOk;
+}
+
 int cola_resnet_add_pair(cola_resnet* h, cola_context* coarse, cola_context* fine)
 {
   if (!h || !valid(coarse) || !valid(fine)) return kErrArg;
@@ -1942,6 +1951,17 @@ int cola_resnet_set_fusion(cola_resnet* h, int on)
     drop_step_graph(h->attached);
   }
   h->r.fuse12 = on != 0;
+  return kOk;
+}
+
+int cola_resnet_set_plane_order(cola_resnet* h, int by_input_planes)
+{
+  if (!h || (by_input_planes != 0 && by_input_planes != 1)) return kErrArg;
+  if (h->attached) {
+    COLA_HIP(hipStreamSynchronize(h->attached->stream));
+    drop_step_graph(h->attached);
+  }
+  h->r.roll = by_input_planes != 0;
   return kOk;
 }
 

@@ -28,6 +28,8 @@ struct ResNet {
   // MFMA B operands: forward (per layer) and data-gradient (layers 2-4) packs
   unsigned short* d_wf[kNetLayers] = {nullptr, nullptr, nullptr, nullptr};
   unsigned short* d_wb[kNetLayers] = {nullptr, nullptr, nullptr, nullptr};
+  // forward A operands of the 16-channel-input layers in k_conv16r's K order
+  unsigned short* d_wr[kNetLayers] = {nullptr, nullptr, nullptr, nullptr};
   // activations: input [n^3][8], h1..h3 [n^3][16] (bf16), output y [3][n^3] fp32
   unsigned short* d_x = nullptr;
   unsigned short* d_f1 = nullptr;  // feature grid [n^3] (bf16) of host inputs (timing, tests)
@@ -44,6 +46,7 @@ struct ResNet {
   float in_scale = 1.0f;     // feature = asinh(delta) * in_scale
   float restrict_side = 0.125f;  // training targets: fine force averaged with weights (w, 1-2w, w) per axis
   bool fuse12 = true;            // inference: features and layers 1-2 in one launch (resnet_layers12)
+  bool roll = true;              // 16-channel-input layers by input planes (k_conv12r, k_conv16r, k_conv_head_r)
 
   // training set: per sample the feature grid (bf16 [n^3]), the target
   // residual (bf16 [3][n^3]) and the scale factor a

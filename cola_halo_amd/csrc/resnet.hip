@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -321,6 +322,424 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
       }
     }
   }
+}
+
+// ---- 16-channel convolution by input planes (inference) -----------------
+// k_conv16 reads the three input planes x - 1, x, x + 1 for every output
+// plane x, six LDS operands per input row for 14 products.  k_conv16r
+// consumes each input plane once: a wave keeps the accumulators of three
+// output planes (p + 1, p, p - 1) and input plane p adds its taps of all
+// three x offsets to them, two LDS operands per input row (the row and its
+// z-shifted copy) for 15 products.  Output plane p - 1 is then complete and
+// stored; its accumulators are reused for plane p + 2.  K order, five chunks
+// per input-plane offset dx (15 in all, one more than k_conv16's 14):
+//   chunk 5 (dx+1) + (dy+1), dy in {-1,0,1}: taps (dx,dy,-1) | (dx,dy,+1)
+//   chunk 5 (dx+1) + 3:                     taps (dx,-1,0) | (dx,0,0)
+//   chunk 5 (dx+1) + 4:                     taps (dx,+1,0) | none
+// With every input plane read once from LDS, the ring holds the plane being
+// read and three planes in flight (loads issued three planes ahead), and a
+// wave waits only for the next plane's loads (vmcnt), not for its stores.
+constexpr int kRollChunks = 15;
+
+__host__ __device__ inline int roll_tap(int c, int h)
+{
+  const int dx = c / 5, k = c % 5;
+  if (k < 3) return 9 * dx + 3 * k + (h ? 2 : 0);  // (dx, dy, -1) | (dx, dy, +1)
+  if (k == 3) return 9 * dx + (h ? 4 : 1);         // (dx, -1, 0) | (dx, 0, 0)
+  return h ? -1 : 9 * dx + 7;                      // (dx, +1, 0) | none
+}
+
+// The rolling kernels' ring: 4 slots of kRollSlotChunks 16-byte chunks
+// (the brick's 2 kPlaneCells chunks padded to whole rounds of kThreads), so
+// that every lane of every wave issues each of its loads: the compiler then
+// knows how many are in flight, and the counts below hold.  The padding
+// chunks receive copies of the last cell.  2 x 81,920 bytes fill a CU's
+// 160 KiB: two workgroups per CU.
+constexpr int kRollLoads = (2 * kPlaneCells + kThreads - 1) / kThreads;  // loads per wave and plane
+constexpr int kRollSlotChunks = kRollLoads * kThreads;
+constexpr size_t kRollRingBytes = sizeof(uint4) * kRingSlots * kRollSlotChunks;
+static_assert(kRollRingBytes * 2 <= 160 * 1024, "two workgroups per CU");
+
+__device__ inline const uint4* roll_slot(const uint4* ring, int p) { return ring + (p & (kRingSlots - 1)) * kRollSlotChunks; }
+
+// Loads of input plane `plane` (periodic) into the ring slot of plane
+// `slot_plane` (16 channels: two chunks per cell), with the slot and the
+// data plane apart so that the tail of a march can refill a free slot
+// (every step then issues the same kRollLoads loads per wave).
+__device__ inline void load_plane_to(uint4* ring, const unsigned short* __restrict__ in, int n, int slot_plane,
+                                     int plane, int y0, int z0)
+{
+  const int lane = threadIdx.x & 63;
+  uint4* slot = ring + (slot_plane & (kRingSlots - 1)) * kRollSlotChunks;
+  const long long xoff = (long long)wrapi(plane, n) * n;
+#pragma unroll
+  for (int k = 0; k < kRollLoads; ++k) {
+    const int base = k * kThreads + (threadIdx.x & ~63);
+    const int c = min(base + lane, 2 * kPlaneCells - 1);
+    const int q = c & 1, cell = c >> 1;
+    const int zz = cell % kBrickZ, yy = cell / kBrickZ;
+    const long long src = (xoff + wrapi(y0 + yy - 1, n)) * n + wrapi(z0 + zz - 1, n);
+    __builtin_amdgcn_global_load_lds(static_cast<const void*>(in + src * 16 + q * 8),
+                                     (__attribute__((address_space(3))) void*)(slot + base), 16, 0, 0);
+  }
+}
+
+// s_waitcnt immediate: wait until at most n vector-memory operations of the
+// wave are outstanding (expcnt and lgkmcnt not waited on).
+constexpr int vmcnt_imm(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+// the counter holds at most 63: a larger allowance waits for more than needed
+constexpr int vmcnt_cap(int n) { return n < 63 ? n : 63; }
+
+// Products of one input plane for the wave's kConvTy output rows: taps
+// dx = -1 into am (output plane p + 1), dx = 0 into a0 (plane p), dx = +1
+// into ap (plane p - 1).  `row` points at the lane's centre cell of brick
+// row 0 (its 16-byte half), `shr` at its z neighbour; brick rows are
+// 2 kBrickZ uint4 apart.
+__device__ inline void roll_plane(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
+                                  const uint4 (&b)[kRollChunks], floatx4 (&am)[kConvTy], floatx4 (&a0)[kConvTy],
+                                  floatx4 (&ap)[kConvTy])
+{
+  constexpr int kRow = 2 * kBrickZ;
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  uint4 prev = zero4;
+  uint4 cen = row[0], sh = shr[0];
+#pragma unroll
+  for (int j = 0; j < kConvTy + 2; ++j) {  // brick row j feeds output rows j - 2, j - 1, j
+    uint4 ncen = zero4, nsh = zero4;
+    if (j + 1 < kConvTy + 2) {
+      ncen = row[(j + 1) * kRow];
+      nsh = shr[(j + 1) * kRow];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the next row's reads ahead of this row's products
+    if (j < kConvTy) {  // dy = -1: output row j
+      am[j] = mfma(sh, b[0], am[j]);
+      a0[j] = mfma(sh, b[5], a0[j]);
+      ap[j] = mfma(sh, b[10], ap[j]);
+    }
+    if (j >= 1 && j - 1 < kConvTy) {  // dy = 0 and (dy,dz) = (-1,0) | (0,0): output row j - 1
+      am[j - 1] = mfma(sh, b[1], am[j - 1]);
+      a0[j - 1] = mfma(sh, b[6], a0[j - 1]);
+      ap[j - 1] = mfma(sh, b[11], ap[j - 1]);
+      const uint4 pc = sel4(lo, prev, cen);
+      am[j - 1] = mfma(pc, b[3], am[j - 1]);
+      a0[j - 1] = mfma(pc, b[8], a0[j - 1]);
+      ap[j - 1] = mfma(pc, b[13], ap[j - 1]);
+    }
+    if (j >= 2) {  // dy = +1 and (dy,dz) = (+1,0) | none: output row j - 2
+      am[j - 2] = mfma(sh, b[2], am[j - 2]);
+      a0[j - 2] = mfma(sh, b[7], a0[j - 2]);
+      ap[j - 2] = mfma(sh, b[12], ap[j - 2]);
+      const uint4 qc = sel4(lo, cen, zero4);
+      am[j - 2] = mfma(qc, b[4], am[j - 2]);
+      a0[j - 2] = mfma(qc, b[9], a0[j - 2]);
+      ap[j - 2] = mfma(qc, b[14], ap[j - 2]);
+    }
+    prev = cen;
+    cen = ncen;
+    sh = nsh;
+  }
+}
+
+// Output plane q of a rolling convolution from its accumulators, by mode
+// (kFwdRelu: bf16 activations; kFwdLinear: fp32 y; kFwdAdd: added to the
+// force grids, old values loaded here), then the accumulators cleared.
+template <int MODE>
+__device__ inline void roll_store(floatx4 (&acc)[kConvTy], bool store, int q, int n, int ybase, int zcell, int g,
+                                  const float (&bch)[4], const ConvOut& o)
+{
+  if (store) {
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) {
+      const long long line = (long long)q * n + ybase + t;
+      const long long cell = line * n + zcell;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[t][i] + bch[i];
+      if (MODE == kFwdRelu) {
+        *reinterpret_cast<uint2*>(o.bf + cell * 16 + 4 * g) =
+            pack_bf16x4(fmaxf(v[0], 0.0f), fmaxf(v[1], 0.0f), fmaxf(v[2], 0.0f), fmaxf(v[3], 0.0f));
+      } else if (MODE == kFwdLinear) {
+        if (g == 0) {
+          const long long nc = (long long)n * n * n;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) o.f[i * nc + cell] = v[i];
+        }
+      } else {
+        if (g == 0) {
+          const long long at = line * o.row + zcell;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) o.add[i][at] = o.add[i][at] + o.scale * v[i];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+// Vector-memory instructions one step of a rolling march issues per wave
+// besides its plane loads (the stores of one output plane; kFwdAdd also
+// loads the old force values).
+template <int MODE>
+constexpr int roll_store_ops()
+{
+  return MODE == kFwdRelu ? kConvTy : (MODE == kFwdLinear ? 3 * kConvTy : 6 * kConvTy);
+}
+
+// Step i of a march (input plane p = x0 - 1 + i) with accumulator sets
+// rotated by R = i mod 3: set (R + 1) % 3 holds output plane p + 1, set R
+// plane p, set (R + 2) % 3 plane p - 1.
+template <int MODE, int R>
+__device__ inline void roll_step(uint4* ring, const unsigned short* __restrict__ in, int n, const ConvUnit& cu, int i,
+                                 int rowoff, int shoff, bool lo, const uint4 (&b)[kRollChunks],
+                                 floatx4 (&acc)[3][kConvTy], int ybase, int zcell, int g, const float (&bch)[4],
+                                 const ConvOut& o)
+{
+  const int p = cu.x0 - 1 + i;
+  // plane p + 3 into the slot plane p - 1 left (the march's last plane again
+  // once past its end, so every step issues the same loads)
+  load_plane_to(ring, in, n, p + 3, min(p + 3, cu.x1), cu.y0, cu.z0);
+  const uint4* slot = roll_slot(ring, p);
+  roll_plane(slot + rowoff, slot + shoff, lo, b, acc[(R + 1) % 3], acc[R], acc[(R + 2) % 3]);
+  roll_store<MODE>(acc[(R + 2) % 3], p - 1 >= cu.x0 && p - 1 < cu.x1, p - 1, n, ybase, zcell, g, bch, o);
+  // plane p + 1 landed: the operations issued after its loads may still be
+  // in flight (the later planes' loads and up to three steps' stores; the
+  // count saturates at the counter's 63, waiting longer)
+  constexpr int L = kRollLoads, S = roll_store_ops<MODE>();
+  constexpr int w0 = vmcnt_imm(2 * L), w1 = vmcnt_imm(vmcnt_cap(2 * L + S));
+  constexpr int w2 = vmcnt_imm(vmcnt_cap(2 * L + 2 * S)), w3 = vmcnt_imm(vmcnt_cap(2 * L + 3 * S));
+  const int nst = min(max(i - 1, 0), 3);
+  if (nst == 3) {
+    __builtin_amdgcn_s_waitcnt(w3);
+  } else if (nst == 2) {
+    __builtin_amdgcn_s_waitcnt(w2);
+  } else if (nst == 1) {
+    __builtin_amdgcn_s_waitcnt(w1);
+  } else {
+    __builtin_amdgcn_s_waitcnt(w0);
+  }
+  // every wave's plane p + 1 is in LDS and every wave is past its reads of
+  // plane p, whose slot the next step's loads take (no fence: the waits
+  // above are the ones needed)
+  asm volatile("s_barrier" ::: "memory");
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_conv16r(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wpk, const float* __restrict__ bias,
+          int cout, int n, int march, ConvOut o)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const bool lo = g < 2;
+  uint4 b[kRollChunks];
+#pragma unroll
+  for (int c = 0; c < kRollChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wpk + ((long long)c * 64 + lane) * 8);
+  float bch[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bch[i] = 4 * g + i >= cout ? 0.0f : bias[4 * g + i];
+  extern __shared__ uint4 ring[];  // [slot][yy][zz][2 halves of 8 channels]
+  const int half = g & 1;
+  const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
+  const int zc = 1 + 16 * wz + r;
+  const int zs = lo ? zc - 1 : zc + 1;
+  const int rowoff = (kConvTy * wy * kBrickZ + zc) * 2 + half;
+  const int shoff = (kConvTy * wy * kBrickZ + zs) * 2 + half;
+  floatx4 acc[3][kConvTy];
+  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const ConvUnit cu = conv_unit(u, n, march);
+    const int ybase = cu.y0 + kConvTy * wy, zcell = cu.z0 + 16 * wz + r;
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) acc[s][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    // the previous unit's refills have landed and its operand reads are done
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // planes x0 - 1, x0, x0 + 1 (x0 + 1 <= x1: a march has at least one plane)
+    for (int k = 0; k < 3; ++k) load_plane_to(ring, in, n, cu.x0 - 1 + k, cu.x0 - 1 + k, cu.y0, cu.z0);
+    constexpr int w_pro = vmcnt_imm(2 * kRollLoads);
+    __builtin_amdgcn_s_waitcnt(w_pro);
+    asm volatile("s_barrier" ::: "memory");
+    // input planes x0 - 1 .. x1, rounded up to whole rotations (the extra
+    // steps reload the last plane and store nothing)
+    const int np = cu.x1 - cu.x0 + 2;
+    for (int i = 0; i < np; i += 3) {
+      roll_step<MODE, 0>(ring, in, n, cu, i, rowoff, shoff, lo, b, acc, ybase, zcell, g, bch, o);
+      roll_step<MODE, 1>(ring, in, n, cu, i + 1, rowoff, shoff, lo, b, acc, ybase, zcell, g, bch, o);
+      roll_step<MODE, 2>(ring, in, n, cu, i + 2, rowoff, shoff, lo, b, acc, ybase, zcell, g, bch, o);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // the tail's refills land before the workgroup's LDS is released
+}
+
+// ---- last layer (16 -> 3) by input planes --------------------------------
+// With three output channels the three x offsets become output rows: row
+// 4 dxi + co of the A operand holds W[tap (dx = dxi - 1, dy, dz)][ci][co]
+// (rows with co = 3 and rows 12-15 zero), so one product per (dy, dz) chunk
+// of an input plane (the five chunks of roll_tap for one dx) makes all of
+// that plane's contributions: lane group g of the result holds, for its
+// cell, the contribution of input plane p to output plane p + 1 - g.  Output
+// plane q = G0(D_{q-1}) + G1(D_q) + G2(D_{q+1}): a wave keeps the last
+// plane's partial sums and adds them one lane group up (ds_bpermute) to the
+// next plane's products, after which lane group 2 holds output plane p - 1
+// complete; its three channels are then spread over lane groups 0-2, one
+// 4-byte access per lane and row (kFwdAdd loads the old force values one
+// step ahead).  Five products per 16-cell tile and input plane instead of
+// fifteen.
+constexpr int kHeadChunks = 5;
+
+// Products of one input plane into d (one tile per output row, fresh).
+__device__ inline void head_plane(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
+                                  const uint4 (&b)[kHeadChunks], floatx4 (&d)[kConvTy])
+{
+  constexpr int kRow = 2 * kBrickZ;
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  const floatx4 fz = {0.0f, 0.0f, 0.0f, 0.0f};
+  uint4 prev = zero4;
+  uint4 cen = row[0], sh = shr[0];
+#pragma unroll
+  for (int j = 0; j < kConvTy + 2; ++j) {
+    uint4 ncen = zero4, nsh = zero4;
+    if (j + 1 < kConvTy + 2) {
+      ncen = row[(j + 1) * kRow];
+      nsh = shr[(j + 1) * kRow];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (j < kConvTy) d[j] = mfma(sh, b[0], fz);  // dy = -1: the first product of output row j
+    if (j >= 1 && j - 1 < kConvTy) {
+      d[j - 1] = mfma(sh, b[1], d[j - 1]);
+      d[j - 1] = mfma(sel4(lo, prev, cen), b[3], d[j - 1]);
+    }
+    if (j >= 2) {
+      d[j - 2] = mfma(sh, b[2], d[j - 2]);
+      d[j - 2] = mfma(sel4(lo, cen, zero4), b[4], d[j - 2]);
+    }
+    prev = cen;
+    cen = ncen;
+    sh = nsh;
+  }
+}
+
+__device__ inline float lane_read(float v, int src_lane)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_lane * 4, __builtin_bit_cast(int, v)));
+}
+
+// s_waitcnt for "at most n vector-memory operations outstanding", n rounded
+// down to the ladder 2L + 8k (L = kRollLoads) that the head's steps produce.
+__device__ inline void head_wait(int n)
+{
+  constexpr int L2 = 2 * kRollLoads;
+  if (n >= L2 + 48) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(vmcnt_cap(L2 + 48)));
+  } else if (n >= L2 + 40) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(L2 + 40));
+  } else if (n >= L2 + 32) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(L2 + 32));
+  } else if (n >= L2 + 24) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(L2 + 24));
+  } else if (n >= L2 + 16) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(L2 + 16));
+  } else if (n >= L2 + 8) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(L2 + 8));
+  } else if (n >= L2) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(L2));
+  } else {
+    __builtin_amdgcn_s_waitcnt(vmcnt_imm(0));
+  }
+}
+
+template <int MODE>  // kFwdAdd or kFwdLinear
+__global__ void __launch_bounds__(kThreads)
+k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wh,
+              const float* __restrict__ bias, int n, int march, ConvOut o)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const bool lo = g < 2;
+  uint4 b[kHeadChunks];
+#pragma unroll
+  for (int c = 0; c < kHeadChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wh + ((long long)c * 64 + lane) * 8);
+  const float bo = g < 3 ? bias[g] : 0.0f;
+  extern __shared__ uint4 ring[];
+  const int half = g & 1;
+  const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
+  const int zc = 1 + 16 * wz + r;
+  const int zs = lo ? zc - 1 : zc + 1;
+  const int rowoff = (kConvTy * wy * kBrickZ + zc) * 2 + half;
+  const int shoff = (kConvTy * wy * kBrickZ + zs) * 2 + half;
+  const int up = lane >= 16 ? lane - 16 : lane;  // one lane group down: source of the partial sums
+  const int g2 = 32 + r;                          // lane of group 2 holding this cell's finished output
+  const long long nc = (long long)n * n * n;
+  // this lane's output channel: its force grid (kFwdAdd) or y row (kFwdLinear),
+  // selected once (indexing the argument struct by g would load it per use)
+  float* const gout = MODE == kFwdAdd ? (g == 0 ? o.add[0] : (g == 1 ? o.add[1] : o.add[2]))
+                                      : o.f + (g < 3 ? g : 0) * nc;
+  constexpr int L = kRollLoads;
+  constexpr int F = MODE == kFwdAdd ? kConvTy : 0, S = kConvTy;
+  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const ConvUnit cu = conv_unit(u, n, march);
+    const int ybase = cu.y0 + kConvTy * wy, zcell = cu.z0 + 16 * wz + r;
+    floatx4 part[kConvTy];
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) part[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) load_plane_to(ring, in, n, cu.x0 - 1 + k, cu.x0 - 1 + k, cu.y0, cu.z0);
+    constexpr int w_pro = vmcnt_imm(2 * kRollLoads);
+    __builtin_amdgcn_s_waitcnt(w_pro);
+    asm volatile("s_barrier" ::: "memory");
+    const int np = cu.x1 - cu.x0 + 2;  // input planes x0 - 1 .. x1
+    for (int i = 0; i < np; ++i) {
+      const int p = cu.x0 - 1 + i;
+      const bool store = p - 1 >= cu.x0 && p - 1 < cu.x1;  // output plane p - 1 completes at this step
+      // kFwdAdd: the old force values of plane p - 1, issued before the plane
+      // loads so that their wait leaves those in flight
+      float fold[kConvTy];
+      if (MODE == kFwdAdd && store) {  // every lane (group 3 reads group 2's grid): no divergent branch
+#pragma unroll
+        for (int t = 0; t < kConvTy; ++t) fold[t] = gout[((long long)(p - 1) * n + ybase + t) * o.row + zcell];
+      }
+      load_plane_to(ring, in, n, p + 3, min(p + 3, cu.x1), cu.y0, cu.z0);
+      const uint4* slot = roll_slot(ring, p);
+      floatx4 d[kConvTy];
+      head_plane(slot + rowoff, slot + shoff, lo, b, d);
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float s = lane_read(part[t][k], up);
+          if (g == 1 || g == 2) d[t][k] += s;
+        }
+        part[t] = d[t];
+      }
+      if (store) {
+#pragma unroll
+        for (int t = 0; t < kConvTy; ++t) {
+          const float v0 = lane_read(d[t][0], g2), v1 = lane_read(d[t][1], g2), v2 = lane_read(d[t][2], g2);
+          const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
+          if (g < 3) {
+            const long long line = (long long)(p - 1) * n + ybase + t;
+            if (MODE == kFwdAdd)
+              gout[line * o.row + zcell] = fold[t] + o.scale * v;
+            else
+              gout[line * n + zcell] = v;
+          }
+        }
+      }
+      // plane p + 1 landed: step j issues the old-value loads F(j), the plane
+      // loads and the stores S(j); count what was issued after the loads of
+      // plane p + 1 (the prologue's for i < 2, step i - 2's after)
+      auto ocount = [&](int j, int per) { return j >= 2 && j <= cu.x1 - cu.x0 + 1 ? per : 0; };
+      int after = 2 * L;
+      if (i >= 2)
+        after += ocount(i - 2, S) + ocount(i - 1, F) + ocount(i - 1, S) + ocount(i, F) + ocount(i, S);
+      head_wait(after);
+      asm volatile("s_barrier" ::: "memory");
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
 }
 
 // 8-channel convolution (the input features) with the operand reuse of
@@ -638,6 +1057,140 @@ k_conv12(const float* __restrict__ delta, long long row, float in_scale, const u
   }
 }
 
+// Features, layer 1 and layer 2 in one pass with layer 2 by input planes
+// (k_conv16r's order): h2 from the real-space density grid.  A unit keeps
+// feature planes (kFeatY x kFeatZ, 4 ring slots) and two h1 planes (the
+// kBrickY x kBrickZ brick) in LDS.  Step i of the march (h1 plane p =
+// x0 - 1 + i): the density of feature plane p + 3 is loaded into
+// registers; layer 2 adds h1 plane p's taps to the accumulators of h2
+// planes p + 1, p, p - 1; h2 plane p - 1 is stored; h1 plane p + 1 is made
+// from feature planes p .. p + 2 into the other h1 slot (layer 1 as in
+// k_conv1); feature plane p + 3 takes the slot of plane p - 1.  One barrier
+// per step.  The arithmetic is that of k_features, k_conv1 and
+// k_conv16r<kFwdRelu>: h2 equals theirs bit for bit.
+constexpr size_t kConv12rBytes = sizeof(uint4) * 2 * 2 * kPlaneCells + sizeof(unsigned short) * kFeatSlots * kFeatCells;
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const unsigned short* __restrict__ wpk1,
+          const float* __restrict__ w1, const float* __restrict__ b1, const float* __restrict__ pa, float a_host,
+          const unsigned short* __restrict__ wr2, const float* __restrict__ b2, int n, int march,
+          unsigned short* __restrict__ out_bf)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const bool lo = g < 2;
+  const int wave = threadIdx.x >> 6;
+  uint4 b[kRollChunks];
+#pragma unroll
+  for (int c = 0; c < kRollChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wr2 + ((long long)c * 64 + lane) * 8);
+  const uint4 bw1 = *reinterpret_cast<const uint4*>(wpk1 + ((long long)kConv1Chunk * 64 + lane) * 8);
+  float bch1[4], bch2[4];
+  conv1_bias(w1, b1, pa ? *pa : a_host, g, bch1);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bch2[i] = b2[4 * g + i];
+  extern __shared__ uint4 smem[];
+  uint4* h1 = smem;  // [2 slots][yy][zz][2 halves of 8 channels]
+  unsigned short* feat = reinterpret_cast<unsigned short*>(smem + 2 * 2 * kPlaneCells);  // [slot][yy][zz]
+  const int half = g & 1;
+  const int wz = wave & 1, wy = wave >> 1;
+  const int zc = 1 + 16 * wz + r;
+  const int zs = lo ? zc - 1 : zc + 1;
+  const int rowoff = (kConvTy * wy * kBrickZ + zc) * 2 + half;
+  const int shoff = (kConvTy * wy * kBrickZ + zs) * 2 + half;
+  // layer-1 taps t = 8g + j of this lane: plane offset dx and feature-brick offset of (dy, dz)
+  int tdx[8], toff[8];
+  bool tval[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int t = 8 * g + j;
+    tval[j] = t < 27;
+    tdx[j] = tval[j] ? t / 9 - 1 : 0;
+    toff[j] = tval[j] ? ((t / 3) % 3 - 1) * kFeatZ + (t % 3 - 1) : 0;
+  }
+  floatx4 acc[3][kConvTy];
+  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const ConvUnit cu = conv_unit(u, n, march);
+    const int ybase = cu.y0 + kConvTy * wy, zcell = cu.z0 + 16 * wz + r;
+    auto fetch = [&](int q, float (&v)[kFeatPer]) {
+      const long long xoff = (long long)wrapi(q, n) * n;
+#pragma unroll
+      for (int k = 0; k < kFeatPer; ++k) {
+        const int c = threadIdx.x + k * kThreads;
+        if (c < kFeatCells) {
+          const int yy = c / kFeatZ, zz = c - yy * kFeatZ;
+          v[k] = delta[(xoff + wrapi(cu.y0 + yy - 2, n)) * row + wrapi(cu.z0 + zz - 2, n)];
+        }
+      }
+    };
+    auto put = [&](int q, const float (&v)[kFeatPer]) {
+      unsigned short* slot = feat + (q & (kFeatSlots - 1)) * kFeatCells;
+#pragma unroll
+      for (int k = 0; k < kFeatPer; ++k) {
+        const int c = threadIdx.x + k * kThreads;
+        if (c < kFeatCells) slot[c] = feature_of(v[k], in_scale);
+      }
+    };
+    // h1 plane p from feature planes p - 1 .. p + 1 into h1 slot p & 1
+    auto layer1 = [&](int p) {
+      int fb[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) fb[j] = ((p + tdx[j]) & (kFeatSlots - 1)) * kFeatCells + toff[j];
+      uint2* dst = reinterpret_cast<uint2*>(h1 + 2 * (p & 1) * kPlaneCells);
+      for (int tt = wave; tt < kH1Tiles; tt += kThreads / 64) {
+        const int cell = tt * 16 + r;
+        const int c = min(cell, kPlaneCells - 1);
+        const int yy = c / kBrickZ, zz = c - yy * kBrickZ;
+        const int centre = (yy + 1) * kFeatZ + zz + 1;
+        unsigned short v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = tval[j] ? feat[fb[j] + centre] : (unsigned short)0;
+        const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
+                                    v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
+        const floatx4 a1 = mfma(op, bw1, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+        if (cell < kPlaneCells)
+          dst[c * 4 + g] = pack_bf16x4(fmaxf(a1[0] + bch1[0], 0.0f), fmaxf(a1[1] + bch1[1], 0.0f),
+                                       fmaxf(a1[2] + bch1[2], 0.0f), fmaxf(a1[3] + bch1[3], 0.0f));
+      }
+    };
+    // one step of the march with the accumulator sets rotated by R = i mod 3
+    auto step = [&](auto rot, int i) {
+      constexpr int R = decltype(rot)::value;
+      const int p = cu.x0 - 1 + i;
+      const bool more = p + 3 <= cu.x1 + 1;  // feature planes x0 - 2 .. x1 + 1
+      float v[kFeatPer];
+      if (more) fetch(p + 3, v);
+      const uint4* slot = h1 + 2 * (p & 1) * kPlaneCells;
+      roll_plane(slot + rowoff, slot + shoff, lo, b, acc[(R + 1) % 3], acc[R], acc[(R + 2) % 3]);
+      roll_store<kFwdRelu>(acc[(R + 2) % 3], p - 1 >= cu.x0 && p - 1 < cu.x1, p - 1, n, ybase, zcell, g, bch2,
+                           ConvOut{out_bf});
+      if (p + 1 <= cu.x1) layer1(p + 1);
+      if (more) put(p + 3, v);
+      __syncthreads();
+    };
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) acc[s][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();  // the previous unit's reads are done
+    {
+      float v[kFeatPer];
+      for (int q = cu.x0 - 2; q <= cu.x0 + 1; ++q) {
+        fetch(q, v);
+        put(q, v);
+      }
+    }
+    __syncthreads();
+    layer1(cu.x0 - 1);
+    __syncthreads();
+    const int np = cu.x1 - cu.x0 + 2;  // h1 planes x0 - 1 .. x1, in whole rotations
+    for (int i = 0; i < np; i += 3) {
+      step(std::integral_constant<int, 0>{}, i);
+      step(std::integral_constant<int, 1>{}, i + 1);
+      step(std::integral_constant<int, 2>{}, i + 2);
+    }
+  }
+}
+
 // Weight gradient of one layer: dW[t][ci][co] = sum_cells X[cell + d_t][ci]
 // G[cell][co] (M = ci, N = co, K = 32 cells along z per MFMA; all 27 taps
 // accumulate in registers) and db[co] = sum_cells G[cell][co].  Each wave
@@ -759,6 +1312,36 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned 
       const float v = (tap >= 0 && co < cout && col < cin) ? w[((long long)(26 - tap) * cin + col) * cout + co] : 0.0f;
       wb[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
     }
+  }
+}
+
+// fp32 master weights W[t][16][co] -> k_conv16r's forward A operands (the
+// 15-chunk K order of roll_tap).
+__global__ void k_pack_roll(const float* __restrict__ w, int cout, unsigned short* __restrict__ wr)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int c = blockIdx.x;
+  const int tap = roll_tap(c, g >> 1);
+  for (int j = 0; j < 8; ++j) {
+    const int ci = 8 * (g & 1) + j;
+    const float v = (tap >= 0 && col < cout) ? w[((long long)tap * 16 + ci) * cout + col] : 0.0f;
+    wr[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
+  }
+}
+
+// Last-layer weights W[t][16][3] -> k_conv_head_r's A operands: row
+// 4 dxi + co of chunk c holds the taps roll_tap(5 dxi + c, h) of output
+// channel co (co = 3 and dxi = 3 zero).
+__global__ void k_pack_head(const float* __restrict__ w, unsigned short* __restrict__ wh)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int c = blockIdx.x;
+  const int dxi = col >> 2, co = col & 3;
+  const int tap = (dxi < 3 && co < 3) ? roll_tap(5 * dxi + c, g >> 1) : -1;
+  for (int j = 0; j < 8; ++j) {
+    const int ci = 8 * (g & 1) + j;
+    const float v = tap >= 0 ? w[((long long)tap * 16 + ci) * 3 + co] : 0.0f;
+    wh[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
   }
 }
 
@@ -950,6 +1533,7 @@ int resnet_init(ResNet* r, int n, unsigned seed)
   for (int l = 0; l < kNetLayers; ++l) {
     COLA_TRY(dev_alloc(&r->d_wf[l], 14LL * 64 * 8));
     COLA_TRY(dev_alloc(&r->d_wb[l], 14LL * 64 * 8));
+    COLA_TRY(dev_alloc(&r->d_wr[l], (long long)kRollChunks * 64 * 8));
   }
   COLA_TRY(dev_alloc(&r->d_x, nc * 8));
   COLA_TRY(dev_alloc(&r->d_f1, nc));
@@ -978,7 +1562,7 @@ void resnet_free(ResNet* r)
     if (p) (void)hipFree(p);
   };
   fr(r->d_param), fr(r->d_m), fr(r->d_v), fr(r->d_grad);
-  for (int l = 0; l < kNetLayers; ++l) fr(r->d_wf[l]), fr(r->d_wb[l]);
+  for (int l = 0; l < kNetLayers; ++l) fr(r->d_wf[l]), fr(r->d_wb[l]), fr(r->d_wr[l]);
   fr(r->d_x), fr(r->d_f1), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a);
   for (int l = 0; l < 3; ++l) fr(r->d_h[l]);
   for (int l = 0; l < 2; ++l) fr(r->d_g[l]);
@@ -992,6 +1576,11 @@ int resnet_pack(ResNet* r, hipStream_t s)
   for (int l = 0; l < kNetLayers; ++l) {
     hipLaunchKernelGGL(k_pack, dim3(14), dim3(64), 0, s, r->d_param + param_offset(l), kNetCin[l], kNetCout[l],
                        r->d_wf[l], l > 0 ? r->d_wb[l] : nullptr);
+    if (kNetCin[l] == 16 && kNetCout[l] == 16)
+      hipLaunchKernelGGL(k_pack_roll, dim3(kRollChunks), dim3(64), 0, s, r->d_param + param_offset(l), kNetCout[l],
+                         r->d_wr[l]);
+    if (kNetCin[l] == 16 && kNetCout[l] == 3)
+      hipLaunchKernelGGL(k_pack_head, dim3(kHeadChunks), dim3(64), 0, s, r->d_param + param_offset(l), r->d_wr[l]);
   }
   COLA_HIP(hipGetLastError());
   return kOk;
@@ -1019,6 +1608,29 @@ ConvLaunch conv_launch(int n)
   return c;
 }
 
+// k_conv16r: a march of m output planes costs about ceil((m + 2) / 3)
+// rotations of three plane steps; the march is the one with the least
+// work on the busiest workgroup (units per workgroup times steps per unit).
+ConvLaunch conv_launch_roll(int n)
+{
+  const long long cols = (long long)(n / kBlockY) * (n / kBlockZ);
+  long long best = -1;
+  ConvLaunch c{1, n};
+  for (int nseg = 1; nseg <= n; ++nseg) {
+    const int march = (n + nseg - 1) / nseg;
+    if ((long long)(nseg - 1) * march >= n) continue;  // the same march as a smaller nseg
+    const long long units = (long long)((n + march - 1) / march) * cols;
+    const long long blocks = std::min<long long>(kConvBlocks, units);
+    const long long cost = (units + blocks - 1) / blocks * (3 * ((march + 4) / 3));
+    if (best < 0 || cost < best) {
+      best = cost;
+      c.blocks = (unsigned)blocks;
+      c.march = march;
+    }
+  }
+  return c;
+}
+
 constexpr size_t kRing16Bytes = sizeof(uint4) * 2 * kRingSlots * kPlaneCells;
 constexpr size_t kRing8Bytes = sizeof(uint4) * kRingSlots * kPlaneCells;
 
@@ -1032,6 +1644,10 @@ int conv_lds_attributes()
                        reinterpret_cast<const void*>(&k_conv16<kBwdData>),
                        reinterpret_cast<const void*>(&k_conv16<kFwdAdd>)};
   for (const void* k : k16) COLA_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRing16Bytes));
+  const void* kr[] = {reinterpret_cast<const void*>(&k_conv16r<kFwdRelu>),
+                      reinterpret_cast<const void*>(&k_conv_head_r<kFwdAdd>),
+                      reinterpret_cast<const void*>(&k_conv_head_r<kFwdLinear>)};
+  for (const void* k : kr) COLA_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRollRingBytes));
   COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv8), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kRing8Bytes));
   done = true;
@@ -1041,17 +1657,27 @@ int conv_lds_attributes()
 namespace {
 
 // layers 2 and 3 (h1 -> h2 -> h3)
-int conv_layers_2_3(ResNet* r, hipStream_t s)
+// hidden layer l (1 or 2: h_{l-1} -> h_l, relu)
+void conv_hidden(ResNet* r, int l, hipStream_t s)
 {
   const int n = r->n;
   const ConvLaunch cl = conv_launch(n);
   ConvOut o;
-  o.bf = r->d_h[1];
-  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[0], r->d_wf[1],
-                     r->d_param + param_offset(1) + 27 * 16 * 16, 16, n, cl.march, o);
-  o.bf = r->d_h[2];
-  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[1], r->d_wf[2],
-                     r->d_param + param_offset(2) + 27 * 16 * 16, 16, n, cl.march, o);
+  o.bf = r->d_h[l];
+  const float* bias = r->d_param + param_offset(l) + 27 * 16 * 16;
+  if (r->roll) {
+    const ConvLaunch cr = conv_launch_roll(n);
+    hipLaunchKernelGGL((k_conv16r<kFwdRelu>), dim3(cr.blocks), dim3(kThreads), kRollRingBytes, s, r->d_h[l - 1],
+                       r->d_wr[l], bias, 16, n, cr.march, o);
+  } else
+    hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[l - 1],
+                       r->d_wf[l], bias, 16, n, cl.march, o);
+}
+
+int conv_layers_2_3(ResNet* r, hipStream_t s)
+{
+  conv_hidden(r, 1, s);
+  conv_hidden(r, 2, s);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
@@ -1062,6 +1688,17 @@ int conv_layer_4(ResNet* r, int mode, const ConvOut& o, hipStream_t s)
   const int n = r->n;
   const ConvLaunch cl = conv_launch(n);
   const float* b4 = r->d_param + param_offset(3) + 27 * 16 * 3;
+  if (r->roll) {
+    const ConvLaunch cr = conv_launch_roll(n);
+    if (mode == kFwdAdd)
+      hipLaunchKernelGGL((k_conv_head_r<kFwdAdd>), dim3(cr.blocks), dim3(kThreads), kRollRingBytes, s, r->d_h[2],
+                         r->d_wr[3], b4, n, cr.march, o);
+    else
+      hipLaunchKernelGGL((k_conv_head_r<kFwdLinear>), dim3(cr.blocks), dim3(kThreads), kRollRingBytes, s, r->d_h[2],
+                         r->d_wr[3], b4, n, cr.march, o);
+    COLA_HIP(hipGetLastError());
+    return kOk;
+  }
   if (mode == kFwdAdd)
     hipLaunchKernelGGL((k_conv16<kFwdAdd>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[2], r->d_wf[3],
                        b4, 3, n, cl.march, o);
@@ -1142,6 +1779,14 @@ int resnet_layers23(ResNet* r, hipStream_t s)
 
 int resnet_layers12(ResNet* r, const float* delta, long long row, const float* pa, hipStream_t s)
 {
+  if (r->roll) {
+    const ConvLaunch cr = conv_launch_roll(r->n);
+    hipLaunchKernelGGL(k_conv12r, dim3(cr.blocks), dim3(kThreads), kConv12rBytes, s, delta, row, r->in_scale,
+                       r->d_wf[0], r->d_param + param_offset(0), r->d_param + param_offset(0) + 27 * 8 * 16, pa, 0.0f,
+                       r->d_wr[1], r->d_param + param_offset(1) + 27 * 16 * 16, r->n, cr.march, r->d_h[1]);
+    COLA_HIP(hipGetLastError());
+    return kOk;
+  }
   const ConvLaunch cl = conv_launch(r->n);
   hipLaunchKernelGGL(k_conv12, dim3(cl.blocks), dim3(kThreads), kConv12Bytes, s, delta, row, r->in_scale, r->d_wf[0],
                      r->d_param + param_offset(0), r->d_param + param_offset(0) + 27 * 8 * 16, pa, 0.0f, r->d_wf[1],
@@ -1153,11 +1798,7 @@ int resnet_layers12(ResNet* r, const float* delta, long long row, const float* p
 int resnet_layer3(ResNet* r, hipStream_t s)
 {
   COLA_TRY(conv_lds_attributes());
-  const ConvLaunch cl = conv_launch(r->n);
-  ConvOut o;
-  o.bf = r->d_h[2];
-  hipLaunchKernelGGL((k_conv16<kFwdRelu>), dim3(cl.blocks), dim3(kThreads), kRing16Bytes, s, r->d_h[1], r->d_wf[2],
-                     r->d_param + param_offset(2) + 27 * 16 * 16, 16, r->n, cl.march, o);
+  conv_hidden(r, 2, s);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

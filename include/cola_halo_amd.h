@@ -286,6 +286,11 @@ int cola_resnet_fit_scale(cola_resnet* net, int count);
    respect to every parameter. */
 int cola_resnet_forward_host(cola_resnet* net, const float* x, float* y);
 int cola_resnet_gradient_host(cola_resnet* net, const float* x, const float* target, double* loss, float* grad);
+/* One Adam step (Kingma & Ba 2015: beta1 0.9, beta2 0.999, eps 1e-8, bias
+   corrected by 1 - beta^t) on the gradient of the last
+   cola_resnet_gradient_host call, the update cola_resnet_train makes after
+   each sample (no reference counterpart; for tests). */
+int cola_resnet_adam_step(cola_resnet* net, float lr);
 /* The inference path on the same host input (no reference counterpart):
    channel 0 of x is the feature and channel 1 of the first cell the scale
    factor a (x must hold a in every cell of channel 1 and zeros in channels
@@ -331,6 +336,15 @@ int cola_set_residual_net(cola_context* ctx, cola_resnet* net);
    in one pass over the density grid, on = 0 runs them as three launches
    through the feature and h1 grids.  Both give the same forces bit for bit. */
 int cola_resnet_set_fusion(cola_resnet* net, int on);
+/* Loop order of the layers with 16 input channels in the forward passes (no
+   reference counterpart): by_input_planes = 1 (default) streams each input
+   plane once and adds its taps to the three output planes it feeds
+   (layers 2-3 and the fused first pass: k_conv16r, k_conv12r; the last
+   layer with the x offsets as output rows: k_conv_head_r); 0 computes each
+   output plane from its three input planes (k_conv16, k_conv12, also the
+   training path's data gradient).  The two orders sum the same products in
+   another order: results agree to float rounding, not bit for bit. */
+int cola_resnet_set_plane_order(cola_resnet* net, int by_input_planes);
 /* Net-weights file: the trained parameters and scales, independent of the
    grid size (the net is a convolution over cells).  Little-endian layout:
    char magic[8] "COLANET1", int32 nparam, int32 0, float in_scale,

@@ -78,6 +78,31 @@ def test_feature_path_matches_general_path(n):
     assert np.abs(y1 - ref).max() <= 1e-2 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("n", [32, 64, 96])
+def test_plane_orders_agree(n):
+    """The hidden layers by input planes (k_conv16r: one march of 1, 1 and 4
+    output planes per unit at n = 32, 64, 96) against by output planes
+    (k_conv16) and the oracle, on both inference paths.  Only the fp32
+    order of the same products differs (and a bfloat16 rounding flip of an
+    activation where it lands on a tie), so the two agree far inside the
+    oracle tolerance; a wrong tap, plane offset or dropped plane would not."""
+    print(f"net plane orders n={n}", flush=True)
+    params, x = _setup(n, 300 + n)
+    x[..., 1] = 0.41
+    x[..., 2:] = 0.0
+    out = {}
+    with ch.ResidualNet(n) as net:
+        net.set_params(params)
+        for order in (False, True):
+            net.set_plane_order(order)
+            out[order] = (net.forward(x), net.forward_features(x))
+    ref, _ = orc.net_forward(params.astype(np.float64), x.astype(np.float64), round_bf16=True)
+    scale = np.abs(ref).max()
+    for i in range(2):
+        assert np.abs(out[True][i] - out[False][i]).max() <= 2e-3 * scale, (i, np.abs(out[True][i] - out[False][i]).max() / scale)
+        assert np.abs(out[True][i] - ref).max() <= 1e-2 * scale
+
+
 def test_gradient_matches_oracle():
     print("net gradient n=32", flush=True)
     n = 32
@@ -94,6 +119,46 @@ def test_gradient_matches_oracle():
     for kind, a, b in _blocks():
         rel = np.linalg.norm(g[a:b] - ref[a:b]) / np.linalg.norm(ref[a:b])
         assert rel <= 5e-2, (kind, a, rel)
+
+
+def test_adam_steps_follow_the_published_update():
+    """Two Adam steps (Kingma & Ba 2015, Algorithm 1: beta1 0.9, beta2 0.999,
+    eps 1e-8, moments m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2, bias
+    corrected by 1 - beta^t) on the device's gradients, against the update
+    computed here in float64 from the same gradients.  The first step alone
+    would be lr sign(g) for any bias-corrected variant; the second pins the
+    moment recursions and the t = 2 corrections (a missing correction, a
+    swapped beta or a square root in the wrong place moves parameters by far
+    more than the 1e-6 lr allowed)."""
+    print("net adam n=32", flush=True)
+    n, lr = 32, 1e-3
+    params, x = _setup(n, 11)
+    rng = np.random.default_rng(12)
+    t1 = rng.standard_normal((n, n, n, 3)).astype(np.float32)
+    t2 = rng.standard_normal((n, n, n, 3)).astype(np.float32)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    with ch.ResidualNet(n) as net:
+        net.set_params(params)
+        net.set_scales(1.0, 1.0)
+        p0 = net.params().astype(np.float64)
+        _, g1 = net.gradient(x, t1)
+        net.adam_step(lr)
+        p1 = net.params().astype(np.float64)
+        _, g2 = net.gradient(x, t2)
+        net.adam_step(lr)
+        p2 = net.params().astype(np.float64)
+    g1 = g1.astype(np.float64)
+    g2 = g2.astype(np.float64)
+    m = (1 - b1) * g1
+    v = (1 - b2) * g1 ** 2
+    want1 = p0 - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    m = b1 * m + (1 - b1) * g2
+    v = b2 * v + (1 - b2) * g2 ** 2
+    want2 = p1 - lr * (m / (1 - b1 ** 2)) / (np.sqrt(v / (1 - b2 ** 2)) + eps)
+    live = np.abs(g1) > 1e-6  # parameters whose step is not dominated by eps
+    assert live.sum() > 0.9 * len(g1)
+    assert np.abs(p1 - want1)[live].max() <= 1e-6 * lr + 1e-7 * np.abs(p0).max()
+    assert np.abs(p2 - want2)[live].max() <= 1e-6 * lr + 1e-7 * np.abs(p0).max()
 
 
 def test_pairs_training_and_corrected_force(eh_table):
@@ -274,16 +339,18 @@ def test_destroying_an_attached_net_detaches_it(eh_table):
         assert np.array_equal(a.get_particles(), b.get_particles())
 
 
+@pytest.mark.parametrize("order", [False, True])
 @pytest.mark.parametrize("nc,ng,box", [(32, 64, 100.0), (256, 768, 500.0)])
-def test_fused_first_layers_equal_separate_launches(eh_table, nc, ng, box):
+def test_fused_first_layers_equal_separate_launches(eh_table, nc, ng, box, order):
     """The inference path's fused pass (features, layer 1 and layer 2 from
     the density grid in one launch) gives the corrected forces of the three
     separate launches bit for bit: at 64^3 every workgroup marches one x
     plane, at 768^3 hundreds of planes (ring slots reused many times)."""
-    print(f"fused layers 1-2, {nc}^3 / {ng}^3", flush=True)
+    print(f"fused layers 1-2, {nc}^3 / {ng}^3, by input planes {order}", flush=True)
     k, p = eh_table
     with ch.ResidualNet(ng, seed=9) as net, ch.Context(nc, ng, box) as c:
         net.set_scales(1.3, 0.7)
+        net.set_plane_order(order)  # k_conv12 / k_conv12r sum layer 2 as k_conv16 / k_conv16r
         c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
         c.lpt_set_displacement(0.1, OM, 4)
         c.set_times(c.times()[0], 0.1)
