@@ -1060,15 +1060,69 @@ k_conv12(const float* __restrict__ delta, long long row, float in_scale, const u
 // Features, layer 1 and layer 2 in one pass with layer 2 by input planes
 // (k_conv16r's order): h2 from the real-space density grid.  A unit keeps
 // feature planes (kFeatY x kFeatZ, 4 ring slots) and two h1 planes (the
-// kBrickY x kBrickZ brick) in LDS.  Step i of the march (h1 plane p =
-// x0 - 1 + i): the density of feature plane p + 3 is loaded into
-// registers; layer 2 adds h1 plane p's taps to the accumulators of h2
-// planes p + 1, p, p - 1; h2 plane p - 1 is stored; h1 plane p + 1 is made
-// from feature planes p .. p + 2 into the other h1 slot (layer 1 as in
-// k_conv1); feature plane p + 3 takes the slot of plane p - 1.  One barrier
-// per step.  The arithmetic is that of k_features, k_conv1 and
-// k_conv16r<kFwdRelu>: h2 equals theirs bit for bit.
-constexpr size_t kConv12rBytes = sizeof(uint4) * 2 * 2 * kPlaneCells + sizeof(unsigned short) * kFeatSlots * kFeatCells;
+// kBrickY x kBrickZ brick, padded to kH1SlotCells) in LDS.  Step i of the
+// march (h1 plane p = x0 - 1 + i): the density of feature plane p + 3 is
+// loaded into registers; layer 2 adds h1 plane p's taps to the
+// accumulators of h2 planes p + 1, p, p - 1 while, row by row, the wave
+// makes one 16-cell tile of h1 plane p + 1 (from feature planes p .. p + 2,
+// into the other h1 slot; layer 1 as in k_conv1); h2 plane p - 1 is stored;
+// feature plane p + 3 takes the slot of plane p - 1.  One barrier per step.
+// Every step makes all 40 tiles (the last ones write the pad) and its next
+// h1 plane even past the march's end (never read for a stored plane), so
+// the loop has no branches inside.  The arithmetic is that of k_features,
+// k_conv1 and k_conv16r<kFwdRelu>: h2 equals theirs bit for bit.
+constexpr int kH1SlotCells = 4 * (kConvTy + 2) * 16;  // one tile per wave and row of the march step
+static_assert(kH1SlotCells >= kPlaneCells, "the h1 slot holds the brick");
+constexpr size_t kConv12rBytes = sizeof(uint4) * 2 * 2 * kH1SlotCells + sizeof(unsigned short) * kFeatSlots * kFeatCells;
+
+// roll_plane with one call of extra(j) per brick row, placed with the row's
+// products (the layer-1 tile of the fused pass).
+template <typename Extra>
+__device__ inline void roll_plane_with(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
+                                       const uint4 (&b)[kRollChunks], floatx4 (&am)[kConvTy],
+                                       floatx4 (&a0)[kConvTy], floatx4 (&ap)[kConvTy], Extra&& extra)
+{
+  constexpr int kRow = 2 * kBrickZ;
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  uint4 prev = zero4;
+  uint4 cen = row[0], sh = shr[0];
+#pragma unroll
+  for (int j = 0; j < kConvTy + 2; ++j) {
+    uint4 ncen = zero4, nsh = zero4;
+    if (j + 1 < kConvTy + 2) {
+      ncen = row[(j + 1) * kRow];
+      nsh = shr[(j + 1) * kRow];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    extra(j);
+    if (j < kConvTy) {
+      am[j] = mfma(sh, b[0], am[j]);
+      a0[j] = mfma(sh, b[5], a0[j]);
+      ap[j] = mfma(sh, b[10], ap[j]);
+    }
+    if (j >= 1 && j - 1 < kConvTy) {
+      am[j - 1] = mfma(sh, b[1], am[j - 1]);
+      a0[j - 1] = mfma(sh, b[6], a0[j - 1]);
+      ap[j - 1] = mfma(sh, b[11], ap[j - 1]);
+      const uint4 pc = sel4(lo, prev, cen);
+      am[j - 1] = mfma(pc, b[3], am[j - 1]);
+      a0[j - 1] = mfma(pc, b[8], a0[j - 1]);
+      ap[j - 1] = mfma(pc, b[13], ap[j - 1]);
+    }
+    if (j >= 2) {
+      am[j - 2] = mfma(sh, b[2], am[j - 2]);
+      a0[j - 2] = mfma(sh, b[7], a0[j - 2]);
+      ap[j - 2] = mfma(sh, b[12], ap[j - 2]);
+      const uint4 qc = sel4(lo, cen, zero4);
+      am[j - 2] = mfma(qc, b[4], am[j - 2]);
+      a0[j - 2] = mfma(qc, b[9], a0[j - 2]);
+      ap[j - 2] = mfma(qc, b[14], ap[j - 2]);
+    }
+    prev = cen;
+    cen = ncen;
+    sh = nsh;
+  }
+}
 
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
 k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const unsigned short* __restrict__ wpk1,
@@ -1088,8 +1142,8 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
 #pragma unroll
   for (int i = 0; i < 4; ++i) bch2[i] = b2[4 * g + i];
   extern __shared__ uint4 smem[];
-  uint4* h1 = smem;  // [2 slots][yy][zz][2 halves of 8 channels]
-  unsigned short* feat = reinterpret_cast<unsigned short*>(smem + 2 * 2 * kPlaneCells);  // [slot][yy][zz]
+  uint4* h1 = smem;  // [2 slots][kH1SlotCells][2 halves of 8 channels]
+  unsigned short* feat = reinterpret_cast<unsigned short*>(smem + 2 * 2 * kH1SlotCells);  // [slot][yy][zz]
   const int half = g & 1;
   const int wz = wave & 1, wy = wave >> 1;
   const int zc = 1 + 16 * wz + r;
@@ -1130,27 +1184,25 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
         if (c < kFeatCells) slot[c] = feature_of(v[k], in_scale);
       }
     };
-    // h1 plane p from feature planes p - 1 .. p + 1 into h1 slot p & 1
-    auto layer1 = [&](int p) {
-      int fb[8];
+    // tile tt (0 .. 39) of h1 plane p from feature planes p - 1 .. p + 1
+    // into h1 slot p & 1 (tiles past the brick write the slot's pad)
+    auto l1tile = [&](int p, int tt) {
+      const int cell = tt * 16 + r;
+      const int c = min(cell, kPlaneCells - 1);
+      const int yy = c / kBrickZ, zz = c - yy * kBrickZ;
+      const int centre = (yy + 1) * kFeatZ + zz + 1;
+      unsigned short v[8];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) fb[j] = ((p + tdx[j]) & (kFeatSlots - 1)) * kFeatCells + toff[j];
-      uint2* dst = reinterpret_cast<uint2*>(h1 + 2 * (p & 1) * kPlaneCells);
-      for (int tt = wave; tt < kH1Tiles; tt += kThreads / 64) {
-        const int cell = tt * 16 + r;
-        const int c = min(cell, kPlaneCells - 1);
-        const int yy = c / kBrickZ, zz = c - yy * kBrickZ;
-        const int centre = (yy + 1) * kFeatZ + zz + 1;
-        unsigned short v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = tval[j] ? feat[fb[j] + centre] : (unsigned short)0;
-        const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
-                                    v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
-        const floatx4 a1 = mfma(op, bw1, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
-        if (cell < kPlaneCells)
-          dst[c * 4 + g] = pack_bf16x4(fmaxf(a1[0] + bch1[0], 0.0f), fmaxf(a1[1] + bch1[1], 0.0f),
-                                       fmaxf(a1[2] + bch1[2], 0.0f), fmaxf(a1[3] + bch1[3], 0.0f));
+      for (int j = 0; j < 8; ++j) {
+        const int fb = ((p + tdx[j]) & (kFeatSlots - 1)) * kFeatCells + toff[j];
+        v[j] = tval[j] ? feat[fb + centre] : (unsigned short)0;
       }
+      const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
+                                  v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
+      const floatx4 a1 = mfma(op, bw1, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+      uint2* dst = reinterpret_cast<uint2*>(h1 + 2 * (p & 1) * kH1SlotCells);
+      dst[cell * 4 + g] = pack_bf16x4(fmaxf(a1[0] + bch1[0], 0.0f), fmaxf(a1[1] + bch1[1], 0.0f),
+                                      fmaxf(a1[2] + bch1[2], 0.0f), fmaxf(a1[3] + bch1[3], 0.0f));
     };
     // one step of the march with the accumulator sets rotated by R = i mod 3
     auto step = [&](auto rot, int i) {
@@ -1159,11 +1211,12 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
       const bool more = p + 3 <= cu.x1 + 1;  // feature planes x0 - 2 .. x1 + 1
       float v[kFeatPer];
       if (more) fetch(p + 3, v);
-      const uint4* slot = h1 + 2 * (p & 1) * kPlaneCells;
+      const uint4* slot = h1 + 2 * (p & 1) * kH1SlotCells;
       roll_plane(slot + rowoff, slot + shoff, lo, b, acc[(R + 1) % 3], acc[R], acc[(R + 2) % 3]);
+#pragma unroll 1
+      for (int j = 0; j < kConvTy + 2; ++j) l1tile(p + 1, wave + 4 * j);
       roll_store<kFwdRelu>(acc[(R + 2) % 3], p - 1 >= cu.x0 && p - 1 < cu.x1, p - 1, n, ybase, zcell, g, bch2,
                            ConvOut{out_bf});
-      if (p + 1 <= cu.x1) layer1(p + 1);
       if (more) put(p + 3, v);
       __syncthreads();
     };
@@ -1180,7 +1233,7 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
       }
     }
     __syncthreads();
-    layer1(cu.x0 - 1);
+    for (int tt = wave; tt < 4 * (kConvTy + 2); tt += kThreads / 64) l1tile(cu.x0 - 1, tt);
     __syncthreads();
     const int np = cu.x1 - cu.x0 + 2;  // h1 planes x0 - 1 .. x1, in whole rotations
     for (int i = 0; i < np; i += 3) {
