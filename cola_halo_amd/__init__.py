@@ -121,6 +121,7 @@ _SIGNATURES = {
     "cola_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "cola_fft_time": (C.c_int, [C.c_int, C.c_int, C.c_void_p]),
     "cola_coarse_grid": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "cola_coarse_grid_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "cola_power_spectrum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
                                       C.c_void_p, C.c_void_p, C.c_void_p]),
     "cola_state_save": (C.c_int, [C.c_void_p]),
@@ -684,6 +685,14 @@ class Context:
         sums = np.zeros(2, np.float64)
         _check(lib().cola_coarse_grid(self._h, n, self._SOURCES[source], out.ctypes.data, sums.ctypes.data))
         return out, (float(sums[0]), float(sums[1]))
+
+    def coarse_grid_noalloc(self, n, delta_dev, sums_dev=0, source="snapshot"):
+        """cola_coarse_grid_noalloc: the coarse grid into caller-owned device
+        memory (delta_dev: address of n^3 floats, sums_dev: 0 or the address
+        of two doubles), accumulated in the library's static scratch (n <=
+        128); enqueued only: synchronize() before reading the outputs."""
+        _check(lib().cola_coarse_grid_noalloc(self._h, n, self._SOURCES[source], C.c_void_p(delta_dev),
+                                              C.c_void_p(sums_dev) if sums_dev else None))
 
     def power_spectrum(self, n, kmin, kmax, nbins, deconvolve=False, subtract_shot=False, source="particles"):
         """Shell-averaged P(k) of the particles on an n^3 CIC grid.  Returns

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1445,6 +1446,49 @@ int deposit_contrast(cola_context* c, const PosSource& src, int n, long long row
 
 }  // namespace
 
+// Scratch of cola_coarse_grid_noalloc: the 64-bit fixed-point CIC
+// accumulators of one grid of up to kNoAllocGrid^3 cells, then two doubles
+// for the density sums when the caller passes none.  Every GPU the library
+// runs on holds its own copy (16 MiB + 256 B) until the process ends.  Host
+// code never passes the array's name to a kernel: it passes the device
+// address hipGetSymbolAddress returns for the current device
+// (noalloc_scratch()).
+constexpr int kNoAllocGrid = 128;
+constexpr long long kNoAllocCells = (long long)kNoAllocGrid * kNoAllocGrid * kNoAllocGrid;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocCells + 32];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned long long* g_scratch_ptr[kNoAllocDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device.
+int noalloc_scratch(unsigned long long** out)
+{
+  int dev = -1;
+  COLA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) {
+    set_error("cola_coarse_grid_noalloc: device index beyond the library's table of 64");
+    return kErrArg;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    COLA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_error("cola_coarse_grid_noalloc: no device address for the scratch array");
+      return kErrHip;
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned long long*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
+}  // namespace
+
 extern "C" {
 
 int cola_coarse_grid(cola_context* c, int n, int source, float* delta_out, double* sums_out)
@@ -1469,6 +1513,26 @@ int cola_coarse_grid(cola_context* c, int n, int source, float* delta_out, doubl
     return kErrState;
   }
   return kOk;
+}
+
+int cola_coarse_grid_noalloc(cola_context* c, int n, int source, float* delta_dev, double* sums_dev)
+{
+  if (!valid(c) || n < 1 || !delta_dev) return kErrArg;
+  if (n > kNoAllocGrid) {
+    set_error("cola_coarse_grid_noalloc: n must be at most 128 (the size of its static scratch)");
+    return kErrArg;
+  }
+  PosSource src;
+  COLA_TRY(position_source(c, source, &src));
+  unsigned long long* fix = nullptr;
+  COLA_TRY(noalloc_scratch(&fix));
+  double* sums = sums_dev ? sums_dev : reinterpret_cast<double*>(fix + kNoAllocCells);
+  // the same kernels as cola_coarse_grid; the accumulators and sums are
+  // cleared by fill kernels on the context's stream
+  COLA_TRY(launch_cic_periodic(src, n, c->boxf, fix, c->stream));
+  COLA_TRY(launch_fill_zero(sums, 2, c->stream));
+  const float nbar_inv = (double)n * n * n / (double)c->np;  // coarse_grid.c:56
+  return launch_density_contrast(n, n, nbar_inv, fix, delta_dev, sums, c->stream);
 }
 
 int cola_power_spectrum(cola_context* c, int n, int source, double kmin, double kmax, int nbins, int flags,

@@ -214,6 +214,20 @@ int cola_reset_phase_stats(cola_context* ctx);
    differs from the reference's serial loop: cells agree to float rounding. */
 int cola_coarse_grid(cola_context* ctx, int n, int source, float* delta_out, double* sums_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): cola_coarse_grid's kernels with the CIC accumulators in
+   a static device array of the library (16 MiB per GPU, held until the
+   process ends), so the call allocates nothing.  n <= 128 (COLA_ERR_ARG
+   beyond).  delta_dev is DEVICE memory for n^3 floats (C order, as
+   cola_coarse_grid's delta_out); sums_dev is NULL or DEVICE memory for the
+   two sums.  Work is only enqueued on the context's stream: the call does
+   not synchronize, and the mass check of cola_coarse_grid (coarse_grid.c:84)
+   is the caller's, from sums_dev after cola_synchronize.  Every call on one
+   GPU shares the one scratch array: calls that overlap on a GPU (contexts
+   on different streams, host threads) race on it. */
+int cola_coarse_grid_noalloc(cola_context* ctx, int n, int source, float* delta_dev, double* sums_dev);
+
 /* Shell-averaged power spectrum of the particle distribution (no reference
    counterpart; the CIC assignment is cola_coarse_grid's): delta on an n^3
    grid (n even, 2^a 3^b 5^c), r2c FFT, P(k) = L^3 abs(delta_k / n^3)^2 over
