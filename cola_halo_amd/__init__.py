@@ -135,6 +135,7 @@ _SIGNATURES = {
     "cola_set_fft_schedule": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_fft_radices": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int]),
     "cola_set_fft_xcd_group": (C.c_int, [C.c_void_p, C.c_int]),
+    "cola_set_readout_group": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_eh98_nowiggle_table": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                            C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "cola_bandwidth_probe": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
@@ -754,6 +755,10 @@ class Context:
         t = Tuning()
         _check(lib().cola_autotune(self._h, reps, C.byref(t)))
         return t.as_dict()
+
+    def set_readout_group(self, group):
+        """Consecutive readout workgroups per XCD (results unchanged)."""
+        _check(lib().cola_set_readout_group(self._h, group))
 
     def set_fft_xcd_group(self, group):
         _check(lib().cola_set_fft_xcd_group(self._h, group))

@@ -439,6 +439,8 @@ int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
   c->lpt_fft.hook = lpt_fft_hook;
   c->lpt_fft.hook_user = c;
   c->cic = make_cic_geom(ngrid, c->boxf, c->wpar, 2LL * c->pm_fft.nzp);
+  // one y band of each lattice slab (nc^2 particles) per XCD
+  c->cic.readout_group = (int)std::max(8LL, (long long)nc * nc / 256 / 8);
   c->grid_floats = (long long)ngrid * ngrid * c->cic.row;
   // default deposit tile 8 x 8 x 32 cells, halved until it divides ngrid
   auto pick = [&](int want) {
@@ -2089,6 +2091,15 @@ int cola_set_fft_xcd_group(cola_context* c, int group)
   drop_step_graph(c);
   c->pm_fft.xgroup = group;
   c->lpt_fft.xgroup = group;
+  return kOk;
+}
+
+int cola_set_readout_group(cola_context* c, int group)
+{
+  if (!valid(c) || group < 1 || group > 4096) return kErrArg;
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  drop_step_graph(c);
+  c->cic.readout_group = group;
   return kOk;
 }
 

@@ -27,6 +27,10 @@ struct CicGeom {
   float x_right = 0.0f;  // pm.c:422-423  ghost threshold L/Ng*Ng - L/Ng
   float shift = 0.0f;    // pm.c:434  -BoxSize
   long long row = 0;     // real row length in floats, >= 2(ng/2+1) (in-place r2c)
+  // readout: consecutive workgroups per XCD (xcd_group_remap); a lattice
+  // slab's rows split into one y band per XCD keeps each XCD's grid rows in
+  // its own L2
+  int readout_group = 8;
 };
 CicGeom make_cic_geom(int ng, float boxsize, float wpar, long long row);
 

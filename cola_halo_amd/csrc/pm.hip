@@ -574,10 +574,9 @@ k_sum_doubles(const double* __restrict__ v, long long n, double* ring, const Ste
 // ---- readout (pm.c:359-412) -------------------------------------------------
 // Particles stay in lattice order (a workgroup's 256 particles are a lattice
 // row at nc = 256), and lattice neighbours gather from overlapping grid
-// rows: consecutive workgroups go in groups of 8 to one XCD, so the shared
-// grid lines are read once into that XCD's L2 (256^3 on 768^3: the fused
-// readout 2.0 -> 1.68 ms; groups of 16 and 32 measured the same).
-constexpr int kReadoutGroup = 8;
+// rows: consecutive workgroups go in groups (CicGeom::readout_group) to one
+// XCD, so the shared grid lines are read once into that XCD's L2 (256^3 on
+// 768^3: groups of 8 took the fused readout from 2.0 to 1.68 ms).
 
 // One axis of the CIC gather at one position, term order of pm.c:397-410.
 struct Gather3 {
@@ -649,7 +648,7 @@ __global__ void __launch_bounds__(kThreads)
 k_readout(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
           const float* __restrict__ g2)
 {
-  const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, kReadoutGroup) * kThreads + threadIdx.x;
+  const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, g.readout_group) * kThreads + threadIdx.x;
   if (i >= p.np) return;
   const Gather3 f = particle_force(p.x[0][i], p.x[1][i], p.x[2][i], g, g0, g1, g2);
 #pragma unroll
@@ -673,7 +672,7 @@ __global__ void __launch_bounds__(kThreads)
 k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
                      const float* __restrict__ g2, const StepParams* __restrict__ prm)
 {
-  const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, kReadoutGroup) * kThreads + threadIdx.x;
+  const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, g.readout_group) * kThreads + threadIdx.x;
   if (i >= p.np) return;
   const KickFactors kf = prm->k;
   const DriftFactors df = prm->d;

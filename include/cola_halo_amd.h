@@ -464,6 +464,9 @@ int cola_write_coarse_grid(const char* filename, float boxsize, int n, const flo
 int cola_set_tiling(cola_context* ctx, int tx, int ty, int tz, int fft_log2_tile);
 /* Adjacent FFT column tiles dispatched to one XCD (a power of two, 1 to 64). */
 int cola_set_fft_xcd_group(cola_context* ctx, int group);
+/* Consecutive readout workgroups dispatched to one XCD (1 to 4096; default
+   one y band of each lattice slab per XCD: max(8, nc^2 / 2048)). */
+int cola_set_readout_group(cola_context* ctx, int group);
 /* Launch shape of the PM transforms: log2 of the transforms per workgroup of
    the y/x column passes and of the z row passes (-1 = default), and the XCD
    group.  Results do not depend on them. */
