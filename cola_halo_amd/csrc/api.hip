@@ -121,6 +121,7 @@ struct cola_context {
   bool have_particles = false;
   bool have_force = false;
   bool have_grids = false;
+  bool fi_force = false;  // the last force was corrected: its grids are the net's interleaved d_fi
   bool density_is_real = false;
 
   DevParticleMin* d_snap = nullptr;
@@ -351,10 +352,18 @@ int pm_fields(cola_context* c)
     COLA_TRY(resnet_head_add(c->net, out, c->cic.row, c->stream));
     phase_end(c, kPhNetHead);
   }
+  c->fi_force = c->net != nullptr;
   return kOk;
 }
 
 bool valid(const cola_context* c) { return c != nullptr; }
+
+// The learned residual's sums over the grid of the last force (its mean is
+// taken off the particles' forces), or null without a net.
+const double* net_sums(const cola_context* c) { return c->net ? c->net->d_fsum : nullptr; }
+// The corrected force grids, interleaved [cell][3] (what the readout
+// gathers), or null without a net (the readout then reads fgrid[0..2]).
+const float* net_fi(const cola_context* c) { return c->net ? c->net->d_fi : nullptr; }
 
 int require_particles(cola_context* c)
 {
@@ -792,7 +801,7 @@ int cola_pm_calculate_forces(cola_context* c)
   COLA_TRY(check_pending(c));
   COLA_TRY(pm_fields(c));
   phase_begin(c, kPhReadout);
-  COLA_TRY(launch_readout(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], c->stream));
+  COLA_TRY(launch_readout(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], net_sums(c), net_fi(c), c->stream));
   phase_end(c, kPhReadout);
   c->have_force = true;
   return kOk;
@@ -841,7 +850,8 @@ int step_kernels(cola_context* c)
   COLA_TRY(assign_kernels(c, true));
   COLA_TRY(pm_fields(c));
   phase_begin(c, kPhReadoutKickDrift);
-  COLA_TRY(launch_readout_kick_drift(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], c->d_prm, c->stream));
+  COLA_TRY(launch_readout_kick_drift(c->p, c->cic, c->fgrid[0], c->fgrid[1], c->fgrid[2], c->d_prm, net_sums(c),
+                                     net_fi(c), c->stream));
   phase_end(c, kPhReadoutKickDrift);
   return kOk;
 }
@@ -1067,6 +1077,18 @@ int cola_get_force_grid(cola_context* c, int axis, float* out, size_t n_out)
   if (!valid(c) || !out || axis < 0 || axis > 2 || n_out < host_grid_floats(c)) return kErrArg;
   if (!c->have_grids) return kErrState;
   COLA_HIP(hipStreamSynchronize(c->stream));
+  if (c->fi_force && c->net) {
+    // the corrected force lives interleaved in the net's grid
+    const long long cells = (long long)c->ng * c->ng * c->ng;
+    std::vector<float> fi(3 * (size_t)cells);
+    COLA_HIP(hipMemcpy(fi.data(), c->net->d_fi, sizeof(float) * fi.size(), hipMemcpyDeviceToHost));
+    const long long hrow = 2 * (c->ng / 2 + 1);
+    for (long long line = 0; line < (long long)c->ng * c->ng; ++line) {
+      for (int z = 0; z < c->ng; ++z) out[line * hrow + z] = fi[(line * c->ng + z) * 3 + axis];
+      out[line * hrow + c->ng] = out[line * hrow + c->ng + 1] = 0.0f;
+    }
+    return kOk;
+  }
   return grid_to_host(c, c->fgrid[axis], out);
 }
 

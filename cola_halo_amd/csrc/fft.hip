@@ -616,9 +616,13 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
     }
   }
 
-  float2* outs[3] = {out0, out1, out2};
+  // Two inverse transforms instead of three: along a column (fixed ky, kz)
+  // the y and z force spectra are n_y and n_z times the same i f1 / k^2
+  // delta (pm.c:336-341 with n_y, n_z constant in x), and the x transform is
+  // linear, so transform 1 is that common spectrum and its outputs are
+  // written twice, scaled by n_y and by n_z; transform 0 carries n_x.
 #pragma unroll 1
-  for (int axis = 0; axis < 3; ++axis) {
+  for (int axis = 0; axis < 2; ++axis) {
     Tile reg;
     // inverse pass 0 (L = 1): pm.c:322-346 operator on the forward outputs b + t*m
 #pragma unroll
@@ -631,8 +635,7 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
 #pragma unroll
       for (int t = 0; t < R_LAST; ++t) {
         const int I0 = wave_pm(b + t * pl.m, n);
-        const float di = (float)(axis == 0 ? I0 : (axis == 1 ? J0 : K));
-        const float f2 = green[q * R_LAST + t] * di;
+        const float f2 = axis == 0 ? green[q * R_LAST + t] * (float)I0 : green[q * R_LAST + t];
         const float2 v = keep[q * R_LAST + t];
         reg[q * R_LAST + t] = (I0 == 0 && J0 == 0 && K == 0) ? make_float2(0.0f, 0.0f)
                                                               : make_float2(-f2 * v.y, f2 * v.x);
@@ -655,15 +658,27 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
         } else {
           // last pass: outputs at b + t*m
           const int nb = p.m << log2T;
-          float2* out = outs[axis] + c0;
 #pragma unroll
           for (int q = 0; q < EMAX / R; ++q) {
             const int u = tid + q * nthr;
             const int c = u & tmask, b = u >> log2T;
             if (u < nb && c0 + c < A.ncols) {
               const unsigned lane = (unsigned)b * (unsigned)A.sr + (unsigned)c;
+              if (axis == 0) {
 #pragma unroll
-              for (int t = 0; t < R; ++t) out[(long long)(t * p.m) * A.sr + lane] = r2[q * R + t];
+                for (int t = 0; t < R; ++t) out0[c0 + (long long)(t * p.m) * A.sr + lane] = r2[q * R + t];
+              } else {
+                const int col = c0 + c;
+                const int iy = (int)fdiv((unsigned)col, (unsigned)A.nz, A.magNz);
+                const float fy = (float)wave_pm(iy, n), fz = (float)(col - iy * A.nz);
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                  const float2 v = r2[q * R + t];
+                  const long long at = c0 + (long long)(t * p.m) * A.sr + lane;
+                  out1[at] = make_float2(fy * v.x, fy * v.y);
+                  out2[at] = make_float2(fz * v.x, fz * v.y);
+                }
+              }
             }
           }
         }

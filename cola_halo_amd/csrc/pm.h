@@ -83,11 +83,15 @@ int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& 
                          const StepParams* prm, hipStream_t s);
 int launch_zero_mass(double* ring, const StepParams* prm, hipStream_t s);
 // pm.c:359-412 + 505-532: readout of the three force grids into p.f.
+// fsum: null, or the three sums of a residual added to the force grids,
+// whose mean is taken off every particle's force.  fi: null (gather from
+// fx, fy, fz), or one interleaved grid fi[cell][3] of the same forces.
 int launch_readout(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy, const float* fz,
-                   hipStream_t s);
+                   const double* fsum, const float* fi, hipStream_t s);
 // Readout fused with cola.c:104-112 kick and cola.c:142-149 drift.
 int launch_readout_kick_drift(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy,
-                              const float* fz, const StepParams* prm, hipStream_t s);
+                              const float* fz, const StepParams* prm, const double* fsum, const float* fi,
+                              hipStream_t s);
 int launch_kick(const DevParticles& p, const KickFactors& k, hipStream_t s);
 int launch_drift(const DevParticles& p, const DriftFactors& d, hipStream_t s);
 

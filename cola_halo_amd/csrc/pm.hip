@@ -630,29 +630,93 @@ __device__ inline Gather3 cic_gather(float x, float y, float z, const CicGeom& g
   return r;
 }
 
+// The same gather from one interleaved grid fi[cell][3] (cells unpadded,
+// C order): the three components of a corner are one 12-byte record, so a
+// particle touches four grid lines instead of twelve.  Same index, weight
+// and term order as cic_gather.
+__device__ inline Gather3 cic_gather_fi(float x, float y, float z, const CicGeom& g, const float* __restrict__ fi)
+{
+  const int ng = g.ng;
+  const float X = x * g.scale;
+  const float Y = y * g.scale;
+  const float Z = z * g.scale;
+  int iI = (int)floorf(X);
+  int J = (int)Y;
+  int K = (int)Z;
+  const float D1 = X - (float)iI;
+  const float D2 = Y - (float)J;
+  const float D3 = Z - (float)K;
+  const float T1 = 1.0f - D1;
+  const float T2 = 1.0f - D2;
+  const float T3 = 1.0f - D3;
+  Gather3 r;
+  r.f[0] = r.f[1] = r.f[2] = 0.0f;
+  if (J < 0 || K < 0) return r;
+  if (J >= ng) J = 0;
+  if (K >= ng) K = 0;
+  const int I1 = iI + 1;
+  int J1 = J + 1;
+  if (J1 >= ng) J1 = 0;
+  int K1 = K + 1;
+  if (K1 >= ng) K1 = 0;
+  auto rec = [&](long long line, int k) { return *reinterpret_cast<const float3*>(fi + (line * ng + k) * 3); };
+  if (0 <= iI && iI < ng) {
+    const long long a = (long long)iI * ng + J, b = (long long)iI * ng + J1;
+    const float3 aK = rec(a, K), aK1 = rec(a, K1), bK = rec(b, K), bK1 = rec(b, K1);
+    const float va[3][4] = {{aK.x, aK1.x, bK.x, bK1.x}, {aK.y, aK1.y, bK.y, bK1.y}, {aK.z, aK1.z, bK.z, bK1.z}};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+      r.f[ax] += va[ax][0] * T3 * T1 * T2 + va[ax][1] * D3 * T1 * T2 + va[ax][2] * T3 * T1 * D2 + va[ax][3] * D3 * T1 * D2;
+  }
+  if (0 <= I1 && I1 < ng) {
+    const long long a = (long long)I1 * ng + J, b = (long long)I1 * ng + J1;
+    const float3 aK = rec(a, K), aK1 = rec(a, K1), bK = rec(b, K), bK1 = rec(b, K1);
+    const float va[3][4] = {{aK.x, aK1.x, bK.x, bK1.x}, {aK.y, aK1.y, bK.y, bK1.y}, {aK.z, aK1.z, bK.z, bK1.z}};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+      r.f[ax] += va[ax][0] * T3 * D1 * T2 + va[ax][1] * D3 * D1 * T2 + va[ax][2] * T3 * D1 * D2 + va[ax][3] * D3 * D1 * D2;
+  }
+  return r;
+}
+
 // Force at a particle: its own gather plus, beyond the ghost threshold, the
 // gather at its ghost position added afterwards (pm.c:526-531).
+template <bool FI>
 __device__ inline Gather3 particle_force(float x, float y, float z, const CicGeom& g, const float* g0, const float* g1,
                                          const float* g2)
 {
-  Gather3 f = cic_gather(x, y, z, g, g0, g1, g2);
+  Gather3 f = FI ? cic_gather_fi(x, y, z, g, g0) : cic_gather(x, y, z, g, g0, g1, g2);
   if (x > g.x_right) {
-    const Gather3 h = cic_gather(x + g.shift, y, z, g, g0, g1, g2);
+    const Gather3 h = FI ? cic_gather_fi(x + g.shift, y, z, g, g0) : cic_gather(x + g.shift, y, z, g, g0, g1, g2);
 #pragma unroll
     for (int a = 0; a < 3; ++a) f.f[a] += h.f[a];
   }
   return f;
 }
 
+// Mean of a residual added to the force grids (fsum: its three sums over the
+// grid, null when there is none): a constant in the grid is a constant in
+// every CIC readout, and the force of a periodic field has none
+// (pm.c:312-313 zeroes the k = 0 mode), so it is taken back off.
+__device__ inline void residual_mean(const double* __restrict__ fsum, float inv_cells, float (&m)[3])
+{
+#pragma unroll
+  for (int a = 0; a < 3; ++a) m[a] = fsum ? (float)(fsum[a] * (double)inv_cells) : 0.0f;
+}
+
+// FI: g0 is the interleaved grid fi[cell][3] (g1, g2 unused).
+template <bool FI>
 __global__ void __launch_bounds__(kThreads)
 k_readout(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
-          const float* __restrict__ g2)
+          const float* __restrict__ g2, const double* __restrict__ fsum, float inv_cells)
 {
   const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, g.readout_group) * kThreads + threadIdx.x;
   if (i >= p.np) return;
-  const Gather3 f = particle_force(p.x[0][i], p.x[1][i], p.x[2][i], g, g0, g1, g2);
+  Gather3 f = particle_force<FI>(p.x[0][i], p.x[1][i], p.x[2][i], g, g0, g1, g2);
+  float m[3];
+  residual_mean(fsum, inv_cells, m);
 #pragma unroll
-  for (int a = 0; a < 3; ++a) p.f[a][i] = f.f[a];
+  for (int a = 0; a < 3; ++a) p.f[a][i] = fsum ? f.f[a] - m[a] : f.f[a];
 }
 
 // cola.c:105-111 for one component
@@ -668,9 +732,11 @@ __device__ inline float drift1(float x, float v, float dx1, float dx2, const Dri
   return x + (v * d.dyyy + 1.0f * (dx1 * d.da1 + dx2 * d.da2));
 }
 
+template <bool FI>
 __global__ void __launch_bounds__(kThreads)
 k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
-                     const float* __restrict__ g2, const StepParams* __restrict__ prm)
+                     const float* __restrict__ g2, const StepParams* __restrict__ prm, const double* __restrict__ fsum,
+                     float inv_cells)
 {
   const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, g.readout_group) * kThreads + threadIdx.x;
   if (i >= p.np) return;
@@ -679,7 +745,13 @@ k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, co
   float x[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) x[a] = p.x[a][i];
-  const Gather3 f = particle_force(x[0], x[1], x[2], g, g0, g1, g2);
+  Gather3 f = particle_force<FI>(x[0], x[1], x[2], g, g0, g1, g2);
+  if (fsum) {
+    float m[3];
+    residual_mean(fsum, inv_cells, m);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f.f[a] -= m[a];
+  }
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float d1 = p.dx1[a][i], d2 = p.dx2[a][i];
@@ -872,19 +944,32 @@ int launch_deposit_tiles(const CicGeom& g, const TileGeom& t, const BinBuffers& 
 }
 
 int launch_readout(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy, const float* fz,
-                   hipStream_t s)
+                   const double* fsum, const float* fi, hipStream_t s)
 {
   if (p.np == 0) return kOk;
-  hipLaunchKernelGGL(k_readout, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz);
+  const float inv_cells = (float)(1.0 / ((double)g.ng * g.ng * g.ng));
+  if (fi)
+    hipLaunchKernelGGL(k_readout<true>, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fi, fi, fi, fsum,
+                       inv_cells);
+  else
+    hipLaunchKernelGGL(k_readout<false>, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz, fsum,
+                       inv_cells);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
 
 int launch_readout_kick_drift(const DevParticles& p, const CicGeom& g, const float* fx, const float* fy,
-                              const float* fz, const StepParams* prm, hipStream_t s)
+                              const float* fz, const StepParams* prm, const double* fsum, const float* fi,
+                              hipStream_t s)
 {
   if (p.np == 0) return kOk;
-  hipLaunchKernelGGL(k_readout_kick_drift, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz, prm);
+  const float inv_cells = (float)(1.0 / ((double)g.ng * g.ng * g.ng));
+  if (fi)
+    hipLaunchKernelGGL(k_readout_kick_drift<true>, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fi, fi, fi, prm,
+                       fsum, inv_cells);
+  else
+    hipLaunchKernelGGL(k_readout_kick_drift<false>, dim3(blocks_for(p.np)), dim3(kThreads), 0, s, p, g, fx, fy, fz,
+                       prm, fsum, inv_cells);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

@@ -41,6 +41,8 @@ struct ResNet {
   int wpart_waves = 0;
   double* d_loss = nullptr;
   float* d_a = nullptr;      // scale factor of standalone evaluations
+  double* d_fsum = nullptr;  // sums of the residual the last corrected force added, per component
+  float* d_fi = nullptr;     // the last corrected force grids, interleaved [cell][3] (read by the readout)
   float out_scale = 1.0f;    // residual = out_scale * y
   bool scale_set = false;    // out_scale fitted to training targets (or set by the caller)
   float in_scale = 1.0f;     // feature = asinh(delta) * in_scale
@@ -83,6 +85,8 @@ int resnet_body(ResNet* r, const float* pa, hipStream_t s);
 // resnet_body in its two launches groups: layer 1 (from d_f1), layers 2-3
 int resnet_layer1(ResNet* r, const float* pa, hipStream_t s);
 int resnet_layers23(ResNet* r, hipStream_t s);
+// out[a] + out_scale * y[a] into d_fi[cell][a] (the force grids keep the
+// plain force; d_fsum gets the sums of the residual added)
 int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s);
 // The inference path with the first two layers fused (fuse12): h2 straight
 // from the real-space density grid (features, layer 1 and layer 2 in one

@@ -86,6 +86,8 @@ struct ConvOut {
   float* add[3] = {nullptr, nullptr, nullptr};
   long long row = 0;
   float scale = 0.0f;
+  double* sum = nullptr;  // kFwdAdd: the three components' sums of what was added
+  float* fi = nullptr;    // kFwdAdd: the corrected forces go here, interleaved [cell][3], instead of add[]
 };
 
 // 16-channel convolution with operand reuse.  A wave computes kConvTy output
@@ -103,9 +105,6 @@ struct ConvOut {
 //   chunk 13:                                  taps (+1,+1,0) | none
 // (left of | for lane groups 0 and 1, right for 2 and 3).
 constexpr int kConvTy = 8;
-// Chunk of the first layer's forward pack holding k_conv1's weights (the
-// 8-channel k_conv8 order uses chunks 0-8).
-constexpr int kConv1Chunk = 9;
 
 // A workgroup (4 waves) owns a column of blocks of 2 x 16 cells along z by
 // 2 kConvTy rows along y and marches along x through `march` consecutive x
@@ -261,6 +260,7 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
   const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
   const int zc = 1 + 16 * wz + r;          // brick z of the lane's cell
   const int zs = lo ? zc - 1 : zc + 1;     // its z neighbour for the dz = -1 | +1 taps
+  float added[3] = {0.0f, 0.0f, 0.0f};     // kFwdAdd: sums of the residual this lane added
   const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     const ConvUnit cu = conv_unit(u, n, march);
@@ -309,7 +309,14 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
           if (g == 0) {
             const long long at = line * o.row + cu.z0 + 16 * wz + r;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) o.add[i][at] = fold[t][i] + o.scale * v[i];
+            for (int i = 0; i < 3; ++i) {
+              const float dv = o.scale * v[i];
+              if (o.fi)
+                o.fi[(line * n + cu.z0 + 16 * wz + r) * 3 + i] = fold[t][i] + dv;
+              else
+                o.add[i][at] = fold[t][i] + dv;
+              added[i] += dv;
+            }
           }
         } else {
           const uint2 hp = *reinterpret_cast<const uint2*>(o.hprev + cell * 16 + 4 * g);
@@ -320,6 +327,15 @@ k_conv16(const unsigned short* __restrict__ in, const unsigned short* __restrict
                           m3 > 0.0f ? v[3] : 0.0f);
         }
       }
+    }
+  }
+  if (MODE == kFwdAdd && o.sum && g == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      float a = added[i];
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) a += __shfl_xor(a, off);
+      if (r == 0) atomicAdd(o.sum + i, (double)a);
     }
   }
 }
@@ -677,6 +693,7 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
                                       : o.f + (g < 3 ? g : 0) * nc;
   constexpr int L = kRollLoads;
   constexpr int F = MODE == kFwdAdd ? kConvTy : 0, S = kConvTy;
+  float added = 0.0f;  // kFwdAdd: this lane's sum of the residual it added (component g)
   const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     const ConvUnit cu = conv_unit(u, n, march);
@@ -721,10 +738,16 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
           const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
           if (g < 3) {
             const long long line = (long long)(p - 1) * n + ybase + t;
-            if (MODE == kFwdAdd)
-              gout[line * o.row + zcell] = fold[t] + o.scale * v;
-            else
+            if (MODE == kFwdAdd) {
+              const float dv = o.scale * v;
+              if (o.fi)
+                o.fi[(line * n + zcell) * 3 + g] = fold[t] + dv;  // lanes g = 0-2 of a cell: 12 contiguous bytes
+              else
+                gout[line * o.row + zcell] = fold[t] + dv;
+              added += dv;
+            } else {
               gout[line * n + zcell] = v;
+            }
           }
         }
       }
@@ -740,6 +763,11 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
     }
   }
   __builtin_amdgcn_s_waitcnt(0);
+  if (MODE == kFwdAdd && o.sum) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) added += __shfl_xor(added, off);  // within the 16-lane group
+    if (r == 0 && g < 3) atomicAdd(o.sum + g, (double)added);
+  }
 }
 
 // 8-channel convolution (the input features) with the operand reuse of
@@ -805,12 +833,37 @@ k_conv8(const unsigned short* __restrict__ in, const unsigned short* __restrict_
 // First layer from the feature channel alone.  The first layer's input is 8
 // channels: the feature, a (the same in every cell) and six zeros.  A
 // constant channel of a periodic convolution adds a * sum_t W[t][1][co] to
-// every cell, so it joins the bias, and the feature's 27 taps fill one K
-// chunk of 32 (taps 27-31 zero): one MFMA per tile instead of nine, and 2
-// bytes per input cell instead of 16.  Lane group g gathers taps 8g .. 8g+7
-// of its cell from a ring of bf16 feature planes (the march of k_conv16);
-// the planes stream through registers, the loads of plane x + 2 in flight
-// during plane x's products.  a comes from device memory (pa) when given.
+// every cell, so it joins the bias, and only the feature's 27 taps remain:
+// two K chunks per 16-cell tile instead of nine, 2 bytes per input cell
+// instead of 16.  The feature planes sit in LDS as triples (f(z - 1), f(z),
+// f(z + 1), -) per cell, so that one 8-byte read gives the three dz taps of
+// a (dx, dy) row: lane group g of chunk c holds tap rows 2s and 2s + 1,
+// s = 4c + g, of the nine rows R = 3 (dx + 1) + (dy + 1), each as its three
+// dz taps and a zero (the fourth element of a triple is masked off); rows
+// past 8 have zero weights and re-read row 8 or 4.  a comes from device
+// memory (pa) when given.
+constexpr int kConv1Tri = 10;  // the first of the two chunks in the layer's forward pack
+
+// tap row of operand slot k (0, 1: chunk 0; 2, 3: chunk 1) in lane group g
+__host__ __device__ inline int tri_row(int g, int k) { return k < 2 ? 2 * g + k : (g == 0 ? 8 : 4); }
+
+// feature f of ring cell c (rows of rowz cells, zz = c mod rowz) into its
+// own triple and those of its z neighbours in the row
+__device__ inline void put_triple(unsigned short* ring16, int c, int zz, int rowz, unsigned short f)
+{
+  ring16[4 * c + 1] = f;
+  if (zz > 0) ring16[4 * (c - 1) + 2] = f;
+  if (zz + 1 < rowz) ring16[4 * (c + 1)] = f;
+}
+
+// layer-1 products of a tile from the lane's four tap-row triples
+__device__ inline floatx4 conv1_tri(const uint2 (&t)[4], const uint4& w0, const uint4& w1)
+{
+  const uint4 op0 = make_uint4(t[0].x, t[0].y & 0xffffu, t[1].x, t[1].y & 0xffffu);
+  const uint4 op1 = make_uint4(t[2].x, t[2].y & 0xffffu, t[3].x, t[3].y & 0xffffu);
+  return mfma(op1, w1, mfma(op0, w0, floatx4{0.0f, 0.0f, 0.0f, 0.0f}));
+}
+
 // Bias of the first layer's output channels 4g..4g+3 with its constant a
 // channel folded in: bias + a * sum_t W[t][1][co], a and W rounded to bf16
 // as the 8-channel layer sees them (the products are exact in fp32).
@@ -828,27 +881,29 @@ __device__ inline void conv1_bias(const float* __restrict__ w, const float* __re
 
 constexpr int kConv1Waves = 3;  // waves per SIMD (= workgroups per CU) requested for k_conv1 (2: 1.5 ms slower at 768^3)
 
+// The march of k_conv16 over a ring of triple planes; the planes stream
+// through registers, the loads of plane x + 2 in flight during plane x's
+// products.
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kConv1Waves)))
 k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__ wpk, const float* __restrict__ w,
         const float* __restrict__ bias, const float* __restrict__ pa, float a_host, int n, int march,
         unsigned short* __restrict__ out_bf)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  const uint4 b = *reinterpret_cast<const uint4*>(wpk + ((long long)kConv1Chunk * 64 + lane) * 8);
+  const uint4 w0 = *reinterpret_cast<const uint4*>(wpk + ((long long)kConv1Tri * 64 + lane) * 8);
+  const uint4 w1 = *reinterpret_cast<const uint4*>(wpk + ((long long)(kConv1Tri + 1) * 64 + lane) * 8);
   float bch[4];
   conv1_bias(w, bias, pa ? *pa : a_host, g, bch);
-  __shared__ unsigned short ring[kRingSlots * kPlaneCells];
+  __shared__ uint2 ring[kRingSlots * kPlaneCells];  // [slot][yy][zz] triples
   const int wz = (threadIdx.x >> 6) & 1, wy = threadIdx.x >> 7;
   const int zc = 1 + 16 * wz + r;
-  // per tap slot j of this lane: plane offset dx, and the ring offset of (dy, dz)
-  int tdx[8], toff[8];
-  bool tval[8];
+  // per operand slot k of this lane: plane offset dx and ring offset of dy
+  int tdx[4], toff[4];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int t = 8 * g + j;
-    tval[j] = t < 27;
-    tdx[j] = tval[j] ? t / 9 - 1 : 0;
-    toff[j] = tval[j] ? ((t / 3) % 3 - 1) * kBrickZ + (t % 3 - 1) : 0;
+  for (int k = 0; k < 4; ++k) {
+    const int R = tri_row(g, k);
+    tdx[k] = R / 3 - 1;
+    toff[k] = (R % 3 - 1) * kBrickZ;
   }
   constexpr int kPer = (kPlaneCells + kThreads - 1) / kThreads;  // cells per thread and plane
   const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
@@ -866,10 +921,11 @@ k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__
       }
     };
     auto put = [&](int p, const unsigned short (&v)[kPer]) {
+      unsigned short* slot = reinterpret_cast<unsigned short*>(ring + ring_cell(p, 0, 0));
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
         const int c = threadIdx.x + k * kThreads;
-        if (c < kPlaneCells) ring[ring_cell(p, 0, 0) + c] = v[k];
+        if (c < kPlaneCells) put_triple(slot, c, c % kBrickZ, kBrickZ, v[k]);
       }
     };
     __syncthreads();  // the previous unit's reads are done
@@ -883,18 +939,16 @@ k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__
       const bool pre = x + 2 <= cu.x1;
       unsigned short nxt[kPer];
       if (pre) fetch(x + 2, nxt);
-      int base[8];
+      int base[4];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) base[j] = ring_cell(x + tdx[j], kConvTy * wy + 1, zc) + toff[j];
+      for (int k = 0; k < 4; ++k) base[k] = ring_cell(x + tdx[k], kConvTy * wy + 1, zc) + toff[k];
       floatx4 acc[kConvTy];
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) {
-        unsigned short v[8];
+        uint2 tr[4];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = tval[j] ? ring[base[j] + t * kBrickZ] : (unsigned short)0;
-        const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
-                                    v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
-        acc[t] = mfma(op, b, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+        for (int k = 0; k < 4; ++k) tr[k] = ring[base[k] + t * kBrickZ];
+        acc[t] = conv1_tri(tr, w0, w1);
       }
       if (pre) put(x + 2, nxt);  // slot of plane x - 2, free since the last barrier
       __syncthreads();
@@ -909,153 +963,12 @@ k_conv1(const unsigned short* __restrict__ f, const unsigned short* __restrict__
   }
 }
 
-// Features, layer 1 and layer 2 in one pass (the inference path): h2 from
-// the real-space density grid (rows of `row` floats) without writing the
-// feature grid or h1 to memory.  The march of k_conv16 with h1 made on the
-// chip: a unit keeps feature planes (bf16 asinh(delta) * scale on the block
-// with two halo cells per side, kFeatY x kFeatZ, 4 ring slots) and h1 planes
-// (the kBrickY x kBrickZ brick, 3 ring slots) in LDS.  At step x the density
-// of feature plane x + 3 is loaded into registers while layer 2 runs on h1
-// planes x - 1 .. x + 1; after a barrier h1 plane x + 2 takes the slot of
-// plane x - 1 (layer 1 as in k_conv1: one K chunk of the 27 feature taps per
-// 16-cell tile of the brick, a folded into the bias).  The arithmetic is
-// that of k_features, k_conv1 and k_conv16<kFwdRelu>, so h2 equals theirs
-// bit for bit.  64,512 bytes of LDS: two workgroups per CU.
+// The fused first pass's feature planes: the 16 x 32 block with two halo
+// cells per side (the first layer's halo on top of the second's).
 constexpr int kFeatY = kBlockY + 4, kFeatZ = kBlockZ + 4;
 constexpr int kFeatCells = kFeatY * kFeatZ;
 constexpr int kFeatSlots = 4;
-constexpr int kH1Slots = 3;
-constexpr int kH1Tiles = (kPlaneCells + 15) / 16;
 constexpr int kFeatPer = (kFeatCells + kThreads - 1) / kThreads;
-constexpr size_t kConv12Bytes =
-    sizeof(uint4) * 2 * kH1Slots * kPlaneCells + sizeof(unsigned short) * kFeatSlots * kFeatCells;
-
-__global__ void __launch_bounds__(kThreads)
-k_conv12(const float* __restrict__ delta, long long row, float in_scale, const unsigned short* __restrict__ wpk1,
-         const float* __restrict__ w1, const float* __restrict__ b1, const float* __restrict__ pa, float a_host,
-         const unsigned short* __restrict__ wpk2, const float* __restrict__ b2, int n, int march,
-         unsigned short* __restrict__ out_bf)
-{
-  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  const bool lo = g < 2;
-  const int wave = threadIdx.x >> 6;
-  uint4 b[14];
-#pragma unroll
-  for (int c = 0; c < 14; ++c) b[c] = *reinterpret_cast<const uint4*>(wpk2 + ((long long)c * 64 + lane) * 8);
-  const uint4 bw1 = *reinterpret_cast<const uint4*>(wpk1 + ((long long)kConv1Chunk * 64 + lane) * 8);
-  float bch1[4], bch2[4];
-  conv1_bias(w1, b1, pa ? *pa : a_host, g, bch1);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) bch2[i] = b2[4 * g + i];
-  extern __shared__ uint4 smem[];
-  uint4* h1 = smem;  // [slot][yy][zz][2 halves of 8 channels]
-  unsigned short* feat = reinterpret_cast<unsigned short*>(smem + 2 * kH1Slots * kPlaneCells);  // [slot][yy][zz]
-  const int half = g & 1;
-  const int wz = wave & 1, wy = wave >> 1;
-  const int zc = 1 + 16 * wz + r;
-  const int zs = lo ? zc - 1 : zc + 1;
-  // layer-1 taps t = 8g + j of this lane: plane offset dx and feature-brick offset of (dy, dz)
-  int tdx[8], toff[8];
-  bool tval[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int t = 8 * g + j;
-    tval[j] = t < 27;
-    tdx[j] = tval[j] ? t / 9 - 1 : 0;
-    toff[j] = tval[j] ? ((t / 3) % 3 - 1) * kFeatZ + (t % 3 - 1) : 0;
-  }
-  const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
-  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
-    const ConvUnit cu = conv_unit(u, n, march);
-    // density of feature plane q (periodic) into registers; its features into ring slot q & 3
-    auto fetch = [&](int q, float (&v)[kFeatPer]) {
-      const long long xoff = (long long)wrapi(q, n) * n;
-#pragma unroll
-      for (int k = 0; k < kFeatPer; ++k) {
-        const int c = threadIdx.x + k * kThreads;
-        if (c < kFeatCells) {
-          const int yy = c / kFeatZ, zz = c - yy * kFeatZ;
-          v[k] = delta[(xoff + wrapi(cu.y0 + yy - 2, n)) * row + wrapi(cu.z0 + zz - 2, n)];
-        }
-      }
-    };
-    auto put = [&](int q, const float (&v)[kFeatPer]) {
-      unsigned short* slot = feat + (q & (kFeatSlots - 1)) * kFeatCells;
-#pragma unroll
-      for (int k = 0; k < kFeatPer; ++k) {
-        const int c = threadIdx.x + k * kThreads;
-        if (c < kFeatCells) slot[c] = feature_of(v[k], in_scale);
-      }
-    };
-    // h1 plane p from feature planes p - 1 .. p + 1 into h1 ring slot p mod 3
-    auto layer1 = [&](int p) {
-      int fb[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) fb[j] = ((p + tdx[j]) & (kFeatSlots - 1)) * kFeatCells + toff[j];
-      uint2* dst = reinterpret_cast<uint2*>(h1 + 2 * ((p + kH1Slots) % kH1Slots) * kPlaneCells);
-      for (int tt = wave; tt < kH1Tiles; tt += kThreads / 64) {
-        const int cell = tt * 16 + r;
-        const int c = min(cell, kPlaneCells - 1);
-        const int yy = c / kBrickZ, zz = c - yy * kBrickZ;
-        const int centre = (yy + 1) * kFeatZ + zz + 1;
-        unsigned short v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = tval[j] ? feat[fb[j] + centre] : (unsigned short)0;
-        const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
-                                    v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
-        const floatx4 acc = mfma(op, bw1, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
-        if (cell < kPlaneCells)
-          dst[c * 4 + g] = pack_bf16x4(fmaxf(acc[0] + bch1[0], 0.0f), fmaxf(acc[1] + bch1[1], 0.0f),
-                                       fmaxf(acc[2] + bch1[2], 0.0f), fmaxf(acc[3] + bch1[3], 0.0f));
-      }
-    };
-    __syncthreads();  // the previous unit's reads are done
-    {
-      float v[kFeatPer];
-      for (int q = cu.x0 - 2; q <= cu.x0 + 1; ++q) {
-        fetch(q, v);
-        put(q, v);
-      }
-    }
-    __syncthreads();
-    layer1(cu.x0 - 1);
-    layer1(cu.x0);
-    __syncthreads();  // feature plane x0 - 2 is free
-    {
-      float v[kFeatPer];
-      fetch(cu.x0 + 2, v);
-      put(cu.x0 + 2, v);
-    }
-    __syncthreads();
-    layer1(cu.x0 + 1);
-    __syncthreads();
-    for (int x = cu.x0; x < cu.x1; ++x) {
-      const bool next = x + 1 < cu.x1;  // h1 plane x + 2 feeds output plane x + 1
-      float v[kFeatPer];
-      if (next) fetch(x + 3, v);
-      floatx4 acc[kConvTy];
-      int sb[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) sb[d] = ((x - 1 + d + kH1Slots) % kH1Slots) * kPlaneCells;
-      conv16_plane(h1, sb, wy, zc, zs, half, lo, b, acc);
-      // feature plane x + 3 takes the slot of plane x - 1, last read by layer 1
-      // of h1 plane x (before the previous barrier)
-      if (next) put(x + 3, v);
-#pragma unroll
-      for (int t = 0; t < kConvTy; ++t) {
-        const long long cell = ((long long)x * n + cu.y0 + kConvTy * wy + t) * n + cu.z0 + 16 * wz + r;
-        *reinterpret_cast<uint2*>(out_bf + cell * 16 + 4 * g) =
-            pack_bf16x4(fmaxf(acc[t][0] + bch2[0], 0.0f), fmaxf(acc[t][1] + bch2[1], 0.0f),
-                        fmaxf(acc[t][2] + bch2[2], 0.0f), fmaxf(acc[t][3] + bch2[3], 0.0f));
-      }
-      __syncthreads();  // layer-2 reads of h1 plane x - 1 done, feature plane x + 3 stored
-      if (next) {
-        layer1(x + 2);
-        __syncthreads();
-      }
-    }
-  }
-}
 
 // Features, layer 1 and layer 2 in one pass with layer 2 by input planes
 // (k_conv16r's order): h2 from the real-space density grid.  A unit keeps
@@ -1073,7 +986,7 @@ k_conv12(const float* __restrict__ delta, long long row, float in_scale, const u
 // k_conv1 and k_conv16r<kFwdRelu>: h2 equals theirs bit for bit.
 constexpr int kH1SlotCells = 4 * (kConvTy + 2) * 16;  // one tile per wave and row of the march step
 static_assert(kH1SlotCells >= kPlaneCells, "the h1 slot holds the brick");
-constexpr size_t kConv12rBytes = sizeof(uint4) * 2 * 2 * kH1SlotCells + sizeof(unsigned short) * kFeatSlots * kFeatCells;
+constexpr size_t kConv12rBytes = sizeof(uint4) * 2 * 2 * kH1SlotCells + sizeof(uint2) * kFeatSlots * kFeatCells;
 
 // roll_plane with one call of extra(j) per brick row, placed with the row's
 // products (the layer-1 tile of the fused pass).
@@ -1136,29 +1049,28 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
   uint4 b[kRollChunks];
 #pragma unroll
   for (int c = 0; c < kRollChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wr2 + ((long long)c * 64 + lane) * 8);
-  const uint4 bw1 = *reinterpret_cast<const uint4*>(wpk1 + ((long long)kConv1Chunk * 64 + lane) * 8);
+  const uint4 w10 = *reinterpret_cast<const uint4*>(wpk1 + ((long long)kConv1Tri * 64 + lane) * 8);
+  const uint4 w11 = *reinterpret_cast<const uint4*>(wpk1 + ((long long)(kConv1Tri + 1) * 64 + lane) * 8);
   float bch1[4], bch2[4];
   conv1_bias(w1, b1, pa ? *pa : a_host, g, bch1);
 #pragma unroll
   for (int i = 0; i < 4; ++i) bch2[i] = b2[4 * g + i];
   extern __shared__ uint4 smem[];
   uint4* h1 = smem;  // [2 slots][kH1SlotCells][2 halves of 8 channels]
-  unsigned short* feat = reinterpret_cast<unsigned short*>(smem + 2 * 2 * kH1SlotCells);  // [slot][yy][zz]
+  uint2* feat = reinterpret_cast<uint2*>(smem + 2 * 2 * kH1SlotCells);  // [slot][yy][zz] triples
   const int half = g & 1;
   const int wz = wave & 1, wy = wave >> 1;
   const int zc = 1 + 16 * wz + r;
   const int zs = lo ? zc - 1 : zc + 1;
   const int rowoff = (kConvTy * wy * kBrickZ + zc) * 2 + half;
   const int shoff = (kConvTy * wy * kBrickZ + zs) * 2 + half;
-  // layer-1 taps t = 8g + j of this lane: plane offset dx and feature-brick offset of (dy, dz)
-  int tdx[8], toff[8];
-  bool tval[8];
+  // layer-1 tap rows of this lane's operand slots: plane offset dx and feature-row offset of dy
+  int tdx[4], toff[4];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int t = 8 * g + j;
-    tval[j] = t < 27;
-    tdx[j] = tval[j] ? t / 9 - 1 : 0;
-    toff[j] = tval[j] ? ((t / 3) % 3 - 1) * kFeatZ + (t % 3 - 1) : 0;
+  for (int k = 0; k < 4; ++k) {
+    const int R = tri_row(g, k);
+    tdx[k] = R / 3 - 1;
+    toff[k] = (R % 3 - 1) * kFeatZ;
   }
   floatx4 acc[3][kConvTy];
   const long long units = (long long)((n + march - 1) / march) * (n / kBlockY) * (n / kBlockZ);
@@ -1177,11 +1089,11 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
       }
     };
     auto put = [&](int q, const float (&v)[kFeatPer]) {
-      unsigned short* slot = feat + (q & (kFeatSlots - 1)) * kFeatCells;
+      unsigned short* slot = reinterpret_cast<unsigned short*>(feat + (q & (kFeatSlots - 1)) * kFeatCells);
 #pragma unroll
       for (int k = 0; k < kFeatPer; ++k) {
         const int c = threadIdx.x + k * kThreads;
-        if (c < kFeatCells) slot[c] = feature_of(v[k], in_scale);
+        if (c < kFeatCells) put_triple(slot, c, c % kFeatZ, kFeatZ, feature_of(v[k], in_scale));
       }
     };
     // tile tt (0 .. 39) of h1 plane p from feature planes p - 1 .. p + 1
@@ -1191,15 +1103,10 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
       const int c = min(cell, kPlaneCells - 1);
       const int yy = c / kBrickZ, zz = c - yy * kBrickZ;
       const int centre = (yy + 1) * kFeatZ + zz + 1;
-      unsigned short v[8];
+      uint2 tr[4];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int fb = ((p + tdx[j]) & (kFeatSlots - 1)) * kFeatCells + toff[j];
-        v[j] = tval[j] ? feat[fb + centre] : (unsigned short)0;
-      }
-      const uint4 op = make_uint4(v[0] | (unsigned)v[1] << 16, v[2] | (unsigned)v[3] << 16,
-                                  v[4] | (unsigned)v[5] << 16, v[6] | (unsigned)v[7] << 16);
-      const floatx4 a1 = mfma(op, bw1, floatx4{0.0f, 0.0f, 0.0f, 0.0f});
+      for (int k = 0; k < 4; ++k) tr[k] = feat[((p + tdx[k]) & (kFeatSlots - 1)) * kFeatCells + toff[k] + centre];
+      const floatx4 a1 = conv1_tri(tr, w10, w11);
       uint2* dst = reinterpret_cast<uint2*>(h1 + 2 * (p & 1) * kH1SlotCells);
       dst[cell * 4 + g] = pack_bf16x4(fmaxf(a1[0] + bch1[0], 0.0f), fmaxf(a1[1] + bch1[1], 0.0f),
                                       fmaxf(a1[2] + bch1[2], 0.0f), fmaxf(a1[3] + bch1[3], 0.0f));
@@ -1343,11 +1250,14 @@ __global__ void k_pack(const float* __restrict__ w, int cin, int cout, unsigned 
         const float v = (g < 3 && col < cout) ? w[((long long)tap * 8 + j) * cout + col] : 0.0f;
         wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
       }
-    // k_conv1: the feature channel's taps t = 8g + j in one chunk
-    if (c == kConv1Chunk)
+    // k_conv1 / k_conv12r: the feature channel's taps by tap-row triples in
+    // two chunks (element j of lane group g: operand slot j / 4, dz j % 4)
+    if (c == kConv1Tri || c == kConv1Tri + 1)
       for (int j = 0; j < 8; ++j) {
-        const int tap = 8 * g + j;
-        const float v = (tap < 27 && col < cout) ? w[((long long)tap * 8) * cout + col] : 0.0f;
+        const int k = 2 * (c - kConv1Tri) + (j >> 2), dzi = j & 3;
+        const bool used = dzi < 3 && (k < 2 || (g == 0 && k == 2));  // rows 0-7, then row 8 once
+        const int tap = 3 * tri_row(g, k) + dzi;
+        const float v = (used && col < cout) ? w[((long long)tap * 8) * cout + col] : 0.0f;
         wf[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
       }
   } else {
@@ -1468,6 +1378,11 @@ k_loss(const float* __restrict__ y, const unsigned short* __restrict__ tb, const
 __global__ void k_zero1(double* p)
 {
   if (threadIdx.x == 0) *p = 0.0;
+}
+
+__global__ void k_zero3(double* p)
+{
+  if (threadIdx.x < 3) p[threadIdx.x] = 0.0;
 }
 
 __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
@@ -1593,6 +1508,8 @@ int resnet_init(ResNet* r, int n, unsigned seed)
   for (int l = 0; l < 3; ++l) COLA_TRY(dev_alloc(&r->d_h[l], nc * 16));
   COLA_TRY(dev_alloc(&r->d_y, 3 * nc));
   COLA_TRY(dev_alloc(&r->d_loss, 1));
+  COLA_TRY(dev_alloc(&r->d_fsum, 3));
+  COLA_TRY(dev_alloc(&r->d_fi, 3 * nc));
   COLA_TRY(dev_alloc(&r->d_a, 1));
   r->wpart_waves = kWgradBlocks * (kThreads / 64);
   // He-normal initial weights (std sqrt(2 / fan_in)), zero biases and moments
@@ -1616,7 +1533,7 @@ void resnet_free(ResNet* r)
   };
   fr(r->d_param), fr(r->d_m), fr(r->d_v), fr(r->d_grad);
   for (int l = 0; l < kNetLayers; ++l) fr(r->d_wf[l]), fr(r->d_wb[l]), fr(r->d_wr[l]);
-  fr(r->d_x), fr(r->d_f1), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a);
+  fr(r->d_x), fr(r->d_f1), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a), fr(r->d_fsum), fr(r->d_fi);
   for (int l = 0; l < 3; ++l) fr(r->d_h[l]);
   for (int l = 0; l < 2; ++l) fr(r->d_g[l]);
   for (auto p : r->s_feat) fr(p);
@@ -1840,10 +1757,10 @@ int resnet_layers12(ResNet* r, const float* delta, long long row, const float* p
     COLA_HIP(hipGetLastError());
     return kOk;
   }
-  const ConvLaunch cl = conv_launch(r->n);
-  hipLaunchKernelGGL(k_conv12, dim3(cl.blocks), dim3(kThreads), kConv12Bytes, s, delta, row, r->in_scale, r->d_wf[0],
-                     r->d_param + param_offset(0), r->d_param + param_offset(0) + 27 * 8 * 16, pa, 0.0f, r->d_wf[1],
-                     r->d_param + param_offset(1) + 27 * 16 * 16, r->n, cl.march, r->d_h[1]);
+  // by output planes there is no fused pass: features, layer 1, layer 2
+  COLA_TRY(resnet_features(r, delta, row, s));
+  COLA_TRY(conv_layer_1(r, r->d_f1, pa, 0.0f, s));
+  conv_hidden(r, 1, s);
   COLA_HIP(hipGetLastError());
   return kOk;
 }
@@ -1862,6 +1779,9 @@ int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s
   for (int a = 0; a < 3; ++a) o.add[a] = out[a];
   o.row = row;
   o.scale = r->out_scale;
+  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, r->d_fsum);
+  o.sum = r->d_fsum;
+  o.fi = r->d_fi;
   return conv_layer_4(r, kFwdAdd, o, s);
 }
 

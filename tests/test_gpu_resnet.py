@@ -211,6 +211,14 @@ def test_pairs_training_and_corrected_force(eh_table):
         y = net.forward(x)
         corr = (g_net.astype(np.float64) - g_plain) / out_scale
         assert np.abs(corr - y).max() <= 1e-2 * np.abs(y).max(), np.abs(corr - y).max() / np.abs(y).max()
+        # the particles read the corrected grids (the oracle's CIC readout)
+        # less the correction's mean over the grid: a periodic force has none
+        parts = coarse.get_particles()
+        for ax in range(3):
+            mean = (g_net[..., ax].astype(np.float64) - g_plain[..., ax]).mean()
+            want = orc.pm_readout(64, box, parts, g_net[..., ax]).astype(np.float64) - mean
+            scale = np.abs(want).max()
+            assert np.abs(f_net[:, ax] - want).max() <= 1e-5 * scale, (ax, mean, np.abs(f_net[:, ax] - want).max())
         coarse.set_residual_net(None)
         coarse.pm_calculate_forces()
         assert np.array_equal(f_plain, coarse.get_forces())
