@@ -76,6 +76,8 @@ _SIGNATURES = {
     "cola_last_mass_sum": (C.c_double, [C.c_void_p]),
     "cola_read_parameters": (C.c_int, [C.c_char_p, C.c_void_p]),
     "cola_confirm_parameters": (C.c_int, [C.c_void_p]),
+    "cola_read_gadget": (C.c_int, [C.c_char_p, C.c_void_p, C.c_longlong, C.c_void_p]),
+    "cola_set_snapshot_records": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "cola_write_gadget": (C.c_int, [C.c_char_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float, C.c_float,
                                     C.c_float, C.c_int]),
     "cola_write_coarse_grid": (C.c_int, [C.c_char_p, C.c_float, C.c_int, C.c_void_p]),
@@ -102,6 +104,7 @@ _SIGNATURES = {
     "cola_resnet_set_restriction": (C.c_int, [C.c_void_p, C.c_float]),
     "cola_resnet_set_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_set_plane_order": (C.c_int, [C.c_void_p, C.c_int]),
+    "cola_resnet_set_head_fusion": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_resnet_gradient_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                             C.c_void_p]),
     "cola_resnet_adam_step": (C.c_int, [C.c_void_p, C.c_float]),
@@ -116,6 +119,11 @@ _SIGNATURES = {
     "cola_resnet_load": (C.c_int, [C.c_void_p, C.c_char_p]),
     "cola_set_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_step_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_float, C.c_void_p, C.c_void_p]),
+    "cola_ensemble_create": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "cola_ensemble_step": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]),
+    "cola_ensemble_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "cola_ensemble_launches": (C.c_int, [C.c_void_p]),
+    "cola_ensemble_destroy": (C.c_int, [C.c_void_p]),
     "cola_set_fft_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "cola_set_fft_tiles_xy": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cola_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
@@ -307,6 +315,15 @@ class Subsampler:
         self.close()
 
 
+class GadgetInfo(C.Structure):
+    """cola_gadget_info."""
+    _fields_ = [("nfiles", C.c_int), ("np_read", C.c_longlong), ("np_total", C.c_longlong), ("a", C.c_float),
+                ("boxsize", C.c_float), ("omega_m", C.c_float), ("h", C.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Tuning(C.Structure):
     """cola_tuning: the launch shapes the autotuner chose."""
     _fields_ = [("tile", C.c_int * 3), ("fft_log2_cols", C.c_int), ("fft_log2_rows", C.c_int),
@@ -398,6 +415,12 @@ class ResidualNet:
         1-2 in one pass (True, the default) or three launches (False); the
         forces are the same bit for bit."""
         _check(lib().cola_resnet_set_fusion(self._h, int(bool(on))))
+
+    def set_head_fusion(self, on):
+        """Layer 3 and the last layer in one launch after the force
+        transforms (True) or two launches through h3 (False, the default);
+        the forces are the same bit for bit."""
+        _check(lib().cola_resnet_set_head_fusion(self._h, int(bool(on))))
 
     def set_plane_order(self, by_input_planes):
         """Loop order of the hidden 16 -> 16 layers: by input planes (True,
@@ -515,6 +538,18 @@ def write_gadget(filename, snap, a, boxsize, omega_m, h, long_id=False):
     snap = np.ascontiguousarray(snap, PARTICLE_MIN_DTYPE)
     _check(lib().cola_write_gadget(os.fsencode(filename), snap.ctypes.data, len(snap), a, boxsize, omega_m, h,
                                    1 if long_id else 0))
+
+
+def read_gadget(filename):
+    """GADGET-1 snapshot (one file or filename.0, .1, ...) as read.c reads
+    it: (records, info dict); records is None when no such file exists."""
+    info = GadgetInfo()
+    _check(lib().cola_read_gadget(os.fsencode(filename), None, 0, C.byref(info)))
+    if info.nfiles == 0:
+        return None, info.as_dict()
+    out = np.zeros(info.np_read, PARTICLE_MIN_DTYPE)
+    _check(lib().cola_read_gadget(os.fsencode(filename), out.ctypes.data, len(out), C.byref(info)))
+    return out, info.as_dict()
 
 
 def write_coarse_grid(filename, boxsize, delta):
@@ -726,6 +761,11 @@ class Context:
     def copy_particles_from(self, other):
         _check(lib().cola_copy_particles(self._h, other._h))
 
+    def set_snapshot_records(self, records):
+        """Snapshot records (e.g. read_gadget's) for the output functions."""
+        records = np.ascontiguousarray(records, PARTICLE_MIN_DTYPE)
+        _check(lib().cola_set_snapshot_records(self._h, records.ctypes.data, len(records)))
+
     def set_residual_net(self, net):
         _check(lib().cola_set_residual_net(self._h, net._h if net is not None else None))
 
@@ -780,10 +820,58 @@ def step_batch(ctxs, omega_m, avel1, apos1):
     _check(lib().cola_step_batch(hs, n, omega_m, a1.ctypes.data, p1.ctypes.data))
 
 
-def run_cola_batch(ctxs, omega_m, a_final, nsteps, seeds, a_out_final=True):
+class Ensemble:
+    """Contexts stepped together with one kernel launch per phase over the
+    workgroups of every realization (cola_ensemble_*).  Close it before its
+    contexts."""
+
+    def __init__(self, ctxs, graph=False):
+        self.ctxs = list(ctxs)
+        n = len(self.ctxs)
+        hs = (C.c_void_p * n)(*[c._h for c in self.ctxs])
+        h = C.c_void_p()
+        _check(lib().cola_ensemble_create(hs, n, C.byref(h)))
+        self._h = h.value
+        if graph:
+            self.set_graph(True)
+
+    def step(self, omega_m, avel1, apos1):
+        n = len(self.ctxs)
+        a1 = np.ascontiguousarray(np.broadcast_to(np.asarray(avel1, np.float32), (n,)))
+        p1 = np.ascontiguousarray(np.broadcast_to(np.asarray(apos1, np.float32), (n,)))
+        _check(lib().cola_ensemble_step(self._h, omega_m, a1.ctypes.data, p1.ctypes.data))
+
+    def set_graph(self, on):
+        _check(lib().cola_ensemble_set_graph(self._h, 1 if on else 0))
+
+    def launches(self):
+        """Kernel launches per ensemble step (0 before the first step)."""
+        return int(lib().cola_ensemble_launches(self._h))
+
+    def close(self):
+        if self._h:
+            _check(lib().cola_ensemble_destroy(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_cola_batch(ctxs, omega_m, a_final, nsteps, seeds, a_out_final=True, ensemble=None):
     """Realizations of the same schedule (main.c:117-207, one output at
     a_final) on several contexts, their steps interleaved so the GPU overlaps
-    them.  Returns one snapshot per context."""
+    them (ensemble=None: each context's step on its own stream), or run as
+    one ensemble (ensemble=Ensemble over ctxs: one launch per phase).
+    Returns one snapshot per context."""
     da = a_final / nsteps
     a_init = da
     Om = float(np.float32(omega_m))
@@ -799,7 +887,10 @@ def run_cola_batch(ctxs, omega_m, a_final, nsteps, seeds, a_out_final=True):
         snap_now = avel0 <= aout <= apos0
         snap_mid = (not snap_now) and apos0 < aout <= avel1
         if not (snap_now or snap_mid):
-            step_batch(ctxs, Om, [avel1] * len(ctxs), [apos1] * len(ctxs))
+            if ensemble is not None:
+                ensemble.step(Om, avel1, apos1)
+            else:
+                step_batch(ctxs, Om, [avel1] * len(ctxs), [apos1] * len(ctxs))
             continue
         for ctx in ctxs:
             ctx.move_particles()

@@ -60,13 +60,14 @@ enum Phase : int {
   kPhNetHidden,
   kPhNetHead,
   kPhNetConv12,
+  kPhNetConv34,
   kNumPhases
 };
 const char* const kPhaseNames[kNumPhases] = {
     "wrap",      "bin_count",  "bin_scan",   "bin_scatter", "deposit",   "fft_z_r2c",     "fft_y_fwd",
     "fft_x_fwd", "fft_x_inv",  "fft_y_inv",  "fft_z_c2r",   "fft_x_fused", "readout",   "readout_kick_drift", "kick",
     "drift",     "snapshot",   "lpt_modes",  "lpt_fft",     "lpt_other", "net_features", "net_conv1",
-    "net_conv_hidden", "net_conv_out_add", "net_conv12"};
+    "net_conv_hidden", "net_conv_out_add", "net_conv12", "net_conv34"};
 
 struct EventPair {
   int phase;
@@ -108,6 +109,9 @@ struct cola_context {
   ResNet* net = nullptr;  // learned force residual added to the force grids (not owned)
   cola_resnet* net_handle = nullptr;  // the handle `net` belongs to
   hipGraphExec_t step_graph = nullptr;  // captured fused step
+  // bumped whenever buffers, tiles or plans a captured step or an ensemble's
+  // tables refer to change
+  unsigned long long config_gen = 0;
   int mass_slot = 0;     // next slot
   int mass_pending = 0;  // slots written since the last check
   double last_mass = 0;
@@ -319,21 +323,31 @@ int pm_assign(cola_context* c, bool wrap_first)
 
 // pm.c:282-355: r2c of the density, then per axis the Poisson gradient and
 // the c2r into the three force grids.
-int pm_fields(cola_context* c)
+KOpParams pm_kop(const cola_context* c)
 {
   const float scale = 2.0 * kPiD / c->boxf;  // pm.c:315
   KOpParams kp;
   kp.f1 = -1.0 / std::pow((double)c->ng, 3.0) / scale;  // pm.c:317
+  return kp;
+}
+
+int pm_fields(cola_context* c)
+{
+  const KOpParams kp = pm_kop(c);
   float* const out[3] = {c->fgrid[0], c->fgrid[1], c->fgrid[2]};
   // the net's features come from the real-space density before the r2c
   // overwrites it; its last layer adds its output to the force grids
+  // with fuse34 (by input planes) layer 3 runs with the last layer after the transforms
+  const bool fused34 = c->net && c->net->fuse12 && c->net->roll && c->net->fuse34;
   if (c->net && c->net->fuse12) {
     phase_begin(c, kPhNetConv12);
     COLA_TRY(resnet_layers12(c->net, c->density, c->cic.row, &c->d_prm->a, c->stream));
     phase_end(c, kPhNetConv12);
-    phase_begin(c, kPhNetHidden);
-    COLA_TRY(resnet_layer3(c->net, c->stream));
-    phase_end(c, kPhNetHidden);
+    if (!fused34) {
+      phase_begin(c, kPhNetHidden);
+      COLA_TRY(resnet_layer3(c->net, c->stream));
+      phase_end(c, kPhNetHidden);
+    }
   } else if (c->net) {
     phase_begin(c, kPhNetFeatures);
     COLA_TRY(resnet_features(c->net, c->density, c->cic.row, c->stream));
@@ -347,7 +361,11 @@ int pm_fields(cola_context* c)
   }
   c->density_is_real = false;
   COLA_TRY(c->pm_fft.forces(c->density, out, kp, c->stream));
-  if (c->net) {
+  if (fused34) {
+    phase_begin(c, kPhNetConv34);
+    COLA_TRY(resnet_layer3_head(c->net, out, c->cic.row, c->stream));
+    phase_end(c, kPhNetConv34);
+  } else if (c->net) {
     phase_begin(c, kPhNetHead);
     COLA_TRY(resnet_head_add(c->net, out, c->cic.row, c->stream));
     phase_end(c, kPhNetHead);
@@ -856,10 +874,18 @@ int step_kernels(cola_context* c)
   return kOk;
 }
 
-void drop_step_graph(cola_context* c)
+void destroy_step_graph(cola_context* c)
 {
   if (c->step_graph) (void)hipGraphExecDestroy(c->step_graph);
   c->step_graph = nullptr;
+}
+
+// The context's buffers, tiles or plans change: captured steps and ensemble
+// tables built on them are stale.
+void drop_step_graph(cola_context* c)
+{
+  destroy_step_graph(c);
+  c->config_gen += 1;
 }
 
 // Captures step_kernels once; every later step replays it after sending its
@@ -926,7 +952,239 @@ int cola_set_step_graph(cola_context* c, int on)
   if (!valid(c)) return kErrArg;
   COLA_HIP(hipStreamSynchronize(c->stream));
   c->graph_mode = on != 0;
-  if (!c->graph_mode) drop_step_graph(c);
+  if (!c->graph_mode) destroy_step_graph(c);
+  return kOk;
+}
+
+}  // extern "C"
+
+// The steps of many contexts with one launch per kernel (batched tables of
+// every member's buffers, tiles and plans).
+struct cola_ensemble {
+  std::vector<cola_context*> ctx;
+  std::vector<unsigned long long> gen;  // each member's config_gen the tables were built for
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<hipEvent_t> ready;  // per member: its queued work before a step
+  PmBatch pm;
+  FftForcesBatch fft;
+  bool built = false;
+  static constexpr int kSlots = 64;
+  StepParams* h_prm = nullptr;  // pinned ring of kSlots entries of n values
+  StepParams* d_prm = nullptr;  // the values of the step in flight, one per member
+  int ring = 0;
+  int since_check = 0;  // steps since the members' mass sums were checked
+  bool graph_mode = false;
+  bool warmed = false;  // the kernels ran eagerly since the tables were built
+  hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+void drop_ensemble_graph(cola_ensemble* e)
+{
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  e->graph = nullptr;
+}
+
+// (Re)builds the batched tables from the members' current buffers and plans.
+int ensemble_build(cola_ensemble* e)
+{
+  COLA_HIP(hipStreamSynchronize(e->stream));
+  drop_ensemble_graph(e);
+  e->pm.destroy();
+  e->fft.destroy();
+  e->built = false;
+  const int n = (int)e->ctx.size();
+  std::vector<PmMember> ms(n);
+  std::vector<const FftPlan*> plans(n);
+  std::vector<float*> dens(n), outs(3 * n);
+  std::vector<KOpParams> kps(n);
+  for (int m = 0; m < n; ++m) {
+    cola_context* c = e->ctx[m];
+    COLA_TRY(require_particles(c));
+    if (c->net) {
+      set_error("ensemble member " + std::to_string(m) + " has a residual net attached");
+      return kErrState;
+    }
+    if (!c->pm_fft.can_fuse()) {
+      set_error("ensemble member " + std::to_string(m) + ": its PM transform's x pass does not fuse");
+      return kErrUnsupported;
+    }
+    COLA_TRY(alloc_grids(c));
+    PmMember& M = ms[m];
+    M.p = c->p;
+    M.g = c->cic;
+    M.t = c->tiles;
+    M.b = c->bins;
+    M.density = c->density;
+    for (int a = 0; a < 3; ++a) M.f[a] = c->fgrid[a];
+    M.ring = c->d_mass;
+    M.prm = e->d_prm + m;
+    M.boxsize = c->boxf;
+    plans[m] = &c->pm_fft;
+    dens[m] = c->density;
+    for (int a = 0; a < 3; ++a) outs[3 * m + a] = c->fgrid[a];
+    kps[m] = pm_kop(c);
+    e->gen[m] = c->config_gen;
+  }
+  COLA_TRY(e->pm.init(ms.data(), n));
+  COLA_TRY(e->fft.init(plans.data(), dens.data(), outs.data(), kps.data(), n));
+  e->built = true;
+  e->warmed = false;
+  return kOk;
+}
+
+int ensemble_kernels(cola_ensemble* e)
+{
+  COLA_TRY(launch_assign_batch(e->pm, e->stream));
+  COLA_TRY(e->fft.run(e->stream));
+  COLA_TRY(launch_readout_kick_drift_batch(e->pm, e->stream));
+  return kOk;
+}
+
+int capture_ensemble_graph(cola_ensemble* e)
+{
+  COLA_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  const int st = ensemble_kernels(e);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+  if (st != kOk || ec != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (st == kOk) set_error(std::string("ensemble capture: ") + hipGetErrorString(ec));
+    return st != kOk ? st : kErrHip;
+  }
+  const hipError_t ei = hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    e->graph = nullptr;
+    set_error(std::string("ensemble graph instantiate: ") + hipGetErrorString(ei));
+    return kErrHip;
+  }
+  return kOk;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cola_ensemble_create(cola_context* const* ctxs, int n, cola_ensemble** out)
+{
+  if (!out || n <= 0 || !ctxs) return kErrArg;
+  *out = nullptr;
+  for (int m = 0; m < n; ++m)
+    if (!valid(ctxs[m])) return kErrArg;
+  cola_ensemble* e = new cola_ensemble;
+  auto fail = [&](int st) {
+    cola_ensemble_destroy(e);
+    return st;
+  };
+  e->ctx.assign(ctxs, ctxs + n);
+  e->gen.assign(n, 0);
+  if (hipStreamCreate(&e->stream) != hipSuccess) return fail(kErrHip);
+  if (hipEventCreateWithFlags(&e->done, hipEventDisableTiming) != hipSuccess) return fail(kErrHip);
+  e->ready.assign(n, nullptr);
+  for (int m = 0; m < n; ++m)
+    if (hipEventCreateWithFlags(&e->ready[m], hipEventDisableTiming) != hipSuccess) return fail(kErrHip);
+  if (hipHostMalloc(&e->h_prm, sizeof(StepParams) * cola_ensemble::kSlots * n) != hipSuccess) return fail(kErrAlloc);
+  std::memset(e->h_prm, 0, sizeof(StepParams) * cola_ensemble::kSlots * n);
+  if (hipMalloc(&e->d_prm, sizeof(StepParams) * n) != hipSuccess) return fail(kErrAlloc);
+  if (hipMemset(e->d_prm, 0, sizeof(StepParams) * n) != hipSuccess) return fail(kErrHip);
+  *out = e;
+  return kOk;
+}
+
+int cola_ensemble_step(cola_ensemble* e, float omega_m, const float* avel1, const float* apos1)
+{
+  if (!e || !avel1 || !apos1) return kErrArg;
+  const int n = (int)e->ctx.size();
+  bool stale = !e->built;
+  for (int m = 0; m < n; ++m) stale = stale || e->ctx[m]->config_gen != e->gen[m] || e->ctx[m]->net;
+  if (stale) COLA_TRY(ensemble_build(e));
+  // after the work queued on the members
+  for (int m = 0; m < n; ++m) {
+    COLA_HIP(hipEventRecord(e->ready[m], e->ctx[m]->stream));
+    COLA_HIP(hipStreamWaitEvent(e->stream, e->ready[m], 0));
+  }
+  // every member's step values (cola_step), in the pinned ring entry of this
+  // step: the ring holds 64 steps and the members are drained every 32
+  StepParams* hp = e->h_prm + (size_t)e->ring * n;
+  for (int m = 0; m < n; ++m) {
+    cola_context* c = e->ctx[m];
+    if (c->mass_pending >= cola_context::kMassSlots - 2) COLA_TRY(check_pending(c));
+    const KickFactors k = kick_factors(omega_m, c->a_x, c->a_v, avel1[m]);
+    const DriftFactors d = drift_factors(omega_m, c->a_x, avel1[m], apos1[m]);
+    hp[m].slot = c->mass_slot;
+    hp[m].a = c->a_x;
+    hp[m].k = k;
+    hp[m].d = d;
+    c->mass_slot = (c->mass_slot + 1) % cola_context::kMassSlots;
+    c->mass_pending += 1;
+  }
+  e->ring = (e->ring + 1) % cola_ensemble::kSlots;
+  COLA_HIP(hipMemcpyAsync(e->d_prm, hp, sizeof(StepParams) * n, hipMemcpyHostToDevice, e->stream));
+  // the first step after a (re)build runs eagerly (one-time kernel set-up
+  // stays out of the capture); later ones replay the captured graph
+  if (e->graph_mode && !e->graph && e->warmed) COLA_TRY(capture_ensemble_graph(e));
+  if (e->graph_mode && e->graph) {
+    COLA_HIP(hipGraphLaunch(e->graph, e->stream));
+  } else {
+    COLA_TRY(ensemble_kernels(e));
+    e->warmed = true;
+  }
+  // the members' later work after this step
+  COLA_HIP(hipEventRecord(e->done, e->stream));
+  for (int m = 0; m < n; ++m) {
+    cola_context* c = e->ctx[m];
+    COLA_HIP(hipStreamWaitEvent(c->stream, e->done, 0));
+    c->density_is_real = false;
+    c->a_v = avel1[m];
+    c->a_x = apos1[m];
+    c->have_force = false;
+    c->fi_force = false;
+  }
+  if (++e->since_check >= cola_ensemble::kSlots / 2) {
+    e->since_check = 0;
+    int st = kOk;
+    for (int m = 0; m < n; ++m) {
+      const int r = check_pending(e->ctx[m]);
+      if (st == kOk) st = r;
+    }
+    return st;
+  }
+  return kOk;
+}
+
+int cola_ensemble_set_graph(cola_ensemble* e, int on)
+{
+  if (!e) return kErrArg;
+  COLA_HIP(hipStreamSynchronize(e->stream));
+  e->graph_mode = on != 0;
+  if (!e->graph_mode) drop_ensemble_graph(e);
+  return kOk;
+}
+
+int cola_ensemble_launches(const cola_ensemble* e)
+{
+  if (!e || !e->built) return 0;
+  // mass slot, count, scan (3), scatter, deposit, sums; the transform passes; readout
+  return 8 + (int)e->fft.stages.size() + 1;
+}
+
+int cola_ensemble_destroy(cola_ensemble* e)
+{
+  if (!e) return kOk;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_ensemble_graph(e);
+  e->pm.destroy();
+  e->fft.destroy();
+  for (hipEvent_t ev : e->ready)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->done) (void)hipEventDestroy(e->done);
+  if (e->h_prm) (void)hipHostFree(e->h_prm);
+  if (e->d_prm) (void)hipFree(e->d_prm);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
   return kOk;
 }
 
@@ -1321,7 +1579,11 @@ int cola_autotune(cola_context* c, int reps, cola_tuning* out)
   COLA_TRY(require_particles(c));
   const bool prof = c->profiling;
   c->profiling = false;
+  // the shapes tuned here are the PM's; the net's kernels have fixed shapes
+  // and would only add their time (and its noise) to every trial
   COLA_TRY(launch_wrap(c->p, c->boxf, c->stream));  // move.c:33-41, as every step begins
+  ResNet* const net = c->net;
+  c->net = nullptr;  // restored below on every path
   cola_tuning best;
   best.tile[0] = c->tiles.tx;
   best.tile[1] = c->tiles.ty;
@@ -1394,6 +1656,7 @@ int cola_autotune(cola_context* c, int reps, cola_tuning* out)
     (void)cola_fft_radices(c->ng, best.fft_schedule, best.fft_radices, 8);
   }
   c->profiling = prof;
+  c->net = net;
   *out = best;
   return st;
 }
@@ -1659,6 +1922,22 @@ int cola_get_snapshot(cola_context* c, cola_particle_min* out, size_t np_out)
   }
   COLA_HIP(hipMemcpyAsync(out, c->d_snap, sizeof(DevParticleMin) * c->np, hipMemcpyDeviceToHost, c->stream));
   COLA_HIP(hipStreamSynchronize(c->stream));
+  return kOk;
+}
+
+int cola_set_snapshot_records(cola_context* c, const cola_particle_min* in, size_t np_in)
+{
+  if (!valid(c) || !in) return kErrArg;
+  if (np_in != (size_t)c->np) {
+    set_error("snapshot of " + std::to_string(np_in) + " records for a context of " + std::to_string(c->np) +
+              " particles");
+    return kErrArg;
+  }
+  if (!c->d_snap) COLA_HIP(hipMalloc(&c->d_snap, sizeof(DevParticleMin) * c->np));
+  static_assert(sizeof(DevParticleMin) == sizeof(cola_particle_min), "snapshot record layout");
+  COLA_HIP(hipMemcpyAsync(c->d_snap, in, sizeof(DevParticleMin) * c->np, hipMemcpyHostToDevice, c->stream));
+  COLA_HIP(hipStreamSynchronize(c->stream));
+  c->have_snapshot = true;
   return kOk;
 }
 
@@ -2039,6 +2318,17 @@ int cola_resnet_set_fusion(cola_resnet* h, int on)
     drop_step_graph(h->attached);
   }
   h->r.fuse12 = on != 0;
+  return kOk;
+}
+
+int cola_resnet_set_head_fusion(cola_resnet* h, int on)
+{
+  if (!h || (on != 0 && on != 1)) return kErrArg;
+  if (h->attached) {
+    COLA_HIP(hipStreamSynchronize(h->attached->stream));
+    drop_step_graph(h->attached);
+  }
+  h->r.fuse34 = on != 0;
   return kOk;
 }
 

@@ -63,6 +63,22 @@ __device__ inline int xcd_group_remap(int b, int nb, int G)
   return span * q + G * (r & 7) + (r >> 3);
 }
 
+// Batched launches list their members' workgroups back to back: member m
+// owns workgroups [start[m], start[m+1]).  The member of workgroup b (the
+// last one whose range starts at or before b; empty ranges are skipped).
+__device__ __forceinline__ int batch_find(const int* __restrict__ start, int n, int b)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 constexpr double kPiD = 3.14159265358979323846;
 
 }  // namespace cola

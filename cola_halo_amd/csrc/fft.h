@@ -100,6 +100,34 @@ struct FftPlan {
   bool can_fuse() const;
 };
 
+// One launch of a batched force transform: one pass of every grid.
+struct FftBatchStage {
+  int pass = 0;  // 0 z r2c, 1 y forward, 2 fused x, 3 y inverse (3 axes), 4 z c2r (3 axes)
+  int emax = 0, rs = 0, threads = 0;
+  size_t lds = 0;
+  int blocks = 0, njobs = 0;
+  const void* jobs = nullptr;  // device job table
+  const int* start = nullptr;  // device: first workgroup of each job, njobs + 1 entries
+};
+
+// The force transforms (FftPlan::forces) of many grids, one launch per pass
+// over the workgroups of every grid: z r2c, y forward, fused x, the y inverse
+// of the three axes, the z c2r of the three axes (five launches instead of
+// nine per grid).  Each grid keeps its own plan's radix schedule and tile
+// shapes; a launch runs every job with the widest workgroup and register
+// tile any of its jobs needs, and the butterflies of a job do not depend on
+// either, so the outputs equal each plan's forces() bit for bit.  Every plan
+// must fuse (FftPlan::can_fuse).
+struct FftForcesBatch {
+  std::vector<FftBatchStage> stages;
+  void* d_mem = nullptr;
+  // plans[i] transforms density[i] (destroyed) into out[3 i + axis] with
+  // the operator parameters kp[i].
+  int init(const FftPlan* const* plans, float* const* density, float* const* out, const KOpParams* kp, int n);
+  int run(hipStream_t s) const;
+  void destroy();
+};
+
 // Radix schedules of n = 2^a 3^b 5^c (the order and grouping of the radix
 // passes; every schedule computes the same DFT, rounding differs):
 //   0  powers of two as radix 8 (a leftover 2 as 4 x 4) first, then 3s, 5s

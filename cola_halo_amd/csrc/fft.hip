@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -220,9 +221,11 @@ constexpr int tile_waves_per_eu(int mode, KOp op, int emax, int rs)
   return 6;
 }
 
+// Workgroup blk of a pass whose grid is nblk workgroups (the kernel's own
+// grid, or one job's share of a batched launch).
 template <int DIR, int MODE, KOp OP, int EMAX, int RS>
-__global__ void __launch_bounds__(kFftMaxThreads) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu(MODE, OP, EMAX, RS))))
-fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __restrict__ tw, TileArgs A)
+__device__ __forceinline__ void tile_run(const void* __restrict__ in_, void* __restrict__ out_,
+                                         const float2* __restrict__ tw, const TileArgs& A, int blk, int nblk)
 {
   extern __shared__ float2 smem[];
   const int n = A.P.n;
@@ -232,14 +235,14 @@ fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __
   const int T = 1 << log2T;
   int c0, batch = 0;
   if (MODE == kCols) {
-    const int lb = xcd_group_remap(blockIdx.x, gridDim.x, A.xgroup);
+    const int lb = xcd_group_remap(blk, nblk, A.xgroup);
     batch = lb / A.tpb_pad;
     const int tile = lb - batch * A.tpb_pad;
     if (tile >= A.tpb) return;  // padding workgroup (before any barrier)
     c0 = tile << log2T;
     if (padding_tile(A, c0, log2T)) return;
   } else {
-    c0 = blockIdx.x << log2T;
+    c0 = blk << log2T;
   }
   float2* tws = smem;
   float2* lds = smem + n;
@@ -468,6 +471,13 @@ fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __
   }
 }
 
+template <int DIR, int MODE, KOp OP, int EMAX, int RS>
+__global__ void __launch_bounds__(kFftMaxThreads) __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu(MODE, OP, EMAX, RS))))
+fft_tile(const void* __restrict__ in_, void* __restrict__ out_, const float2* __restrict__ tw, TileArgs A)
+{
+  tile_run<DIR, MODE, OP, EMAX, RS>(in_, out_, tw, A, blockIdx.x, gridDim.x);
+}
+
 // ---- fused x pass: forward x FFT, then per axis Poisson + inverse x FFT ----
 // Column layout helpers (LDS [r][c], column fastest).
 struct ColCtx {
@@ -541,15 +551,16 @@ __device__ inline void col_load_lds(float2 (&reg)[EMAX], const FftPass& p, const
 // butterflies are exactly the inverse's first, so delta(k) stays in registers
 // and never goes to memory.  Outputs: the three x-inverted force spectra.
 template <int R_LAST, int EMAX, int RS>
-__global__ void __launch_bounds__(kFftMaxThreads)
-fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __restrict__ out1,
-            float2* __restrict__ out2, const float2* __restrict__ tw, TileArgs A, FftRadices Pinv)
+__device__ __forceinline__ void fused_run(const float2* __restrict__ in, float2* __restrict__ out0,
+                                          float2* __restrict__ out1, float2* __restrict__ out2,
+                                          const float2* __restrict__ tw, const TileArgs& A, const FftRadices& Pinv,
+                                          int blk, int nblk)
 {
   extern __shared__ float2 smem[];
   const int n = A.P.n;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int log2T = A.log2T;
-  const int lb = xcd_group_remap(blockIdx.x, gridDim.x, A.xgroup);
+  const int lb = xcd_group_remap(blk, nblk, A.xgroup);
   if (lb >= A.tpb) return;  // padding workgroup (before any barrier)
   const int c0 = lb << log2T;
   if (padding_tile(A, c0, log2T)) return;
@@ -688,6 +699,14 @@ fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __
   }
 }
 
+template <int R_LAST, int EMAX, int RS>
+__global__ void __launch_bounds__(kFftMaxThreads)
+fft_x_fused(const float2* __restrict__ in, float2* __restrict__ out0, float2* __restrict__ out1,
+            float2* __restrict__ out2, const float2* __restrict__ tw, TileArgs A, FftRadices Pinv)
+{
+  fused_run<R_LAST, EMAX, RS>(in, out0, out1, out2, tw, A, Pinv, blockIdx.x, gridDim.x);
+}
+
 unsigned magic_for(int d)
 {
   if (d <= 1) return 0;
@@ -743,6 +762,65 @@ void set_groups(TileArgs& A, int tpb, int group)
   A.tpb = tpb;
   A.xgroup = group;
   A.tpb_pad = (tpb + group - 1) / group * group;
+}
+
+// ---- batched passes: one launch over the workgroups of many grids ---------
+// A job is one pass of one grid as its own launch runs it: that launch's
+// arguments and workgroup count.  A batched launch lists the jobs'
+// workgroups back to back (start[j], batch_find); each job starts on a
+// multiple of 8 x its XCD group, so its workgroups land on the XCDs its own
+// launch would put them on, and the tile arithmetic is the job's own, so the
+// outputs are those of the separate launches bit for bit.
+struct TileJob {
+  const void* in;
+  void* out;
+  const float2* tw;
+  TileArgs A;
+  int blocks;
+};
+
+struct FusedJob {
+  const float2* in;
+  float2* out[3];
+  const float2* tw;
+  TileArgs A;
+  FftRadices Pinv;
+  int blocks;
+};
+
+template <int DIR, int MODE, int EMAX, int RS>
+__global__ void __launch_bounds__(kFftMaxThreads)
+    __attribute__((amdgpu_waves_per_eu(tile_waves_per_eu(MODE, KOp::None, EMAX, RS))))
+fft_tile_jobs(const TileJob* __restrict__ jobs, const int* __restrict__ start, int njobs)
+{
+  const int j = batch_find(start, njobs, blockIdx.x);
+  const TileJob& J = jobs[j];
+  const int lb = (int)blockIdx.x - start[j];
+  if (lb >= J.blocks) return;  // alignment padding (before any barrier)
+  tile_run<DIR, MODE, KOp::None, EMAX, RS>(J.in, J.out, J.tw, J.A, lb, J.blocks);
+}
+
+template <int EMAX, int RS>
+__global__ void __launch_bounds__(kFftMaxThreads)
+fft_x_fused_jobs(const FusedJob* __restrict__ jobs, const int* __restrict__ start, int njobs)
+{
+  const int j = batch_find(start, njobs, blockIdx.x);
+  const FusedJob& J = jobs[j];
+  const int lb = (int)blockIdx.x - start[j];
+  if (lb >= J.blocks) return;
+  // the last forward radix is a template constant of the fused pass
+  // (FftPlan::can_fuse admits 3, 4, 5 and 8)
+  auto run = [&](auto Rc) {
+    fused_run<decltype(Rc)::value, EMAX, RS>(J.in, J.out[0], J.out[1], J.out[2], J.tw, J.A, J.Pinv, lb, J.blocks);
+  };
+  switch (J.A.P.pass[J.A.P.npass - 1].R) {
+    case 3: run(std::integral_constant<int, 3>{}); break;
+    case 4: run(std::integral_constant<int, 4>{}); break;
+    case 5:
+      if constexpr (RS == kRsAll) run(std::integral_constant<int, 5>{});
+      break;
+    default: run(std::integral_constant<int, 8>{}); break;
+  }
 }
 
 }  // namespace
@@ -993,6 +1071,116 @@ bool FftPlan::can_fuse() const
   return RL == 3 || RL == 4 || RL == 5 || RL == 8;
 }
 
+namespace {
+
+// The launches of FftPlan::forces (fused path) for one grid: z r2c, y
+// forward, fused x, then per axis y inverse and z c2r.
+struct ForceJobs {
+  TileJob zr2c, yfwd, yinv[3], zc2r[3];
+  FusedJob x;
+};
+
+void make_force_jobs(const FftPlan& P, float* density, float* const out[3], const KOpParams& kp, ForceJobs* J)
+{
+  const int n = P.n;
+  const long long nz = n / 2 + 1, np_ = P.nzp;
+  const int npairs = n * n / 2;
+  float2* cdata = reinterpret_cast<float2*>(density);
+  TileArgs A;
+  A.P = P.rad;
+  A.kp = kp;
+  A.nz = (int)np_;
+  A.magNz = magic_for((int)np_);
+  A.kz_valid = 0;
+  // z and y forward
+  A.log2T = P.grows.log2T;
+  A.ncols = npairs;
+  A.rs_in = 2 * np_;
+  A.rs_out = np_;
+  J->zr2c = TileJob{density, cdata, P.d_tw, A, grid_x(npairs, P.grows.log2T)};
+  A.log2T = P.gy.log2T;
+  A.ncols = (int)nz;
+  A.sr = np_;
+  A.bs = (long long)n * np_;
+  set_groups(A, grid_x(nz, P.gy.log2T), P.xgroup);
+  J->yfwd = TileJob{cdata, cdata, P.d_tw, A, A.tpb_pad * n};
+  // fused x: forward, Poisson per axis, inverse (row padding tiles return at once)
+  const long long ncols = (long long)n * np_;
+  A.log2T = P.gx.log2T;
+  A.ncols = (int)ncols;
+  A.sr = ncols;
+  A.bs = 0;
+  A.kz_valid = np_ > nz ? (int)nz : 0;
+  set_groups(A, grid_x(ncols, P.gx.log2T), P.xgroup);
+  FusedJob& x = J->x;
+  x.in = cdata;
+  for (int a = 0; a < 3; ++a) x.out[a] = reinterpret_cast<float2*>(out[a]);
+  x.tw = P.d_tw;
+  x.A = A;
+  x.Pinv = reversed(P.rad);
+  x.blocks = A.tpb_pad;
+  // per axis: y inverse, z c2r
+  for (int a = 0; a < 3; ++a) {
+    TileArgs B;
+    B.P = P.rad;
+    B.kz_valid = 0;
+    B.log2T = P.gy.log2T;
+    B.ncols = (int)nz;
+    B.sr = np_;
+    B.bs = (long long)n * np_;
+    set_groups(B, grid_x(nz, P.gy.log2T), P.xgroup);
+    J->yinv[a] = TileJob{x.out[a], x.out[a], P.d_tw, B, B.tpb_pad * n};
+    B.log2T = P.grows.log2T;
+    B.ncols = npairs;
+    B.rs_in = np_;
+    B.rs_out = 2 * np_;
+    B.zero_pad = np_ == nz;
+    J->zc2r[a] = TileJob{x.out[a], out[a], P.d_tw, B, grid_x(npairs, P.grows.log2T)};
+  }
+}
+
+template <int DIR, int MODE, int E, int RS>
+int launch_jobs_e(const FftBatchStage& st, hipStream_t s)
+{
+  static bool attr = false;
+  if (!attr) {
+    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_tile_jobs<DIR, MODE, E, RS>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  hipLaunchKernelGGL((fft_tile_jobs<DIR, MODE, E, RS>), dim3(st.blocks), dim3(st.threads), st.lds, s,
+                     static_cast<const TileJob*>(st.jobs), st.start, st.njobs);
+  return kOk;
+}
+
+template <int DIR, int MODE>
+int launch_jobs(const FftBatchStage& st, hipStream_t s)
+{
+  if (st.emax <= 8) {
+    if (st.rs == kRs348) return launch_jobs_e<DIR, MODE, 8, kRs348>(st, s);
+    return launch_jobs_e<DIR, MODE, 8, kRsAll>(st, s);
+  }
+  if (st.emax <= 12) return launch_jobs_e<DIR, MODE, 12, kRsAll>(st, s);
+  if (st.emax <= 16) return launch_jobs_e<DIR, MODE, 16, kRsAll>(st, s);
+  return launch_jobs_e<DIR, MODE, 24, kRsAll>(st, s);
+}
+
+template <int E, int RS>
+int launch_fused_jobs_e(const FftBatchStage& st, hipStream_t s)
+{
+  static bool attr = false;
+  if (!attr) {
+    COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_x_fused_jobs<E, RS>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr = true;
+  }
+  hipLaunchKernelGGL((fft_x_fused_jobs<E, RS>), dim3(st.blocks), dim3(st.threads), st.lds, s,
+                     static_cast<const FusedJob*>(st.jobs), st.start, st.njobs);
+  return kOk;
+}
+
+}  // namespace
+
 int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hipStream_t s) const
 {
   if (!can_fuse()) {
@@ -1004,77 +1192,176 @@ int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hi
     }
     return kOk;
   }
-  const long long nz = n / 2 + 1, np_ = nzp;
-  const int npairs = n * n / 2;
-  float2* cdata = reinterpret_cast<float2*>(density);
   auto mark = [&](int pass, int begin) {
     if (hook) hook(hook_user, pass, begin);
   };
-  TileArgs A;
-  A.P = rad;
-  A.kp = kp;
-  A.nz = (int)np_;
-  A.magNz = magic_for((int)np_);
-  A.kz_valid = 0;
-  // z and y forward
-  A.log2T = grows.log2T;
-  A.ncols = npairs;
-  A.rs_in = 2 * np_;
-  A.rs_out = np_;
+  ForceJobs J;
+  make_force_jobs(*this, density, out, kp, &J);
+  auto tile = [&](auto dir, auto mode, const FftPassGeom& g, const TileJob& t) {
+    return launch_tile<decltype(dir)::value, decltype(mode)::value, KOp::None>(g, dim3(t.blocks), s, t.in, t.out,
+                                                                              t.tw, t.A);
+  };
+  using Fwd = std::integral_constant<int, -1>;
+  using Inv = std::integral_constant<int, 1>;
   mark(0, 1);
-  COLA_TRY((launch_tile<-1, kR2C, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, density, cdata, d_tw, A)));
+  COLA_TRY(tile(Fwd{}, std::integral_constant<int, kR2C>{}, grows, J.zr2c));
   mark(0, 0);
-  A.log2T = gy.log2T;
-  A.ncols = (int)nz;
-  A.sr = np_;
-  A.bs = (long long)n * np_;
-  set_groups(A, grid_x(nz, gy.log2T), xgroup);
   mark(1, 1);
-  COLA_TRY((launch_tile<-1, kCols, KOp::None>(gy, dim3(A.tpb_pad * n), s, cdata, cdata, d_tw, A)));
+  COLA_TRY(tile(Fwd{}, std::integral_constant<int, kCols>{}, gy, J.yfwd));
   mark(1, 0);
-  // fused x: forward, Poisson per axis, inverse (row padding tiles return at once)
-  const long long ncols = (long long)n * np_;
-  A.log2T = gx.log2T;
-  A.ncols = (int)ncols;
-  A.sr = ncols;
-  A.bs = 0;
-  A.kz_valid = np_ > nz ? (int)nz : 0;
-  set_groups(A, grid_x(ncols, gx.log2T), xgroup);
-  const FftRadices Pinv = reversed(rad);
-  float2* o[3] = {reinterpret_cast<float2*>(out[0]), reinterpret_cast<float2*>(out[1]),
-                  reinterpret_cast<float2*>(out[2])};
+  const FusedJob& x = J.x;
   mark(6, 1);
   switch (rad.pass[rad.npass - 1].R) {
-    case 3: COLA_TRY(launch_fused_r<3>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
-    case 4: COLA_TRY(launch_fused_r<4>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
-    case 5: COLA_TRY(launch_fused_r<5>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
-    default: COLA_TRY(launch_fused_r<8>(gx, A.tpb_pad, s, cdata, o[0], o[1], o[2], d_tw, A, Pinv)); break;
+    case 3: COLA_TRY(launch_fused_r<3>(gx, x.blocks, s, x.in, x.out[0], x.out[1], x.out[2], d_tw, x.A, x.Pinv)); break;
+    case 4: COLA_TRY(launch_fused_r<4>(gx, x.blocks, s, x.in, x.out[0], x.out[1], x.out[2], d_tw, x.A, x.Pinv)); break;
+    case 5: COLA_TRY(launch_fused_r<5>(gx, x.blocks, s, x.in, x.out[0], x.out[1], x.out[2], d_tw, x.A, x.Pinv)); break;
+    default: COLA_TRY(launch_fused_r<8>(gx, x.blocks, s, x.in, x.out[0], x.out[1], x.out[2], d_tw, x.A, x.Pinv)); break;
   }
   mark(6, 0);
-  // per axis: y inverse, z c2r
   for (int a = 0; a < 3; ++a) {
-    TileArgs B;
-    B.P = rad;
-    B.kz_valid = 0;
-    B.log2T = gy.log2T;
-    B.ncols = (int)nz;
-    B.sr = np_;
-    B.bs = (long long)n * np_;
-    set_groups(B, grid_x(nz, gy.log2T), xgroup);
     mark(4, 1);
-    COLA_TRY((launch_tile<1, kCols, KOp::None>(gy, dim3(B.tpb_pad * n), s, o[a], o[a], d_tw, B)));
+    COLA_TRY(tile(Inv{}, std::integral_constant<int, kCols>{}, gy, J.yinv[a]));
     mark(4, 0);
-    B.log2T = grows.log2T;
-    B.ncols = npairs;
-    B.rs_in = np_;
-    B.rs_out = 2 * np_;
-    B.zero_pad = np_ == nz;
     mark(5, 1);
-    COLA_TRY((launch_tile<1, kC2R, KOp::None>(grows, dim3(grid_x(npairs, grows.log2T)), s, o[a], out[a], d_tw, B)));
+    COLA_TRY(tile(Inv{}, std::integral_constant<int, kC2R>{}, grows, J.zc2r[a]));
     mark(5, 0);
   }
   COLA_HIP(hipGetLastError());
   return kOk;
+}
+
+int FftForcesBatch::init(const FftPlan* const* plans, float* const* density, float* const* out, const KOpParams* kp,
+                         int n)
+{
+  destroy();
+  if (n <= 0 || !plans || !density || !out || !kp) return kErrArg;
+  std::vector<ForceJobs> fj(n);
+  for (int i = 0; i < n; ++i) {
+    if (!plans[i] || !plans[i]->can_fuse()) {
+      set_error("batched force transforms need plans whose x pass fuses");
+      return kErrUnsupported;
+    }
+    make_force_jobs(*plans[i], density[i], out + 3 * i, kp[i], &fj[i]);
+  }
+  // the jobs of the five passes and each job's own launch geometry
+  struct Pass {
+    int mode;
+    std::vector<TileJob> tj;
+    std::vector<FusedJob> xj;
+    std::vector<const FftPassGeom*> g;
+  };
+  Pass ps[5] = {{kR2C, {}, {}, {}}, {kCols, {}, {}, {}}, {kCols, {}, {}, {}}, {kCols, {}, {}, {}}, {kC2R, {}, {}, {}}};
+  for (int i = 0; i < n; ++i) {
+    const FftPlan& P = *plans[i];
+    ps[0].tj.push_back(fj[i].zr2c);
+    ps[0].g.push_back(&P.grows);
+    ps[1].tj.push_back(fj[i].yfwd);
+    ps[1].g.push_back(&P.gy);
+    ps[2].xj.push_back(fj[i].x);
+    ps[2].g.push_back(&P.gx);
+    for (int a = 0; a < 3; ++a) {
+      ps[3].tj.push_back(fj[i].yinv[a]);
+      ps[3].g.push_back(&P.gy);
+      ps[4].tj.push_back(fj[i].zc2r[a]);
+      ps[4].g.push_back(&P.grows);
+    }
+  }
+  auto align_up = [](size_t v, size_t a) { return (v + a - 1) / a * a; };
+  std::vector<char> host;
+  std::vector<std::pair<size_t, size_t>> offs;  // job table, start table
+  for (int k = 0; k < 5; ++k) {
+    Pass& p = ps[k];
+    const bool fused = k == 2;
+    const int nj = fused ? (int)p.xj.size() : (int)p.tj.size();
+    auto args = [&](int j) -> const TileArgs& { return fused ? p.xj[j].A : p.tj[j].A; };
+    auto blocks = [&](int j) { return fused ? p.xj[j].blocks : p.tj[j].blocks; };
+    FftBatchStage st;
+    st.pass = k;
+    st.njobs = nj;
+    for (const FftPassGeom* g : p.g) st.threads = std::max(st.threads, g->threads);
+    int emax = 0;
+    bool all348 = true;
+    for (int j = 0; j < nj; ++j) {
+      const TileArgs& A = args(j);
+      const int e = fft_emax(A.P, A.log2T, st.threads);
+      if (e == 0) {
+        set_error("batched force transforms: a tile does not fit the common workgroup");
+        return kErrUnsupported;
+      }
+      emax = std::max(emax, e);
+      all348 = all348 && radices_348(A.P);
+      st.lds = std::max(st.lds, tile_lds_bytes(p.mode, A.P.n, A.log2T));
+    }
+    st.emax = emax <= 8 ? 8 : emax <= 12 ? 12 : emax <= 16 ? 16 : 24;
+    if (fused && st.emax > 12) {
+      set_error("batched force transforms: fused x tile wider than 12 values per thread");
+      return kErrUnsupported;
+    }
+    st.rs = st.emax == 8 && all348 ? kRs348 : kRsAll;
+    std::vector<int> start(nj + 1);
+    long long at = 0;
+    for (int j = 0; j < nj; ++j) {
+      const long long al = p.mode == kCols ? 8LL * std::max(1, args(j).xgroup) : 1LL;
+      at = (at + al - 1) / al * al;
+      start[j] = (int)at;
+      at += blocks(j);
+      if (at > 0x7fffffffLL || at * st.threads > 0xffffffffLL) {
+        set_error("batched force transforms: too many workgroups for one launch");
+        return kErrUnsupported;
+      }
+    }
+    start[nj] = (int)at;
+    st.blocks = (int)at;
+    const size_t jo = align_up(host.size(), 256);
+    const size_t jb = fused ? sizeof(FusedJob) * nj : sizeof(TileJob) * nj;
+    const size_t so = align_up(jo + jb, 256);
+    host.resize(so + sizeof(int) * (nj + 1));
+    if (fused)
+      std::memcpy(host.data() + jo, p.xj.data(), jb);
+    else
+      std::memcpy(host.data() + jo, p.tj.data(), jb);
+    std::memcpy(host.data() + so, start.data(), sizeof(int) * (nj + 1));
+    offs.push_back({jo, so});
+    stages.push_back(st);
+  }
+  COLA_HIP(hipMalloc(&d_mem, host.size()));
+  COLA_HIP(hipMemcpy(d_mem, host.data(), host.size(), hipMemcpyHostToDevice));
+  char* base = static_cast<char*>(d_mem);
+  for (size_t k = 0; k < stages.size(); ++k) {
+    stages[k].jobs = base + offs[k].first;
+    stages[k].start = reinterpret_cast<const int*>(base + offs[k].second);
+  }
+  return kOk;
+}
+
+int FftForcesBatch::run(hipStream_t s) const
+{
+  for (const FftBatchStage& st : stages) {
+    switch (st.pass) {
+      case 0: COLA_TRY((launch_jobs<-1, kR2C>(st, s))); break;
+      case 1: COLA_TRY((launch_jobs<-1, kCols>(st, s))); break;
+      case 2:
+        if (st.emax <= 8 && st.rs == kRs348) {
+          COLA_TRY((launch_fused_jobs_e<8, kRs348>(st, s)));
+        } else if (st.emax <= 8) {
+          COLA_TRY((launch_fused_jobs_e<8, kRsAll>(st, s)));
+        } else {
+          COLA_TRY((launch_fused_jobs_e<12, kRsAll>(st, s)));
+        }
+        break;
+      case 3: COLA_TRY((launch_jobs<1, kCols>(st, s))); break;
+      default: COLA_TRY((launch_jobs<1, kC2R>(st, s))); break;
+    }
+  }
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+void FftForcesBatch::destroy()
+{
+  if (d_mem) (void)hipFree(d_mem);
+  d_mem = nullptr;
+  stages.clear();
 }
 
 int FftPlan::c2r(const float2* in_k, float* out, KOp op, const KOpParams& kp, hipStream_t s) const

@@ -106,4 +106,39 @@ int launch_noncola_initial(const DevParticles& p, const SnapshotFactors& f, DevP
 
 int launch_zero_doubles(double* p, int n, hipStream_t s);
 
+// ---- batched step kernels: one launch per kernel for many realizations ----
+// What the step kernels of one realization read.
+struct PmMember {
+  DevParticles p;
+  CicGeom g;
+  TileGeom t;
+  BinBuffers b;
+  float* density;
+  const float* f[3];       // force grids the readout gathers from
+  double* ring;            // mass-check ring (ring[prm->slot] gets the grid sum)
+  const StepParams* prm;   // this step's values of the realization
+  float boxsize;
+};
+
+// Device tables of a batched step: the members and, per kernel, the first
+// workgroup of every member (kernel k: start[k * (n + 1) + m], m <= n).
+enum PmBatchKernel : int { kBatchBin = 0, kBatchScan, kBatchTile, kBatchSum, kBatchRead, kPmBatchKernels };
+struct PmBatch {
+  int n = 0;
+  const PmMember* members = nullptr;
+  const int* start = nullptr;
+  int blocks[kPmBatchKernels] = {};
+  size_t deposit_lds = 0;
+  void* d_mem = nullptr;
+  int init(const PmMember* members, int n);
+  void destroy();
+};
+// The kernels of assign_kernels for every member, each one launch: mass
+// slot cleared, wrap + tile counts, scan (3 launches), scatter, deposit,
+// grid sums.  Same per-particle and per-tile arithmetic as the single
+// launches: the density grids are the same bit for bit.
+int launch_assign_batch(const PmBatch& b, hipStream_t s);
+// The fused readout + kick + drift of every member (plain force grids).
+int launch_readout_kick_drift_batch(const PmBatch& b, hipStream_t s);
+
 }  // namespace cola

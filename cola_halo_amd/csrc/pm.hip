@@ -15,6 +15,7 @@
 #include "pm.h"
 
 #include <algorithm>
+#include <vector>
 
 namespace cola {
 
@@ -243,16 +244,18 @@ __device__ inline LaneRun lane_run(int key, bool active)
 // lane of a workgroup's chunk keeps adjacent lanes on adjacent particles.
 constexpr int kBinPer = 4;
 
-__global__ void __launch_bounds__(kThreads)
-k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
-            float L)
+// Block blk of the counting pass (the kernels below run it for their own
+// block index or for a block of one member of a batched launch).
+__device__ __forceinline__ void bin_count_block(const DevParticles& p, const CicGeom& g, const TileGeom& t,
+                                                int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
+                                                float L, long long blk)
 {
   __shared__ BinHash h;
   float px[kBinPer], py[kBinPer], pz[kBinPer];
   bool live[kBinPer];
 #pragma unroll
   for (int q = 0; q < kBinPer; ++q) {
-    const long long i = ((long long)blockIdx.x * kBinPer + q) * kThreads + threadIdx.x;
+    const long long i = (blk * kBinPer + q) * kThreads + threadIdx.x;
     live[q] = i < p.np;
     px[q] = live[q] ? p.x[0][i] : 0.0f;
     py[q] = live[q] ? p.x[1][i] : 0.0f;
@@ -267,7 +270,7 @@ k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int
     if (live[q]) {
       float x = px[q], y = py[q], z = pz[q];
       if (wrap_first) {
-        const long long i = ((long long)blockIdx.x * kBinPer + q) * kThreads + threadIdx.x;
+        const long long i = (blk * kBinPer + q) * kThreads + threadIdx.x;
         wrap1(x, L);
         wrap1(y, L);
         wrap1(z, L);
@@ -303,14 +306,22 @@ k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int
     if (h.key[s] != kHashEmpty) atomicAdd(&counts[h.key[s]], h.cnt[s]);
 }
 
+__global__ void __launch_bounds__(kThreads)
+k_bin_count(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, int* __restrict__ overflow, int wrap_first,
+            float L)
+{
+  bin_count_block(p, g, t, counts, overflow, wrap_first, L, blockIdx.x);
+}
+
 // Claims the entries' slots by counting each tile's counter back down to zero
 // (ready for the next step): a workgroup takes the cnt slots below the
 // counter's value c, and its entry of rank r in the tile gets position
 // offsets[tile] + c - 1 - r.  Entries that found no table slot claim theirs
 // one by one from the counter.
-__global__ void __launch_bounds__(kThreads)
-k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, const int* __restrict__ offsets,
-              float* __restrict__ sorted, long long capacity, int* __restrict__ overflow)
+__device__ __forceinline__ void bin_scatter_block(const DevParticles& p, const CicGeom& g, const TileGeom& t,
+                                                  int* __restrict__ counts, const int* __restrict__ offsets,
+                                                  float* __restrict__ sorted, long long capacity,
+                                                  int* __restrict__ overflow, long long blk)
 {
   __shared__ BinHash h;
   __shared__ long long base[kHashSlots];
@@ -318,7 +329,7 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
   bool live[kBinPer];
 #pragma unroll
   for (int q = 0; q < kBinPer; ++q) {
-    const long long i = ((long long)blockIdx.x * kBinPer + q) * kThreads + threadIdx.x;
+    const long long i = (blk * kBinPer + q) * kThreads + threadIdx.x;
     live[q] = i < p.np;
     px[q] = live[q] ? p.x[0][i] : 0.0f;
     py[q] = live[q] ? p.x[1][i] : 0.0f;
@@ -393,11 +404,19 @@ k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, c
   }
 }
 
+__global__ void __launch_bounds__(kThreads)
+k_bin_scatter(DevParticles p, CicGeom g, TileGeom t, int* __restrict__ counts, const int* __restrict__ offsets,
+              float* __restrict__ sorted, long long capacity, int* __restrict__ overflow)
+{
+  bin_scatter_block(p, g, t, counts, offsets, sorted, capacity, overflow, blockIdx.x);
+}
+
 // ---- exclusive scan of the tile counts -------------------------------------
-__global__ void __launch_bounds__(kThreads) k_scan_reduce(const int* __restrict__ in, long long n, int* __restrict__ sums)
+__device__ __forceinline__ void scan_reduce_block(const int* __restrict__ in, long long n, int* __restrict__ sums,
+                                                  int blk)
 {
   __shared__ int red[kThreads];
-  const long long base = (long long)blockIdx.x * kScanChunk;
+  const long long base = (long long)blk * kScanChunk;
   int s = 0;
   for (int k = threadIdx.x; k < kScanChunk; k += kThreads) {
     const long long i = base + k;
@@ -409,7 +428,12 @@ __global__ void __launch_bounds__(kThreads) k_scan_reduce(const int* __restrict_
     if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) sums[blk] = red[0];
+}
+
+__global__ void __launch_bounds__(kThreads) k_scan_reduce(const int* __restrict__ in, long long n, int* __restrict__ sums)
+{
+  scan_reduce_block(in, n, sums, blockIdx.x);
 }
 
 __device__ inline int block_exclusive_scan(int v, int* sh, int* total)
@@ -430,7 +454,7 @@ __device__ inline int block_exclusive_scan(int v, int* sh, int* total)
 }
 
 // single workgroup: exclusive scan of the chunk sums in place
-__global__ void __launch_bounds__(kThreads) k_scan_top(int* __restrict__ sums, int nsums)
+__device__ __forceinline__ void scan_top_block(int* __restrict__ sums, int nsums)
 {
   __shared__ int sh[kThreads];
   int carry = 0;
@@ -444,11 +468,13 @@ __global__ void __launch_bounds__(kThreads) k_scan_top(int* __restrict__ sums, i
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_scan_final(const int* __restrict__ in, long long n, const int* __restrict__ sums, int* __restrict__ out)
+__global__ void __launch_bounds__(kThreads) k_scan_top(int* __restrict__ sums, int nsums) { scan_top_block(sums, nsums); }
+
+__device__ __forceinline__ void scan_final_block(const int* __restrict__ in, long long n, const int* __restrict__ sums,
+                                                 int* __restrict__ out, int blk)
 {
   __shared__ int sh[kThreads];
-  const long long base = (long long)blockIdx.x * kScanChunk;
+  const long long base = (long long)blk * kScanChunk;
   constexpr int per = kScanChunk / kThreads;
   int v[per];
   int s = 0;
@@ -459,7 +485,7 @@ k_scan_final(const int* __restrict__ in, long long n, const int* __restrict__ su
     s += v[k];
   }
   int total;
-  int run = sums[blockIdx.x] + block_exclusive_scan(s, sh, &total);
+  int run = sums[blk] + block_exclusive_scan(s, sh, &total);
 #pragma unroll
   for (int k = 0; k < per; ++k) {
     const long long i = base + (long long)threadIdx.x * per + k;
@@ -467,6 +493,12 @@ k_scan_final(const int* __restrict__ in, long long n, const int* __restrict__ su
     run += v[k];
     if (i == n - 1) out[n] = run;
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_scan_final(const int* __restrict__ in, long long n, const int* __restrict__ sums, int* __restrict__ out)
+{
+  scan_final_block(in, n, sums, out, blockIdx.x);
 }
 
 // ---- deposit: one workgroup per tile ----------------------------------------
@@ -486,13 +518,12 @@ constexpr float kFix = 4294967296.0f;  // 2^32
 constexpr double kUnfix = 1.0 / 4294967296.0;
 constexpr int kDepositThreads = 256;
 
-__global__ void __launch_bounds__(kDepositThreads)
-k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const float* __restrict__ sorted,
-                float* __restrict__ density, double* __restrict__ tile_sums)
+// Tile `tile` of the deposit; cells: the tile's LDS accumulators.
+__device__ __forceinline__ void deposit_tile(const CicGeom& g, const TileGeom& t, const int* __restrict__ offsets,
+                                             const float* __restrict__ sorted, float* __restrict__ density,
+                                             double* __restrict__ tile_sums, unsigned long long* cells, long long tile)
 {
-  extern __shared__ unsigned long long cells[];
   __shared__ double wsum[kDepositThreads / 64];
-  const long long tile = blockIdx.x;
   const int tz = (int)(tile % t.ntz);
   const int ty = (int)((tile / t.ntz) % t.nty);
   const int tx = (int)(tile / ((long long)t.ntz * t.nty));
@@ -549,13 +580,21 @@ k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const fl
   }
 }
 
+__global__ void __launch_bounds__(kDepositThreads)
+k_deposit_tiles(CicGeom g, TileGeom t, const int* __restrict__ offsets, const float* __restrict__ sorted,
+                float* __restrict__ density, double* __restrict__ tile_sums)
+{
+  extern __shared__ unsigned long long cells[];
+  deposit_tile(g, t, offsets, sorted, density, tile_sums, cells, blockIdx.x);
+}
+
 // pm.c:535-562: sum of the per-tile sums into ring[prm->slot] (zeroed first)
-__global__ void __launch_bounds__(kThreads)
-k_sum_doubles(const double* __restrict__ v, long long n, double* ring, const StepParams* __restrict__ prm)
+__device__ __forceinline__ void sum_doubles_block(const double* __restrict__ v, long long n, double* ring,
+                                                  const StepParams* __restrict__ prm, int blk)
 {
   double* const out = ring + prm->slot;
   __shared__ double wsum[kThreads / 64];
-  const long long base = (long long)blockIdx.x * kThreads * 16;
+  const long long base = (long long)blk * kThreads * 16;
   double s = 0.0;
   for (int k = 0; k < 16; ++k) {
     const long long i = base + (long long)k * kThreads + threadIdx.x;
@@ -569,6 +608,12 @@ k_sum_doubles(const double* __restrict__ v, long long n, double* ring, const Ste
     for (int w = 0; w < kThreads / 64; ++w) tot += wsum[w];
     atomicAdd(out, tot);
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_sum_doubles(const double* __restrict__ v, long long n, double* ring, const StepParams* __restrict__ prm)
+{
+  sum_doubles_block(v, n, ring, prm, blockIdx.x);
 }
 
 // ---- readout (pm.c:359-412) -------------------------------------------------
@@ -732,13 +777,16 @@ __device__ inline float drift1(float x, float v, float dx1, float dx2, const Dri
   return x + (v * d.dyyy + 1.0f * (dx1 * d.da1 + dx2 * d.da2));
 }
 
+// Block blk of nblk (the member's own grid) of the fused readout.
 template <bool FI>
-__global__ void __launch_bounds__(kThreads)
-k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
-                     const float* __restrict__ g2, const StepParams* __restrict__ prm, const double* __restrict__ fsum,
-                     float inv_cells)
+__device__ __forceinline__ void readout_kick_drift_block(const DevParticles& p, const CicGeom& g,
+                                                         const float* __restrict__ g0, const float* __restrict__ g1,
+                                                         const float* __restrict__ g2,
+                                                         const StepParams* __restrict__ prm,
+                                                         const double* __restrict__ fsum, float inv_cells, int blk,
+                                                         int nblk)
 {
-  const long long i = (long long)xcd_group_remap(blockIdx.x, gridDim.x, g.readout_group) * kThreads + threadIdx.x;
+  const long long i = (long long)xcd_group_remap(blk, nblk, g.readout_group) * kThreads + threadIdx.x;
   if (i >= p.np) return;
   const KickFactors kf = prm->k;
   const DriftFactors df = prm->d;
@@ -759,6 +807,15 @@ k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, co
     p.v[a][i] = v;
     p.x[a][i] = drift1(x[a], v, d1, d2, df);
   }
+}
+
+template <bool FI>
+__global__ void __launch_bounds__(kThreads)
+k_readout_kick_drift(DevParticles p, CicGeom g, const float* __restrict__ g0, const float* __restrict__ g1,
+                     const float* __restrict__ g2, const StepParams* __restrict__ prm, const double* __restrict__ fsum,
+                     float inv_cells)
+{
+  readout_kick_drift_block<FI>(p, g, g0, g1, g2, prm, fsum, inv_cells, blockIdx.x, gridDim.x);
 }
 
 __global__ void __launch_bounds__(kThreads) k_kick(DevParticles p, KickFactors kf)
@@ -818,6 +875,83 @@ __global__ void k_zero_doubles(double* p, int n)
 __global__ void k_zero_mass(double* ring, const StepParams* __restrict__ prm)
 {
   if (threadIdx.x == 0) ring[prm->slot] = 0.0;
+}
+
+// ---- batched kernels: workgroup b runs workgroup b - start[m] of member m --
+__global__ void __launch_bounds__(kThreads) k_zero_mass_batch(const PmMember* __restrict__ ms, int n)
+{
+  const int m = blockIdx.x * kThreads + threadIdx.x;
+  if (m < n) ms[m].ring[ms[m].prm->slot] = 0.0;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bin_count_batch(const PmMember* __restrict__ ms, const int* __restrict__ start, int n)
+{
+  const int m = batch_find(start, n, blockIdx.x);
+  const PmMember& M = ms[m];
+  bin_count_block(M.p, M.g, M.t, M.b.counts, M.b.overflow, 1, M.boxsize, (long long)blockIdx.x - start[m]);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_scan_reduce_batch(const PmMember* __restrict__ ms, const int* __restrict__ start, int n)
+{
+  const int m = batch_find(start, n, blockIdx.x);
+  const PmMember& M = ms[m];
+  scan_reduce_block(M.b.counts, M.t.ntiles, M.b.block_sums, (int)blockIdx.x - start[m]);
+}
+
+// one workgroup per member
+__global__ void __launch_bounds__(kThreads) k_scan_top_batch(const PmMember* __restrict__ ms)
+{
+  const PmMember& M = ms[blockIdx.x];
+  scan_top_block(M.b.block_sums, (int)((M.t.ntiles + kScanChunk - 1) / kScanChunk));
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_scan_final_batch(const PmMember* __restrict__ ms, const int* __restrict__ start, int n)
+{
+  const int m = batch_find(start, n, blockIdx.x);
+  const PmMember& M = ms[m];
+  scan_final_block(M.b.counts, M.t.ntiles, M.b.block_sums, M.b.offsets, (int)blockIdx.x - start[m]);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_bin_scatter_batch(const PmMember* __restrict__ ms, const int* __restrict__ start, int n)
+{
+  const int m = batch_find(start, n, blockIdx.x);
+  const PmMember& M = ms[m];
+  bin_scatter_block(M.p, M.g, M.t, M.b.counts, M.b.offsets, M.b.sorted, M.b.capacity, M.b.overflow,
+                    (long long)blockIdx.x - start[m]);
+}
+
+__global__ void __launch_bounds__(kDepositThreads)
+k_deposit_tiles_batch(const PmMember* __restrict__ ms, const int* __restrict__ start, int n)
+{
+  extern __shared__ unsigned long long cells[];
+  const int m = batch_find(start, n, blockIdx.x);
+  const PmMember& M = ms[m];
+  deposit_tile(M.g, M.t, M.b.offsets, M.b.sorted, M.density, M.b.tile_sums, cells, (long long)blockIdx.x - start[m]);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_sum_doubles_batch(const PmMember* __restrict__ ms, const int* __restrict__ start, int n)
+{
+  const int m = batch_find(start, n, blockIdx.x);
+  const PmMember& M = ms[m];
+  sum_doubles_block(M.b.tile_sums, M.t.ntiles, M.ring, M.prm, (int)blockIdx.x - start[m]);
+}
+
+// Members start on a multiple of 8 x their readout group, so the XCD
+// grouping of each member's workgroups is that of its own launch.
+__global__ void __launch_bounds__(kThreads)
+k_readout_kick_drift_batch(const PmMember* __restrict__ ms, const int* __restrict__ start, int n)
+{
+  const int m = batch_find(start, n, blockIdx.x);
+  const PmMember& M = ms[m];
+  const int nb = (int)((M.p.np + kThreads - 1) / kThreads);
+  const int lb = (int)blockIdx.x - start[m];
+  if (lb >= nb) return;  // alignment padding
+  readout_kick_drift_block<false>(M.p, M.g, M.f[0], M.f[1], M.f[2], M.prm, nullptr, 0.0f, lb, nb);
 }
 
 unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
@@ -1009,6 +1143,85 @@ int launch_noncola_initial(const DevParticles& p, const SnapshotFactors& f, DevP
 int launch_zero_mass(double* ring, const StepParams* prm, hipStream_t s)
 {
   hipLaunchKernelGGL(k_zero_mass, dim3(1), dim3(64), 0, s, ring, prm);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int PmBatch::init(const PmMember* ms, int nm)
+{
+  destroy();
+  if (nm <= 0 || !ms) return kErrArg;
+  n = nm;
+  std::vector<int> st((size_t)kPmBatchKernels * (nm + 1));
+  size_t lds = 0;
+  long long at[kPmBatchKernels] = {};
+  for (int m = 0; m < nm; ++m) {
+    const PmMember& M = ms[m];
+    const long long nsums = (M.t.ntiles + kScanChunk - 1) / kScanChunk;
+    const long long per_sum = (long long)kThreads * 16;
+    const long long cnt[kPmBatchKernels] = {(long long)bin_blocks(M.p.np), nsums, M.t.ntiles,
+                                            (M.t.ntiles + per_sum - 1) / per_sum, (long long)blocks_for(M.p.np)};
+    const long long align[kPmBatchKernels] = {1, 1, 1, 1, 8LL * std::max(1, M.g.readout_group)};
+    for (int k = 0; k < kPmBatchKernels; ++k) {
+      at[k] = (at[k] + align[k] - 1) / align[k] * align[k];
+      st[(size_t)k * (nm + 1) + m] = (int)at[k];
+      at[k] += cnt[k];
+      if (at[k] > 0x7fffffffLL || at[k] * kThreads > 0xffffffffLL) {
+        set_error("batched step: too many workgroups for one launch");
+        return kErrUnsupported;
+      }
+    }
+    lds = std::max(lds, sizeof(unsigned long long) * M.t.tx * M.t.ty * M.t.tz);
+  }
+  for (int k = 0; k < kPmBatchKernels; ++k) {
+    st[(size_t)k * (nm + 1) + nm] = (int)at[k];
+    blocks[k] = (int)at[k];
+  }
+  deposit_lds = lds;
+  const size_t mb = (sizeof(PmMember) * nm + 255) / 256 * 256;
+  COLA_HIP(hipMalloc(&d_mem, mb + sizeof(int) * st.size()));
+  COLA_HIP(hipMemcpy(d_mem, ms, sizeof(PmMember) * nm, hipMemcpyHostToDevice));
+  COLA_HIP(hipMemcpy(static_cast<char*>(d_mem) + mb, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
+  members = static_cast<const PmMember*>(d_mem);
+  start = reinterpret_cast<const int*>(static_cast<char*>(d_mem) + mb);
+  return kOk;
+}
+
+void PmBatch::destroy()
+{
+  if (d_mem) (void)hipFree(d_mem);
+  *this = PmBatch{};
+}
+
+int launch_assign_batch(const PmBatch& b, hipStream_t s)
+{
+  const int n = b.n;
+  auto tab = [&](int k) { return b.start + (size_t)k * (n + 1); };
+  hipLaunchKernelGGL(k_zero_mass_batch, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, s, b.members, n);
+  if (b.blocks[kBatchBin] > 0)
+    hipLaunchKernelGGL(k_bin_count_batch, dim3(b.blocks[kBatchBin]), dim3(kThreads), 0, s, b.members, tab(kBatchBin),
+                       n);
+  hipLaunchKernelGGL(k_scan_reduce_batch, dim3(b.blocks[kBatchScan]), dim3(kThreads), 0, s, b.members,
+                     tab(kBatchScan), n);
+  hipLaunchKernelGGL(k_scan_top_batch, dim3(n), dim3(kThreads), 0, s, b.members);
+  hipLaunchKernelGGL(k_scan_final_batch, dim3(b.blocks[kBatchScan]), dim3(kThreads), 0, s, b.members,
+                     tab(kBatchScan), n);
+  if (b.blocks[kBatchBin] > 0)
+    hipLaunchKernelGGL(k_bin_scatter_batch, dim3(b.blocks[kBatchBin]), dim3(kThreads), 0, s, b.members,
+                       tab(kBatchBin), n);
+  hipLaunchKernelGGL(k_deposit_tiles_batch, dim3(b.blocks[kBatchTile]), dim3(kDepositThreads), b.deposit_lds, s,
+                     b.members, tab(kBatchTile), n);
+  hipLaunchKernelGGL(k_sum_doubles_batch, dim3(b.blocks[kBatchSum]), dim3(kThreads), 0, s, b.members, tab(kBatchSum),
+                     n);
+  COLA_HIP(hipGetLastError());
+  return kOk;
+}
+
+int launch_readout_kick_drift_batch(const PmBatch& b, hipStream_t s)
+{
+  if (b.blocks[kBatchRead] == 0) return kOk;
+  hipLaunchKernelGGL(k_readout_kick_drift_batch, dim3(b.blocks[kBatchRead]), dim3(kThreads), 0, s, b.members,
+                     b.start + (size_t)kBatchRead * (b.n + 1), b.n);
   COLA_HIP(hipGetLastError());
   return kOk;
 }

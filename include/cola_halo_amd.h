@@ -153,6 +153,32 @@ int cola_set_step_graph(cola_context* ctx, int on);
    Equals n separate cola_step calls bit for bit. */
 int cola_step_batch(cola_context* const* ctxs, int n, float omega_m, const float* avel1, const float* apos1);
 
+/* Ensembles (the reference runs its realizations one after another,
+   main.c:117-121 with seed = random_seed + i; no counterpart for running
+   them together).  An ensemble steps n contexts of any sizes with one
+   kernel launch per phase over the workgroups of every realization: mass
+   slot, wrap + tile counts, scan (3), scatter, deposit, grid sums, z r2c,
+   y forward, fused x, y inverse (3 axes), z c2r (3 axes), readout + kick +
+   drift: 14 launches per ensemble step, whatever n.  Each realization keeps
+   its own tiles, radix schedule and launch shapes, and the particles equal
+   those of cola_step on each context bit for bit.  The contexts need their
+   particles set, no residual net attached, and an x pass that fuses (every
+   grid of 2^a 3^b 5^c with a last radix of 3, 4, 5 or 8 does).  An ensemble
+   step orders after the work queued on the member contexts and before
+   their later work; changing a member's tiles or FFT shapes rebuilds the
+   ensemble's tables at its next step.  Destroy the ensemble before its
+   contexts. */
+typedef struct cola_ensemble cola_ensemble;
+int cola_ensemble_create(cola_context* const* ctxs, int n, cola_ensemble** out);
+/* One step of every member (cola_step's meaning, per-member avel1 / apos1). */
+int cola_ensemble_step(cola_ensemble* e, float omega_m, const float* avel1, const float* apos1);
+/* Graph mode: the 14 launches are captured once and replayed; per-step values
+   reach the kernels through one host-to-device copy before each replay. */
+int cola_ensemble_set_graph(cola_ensemble* e, int on);
+/* Kernel launches per ensemble step (0 before the first step). */
+int cola_ensemble_launches(const cola_ensemble* e);
+int cola_ensemble_destroy(cola_ensemble* e);
+
 /* cola.c:341-416 cola_set_snapshot (cola.h:18, main.c:346): kick/drift to
    aout, add the 2LPT velocity, km/s; writes np records in lattice-ID order
    into the caller's host array. */
@@ -359,6 +385,12 @@ int cola_resnet_set_fusion(cola_resnet* net, int on);
    training path's data gradient).  The two orders sum the same products in
    another order: results agree to float rounding, not bit for bit. */
 int cola_resnet_set_plane_order(cola_resnet* net, int by_input_planes);
+/* Launch shape of the last two layers in a context's force (no reference
+   counterpart): on = 1 (with the fused first pass and input-plane order)
+   runs layer 3 and the last layer as one launch after the force
+   transforms, h3 kept in LDS; on = 0 (default) writes h3 to memory between
+   two launches.  Both give the same forces bit for bit. */
+int cola_resnet_set_head_fusion(cola_resnet* net, int on);
 /* Net-weights file: the trained parameters and scales, independent of the
    grid size (the net is a convolution over cells).  Little-endian layout:
    char magic[8] "COLANET1", int32 nparam, int32 0, float in_scale,
@@ -437,6 +469,31 @@ int cola_confirm_parameters(const cola_params* p);
    when the file cannot be opened (write.c:23-24). */
 int cola_write_gadget(const char* filename, const cola_particle_min* p, long long np, float a, float boxsize,
                       float omega_m, float h, int use_long_id);
+
+/* read.c:30-168 read_snapshot on one rank: a GADGET-1 snapshot written as
+   one file `filename` or as filename.0, filename.1, ... (header num_files;
+   find_files, read.c:171-201).  Type-1 particles of every file in file
+   order: positions wrapped into [0, L) (read.c:99-102), velocities times
+   sqrt(a) back to km/s (read.c:76, 130-132), id 0 (read.c:137).  info gets
+   the file count (0 when neither name exists: not an error, as read.c
+   returns 0), the particles read and the last header's np_total, a,
+   boxsize, omega_m and h.  With out == NULL only the headers are read and
+   np_read is the particle count.  Errors: 15000 a file cannot be opened,
+   15001 more particles than np_out, 15005 a record marker does not match
+   (read.c:16-27). */
+typedef struct {
+  int nfiles;
+  long long np_read;
+  long long np_total;
+  float a, boxsize, omega_m, h;
+} cola_gadget_info;
+int cola_read_gadget(const char* filename, cola_particle_min* out, long long np_out, cola_gadget_info* info);
+
+/* The snapshot records the output functions read (COLA_SOURCE_SNAPSHOT,
+   cola_fof_find_halos, cola_get_snapshot) set from host records, e.g. a
+   snapshot read back from a file (halo_main.c's read_snapshot; no separate
+   reference function).  np_in must equal the context's particle count. */
+int cola_set_snapshot_records(cola_context* ctx, const cola_particle_min* in, size_t np_in);
 
 /* Random particle subsample (subsample.c).  The generator is a RANLUX
    (ranlxd1) stream seeded with 2 * seed (subsample_init, subsample.c:19-29,

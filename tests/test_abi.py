@@ -61,3 +61,11 @@ def test_fft_radix_schedules():
     assert ch.fft_radices(768, 3) == [4, 8, 8, 3]
     assert ch.fft_radices(768, 4) == [8, 4, 8, 3]
     assert ch.fft_radices(7 * 16, 0) == [] and ch.fft_radices(768, 5) == []
+
+
+def test_ensemble_argument_errors_without_gpu():
+    h = C.c_void_p()
+    assert ch.lib().cola_ensemble_create(None, 0, C.byref(h)) == ch.ERR_ARG
+    assert ch.lib().cola_ensemble_step(None, C.c_float(0.3), None, None) == ch.ERR_ARG
+    assert ch.lib().cola_ensemble_launches(None) == 0
+    assert ch.lib().cola_ensemble_destroy(None) == 0

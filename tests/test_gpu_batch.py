@@ -83,3 +83,92 @@ def test_batch_snapshot_after_kick_matches_single_run(eh_table):
         with _ctx(nc, ng, box, eh_table) as c:
             (a, ref), = ch.run_cola(c, OM, 1.0, 49, seed, [0.0])
         assert np.array_equal(snap, ref), (nc, seed)
+
+
+def _ens_members(eh_table):
+    """Realizations of different sizes (grids 96 to 192, one with radix 5)
+    and their own launch shapes and radix schedules."""
+    shapes = [(32, 96, 100.0, 3), (40, 120, 125.0, 4), (48, 144, 150.0, 5), (64, 192, 200.0, 6), (32, 96, 100.0, 7)]
+
+    def make(i):
+        nc, ng, box, _ = shapes[i]
+        c = _ctx(nc, ng, box, eh_table)
+        if i == 1:
+            c.set_fft_schedule(3)
+        if i == 2:
+            c.set_tiling(16, 16, 16, 2)
+        if i == 3:
+            c.set_fft_tiles(4, 2, 4, 2)
+        if i == 4:
+            c.set_fft_schedule(1)
+        return c
+    return shapes, make
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_ensemble_equals_single_runs(eh_table, graph):
+    """One launch per phase over all realizations gives each realization's
+    own run bit for bit."""
+    print(f"ensemble graph={graph}", flush=True)
+    shapes, make = _ens_members(eh_table)
+    ctxs = [make(i) for i in range(len(shapes))]
+    try:
+        with ch.Ensemble(ctxs, graph=graph) as ens:
+            snaps = ch.run_cola_batch(ctxs, OM, 1.0, 10, [s[3] for s in shapes], ensemble=ens)
+            assert ens.launches() == 14
+    finally:
+        for c in ctxs:
+            c.close()
+    for i, ((nc, ng, box, seed), snap) in enumerate(zip(shapes, snaps)):
+        with make(i) as c:
+            (a, ref), = ch.run_cola(c, OM, 1.0, 10, seed, [0.0])
+        assert np.array_equal(snap, ref), (nc, seed)
+
+
+def test_ensemble_rebuilds_after_member_changes(eh_table):
+    """Retiling a member between ensemble steps rebuilds the tables; every
+    step still equals the member's own cola_step."""
+    print("ensemble retile", flush=True)
+    shapes = [(32, 96, 100.0, 3), (48, 144, 150.0, 4)]
+    ctxs = [_ctx(nc, ng, box, eh_table) for nc, ng, box, _ in shapes]
+    refs = [_ctx(nc, ng, box, eh_table) for nc, ng, box, _ in shapes]
+    try:
+        for c, r, s in zip(ctxs, refs, shapes):
+            for x in (c, r):
+                x.lpt_set_displacement(0.1, OM, s[3])
+                x.set_times(x.times()[0], 0.1)
+        with ch.Ensemble(ctxs) as ens:
+            da = 0.1
+            for istep in range(1, 7):
+                avel1, apos1 = (istep + 0.5) * da, (istep + 1.0) * da
+                if istep == 3:
+                    ctxs[0].set_tiling(16, 16, 16, 2)
+                    ctxs[1].set_fft_tiles(4, 1, 4, 3)
+                    refs[0].set_tiling(16, 16, 16, 2)
+                    refs[1].set_fft_tiles(4, 1, 4, 3)
+                ens.step(OMF, avel1, apos1)
+                for r in refs:
+                    r.step(OMF, avel1, apos1)
+                for c, r in zip(ctxs, refs):
+                    assert np.array_equal(c.get_particles(), r.get_particles()), istep
+                    assert c.times() == r.times()
+                    assert c.last_mass_sum() == pytest.approx(r.last_mass_sum(), abs=1e-6)
+    finally:
+        for c in ctxs + refs:
+            c.close()
+
+
+def test_ensemble_refuses_member_with_net(eh_table):
+    print("ensemble net refusal", flush=True)
+    with _ctx(32, 96, 100.0, eh_table) as a:
+        a.lpt_set_displacement(0.1, OM, 3)
+        a.set_times(a.times()[0], 0.1)
+        net = ch.ResidualNet(96)
+        a.set_residual_net(net)
+        with ch.Ensemble([a]) as ens:
+            with pytest.raises(ch.ColaError):
+                ens.step(OMF, 0.15, 0.2)
+            a.set_residual_net(None)
+            ens.step(OMF, 0.15, 0.2)
+            assert a.times() == (np.float32(0.2), np.float32(0.15))
+        net.close()
