@@ -43,12 +43,14 @@ struct ResNet {
   float* d_a = nullptr;      // scale factor of standalone evaluations
   double* d_fsum = nullptr;  // sums of the residual the last corrected force added, per component
   float* d_fi = nullptr;     // the last corrected force grids, interleaved [cell][3] (read by the readout)
+  float* d_trash = nullptr;  // stores of lanes without an output cell (k_conv3h)
   float out_scale = 1.0f;    // residual = out_scale * y
   bool scale_set = false;    // out_scale fitted to training targets (or set by the caller)
   float in_scale = 1.0f;     // feature = asinh(delta) * in_scale
   float restrict_side = 0.125f;  // training targets: fine force averaged with weights (w, 1-2w, w) per axis
   bool fuse12 = true;            // inference: features and layers 1-2 in one launch (resnet_layers12)
   bool roll = true;              // 16-channel-input layers by input planes (k_conv12r, k_conv16r, k_conv_head_r)
+  bool fuse34 = false;           // inference by input planes: layer 3 and the last layer in one launch (k_conv3h)
 
   // training set: per sample the feature grid (bf16 [n^3]), the target
   // residual (bf16 [3][n^3]) and the scale factor a
@@ -94,6 +96,10 @@ int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s
 // resnet_layer1 and layer 2 bit for bit.
 int resnet_layers12(ResNet* r, const float* delta, long long row, const float* pa, hipStream_t s);
 int resnet_layer3(ResNet* r, hipStream_t s);
+// Layer 3 and the last layer in one launch (from h2, after the force
+// transforms): the corrected force into d_fi as resnet_layer3 followed by
+// resnet_head_add, bit for bit, without writing h3.  By input planes only.
+int resnet_layer3_head(ResNet* r, float* const out[3], long long row, hipStream_t s);
 // loss against target (bf16 [3][n^3] or fp32 [3][n^3]) and gradient into d_grad
 int resnet_backward(ResNet* r, const unsigned short* target_bf16, const float* target_f32, hipStream_t s);
 int resnet_adam(ResNet* r, float lr, hipStream_t s);

@@ -88,6 +88,7 @@ struct ConvOut {
   float scale = 0.0f;
   double* sum = nullptr;  // kFwdAdd: the three components' sums of what was added
   float* fi = nullptr;    // kFwdAdd: the corrected forces go here, interleaved [cell][3], instead of add[]
+  float* trash = nullptr; // k_conv3h: 64 floats the lanes without an output cell store to
 };
 
 // 16-channel convolution with operand reuse.  A wave computes kConvTy output
@@ -411,28 +412,28 @@ constexpr int vmcnt_cap(int n) { return n < 63 ? n : 63; }
 // into ap (plane p - 1).  `row` points at the lane's centre cell of brick
 // row 0 (its 16-byte half), `shr` at its z neighbour; brick rows are
 // 2 kBrickZ uint4 apart.
+template <int T>
 __device__ inline void roll_plane(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
-                                  const uint4 (&b)[kRollChunks], floatx4 (&am)[kConvTy], floatx4 (&a0)[kConvTy],
-                                  floatx4 (&ap)[kConvTy])
+                                  const uint4 (&b)[kRollChunks], floatx4 (&am)[T], floatx4 (&a0)[T], floatx4 (&ap)[T])
 {
   constexpr int kRow = 2 * kBrickZ;
   const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
   uint4 prev = zero4;
   uint4 cen = row[0], sh = shr[0];
 #pragma unroll
-  for (int j = 0; j < kConvTy + 2; ++j) {  // brick row j feeds output rows j - 2, j - 1, j
+  for (int j = 0; j < T + 2; ++j) {  // brick row j feeds output rows j - 2, j - 1, j
     uint4 ncen = zero4, nsh = zero4;
-    if (j + 1 < kConvTy + 2) {
+    if (j + 1 < T + 2) {
       ncen = row[(j + 1) * kRow];
       nsh = shr[(j + 1) * kRow];
     }
     __builtin_amdgcn_sched_barrier(0);  // keep the next row's reads ahead of this row's products
-    if (j < kConvTy) {  // dy = -1: output row j
+    if (j < T) {  // dy = -1: output row j
       am[j] = mfma(sh, b[0], am[j]);
       a0[j] = mfma(sh, b[5], a0[j]);
       ap[j] = mfma(sh, b[10], ap[j]);
     }
-    if (j >= 1 && j - 1 < kConvTy) {  // dy = 0 and (dy,dz) = (-1,0) | (0,0): output row j - 1
+    if (j >= 1 && j - 1 < T) {  // dy = 0 and (dy,dz) = (-1,0) | (0,0): output row j - 1
       am[j - 1] = mfma(sh, b[1], am[j - 1]);
       a0[j - 1] = mfma(sh, b[6], a0[j - 1]);
       ap[j - 1] = mfma(sh, b[11], ap[j - 1]);
@@ -606,10 +607,12 @@ k_conv16r(const unsigned short* __restrict__ in, const unsigned short* __restric
 constexpr int kHeadChunks = 5;
 
 // Products of one input plane into d (one tile per output row, fresh).
+// ROW: uint4 between input rows (two per cell).
+template <int ROW = 2 * kBrickZ>
 __device__ inline void head_plane(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
                                   const uint4 (&b)[kHeadChunks], floatx4 (&d)[kConvTy])
 {
-  constexpr int kRow = 2 * kBrickZ;
+  constexpr int kRow = ROW;
   const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
   const floatx4 fz = {0.0f, 0.0f, 0.0f, 0.0f};
   uint4 prev = zero4;
@@ -764,6 +767,220 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
   }
   __builtin_amdgcn_s_waitcnt(0);
   if (MODE == kFwdAdd && o.sum) {
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) added += __shfl_xor(added, off);  // within the 16-lane group
+    if (r == 0 && g < 3) atomicAdd(o.sum + g, (double)added);
+  }
+}
+
+// ---- layer 3 and the last layer in one pass -------------------------------
+// k_conv16r<kFwdRelu> (h2 -> h3) and k_conv_head_r<kFwdAdd> (h3 -> corrected
+// force) fused, so that h3 never goes to memory (32 bytes per cell written
+// and read back).  A unit owns the output cells y0 .. y0 + 15, z0 .. z0 + 29
+// (z blocks of 30, the last one ragged: its cells wrap around and are not
+// stored) of planes x0 .. x1 - 1.  Layer 3 makes h3 on the region the last
+// layer reads, rows y0 - 1 .. y0 + 16 and cells z0 - 1 .. z0 + 30 (18 x 32,
+// 1.2 times the cells it owns): each wave 9 rows of one 16-cell column,
+// from h2 bricks of 20 x 34 cells in a 4-slot ring loaded three planes
+// ahead, as in k_conv16r.  Step i (h2 plane p = x0 - 2 + i) rolls plane p
+// into the three accumulator sets and writes the completed h3 plane p - 1
+// (bias, relu, bf16) into LDS slot (p - 1) & 1; after the step's barrier the
+// last layer consumes that slot as k_conv_head_r consumes an input plane and
+// output plane p - 2 is complete.  Per output cell the products and sums
+// run in the order of the two separate launches: the force is theirs bit
+// for bit.  The first steps' h3 planes (before x0 - 1) are incomplete; they
+// only reach output planes before x0, which are not stored.
+constexpr int kT3 = 9;                                  // layer-3 rows per wave
+constexpr int kBrick3Y = 2 * kT3 + 2;                   // h2 brick rows
+constexpr int kPlane3Cells = kBrick3Y * kBrickZ;        // 680
+constexpr int kRoll3Loads = (2 * kPlane3Cells + kThreads - 1) / kThreads;  // 6 loads per wave and plane
+constexpr int kRoll3SlotChunks = kRoll3Loads * kThreads;
+constexpr int kH3Y = 2 * kT3, kH3Z = 32;                // h3 region of a unit
+constexpr int kOut3Z = kH3Z - 2;                        // output cells along z
+constexpr int kH3SlotChunks = 2 * kH3Y * kH3Z;
+constexpr size_t kConv3hBytes = sizeof(uint4) * (kRingSlots * kRoll3SlotChunks + 2 * kH3SlotChunks);
+static_assert(kConv3hBytes <= 160 * 1024, "one workgroup per CU");
+
+// h2 plane `plane` (periodic) into the ring slot of plane `slot_plane`: brick
+// rows y0 - 2 .. y0 + 17, cells z0 - 2 .. z0 + 31 (padding chunks repeat the
+// last cell, so every lane issues every load).
+__device__ inline void load_plane3_to(uint4* ring, const unsigned short* __restrict__ in, int n, int slot_plane,
+                                      int plane, int y0, int z0)
+{
+  const int lane = threadIdx.x & 63;
+  uint4* slot = ring + (slot_plane & (kRingSlots - 1)) * kRoll3SlotChunks;
+  const long long xoff = (long long)wrapi(plane, n) * n;
+#pragma unroll
+  for (int k = 0; k < kRoll3Loads; ++k) {
+    const int base = k * kThreads + (threadIdx.x & ~63);
+    const int c = min(base + lane, 2 * kPlane3Cells - 1);
+    const int q = c & 1, cell = c >> 1;
+    const int zz = cell % kBrickZ, yy = cell / kBrickZ;
+    const long long src = (xoff + wrapi(y0 + yy - 2, n)) * n + wrapi(z0 + zz - 2, n);
+    __builtin_amdgcn_global_load_lds(static_cast<const void*>(in + src * 16 + q * 8),
+                                     (__attribute__((address_space(3))) void*)(slot + base), 16, 0, 0);
+  }
+}
+
+// s_waitcnt: at most n vector-memory operations outstanding and every LDS
+// access done.
+constexpr int vmcnt_lgkm0_imm(int n) { return (n & 15) | (7 << 4) | (0 << 8) | ((n >> 4) << 14); }
+
+// The wait of step i: n = the operations issued after the next plane's
+// loads, on the ladder 2 L + 8 k (L = kRoll3Loads, k <= 4) the steps make.
+__device__ inline void conv3h_wait(int n)
+{
+  constexpr int L2 = 2 * kRoll3Loads;
+  if (n >= L2 + 32) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 32));
+  } else if (n >= L2 + 24) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 24));
+  } else if (n >= L2 + 16) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 16));
+  } else if (n >= L2 + 8) {
+    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 8));
+  } else {
+    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2));
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1)))
+k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wr3, const float* __restrict__ b3,
+         const unsigned short* __restrict__ wh, const float* __restrict__ b4, int n, int march, ConvOut o)
+{
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const bool lo = g < 2;
+  const int wave = threadIdx.x >> 6;
+  const int wz = wave & 1, wy = wave >> 1;
+  const int half = g & 1;
+  uint4 b[kRollChunks];
+#pragma unroll
+  for (int c = 0; c < kRollChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wr3 + ((long long)c * 64 + lane) * 8);
+  uint4 bh[kHeadChunks];
+#pragma unroll
+  for (int c = 0; c < kHeadChunks; ++c) bh[c] = *reinterpret_cast<const uint4*>(wh + ((long long)c * 64 + lane) * 8);
+  float bch[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) bch[i] = b3[4 * g + i];
+  const float bo = g < 3 ? b4[g] : 0.0f;
+  extern __shared__ uint4 smem[];
+  uint4* ring = smem;                                   // [slot][20 rows][34 cells][2 halves]
+  uint4* h3 = smem + kRingSlots * kRoll3SlotChunks;     // [2 slots][18 rows][32 cells][2 halves]
+  // layer 3: rows 9 wy .. 9 wy + 8 of the h3 region, cells 16 wz + r
+  const int zc = 1 + 16 * wz + r;
+  const int zs = lo ? zc - 1 : zc + 1;
+  const int rowoff = (kT3 * wy * kBrickZ + zc) * 2 + half;
+  const int shoff = (kT3 * wy * kBrickZ + zs) * 2 + half;
+  // last layer: output rows 8 wy .. 8 wy + 7, cells z0 + 14 wz + r (the
+  // second column's first two cells are the first column's last two)
+  const int hz = 1 + 14 * wz + r;
+  const int hzs = lo ? hz - 1 : hz + 1;
+  const int hrowoff = (kConvTy * wy * kH3Z + hz) * 2 + half;
+  const int hshoff = (kConvTy * wy * kH3Z + hzs) * 2 + half;
+  const int up = lane >= 16 ? lane - 16 : lane;
+  const int g2 = 32 + r;
+  float* const gout = g == 0 ? o.add[0] : (g == 1 ? o.add[1] : o.add[2]);
+  constexpr int L = kRoll3Loads, F = kConvTy, S = kConvTy;
+  float added = 0.0f;
+  floatx4 acc[3][kT3];
+  const int nzb = (n + kOut3Z - 1) / kOut3Z, nyb = n / kBlockY;
+  const long long units = (long long)((n + march - 1) / march) * nyb * nzb;
+  for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+    const int z0 = (int)(u % nzb) * kOut3Z;
+    const long long t2 = u / nzb;
+    const int y0 = (int)(t2 % nyb) * kBlockY;
+    const int x0 = (int)(t2 / nyb) * march, x1 = min(x0 + march, n);
+    const int ybase = y0 + kConvTy * wy;
+    const int zcell = z0 + 14 * wz + r;
+    const bool zok = zcell < n && (wz == 0 || r >= 2);  // a cell of the grid this lane alone stores
+    const int zld = zcell < n ? zcell : zcell - n;       // in range for the old-value loads
+    floatx4 part[kConvTy];
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) part[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s3 = 0; s3 < 3; ++s3)
+#pragma unroll
+      for (int t = 0; t < kT3; ++t) acc[s3][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    // the previous unit's stores, refills and LDS reads are done
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) load_plane3_to(ring, in, n, x0 - 2 + k, x0 - 2 + k, y0, z0);
+    constexpr int w_pro = vmcnt_imm(2 * kRoll3Loads);
+    __builtin_amdgcn_s_waitcnt(w_pro);
+    asm volatile("s_barrier" ::: "memory");
+    // output plane p - 2 is stored at steps i = 4 .. x1 - x0 + 3
+    auto stores = [&](int j) { return j >= 4 && j < x1 - x0 + 4; };
+    auto step = [&](auto rot, int i) {
+      constexpr int R = decltype(rot)::value;
+      const int p = x0 - 2 + i;
+      const bool store = stores(i);
+      float fold[kConvTy];
+      if (store) {  // every lane (group 3 reads group 2's grid; wrapped cells a valid one): F loads
+#pragma unroll
+        for (int t = 0; t < kConvTy; ++t) fold[t] = gout[((long long)(p - 2) * n + ybase + t) * o.row + zld];
+      }
+      // h2 plane p + 3 into the slot plane p - 1 left (the last plane again past x1 + 1)
+      load_plane3_to(ring, in, n, p + 3, min(p + 3, x1 + 1), y0, z0);
+      const uint4* slot = ring + (p & (kRingSlots - 1)) * kRoll3SlotChunks;
+      roll_plane<kT3>(slot + rowoff, slot + shoff, lo, b, acc[(R + 1) % 3], acc[R], acc[(R + 2) % 3]);
+      // h3 plane p - 1 complete: bias, relu, bf16 into its LDS slot
+      {
+        uint2* dst = reinterpret_cast<uint2*>(h3 + ((p - 1) & 1) * kH3SlotChunks);
+        floatx4(&a)[kT3] = acc[(R + 2) % 3];
+#pragma unroll
+        for (int t = 0; t < kT3; ++t) {
+          const int cell = (kT3 * wy + t) * kH3Z + 16 * wz + r;
+          dst[cell * 4 + g] = pack_bf16x4(fmaxf(a[t][0] + bch[0], 0.0f), fmaxf(a[t][1] + bch[1], 0.0f),
+                                          fmaxf(a[t][2] + bch[2], 0.0f), fmaxf(a[t][3] + bch[3], 0.0f));
+          a[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+      }
+      // h2 plane p + 1 landed (issued at step i - 2 or in the prologue; after
+      // it: that step's stores, step i - 1's old values, loads and stores,
+      // this step's old values and loads) and every wave's h3 writes are done
+      int after = 2 * L;
+      if (stores(i - 2)) after += S;
+      if (stores(i - 1)) after += F + S;
+      if (store) after += F;
+      conv3h_wait(after);
+      asm volatile("s_barrier" ::: "memory");
+      // the last layer on h3 plane p - 1
+      const uint4* hs = h3 + ((p - 1) & 1) * kH3SlotChunks;
+      floatx4 d[kConvTy];
+      head_plane<2 * kH3Z>(hs + hrowoff, hs + hshoff, lo, bh, d);
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const float sv = lane_read(part[t][k], up);
+          if (g == 1 || g == 2) d[t][k] += sv;
+        }
+        part[t] = d[t];
+      }
+      if (store) {  // S stores: every lane of groups 0-2 (the ones without a cell into trash)
+#pragma unroll
+        for (int t = 0; t < kConvTy; ++t) {
+          const float v0 = lane_read(d[t][0], g2), v1 = lane_read(d[t][1], g2), v2 = lane_read(d[t][2], g2);
+          const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
+          if (g < 3) {
+            const long long line = (long long)(p - 2) * n + ybase + t;
+            const float dv = o.scale * v;
+            float* dst = zok ? o.fi + (line * n + zcell) * 3 + g : o.trash + lane;
+            *dst = fold[t] + dv;
+            if (zok) added += dv;
+          }
+        }
+      }
+    };
+    const int np = x1 - x0 + 4;  // h2 planes x0 - 2 .. x1 + 1, in whole rotations
+    for (int i = 0; i < np; i += 3) {
+      step(std::integral_constant<int, 0>{}, i);
+      step(std::integral_constant<int, 1>{}, i + 1);
+      step(std::integral_constant<int, 2>{}, i + 2);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  if (o.sum) {
 #pragma unroll
     for (int off = 8; off > 0; off >>= 1) added += __shfl_xor(added, off);  // within the 16-lane group
     if (r == 0 && g < 3) atomicAdd(o.sum + g, (double)added);
@@ -1510,6 +1727,7 @@ int resnet_init(ResNet* r, int n, unsigned seed)
   COLA_TRY(dev_alloc(&r->d_loss, 1));
   COLA_TRY(dev_alloc(&r->d_fsum, 3));
   COLA_TRY(dev_alloc(&r->d_fi, 3 * nc));
+  COLA_TRY(dev_alloc(&r->d_trash, 256));
   COLA_TRY(dev_alloc(&r->d_a, 1));
   r->wpart_waves = kWgradBlocks * (kThreads / 64);
   // He-normal initial weights (std sqrt(2 / fan_in)), zero biases and moments
@@ -1533,7 +1751,7 @@ void resnet_free(ResNet* r)
   };
   fr(r->d_param), fr(r->d_m), fr(r->d_v), fr(r->d_grad);
   for (int l = 0; l < kNetLayers; ++l) fr(r->d_wf[l]), fr(r->d_wb[l]), fr(r->d_wr[l]);
-  fr(r->d_x), fr(r->d_f1), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a), fr(r->d_fsum), fr(r->d_fi);
+  fr(r->d_x), fr(r->d_f1), fr(r->d_y), fr(r->d_loss), fr(r->d_wpart), fr(r->d_a), fr(r->d_fsum), fr(r->d_fi), fr(r->d_trash);
   for (int l = 0; l < 3; ++l) fr(r->d_h[l]);
   for (int l = 0; l < 2; ++l) fr(r->d_g[l]);
   for (auto p : r->s_feat) fr(p);
@@ -1601,6 +1819,30 @@ ConvLaunch conv_launch_roll(int n)
   return c;
 }
 
+// k_conv3h: one workgroup per CU; a march of m output planes takes
+// ceil((m + 4) / 3) rotations; the march with the least work on the busiest
+// workgroup.
+ConvLaunch conv_launch_3h(int n)
+{
+  const long long cols = (long long)(n / kBlockY) * ((n + kOut3Z - 1) / kOut3Z);
+  const long long wgs = 256;
+  long long best = -1;
+  ConvLaunch c{1, n};
+  for (int nseg = 1; nseg <= n; ++nseg) {
+    const int march = (n + nseg - 1) / nseg;
+    if ((long long)(nseg - 1) * march >= n) continue;
+    const long long units = (long long)((n + march - 1) / march) * cols;
+    const long long blocks = std::min<long long>(wgs, units);
+    const long long cost = (units + blocks - 1) / blocks * (3 * ((march + 6) / 3));
+    if (best < 0 || cost < best) {
+      best = cost;
+      c.blocks = (unsigned)blocks;
+      c.march = march;
+    }
+  }
+  return c;
+}
+
 constexpr size_t kRing16Bytes = sizeof(uint4) * 2 * kRingSlots * kPlaneCells;
 constexpr size_t kRing8Bytes = sizeof(uint4) * kRingSlots * kPlaneCells;
 
@@ -1618,6 +1860,8 @@ int conv_lds_attributes()
                       reinterpret_cast<const void*>(&k_conv_head_r<kFwdAdd>),
                       reinterpret_cast<const void*>(&k_conv_head_r<kFwdLinear>)};
   for (const void* k : kr) COLA_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRollRingBytes));
+  COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv3h), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kConv3hBytes));
   COLA_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv8), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)kRing8Bytes));
   done = true;
@@ -1783,6 +2027,25 @@ int resnet_head_add(ResNet* r, float* const out[3], long long row, hipStream_t s
   o.sum = r->d_fsum;
   o.fi = r->d_fi;
   return conv_layer_4(r, kFwdAdd, o, s);
+}
+
+int resnet_layer3_head(ResNet* r, float* const out[3], long long row, hipStream_t s)
+{
+  COLA_TRY(conv_lds_attributes());
+  ConvOut o;
+  for (int a = 0; a < 3; ++a) o.add[a] = out[a];
+  o.row = row;
+  o.scale = r->out_scale;
+  hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, s, r->d_fsum);
+  o.sum = r->d_fsum;
+  o.fi = r->d_fi;
+  o.trash = r->d_trash;
+  const ConvLaunch cl = conv_launch_3h(r->n);
+  hipLaunchKernelGGL(k_conv3h, dim3(cl.blocks), dim3(kThreads), kConv3hBytes, s, r->d_h[1], r->d_wr[2],
+                     r->d_param + param_offset(2) + 27 * 16 * 16, r->d_wr[3], r->d_param + param_offset(3) + 27 * 16 * 3,
+                     r->n, cl.march, o);
+  COLA_HIP(hipGetLastError());
+  return kOk;
 }
 
 int resnet_backward(ResNet* r, const unsigned short* tb, const float* tf, hipStream_t s)

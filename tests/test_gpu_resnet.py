@@ -380,3 +380,36 @@ def test_fused_first_layers_equal_separate_launches(eh_table, nc, ng, box, order
     assert not np.array_equal(plain, fused)
     assert np.array_equal(fused, separate)
     assert np.array_equal(fused, again)
+
+
+@pytest.mark.parametrize("nc,ng,box", [(32, 64, 100.0), (48, 96, 150.0), (256, 768, 500.0)])
+def test_fused_last_layers_equal_separate_launches(eh_table, nc, ng, box):
+    """Layer 3 and the last layer in one launch (h3 kept in LDS, z blocks of
+    30 cells with a ragged last block at every size here) give the corrected
+    force grid of the two separate launches bit for bit; the particles'
+    forces agree up to the rounding of the residual's mean (its double sums
+    run in another order)."""
+    print(f"fused layers 3-4, {nc}^3 / {ng}^3", flush=True)
+    k, p = eh_table
+    with ch.ResidualNet(ng, seed=9) as net, ch.Context(nc, ng, box) as c:
+        net.set_scales(1.3, 0.7)
+        c.power_init_table(k, p, 0.1, 0.8, OM, 1.0 - OM)
+        c.lpt_set_displacement(0.1, OM, 4)
+        c.set_times(c.times()[0], 0.1)
+        omf = float(np.float32(OM))
+        for istep in range(1, 4):
+            c.step(omf, (istep + 0.5) * 0.1, (istep + 1.0) * 0.1)
+        c.move_particles()
+        c.set_residual_net(net)
+        grids = {}
+        forces = {}
+        for fused in (True, False, True):
+            net.set_head_fusion(fused)
+            c.pm_calculate_forces()
+            grids.setdefault(fused, []).append(np.stack([c.get_force_grid(a)[:, :, :ng] for a in range(3)]))
+            forces.setdefault(fused, []).append(c.get_forces())
+    assert np.array_equal(grids[True][0], grids[False][0])
+    assert np.array_equal(grids[True][0], grids[True][1])
+    scale = np.abs(forces[False][0]).max()
+    assert np.abs(forces[True][0] - forces[False][0]).max() <= 1e-6 * scale
+    assert np.array_equal(forces[True][0], forces[True][1])
