@@ -152,6 +152,136 @@ int cola_write_gadget(const char* filename, const cola_particle_min* p, long lon
   return COLA_OK;
 }
 
+namespace {
+
+// A GADGET-1 file being read: 4-byte record markers around each block.
+class Reader {
+ public:
+  explicit Reader(const std::string& name) : fp_(std::fopen(name.c_str(), "rb")) {}
+  ~Reader()
+  {
+    if (fp_) std::fclose(fp_);
+  }
+  bool ok() const { return fp_ != nullptr; }
+  bool get(void* data, size_t bytes) { return bytes == 0 || std::fread(data, 1, bytes, fp_) == bytes; }
+  // read.c:16-27 check_separator: the marker must equal the block length
+  int separator(long long expected)
+  {
+    int32_t v = -1;
+    if (!get(&v, sizeof(v)) || (long long)v != expected) {
+      set_error("Error: Unable to read snapshot correctly. Separator " + std::to_string(v) +
+                " != " + std::to_string(expected));
+      return 15005;
+    }
+    return COLA_OK;
+  }
+
+ private:
+  FILE* fp_;
+};
+
+int read_header(Reader& r, GadgetHeader* h)
+{
+  int st = r.separator(256);
+  if (st) return st;
+  if (!r.get(h, sizeof(*h))) {
+    set_error("Error: Unable to read snapshot header");
+    return 15005;
+  }
+  return r.separator(256);
+}
+
+// read.c:171-201 find_files: the file count of a snapshot written as one
+// file `base` or as base.0, base.1, ... (0 when neither exists).
+int find_files(const std::string& base, int* nfiles)
+{
+  *nfiles = 0;
+  for (const std::string& name : {base, base + ".0"}) {
+    Reader r(name);
+    if (!r.ok()) continue;
+    GadgetHeader h;
+    const int st = read_header(r, &h);
+    if (st) return st;
+    *nfiles = h.num_files;
+    return COLA_OK;
+  }
+  return COLA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cola_read_gadget(const char* filename, cola_particle_min* out, long long np_out, cola_gadget_info* info)
+{
+  if (!filename || !info || np_out < 0 || (np_out > 0 && !out)) return COLA_ERR_ARG;
+  std::memset(info, 0, sizeof(*info));
+  const std::string base = filename;
+  int nfiles = 0;
+  int st = find_files(base, &nfiles);
+  if (st) return st;
+  if (nfiles <= 0) return COLA_OK;  // read.c:46-47: not found
+  info->nfiles = nfiles;
+  GadgetHeader h;
+  long long nread = 0;
+  for (int num = 0; num < nfiles; ++num) {
+    const std::string name = nfiles > 1 ? base + "." + std::to_string(num) : base;
+    Reader r(name);
+    if (!r.ok()) {
+      set_error("Error: can't open snapshot file " + name + " (read.c)");
+      return 15000;
+    }
+    if ((st = read_header(r, &h))) return st;
+    const long long np = h.np[1];  // type-1 particles only (read.c:75)
+    if (!out) {                   // counting pass: headers only
+      nread += np;
+      continue;
+    }
+    if (nread + np > np_out) {
+      set_error("Error: Not enough space to read snapshot. " + std::to_string(np) + " particles in " + name);
+      return 15001;
+    }
+    const float vfac_back = std::sqrt(h.time);  // read.c:76, GADGET convention back to km/s
+    const float boxsize = (float)h.boxsize;
+    std::vector<float> buf(3 * (size_t)np);
+    if ((st = r.separator(12 * np))) return st;
+    if (!r.get(buf.data(), sizeof(float) * buf.size())) {
+      set_error("Error: short position block in " + name);
+      return 15005;
+    }
+    if ((st = r.separator(12 * np))) return st;
+    for (long long i = 0; i < np; ++i) {
+      cola_particle_min& q = out[nread + i];
+      for (int k = 0; k < 3; ++k) {
+        float x = buf[3 * i + k];
+        if (x < 0.0f) x += boxsize;  // read.c:99-102 periodic wrap
+        if (x >= boxsize) x -= boxsize;
+        q.x[k] = x;
+      }
+      q.id = 0;  // read.c:137
+    }
+    if ((st = r.separator(12 * np))) return st;
+    if (!r.get(buf.data(), sizeof(float) * buf.size())) {
+      set_error("Error: short velocity block in " + name);
+      return 15005;
+    }
+    if ((st = r.separator(12 * np))) return st;
+    for (long long i = 0; i < np; ++i)
+      for (int k = 0; k < 3; ++k) out[nread + i].v[k] = vfac_back * buf[3 * i + k];
+    nread += np;
+  }
+  // read.c:149-156 (the last file's header)
+  info->np_read = nread;
+  info->np_total = ((long long)h.np_total_highword[1] << 32) + (long long)h.np_total[1];
+  info->a = (float)h.time;
+  info->boxsize = (float)h.boxsize;
+  info->omega_m = (float)h.omega0;
+  info->h = (float)h.hubble_param;
+  return COLA_OK;
+}
+
+}  // extern "C"
+
 struct cola_subsampler {
   cola::RanluxStream* rng = nullptr;
   double factor = 0.0;

@@ -193,3 +193,77 @@ residual_net = "{net_file}"
     d = np.abs(pos.astype(np.float64) - plain["x"]) % box
     d = np.minimum(d, box - d)  # periodic distance
     assert np.sqrt(np.mean(d ** 2)) < 0.5 * box / nc  # a correction, not a different run
+
+
+HALO = os.path.join(os.path.dirname(HERE), "cola_halo_amd", "halo")
+
+PARAM_RUN = """nc = 32
+boxsize = 64.0
+random_seed= 7
+nrealization= 1
+ntimestep= 10
+a_final= 1.0
+output_redshifts= {{0.0}}
+omega_m = 0.273
+h       = 0.705
+sigma8  = 0.812
+pm_nc_factor= 3
+loglevel=2
+powerspectrum= "{pk}"
+fof= "{d}/fof"
+linking_factor= 0.2
+snapshot= "{d}/snp"
+subsample= "{d}/sub"
+subsample_factor= 0.05
+coarse_grid= "{d}/grid"
+coarse_grid_nc= 16
+"""
+
+
+def _halo_lines(path):
+    rows = [line.split() for line in open(path)]
+    return np.array([int(r[0]) for r in rows]), np.array([[float(v) for v in r[1:4]] for r in rows]), \
+        np.array([[float(v) for v in r[4:7]] for r in rows])
+
+
+@pytest.mark.skipif(not (os.path.exists(DRIVER) and os.path.exists(HALO)), reason="programs not built")
+def test_halo_program_reads_driver_snapshots(tmp_path):
+    """halo (halo_main.c) on the driver's own snapshot: the snapshot read
+    back (read.c) gives the driver's FoF catalogue (same groups and centres;
+    velocities through v / sqrt(a) * sqrt(a) in float), its coarse grid and
+    the same subsample records."""
+    print("halo program", flush=True)
+    run = tmp_path / "run"
+    run.mkdir()
+    param = tmp_path / "param.lua"
+    param.write_text(PARAM_RUN.format(pk=PK_FILE, d=run))
+    r = subprocess.run([DRIVER, str(param)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr + r.stdout
+    post = tmp_path / "post"
+    post.mkdir()
+    # halo_main.c names snapshots <dir>/<snapshot>_NNN; one file, no .0
+    os.replace(run / "snp00007a.0", post / "snp_000")
+    (tmp_path / "dirs.txt").write_text(f"{post}\n")
+    hparam = tmp_path / "halo.lua"
+    hparam.write_text(PARAM_RUN.format(pk=PK_FILE, d=run).replace(f'"{run}/fof"', '"fof"')
+                      .replace(f'"{run}/snp"', '"snp"').replace(f'"{run}/sub"', '"sub"')
+                      .replace(f'"{run}/grid"', '"grid"'))
+    r = subprocess.run([HALO, str(hparam)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "halo done" in r.stdout
+    n0, x0, v0 = _halo_lines(run / "fof00007a.txt")
+    n1, x1, v1 = _halo_lines(post / "fof_000.txt")
+    assert len(n0) > 10
+    assert np.array_equal(n0, n1)
+    assert np.array_equal(x0, x1)
+    assert np.allclose(v1, v0, rtol=1e-5, atol=1e-3)
+    assert open(run / "grid00007a.b", "rb").read() == open(post / "grid_000.b", "rb").read()
+    s0 = np.frombuffer(open(run / "sub00007a.b", "rb").read()[28:-4], "<f4").reshape(-1, 6)
+    s1 = np.frombuffer(open(post / "sub_000.b", "rb").read()[28:-4], "<f4").reshape(-1, 6)
+    assert s0.shape == s1.shape and len(s0) > 100
+    assert np.array_equal(s0[:, :3], s1[:, :3])
+    assert np.allclose(s0[:, 3:], s1[:, 3:], rtol=1e-5, atol=1e-3)
+    # no snapshot at all: abort code 2 (halo_main.c:139-141)
+    (tmp_path / "dirs.txt").write_text(f"{run}\n")
+    r = subprocess.run([HALO, str(hparam)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 2

@@ -140,6 +140,93 @@ def test_gadget_writer_unwritable_path():
     assert e.value.code == 9000
 
 
+def write_gadget_parts(base, parts, a, box, om=0.273, h=0.705, bad_marker=False):
+    """Independent writer of a snapshot split over len(parts) GADGET-1 files
+    base.0, base.1, ... (num_files in every header, np_total the sum)."""
+    ntot = sum(len(x) for x, _ in parts)
+    for num, (x, v) in enumerate(parts):
+        n = len(x)
+        hdr = bytearray(256)
+        struct.pack_into("<6i", hdr, 0, 0, n, 0, 0, 0, 0)
+        struct.pack_into("<d", hdr, 72, float(a))
+        struct.pack_into("<6I", hdr, 96, 0, ntot & 0xffffffff, 0, 0, 0, 0)
+        struct.pack_into("<i", hdr, 124, len(parts))
+        struct.pack_into("<4d", hdr, 128, float(box), float(om), 1.0 - float(om), float(h))
+        struct.pack_into("<6I", hdr, 168, 0, ntot >> 32, 0, 0, 0, 0)
+        with open(f"{base}.{num}", "wb") as f:
+            for body in (bytes(hdr), np.asarray(x, "<f4").tobytes(), np.asarray(v, "<f4").tobytes(),
+                         np.arange(n, dtype="<u4").tobytes()):
+                f.write(struct.pack("<i", len(body)))
+                f.write(body)
+                f.write(struct.pack("<i", len(body) + (1 if bad_marker and len(body) == 12 * n else 0)))
+
+
+def test_gadget_reader_round_trip(tmp_path):
+    """read.c on the writer's file: positions as written (already in the
+    box), velocities v / sqrt(a) * sqrt(a) with float rounding, ids 0."""
+    rng = np.random.default_rng(5)
+    n = 777
+    snap = np.zeros(n, ch.PARTICLE_MIN_DTYPE)
+    snap["x"] = rng.uniform(0, 100, (n, 3))
+    snap["v"] = rng.normal(0, 300, (n, 3))
+    snap["id"] = np.arange(1, n + 1)
+    a = np.float32(0.5780347)
+    path = str(tmp_path / "snp_000")
+    ch.write_gadget(path, snap, a, 100.0, 0.273, 0.705)
+    rec, info = ch.read_gadget(path)
+    assert info["nfiles"] == 1 and info["np_read"] == n and info["np_total"] == n
+    assert info["a"] == a and info["boxsize"] == np.float32(100.0)
+    assert info["omega_m"] == np.float32(0.273) and info["h"] == np.float32(0.705)
+    assert np.array_equal(rec["x"], snap["x"])
+    vfac = np.float32(1.0 / np.sqrt(np.float64(a)))
+    back = np.float32(np.sqrt(np.float64(a)))          # read.c:76, double sqrt stored to float
+    assert np.array_equal(rec["v"], back * (vfac * snap["v"]).astype(np.float32))
+    assert np.abs(rec["v"] - snap["v"]).max() <= 1e-6 * np.abs(snap["v"]).max()
+    assert not rec["id"].any()
+    assert ch.read_gadget(str(tmp_path / "snp_001"))[0] is None   # missing: not an error
+
+
+def test_gadget_reader_multi_file_and_wrap(tmp_path):
+    """A snapshot in two files (base.0, base.1) is read in file order;
+    positions outside [0, L) are wrapped as read.c:99-102 does."""
+    box, a = np.float32(50.0), np.float32(0.25)
+    x0 = np.array([[-1.0, 10.0, 49.999], [50.0, 0.0, 25.0]], np.float32)
+    x1 = np.array([[51.5, -0.0001, 3.0], [1.0, 2.0, 3.0], [49.0, 60.0, -50.0]], np.float32)
+    v0 = np.full((2, 3), 2.0, np.float32)
+    v1 = np.full((3, 3), -4.0, np.float32)
+    base = str(tmp_path / "snp_000")
+    write_gadget_parts(base, [(x0, v0), (x1, v1)], a, box)
+    rec, info = ch.read_gadget(base)
+    assert info["nfiles"] == 2 and info["np_read"] == 5 and info["np_total"] == 5
+    want = np.concatenate([x0, x1])
+    want = np.where(want < 0, want + box, want)
+    want = np.where(want >= box, want - box, want)
+    assert np.array_equal(rec["x"], want)
+    assert (rec["x"] >= 0).all() and (rec["x"] < box).all()
+    back = np.float32(np.sqrt(np.float64(a)))
+    assert np.array_equal(rec["v"], back * np.concatenate([v0, v1]))
+
+
+def test_gadget_reader_errors(tmp_path):
+    base = str(tmp_path / "snp_000")
+    two = [(np.zeros((4, 3)), np.zeros((4, 3))), (np.zeros((2, 3)), np.zeros((2, 3)))]
+    write_gadget_parts(base, two, 0.5, 10.0, bad_marker=True)
+    with pytest.raises(ch.ColaError) as e:
+        ch.read_gadget(base)
+    assert e.value.code == 15005   # read.c:16-27
+    # one file named base.0 with num_files = 1: read.c then opens `base`
+    # itself (read.c:60-63), which does not exist
+    write_gadget_parts(base, [(np.zeros((4, 3)), np.zeros((4, 3)))], 0.5, 10.0)
+    with pytest.raises(ch.ColaError) as e:
+        ch.read_gadget(base)
+    assert e.value.code == 15000
+    write_gadget_parts(base, two, 0.5, 10.0)
+    out = np.zeros(5, ch.PARTICLE_MIN_DTYPE)
+    info = ch.GadgetInfo()
+    code = ch.lib().cola_read_gadget(base.encode(), out.ctypes.data, 5, ch.C.byref(info))
+    assert code == 15001           # read.c:80-84: more particles than room
+
+
 def test_coarse_grid_file_layout(tmp_path):
     """tools/density_grid.c:1-28: float boxsize, int nc, nc^3 floats in C
     order, int nc."""
