@@ -123,6 +123,7 @@ _SIGNATURES = {
     "cola_ensemble_step": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p, C.c_void_p]),
     "cola_ensemble_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_ensemble_launches": (C.c_int, [C.c_void_p]),
+    "cola_ensemble_set_merge": (C.c_int, [C.c_void_p, C.c_int]),
     "cola_ensemble_destroy": (C.c_int, [C.c_void_p]),
     "cola_set_fft_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "cola_set_fft_tiles_xy": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -825,7 +826,7 @@ class Ensemble:
     workgroups of every realization (cola_ensemble_*).  Close it before its
     contexts."""
 
-    def __init__(self, ctxs, graph=False):
+    def __init__(self, ctxs, graph=False, merge=False):
         self.ctxs = list(ctxs)
         n = len(self.ctxs)
         hs = (C.c_void_p * n)(*[c._h for c in self.ctxs])
@@ -834,6 +835,8 @@ class Ensemble:
         self._h = h.value
         if graph:
             self.set_graph(True)
+        if merge:
+            self.set_merge(True)
 
     def step(self, omega_m, avel1, apos1):
         n = len(self.ctxs)
@@ -843,6 +846,11 @@ class Ensemble:
 
     def set_graph(self, on):
         _check(lib().cola_ensemble_set_graph(self._h, 1 if on else 0))
+
+    def set_merge(self, on):
+        """One launch per transform pass for all members (True) or per pass
+        and FFT kernel variant (False, the default)."""
+        _check(lib().cola_ensemble_set_merge(self._h, 1 if on else 0))
 
     def launches(self):
         """Kernel launches per ensemble step (0 before the first step)."""

@@ -975,6 +975,7 @@ struct cola_ensemble {
   int ring = 0;
   int since_check = 0;  // steps since the members' mass sums were checked
   bool graph_mode = false;
+  bool merge_all = false;  // one transform launch per pass for every member (else per kernel variant)
   bool warmed = false;  // the kernels ran eagerly since the tables were built
   hipGraphExec_t graph = nullptr;
 };
@@ -1029,7 +1030,7 @@ int ensemble_build(cola_ensemble* e)
     e->gen[m] = c->config_gen;
   }
   COLA_TRY(e->pm.init(ms.data(), n));
-  COLA_TRY(e->fft.init(plans.data(), dens.data(), outs.data(), kps.data(), n));
+  COLA_TRY(e->fft.init(plans.data(), dens.data(), outs.data(), kps.data(), n, e->merge_all));
   e->built = true;
   e->warmed = false;
   return kOk;
@@ -1161,6 +1162,17 @@ int cola_ensemble_set_graph(cola_ensemble* e, int on)
   COLA_HIP(hipStreamSynchronize(e->stream));
   e->graph_mode = on != 0;
   if (!e->graph_mode) drop_ensemble_graph(e);
+  return kOk;
+}
+
+int cola_ensemble_set_merge(cola_ensemble* e, int on)
+{
+  if (!e || (on != 0 && on != 1)) return kErrArg;
+  if (e->merge_all == (on != 0)) return kOk;
+  COLA_HIP(hipStreamSynchronize(e->stream));
+  e->merge_all = on != 0;
+  drop_ensemble_graph(e);
+  e->built = false;  // rebuilt at the next step
   return kOk;
 }
 

@@ -110,20 +110,24 @@ struct FftBatchStage {
   const int* start = nullptr;  // device: first workgroup of each job, njobs + 1 entries
 };
 
-// The force transforms (FftPlan::forces) of many grids, one launch per pass
+// The force transforms (FftPlan::forces) of many grids, batched per pass
 // over the workgroups of every grid: z r2c, y forward, fused x, the y inverse
-// of the three axes, the z c2r of the three axes (five launches instead of
-// nine per grid).  Each grid keeps its own plan's radix schedule and tile
-// shapes; a launch runs every job with the widest workgroup and register
-// tile any of its jobs needs, and the butterflies of a job do not depend on
-// either, so the outputs equal each plan's forces() bit for bit.  Every plan
-// must fuse (FftPlan::can_fuse).
+// of the three axes, the z c2r of the three axes (instead of nine launches
+// per grid).  Each grid keeps its own plan's radix schedule and tile
+// shapes; a launch runs its jobs with the widest workgroup and register
+// tile any of them needs, and the butterflies of a job do not depend on
+// either, so the outputs equal each plan's forces() bit for bit.
+// merge_all: one launch per pass for all grids (the kernel variant with the
+// most registers any grid needs); else one launch per pass and kernel
+// variant (register tile, radix set) that the grids' own launches use.
+// Every plan must fuse (FftPlan::can_fuse).
 struct FftForcesBatch {
   std::vector<FftBatchStage> stages;
   void* d_mem = nullptr;
   // plans[i] transforms density[i] (destroyed) into out[3 i + axis] with
   // the operator parameters kp[i].
-  int init(const FftPlan* const* plans, float* const* density, float* const* out, const KOpParams* kp, int n);
+  int init(const FftPlan* const* plans, float* const* density, float* const* out, const KOpParams* kp, int n,
+           bool merge_all);
   int run(hipStream_t s) const;
   void destroy();
 };

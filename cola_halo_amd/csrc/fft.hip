@@ -1231,7 +1231,7 @@ int FftPlan::forces(float* density, float* const out[3], const KOpParams& kp, hi
 }
 
 int FftForcesBatch::init(const FftPlan* const* plans, float* const* density, float* const* out, const KOpParams* kp,
-                         int n)
+                         int n, bool merge_all)
 {
   destroy();
   if (n <= 0 || !plans || !density || !out || !kp) return kErrArg;
@@ -1267,18 +1267,20 @@ int FftForcesBatch::init(const FftPlan* const* plans, float* const* density, flo
     }
   }
   auto align_up = [](size_t v, size_t a) { return (v + a - 1) / a * a; };
+  auto round_e = [](int e) { return e <= 8 ? 8 : e <= 12 ? 12 : e <= 16 ? 16 : 24; };
   std::vector<char> host;
   std::vector<std::pair<size_t, size_t>> offs;  // job table, start table
-  for (int k = 0; k < 5; ++k) {
+  // One stage: the jobs idx of pass k in one launch.
+  auto build = [&](int k, const std::vector<int>& idx) -> int {
     Pass& p = ps[k];
     const bool fused = k == 2;
-    const int nj = fused ? (int)p.xj.size() : (int)p.tj.size();
-    auto args = [&](int j) -> const TileArgs& { return fused ? p.xj[j].A : p.tj[j].A; };
-    auto blocks = [&](int j) { return fused ? p.xj[j].blocks : p.tj[j].blocks; };
+    const int nj = (int)idx.size();
+    auto args = [&](int j) -> const TileArgs& { return fused ? p.xj[idx[j]].A : p.tj[idx[j]].A; };
+    auto blocks = [&](int j) { return fused ? p.xj[idx[j]].blocks : p.tj[idx[j]].blocks; };
     FftBatchStage st;
     st.pass = k;
     st.njobs = nj;
-    for (const FftPassGeom* g : p.g) st.threads = std::max(st.threads, g->threads);
+    for (int j = 0; j < nj; ++j) st.threads = std::max(st.threads, p.g[idx[j]]->threads);
     int emax = 0;
     bool all348 = true;
     for (int j = 0; j < nj; ++j) {
@@ -1292,7 +1294,7 @@ int FftForcesBatch::init(const FftPlan* const* plans, float* const* density, flo
       all348 = all348 && radices_348(A.P);
       st.lds = std::max(st.lds, tile_lds_bytes(p.mode, A.P.n, A.log2T));
     }
-    st.emax = emax <= 8 ? 8 : emax <= 12 ? 12 : emax <= 16 ? 16 : 24;
+    st.emax = round_e(emax);
     if (fused && st.emax > 12) {
       set_error("batched force transforms: fused x tile wider than 12 values per thread");
       return kErrUnsupported;
@@ -1316,13 +1318,41 @@ int FftForcesBatch::init(const FftPlan* const* plans, float* const* density, flo
     const size_t jb = fused ? sizeof(FusedJob) * nj : sizeof(TileJob) * nj;
     const size_t so = align_up(jo + jb, 256);
     host.resize(so + sizeof(int) * (nj + 1));
-    if (fused)
-      std::memcpy(host.data() + jo, p.xj.data(), jb);
-    else
-      std::memcpy(host.data() + jo, p.tj.data(), jb);
+    for (int j = 0; j < nj; ++j) {
+      if (fused)
+        std::memcpy(host.data() + jo + sizeof(FusedJob) * j, &p.xj[idx[j]], sizeof(FusedJob));
+      else
+        std::memcpy(host.data() + jo + sizeof(TileJob) * j, &p.tj[idx[j]], sizeof(TileJob));
+    }
     std::memcpy(host.data() + so, start.data(), sizeof(int) * (nj + 1));
     offs.push_back({jo, so});
     stages.push_back(st);
+    return kOk;
+  };
+  for (int k = 0; k < 5; ++k) {
+    Pass& p = ps[k];
+    const int nj = k == 2 ? (int)p.xj.size() : (int)p.tj.size();
+    if (merge_all) {
+      std::vector<int> all(nj);
+      for (int j = 0; j < nj; ++j) all[j] = j;
+      COLA_TRY(build(k, all));
+      continue;
+    }
+    // jobs grouped by the kernel their own launch runs (register tile and
+    // radix set), first appearance first
+    std::vector<std::pair<int, std::vector<int>>> groups;
+    for (int j = 0; j < nj; ++j) {
+      const TileArgs& A = k == 2 ? p.xj[j].A : p.tj[j].A;
+      const int e = round_e(p.g[j]->emax);
+      const int key = 2 * e + (e == 8 && radices_348(A.P) ? 1 : 0);
+      auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
+      if (it == groups.end()) {
+        groups.push_back({key, {}});
+        it = groups.end() - 1;
+      }
+      it->second.push_back(j);
+    }
+    for (const auto& g : groups) COLA_TRY(build(k, g.second));
   }
   COLA_HIP(hipMalloc(&d_mem, host.size()));
   COLA_HIP(hipMemcpy(d_mem, host.data(), host.size(), hipMemcpyHostToDevice));

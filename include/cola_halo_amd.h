@@ -159,9 +159,13 @@ int cola_step_batch(cola_context* const* ctxs, int n, float omega_m, const float
    kernel launch per phase over the workgroups of every realization: mass
    slot, wrap + tile counts, scan (3), scatter, deposit, grid sums, z r2c,
    y forward, fused x, y inverse (3 axes), z c2r (3 axes), readout + kick +
-   drift: 14 launches per ensemble step, whatever n.  Each realization keeps
-   its own tiles, radix schedule and launch shapes, and the particles equal
-   those of cola_step on each context bit for bit.  The contexts need their
+   drift.  The transform passes launch once per FFT kernel variant the
+   members' own launches use (register tile and radix set: one per pass when
+   all members share one); cola_ensemble_set_merge(e, 1) makes every pass
+   one launch with the widest variant (14 launches per step whatever the
+   sizes; slower where the variants differ).  Each realization keeps its own
+   tiles, radix schedule and launch shapes, and the particles equal those of
+   cola_step on each context bit for bit.  The contexts need their
    particles set, no residual net attached, and an x pass that fuses (every
    grid of 2^a 3^b 5^c with a last radix of 3, 4, 5 or 8 does).  An ensemble
    step orders after the work queued on the member contexts and before
@@ -175,6 +179,9 @@ int cola_ensemble_step(cola_ensemble* e, float omega_m, const float* avel1, cons
 /* Graph mode: the 14 launches are captured once and replayed; per-step values
    reach the kernels through one host-to-device copy before each replay. */
 int cola_ensemble_set_graph(cola_ensemble* e, int on);
+/* on = 1: one launch per transform pass for all members; 0 (default): one
+   per pass and kernel variant.  Takes effect at the next step. */
+int cola_ensemble_set_merge(cola_ensemble* e, int on);
 /* Kernel launches per ensemble step (0 before the first step). */
 int cola_ensemble_launches(const cola_ensemble* e);
 int cola_ensemble_destroy(cola_ensemble* e);
