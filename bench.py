@@ -53,7 +53,8 @@ PK_KMAX, PK_BINS, PK_TOL = 1.0, 25, 0.01
 
 # launches of one kernel inside one profiling phase: the hidden layers are one
 # phase of two k_conv16 launches (layers 2 and 3), or of one (layer 3) when
-# layers 1-2 run fused (phase net_conv12)
+# layers 1-2 run fused (phase net_conv12); with layers 3-4 fused (phase
+# net_conv34) there is none
 def launches_per_phase(phase, stats):
     if phase == "net_conv_hidden":
         return 1 if stats.get("net_conv12", (0.0, 0))[1] > 0 else 2

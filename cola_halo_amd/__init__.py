@@ -419,7 +419,7 @@ class ResidualNet:
 
     def set_head_fusion(self, on):
         """Layer 3 and the last layer in one launch after the force
-        transforms (True) or two launches through h3 (False, the default);
+        transforms (True, the default) or two launches through h3 (False);
         the forces are the same bit for bit."""
         _check(lib().cola_resnet_set_head_fusion(self._h, int(bool(on))))
 

@@ -50,7 +50,7 @@ struct ResNet {
   float restrict_side = 0.125f;  // training targets: fine force averaged with weights (w, 1-2w, w) per axis
   bool fuse12 = true;            // inference: features and layers 1-2 in one launch (resnet_layers12)
   bool roll = true;              // 16-channel-input layers by input planes (k_conv12r, k_conv16r, k_conv_head_r)
-  bool fuse34 = false;           // inference by input planes: layer 3 and the last layer in one launch (k_conv3h)
+  bool fuse34 = true;            // inference by input planes: layer 3 and the last layer in one launch (k_conv3h)
 
   // training set: per sample the feature grid (bf16 [n^3]), the target
   // residual (bf16 [3][n^3]) and the scale factor a

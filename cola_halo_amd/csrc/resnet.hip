@@ -457,6 +457,55 @@ __device__ inline void roll_plane(const uint4* __restrict__ row, const uint4* __
   }
 }
 
+// roll_plane with one call of extra(j) per brick row, placed with the row's
+// products (the layer-1 tile of the fused pass).
+template <int T, typename Extra>
+__device__ inline void roll_plane_with(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
+                                       const uint4 (&b)[kRollChunks], floatx4 (&am)[T], floatx4 (&a0)[T],
+                                       floatx4 (&ap)[T], Extra&& extra)
+{
+  constexpr int kRow = 2 * kBrickZ;
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  uint4 prev = zero4;
+  uint4 cen = row[0], sh = shr[0];
+#pragma unroll
+  for (int j = 0; j < T + 2; ++j) {
+    uint4 ncen = zero4, nsh = zero4;
+    if (j + 1 < T + 2) {
+      ncen = row[(j + 1) * kRow];
+      nsh = shr[(j + 1) * kRow];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    extra(j);
+    if (j < T) {
+      am[j] = mfma(sh, b[0], am[j]);
+      a0[j] = mfma(sh, b[5], a0[j]);
+      ap[j] = mfma(sh, b[10], ap[j]);
+    }
+    if (j >= 1 && j - 1 < T) {
+      am[j - 1] = mfma(sh, b[1], am[j - 1]);
+      a0[j - 1] = mfma(sh, b[6], a0[j - 1]);
+      ap[j - 1] = mfma(sh, b[11], ap[j - 1]);
+      const uint4 pc = sel4(lo, prev, cen);
+      am[j - 1] = mfma(pc, b[3], am[j - 1]);
+      a0[j - 1] = mfma(pc, b[8], a0[j - 1]);
+      ap[j - 1] = mfma(pc, b[13], ap[j - 1]);
+    }
+    if (j >= 2) {
+      am[j - 2] = mfma(sh, b[2], am[j - 2]);
+      a0[j - 2] = mfma(sh, b[7], a0[j - 2]);
+      ap[j - 2] = mfma(sh, b[12], ap[j - 2]);
+      const uint4 qc = sel4(lo, cen, zero4);
+      am[j - 2] = mfma(qc, b[4], am[j - 2]);
+      a0[j - 2] = mfma(qc, b[9], a0[j - 2]);
+      ap[j - 2] = mfma(qc, b[14], ap[j - 2]);
+    }
+    prev = cen;
+    cen = ncen;
+    sh = nsh;
+  }
+}
+
 // Output plane q of a rolling convolution from its accumulators, by mode
 // (kFwdRelu: bf16 activations; kFwdLinear: fp32 y; kFwdAdd: added to the
 // force grids, old values loaded here), then the accumulators cleared.
@@ -608,9 +657,9 @@ constexpr int kHeadChunks = 5;
 
 // Products of one input plane into d (one tile per output row, fresh).
 // ROW: uint4 between input rows (two per cell).
-template <int ROW = 2 * kBrickZ>
+template <int ROW = 2 * kBrickZ, int T = kConvTy>
 __device__ inline void head_plane(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
-                                  const uint4 (&b)[kHeadChunks], floatx4 (&d)[kConvTy])
+                                  const uint4 (&b)[kHeadChunks], floatx4 (&d)[T])
 {
   constexpr int kRow = ROW;
   const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
@@ -618,15 +667,15 @@ __device__ inline void head_plane(const uint4* __restrict__ row, const uint4* __
   uint4 prev = zero4;
   uint4 cen = row[0], sh = shr[0];
 #pragma unroll
-  for (int j = 0; j < kConvTy + 2; ++j) {
+  for (int j = 0; j < T + 2; ++j) {
     uint4 ncen = zero4, nsh = zero4;
-    if (j + 1 < kConvTy + 2) {
+    if (j + 1 < T + 2) {
       ncen = row[(j + 1) * kRow];
       nsh = shr[(j + 1) * kRow];
     }
     __builtin_amdgcn_sched_barrier(0);
-    if (j < kConvTy) d[j] = mfma(sh, b[0], fz);  // dy = -1: the first product of output row j
-    if (j >= 1 && j - 1 < kConvTy) {
+    if (j < T) d[j] = mfma(sh, b[0], fz);  // dy = -1: the first product of output row j
+    if (j >= 1 && j - 1 < T) {
       d[j - 1] = mfma(sh, b[1], d[j - 1]);
       d[j - 1] = mfma(sel4(lo, prev, cen), b[3], d[j - 1]);
     }
@@ -780,23 +829,29 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
 // (z blocks of 30, the last one ragged: its cells wrap around and are not
 // stored) of planes x0 .. x1 - 1.  Layer 3 makes h3 on the region the last
 // layer reads, rows y0 - 1 .. y0 + 16 and cells z0 - 1 .. z0 + 30 (18 x 32,
-// 1.2 times the cells it owns): each wave 9 rows of one 16-cell column,
-// from h2 bricks of 20 x 34 cells in a 4-slot ring loaded three planes
-// ahead, as in k_conv16r.  Step i (h2 plane p = x0 - 2 + i) rolls plane p
-// into the three accumulator sets and writes the completed h3 plane p - 1
-// (bias, relu, bf16) into LDS slot (p - 1) & 1; after the step's barrier the
-// last layer consumes that slot as k_conv_head_r consumes an input plane and
-// output plane p - 2 is complete.  Per output cell the products and sums
-// run in the order of the two separate launches: the force is theirs bit
-// for bit.  The first steps' h3 planes (before x0 - 1) are incomplete; they
-// only reach output planes before x0, which are not stored.
+// 1.2 times the cells it owns), from h2 bricks of 20 x 34 cells in a 4-slot
+// ring loaded three planes ahead, as in k_conv16r.  Eight waves in two
+// roles, one of each per SIMD: waves 0-3 run layer 3 (wave (wy, wz): rows
+// 9 wy .. 9 wy + 8 of the 16-cell column wz), waves 4-7 the last layer one
+// plane behind (output rows 8 wy .. 8 wy + 7 of output column wz).  At step
+// i (h2 plane p = x0 - 2 + i) layer 3 rolls plane p into its three
+// accumulator sets and writes the completed h3 plane p - 1 (bias, relu,
+// bf16) into LDS slot (p - 1) & 1, while the last layer consumes h3 plane
+// p - 2 (written before the previous step's barrier) as k_conv_head_r
+// consumes an input plane and stores output plane p - 3; one barrier per
+// step.  The last layer's shuffles, loads and stores run beside layer 3's
+// products on the same SIMD.  Per output cell the products and sums run in
+// the order of the two separate launches: the force is theirs bit for bit.
+// The first steps' h3 planes (before x0 - 1, and slot contents of a
+// previous unit) only reach output planes before x0, which are not stored.
+constexpr int kThreads3 = 512;
 constexpr int kT3 = 9;                                  // layer-3 rows per wave
-constexpr int kBrick3Y = 2 * kT3 + 2;                   // h2 brick rows
-constexpr int kPlane3Cells = kBrick3Y * kBrickZ;        // 680
-constexpr int kRoll3Loads = (2 * kPlane3Cells + kThreads - 1) / kThreads;  // 6 loads per wave and plane
-constexpr int kRoll3SlotChunks = kRoll3Loads * kThreads;
 constexpr int kH3Y = 2 * kT3, kH3Z = 32;                // h3 region of a unit
 constexpr int kOut3Z = kH3Z - 2;                        // output cells along z
+constexpr int kBrick3Y = kH3Y + 2;                      // h2 brick rows
+constexpr int kPlane3Cells = kBrick3Y * kBrickZ;        // 680
+constexpr int kRoll3Loads = (2 * kPlane3Cells + kThreads3 - 1) / kThreads3;  // 3 loads per wave and plane
+constexpr int kRoll3SlotChunks = kRoll3Loads * kThreads3;
 constexpr int kH3SlotChunks = 2 * kH3Y * kH3Z;
 constexpr size_t kConv3hBytes = sizeof(uint4) * (kRingSlots * kRoll3SlotChunks + 2 * kH3SlotChunks);
 static_assert(kConv3hBytes <= 160 * 1024, "one workgroup per CU");
@@ -812,7 +867,7 @@ __device__ inline void load_plane3_to(uint4* ring, const unsigned short* __restr
   const long long xoff = (long long)wrapi(plane, n) * n;
 #pragma unroll
   for (int k = 0; k < kRoll3Loads; ++k) {
-    const int base = k * kThreads + (threadIdx.x & ~63);
+    const int base = k * kThreads3 + (threadIdx.x & ~63);
     const int c = min(base + lane, 2 * kPlane3Cells - 1);
     const int q = c & 1, cell = c >> 1;
     const int zz = cell % kBrickZ, yy = cell / kBrickZ;
@@ -826,53 +881,88 @@ __device__ inline void load_plane3_to(uint4* ring, const unsigned short* __restr
 // access done.
 constexpr int vmcnt_lgkm0_imm(int n) { return (n & 15) | (7 << 4) | (0 << 8) | ((n >> 4) << 14); }
 
-// The wait of step i: n = the operations issued after the next plane's
-// loads, on the ladder 2 L + 8 k (L = kRoll3Loads, k <= 4) the steps make.
-__device__ inline void conv3h_wait(int n)
-{
-  constexpr int L2 = 2 * kRoll3Loads;
-  if (n >= L2 + 32) {
-    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 32));
-  } else if (n >= L2 + 24) {
-    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 24));
-  } else if (n >= L2 + 16) {
-    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 16));
-  } else if (n >= L2 + 8) {
-    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2 + 8));
-  } else {
-    __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(L2));
-  }
-}
-
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1)))
+__global__ void __launch_bounds__(kThreads3) __attribute__((amdgpu_waves_per_eu(2)))
 k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict__ wr3, const float* __restrict__ b3,
          const unsigned short* __restrict__ wh, const float* __restrict__ b4, int n, int march, ConvOut o)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const bool lo = g < 2;
   const int wave = threadIdx.x >> 6;
-  const int wz = wave & 1, wy = wave >> 1;
+  const bool head = wave >= 4;  // role (wave-uniform)
+  const int wz = wave & 1, wy = (wave >> 1) & 1;
   const int half = g & 1;
-  uint4 b[kRollChunks];
-#pragma unroll
-  for (int c = 0; c < kRollChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wr3 + ((long long)c * 64 + lane) * 8);
-  uint4 bh[kHeadChunks];
-#pragma unroll
-  for (int c = 0; c < kHeadChunks; ++c) bh[c] = *reinterpret_cast<const uint4*>(wh + ((long long)c * 64 + lane) * 8);
-  float bch[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) bch[i] = b3[4 * g + i];
-  const float bo = g < 3 ? b4[g] : 0.0f;
   extern __shared__ uint4 smem[];
   uint4* ring = smem;                                   // [slot][20 rows][34 cells][2 halves]
   uint4* h3 = smem + kRingSlots * kRoll3SlotChunks;     // [2 slots][18 rows][32 cells][2 halves]
-  // layer 3: rows 9 wy .. 9 wy + 8 of the h3 region, cells 16 wz + r
-  const int zc = 1 + 16 * wz + r;
-  const int zs = lo ? zc - 1 : zc + 1;
-  const int rowoff = (kT3 * wy * kBrickZ + zc) * 2 + half;
-  const int shoff = (kT3 * wy * kBrickZ + zs) * 2 + half;
-  // last layer: output rows 8 wy .. 8 wy + 7, cells z0 + 14 wz + r (the
-  // second column's first two cells are the first column's last two)
+  const int nzb = (n + kOut3Z - 1) / kOut3Z, nyb = n / kBlockY;
+  const long long units = (long long)((n + march - 1) / march) * nyb * nzb;
+  constexpr int L = kRoll3Loads;
+  if (!head) {
+    // ---- layer 3: rows 9 wy .. 9 wy + 8 of the h3 region, cells 16 wz + r
+    uint4 b[kRollChunks];
+#pragma unroll
+    for (int c = 0; c < kRollChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wr3 + ((long long)c * 64 + lane) * 8);
+    float bch[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bch[i] = b3[4 * g + i];
+    const int zc = 1 + 16 * wz + r;
+    const int zs = lo ? zc - 1 : zc + 1;
+    const int rowoff = (kT3 * wy * kBrickZ + zc) * 2 + half;
+    const int shoff = (kT3 * wy * kBrickZ + zs) * 2 + half;
+    floatx4 acc[3][kT3];
+    for (long long u = blockIdx.x; u < units; u += gridDim.x) {
+      const int z0 = (int)(u % nzb) * kOut3Z;
+      const long long t2 = u / nzb;
+      const int y0 = (int)(t2 % nyb) * kBlockY;
+      const int x0 = (int)(t2 / nyb) * march, x1 = min(x0 + march, n);
+#pragma unroll
+      for (int s3 = 0; s3 < 3; ++s3)
+#pragma unroll
+        for (int t = 0; t < kT3; ++t) acc[s3][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();  // the previous unit is done
+      for (int k = 0; k < 3; ++k) load_plane3_to(ring, in, n, x0 - 2 + k, x0 - 2 + k, y0, z0);
+      constexpr int w_pro = vmcnt_imm(2 * L);
+      __builtin_amdgcn_s_waitcnt(w_pro);
+      asm volatile("s_barrier" ::: "memory");
+      auto step = [&](auto rot, int i) {
+        constexpr int R = decltype(rot)::value;
+        const int p = x0 - 2 + i;
+        // h2 plane p + 3 into the slot plane p - 1 left (the last plane again past x1 + 1)
+        load_plane3_to(ring, in, n, p + 3, min(p + 3, x1 + 1), y0, z0);
+        const uint4* slot = ring + (p & (kRingSlots - 1)) * kRoll3SlotChunks;
+        roll_plane<kT3>(slot + rowoff, slot + shoff, lo, b, acc[(R + 1) % 3], acc[R], acc[(R + 2) % 3]);
+        // h3 plane p - 1 complete: bias, relu, bf16 into its LDS slot
+        uint2* dst = reinterpret_cast<uint2*>(h3 + ((p - 1) & 1) * kH3SlotChunks);
+        floatx4(&a)[kT3] = acc[(R + 2) % 3];
+#pragma unroll
+        for (int t = 0; t < kT3; ++t) {
+          const int cell = (kT3 * wy + t) * kH3Z + 16 * wz + r;
+          dst[cell * 4 + g] = pack_bf16x4(fmaxf(a[t][0] + bch[0], 0.0f), fmaxf(a[t][1] + bch[1], 0.0f),
+                                          fmaxf(a[t][2] + bch[2], 0.0f), fmaxf(a[t][3] + bch[3], 0.0f));
+          a[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        // h2 plane p + 1 landed (its loads and this wave's later ones: two
+        // steps' L) and the h3 writes are done
+        __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(2 * L));
+        asm volatile("s_barrier" ::: "memory");
+      };
+      const int np = x1 - x0 + 5;  // h2 planes x0 - 2 .. x1 + 2 (the last a repeat), in whole rotations
+      for (int i = 0; i < np; i += 3) {
+        step(std::integral_constant<int, 0>{}, i);
+        step(std::integral_constant<int, 1>{}, i + 1);
+        step(std::integral_constant<int, 2>{}, i + 2);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    return;
+  }
+  // ---- the last layer: output rows 8 wy .. 8 wy + 7, cells z0 + 14 wz + r
+  // (the second column's first two cells are the first column's last two)
+  uint4 bh[kHeadChunks];
+#pragma unroll
+  for (int c = 0; c < kHeadChunks; ++c) bh[c] = *reinterpret_cast<const uint4*>(wh + ((long long)c * 64 + lane) * 8);
+  const float bo = g < 3 ? b4[g] : 0.0f;
   const int hz = 1 + 14 * wz + r;
   const int hzs = lo ? hz - 1 : hz + 1;
   const int hrowoff = (kConvTy * wy * kH3Z + hz) * 2 + half;
@@ -880,11 +970,9 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
   const int up = lane >= 16 ? lane - 16 : lane;
   const int g2 = 32 + r;
   float* const gout = g == 0 ? o.add[0] : (g == 1 ? o.add[1] : o.add[2]);
-  constexpr int L = kRoll3Loads, F = kConvTy, S = kConvTy;
+  constexpr int F = kConvTy, S = kConvTy;
+  static_assert(2 * kRoll3Loads + 3 * S + 2 * F < 64, "the wait's count fits the counter");
   float added = 0.0f;
-  floatx4 acc[3][kT3];
-  const int nzb = (n + kOut3Z - 1) / kOut3Z, nyb = n / kBlockY;
-  const long long units = (long long)((n + march - 1) / march) * nyb * nzb;
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     const int z0 = (int)(u % nzb) * kOut3Z;
     const long long t2 = u / nzb;
@@ -897,55 +985,30 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
     floatx4 part[kConvTy];
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) part[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int s3 = 0; s3 < 3; ++s3)
-#pragma unroll
-      for (int t = 0; t < kT3; ++t) acc[s3][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-    // the previous unit's stores, refills and LDS reads are done
     __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    __syncthreads();  // the previous unit is done
     for (int k = 0; k < 3; ++k) load_plane3_to(ring, in, n, x0 - 2 + k, x0 - 2 + k, y0, z0);
-    constexpr int w_pro = vmcnt_imm(2 * kRoll3Loads);
+    constexpr int w_pro = vmcnt_imm(2 * L);
     __builtin_amdgcn_s_waitcnt(w_pro);
     asm volatile("s_barrier" ::: "memory");
-    // output plane p - 2 is stored at steps i = 4 .. x1 - x0 + 3
-    auto stores = [&](int j) { return j >= 4 && j < x1 - x0 + 4; };
-    auto step = [&](auto rot, int i) {
-      constexpr int R = decltype(rot)::value;
+    // output plane p - 3 is stored at steps i = 5 .. x1 - x0 + 4.  Every
+    // step issues the same vector-memory operations, so the waits are
+    // fixed: F old-value loads for the next step (its plane clamped into
+    // the march), L plane loads, S stores (into trash at steps that store
+    // nothing); the old values arrive a whole step before their use.
+    auto stores = [&](int j) { return j >= 5 && j < x1 - x0 + 5; };
+    auto old_plane = [&](int j) { return min(max(x0 - 5 + j, x0), x1 - 1); };  // step j's output plane
+    const int np = x1 - x0 + 5;
+    float fold[kConvTy], fnext[kConvTy];
+#pragma unroll
+    for (int t = 0; t < kConvTy; ++t) fold[t] = 0.0f;
+    for (int i = 0; i < np; ++i) {  // the same number of steps (and barriers) as layer 3's rotations
       const int p = x0 - 2 + i;
       const bool store = stores(i);
-      float fold[kConvTy];
-      if (store) {  // every lane (group 3 reads group 2's grid; wrapped cells a valid one): F loads
 #pragma unroll
-        for (int t = 0; t < kConvTy; ++t) fold[t] = gout[((long long)(p - 2) * n + ybase + t) * o.row + zld];
-      }
-      // h2 plane p + 3 into the slot plane p - 1 left (the last plane again past x1 + 1)
+      for (int t = 0; t < kConvTy; ++t) fnext[t] = gout[((long long)old_plane(i + 1) * n + ybase + t) * o.row + zld];
       load_plane3_to(ring, in, n, p + 3, min(p + 3, x1 + 1), y0, z0);
-      const uint4* slot = ring + (p & (kRingSlots - 1)) * kRoll3SlotChunks;
-      roll_plane<kT3>(slot + rowoff, slot + shoff, lo, b, acc[(R + 1) % 3], acc[R], acc[(R + 2) % 3]);
-      // h3 plane p - 1 complete: bias, relu, bf16 into its LDS slot
-      {
-        uint2* dst = reinterpret_cast<uint2*>(h3 + ((p - 1) & 1) * kH3SlotChunks);
-        floatx4(&a)[kT3] = acc[(R + 2) % 3];
-#pragma unroll
-        for (int t = 0; t < kT3; ++t) {
-          const int cell = (kT3 * wy + t) * kH3Z + 16 * wz + r;
-          dst[cell * 4 + g] = pack_bf16x4(fmaxf(a[t][0] + bch[0], 0.0f), fmaxf(a[t][1] + bch[1], 0.0f),
-                                          fmaxf(a[t][2] + bch[2], 0.0f), fmaxf(a[t][3] + bch[3], 0.0f));
-          a[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
-        }
-      }
-      // h2 plane p + 1 landed (issued at step i - 2 or in the prologue; after
-      // it: that step's stores, step i - 1's old values, loads and stores,
-      // this step's old values and loads) and every wave's h3 writes are done
-      int after = 2 * L;
-      if (stores(i - 2)) after += S;
-      if (stores(i - 1)) after += F + S;
-      if (store) after += F;
-      conv3h_wait(after);
-      asm volatile("s_barrier" ::: "memory");
-      // the last layer on h3 plane p - 1
-      const uint4* hs = h3 + ((p - 1) & 1) * kH3SlotChunks;
+      const uint4* hs = h3 + ((p - 2) & 1) * kH3SlotChunks;
       floatx4 d[kConvTy];
       head_plane<2 * kH3Z>(hs + hrowoff, hs + hshoff, lo, bh, d);
 #pragma unroll
@@ -957,26 +1020,39 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
         }
         part[t] = d[t];
       }
-      if (store) {  // S stores: every lane of groups 0-2 (the ones without a cell into trash)
+      const bool put = store && zok;
 #pragma unroll
-        for (int t = 0; t < kConvTy; ++t) {
-          const float v0 = lane_read(d[t][0], g2), v1 = lane_read(d[t][1], g2), v2 = lane_read(d[t][2], g2);
-          const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
-          if (g < 3) {
-            const long long line = (long long)(p - 2) * n + ybase + t;
-            const float dv = o.scale * v;
-            float* dst = zok ? o.fi + (line * n + zcell) * 3 + g : o.trash + lane;
-            *dst = fold[t] + dv;
-            if (zok) added += dv;
-          }
+      for (int t = 0; t < kConvTy; ++t) {
+        const float v0 = lane_read(d[t][0], g2), v1 = lane_read(d[t][1], g2), v2 = lane_read(d[t][2], g2);
+        const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
+        if (g < 3) {
+          const long long line = (long long)(put ? p - 3 : x0) * n + ybase + t;
+          const float dv = o.scale * v;
+          float* dst2 = put ? o.fi + (line * n + zcell) * 3 + g : o.trash + lane;
+          *dst2 = fold[t] + dv;
+          if (put) added += dv;
         }
       }
-    };
-    const int np = x1 - x0 + 4;  // h2 planes x0 - 2 .. x1 + 1, in whole rotations
-    for (int i = 0; i < np; i += 3) {
-      step(std::integral_constant<int, 0>{}, i);
-      step(std::integral_constant<int, 1>{}, i + 1);
-      step(std::integral_constant<int, 2>{}, i + 2);
+#pragma unroll
+      for (int t = 0; t < kConvTy; ++t) fold[t] = fnext[t];
+      // h2 plane p + 1 landed: after its loads (step i - 2's L) came that
+      // step's S and two more steps' F, L, S; for i < 2 its loads were the
+      // prologue's second and third, followed by L (i = 0) and the steps'
+      if (i >= 2) {
+        __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(2 * L + 3 * S + 2 * F));
+      } else if (i == 1) {
+        __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(2 * L + 2 * S + 2 * F));
+      } else {
+        __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(2 * L + S + F));
+      }
+      asm volatile("s_barrier" ::: "memory");
+    }
+    // layer 3's steps come in whole rotations: the remaining barriers
+    for (int i = np; i < (np + 2) / 3 * 3; ++i) {
+      const int p = x0 - 2 + i;
+      load_plane3_to(ring, in, n, p + 3, min(p + 3, x1 + 1), y0, z0);
+      __builtin_amdgcn_s_waitcnt(vmcnt_lgkm0_imm(0));
+      asm volatile("s_barrier" ::: "memory");
     }
   }
   __builtin_amdgcn_s_waitcnt(0);
@@ -1204,55 +1280,6 @@ constexpr int kFeatPer = (kFeatCells + kThreads - 1) / kThreads;
 constexpr int kH1SlotCells = 4 * (kConvTy + 2) * 16;  // one tile per wave and row of the march step
 static_assert(kH1SlotCells >= kPlaneCells, "the h1 slot holds the brick");
 constexpr size_t kConv12rBytes = sizeof(uint4) * 2 * 2 * kH1SlotCells + sizeof(uint2) * kFeatSlots * kFeatCells;
-
-// roll_plane with one call of extra(j) per brick row, placed with the row's
-// products (the layer-1 tile of the fused pass).
-template <typename Extra>
-__device__ inline void roll_plane_with(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
-                                       const uint4 (&b)[kRollChunks], floatx4 (&am)[kConvTy],
-                                       floatx4 (&a0)[kConvTy], floatx4 (&ap)[kConvTy], Extra&& extra)
-{
-  constexpr int kRow = 2 * kBrickZ;
-  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-  uint4 prev = zero4;
-  uint4 cen = row[0], sh = shr[0];
-#pragma unroll
-  for (int j = 0; j < kConvTy + 2; ++j) {
-    uint4 ncen = zero4, nsh = zero4;
-    if (j + 1 < kConvTy + 2) {
-      ncen = row[(j + 1) * kRow];
-      nsh = shr[(j + 1) * kRow];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    extra(j);
-    if (j < kConvTy) {
-      am[j] = mfma(sh, b[0], am[j]);
-      a0[j] = mfma(sh, b[5], a0[j]);
-      ap[j] = mfma(sh, b[10], ap[j]);
-    }
-    if (j >= 1 && j - 1 < kConvTy) {
-      am[j - 1] = mfma(sh, b[1], am[j - 1]);
-      a0[j - 1] = mfma(sh, b[6], a0[j - 1]);
-      ap[j - 1] = mfma(sh, b[11], ap[j - 1]);
-      const uint4 pc = sel4(lo, prev, cen);
-      am[j - 1] = mfma(pc, b[3], am[j - 1]);
-      a0[j - 1] = mfma(pc, b[8], a0[j - 1]);
-      ap[j - 1] = mfma(pc, b[13], ap[j - 1]);
-    }
-    if (j >= 2) {
-      am[j - 2] = mfma(sh, b[2], am[j - 2]);
-      a0[j - 2] = mfma(sh, b[7], a0[j - 2]);
-      ap[j - 2] = mfma(sh, b[12], ap[j - 2]);
-      const uint4 qc = sel4(lo, cen, zero4);
-      am[j - 2] = mfma(qc, b[4], am[j - 2]);
-      a0[j - 2] = mfma(qc, b[9], a0[j - 2]);
-      ap[j - 2] = mfma(qc, b[14], ap[j - 2]);
-    }
-    prev = cen;
-    cen = ncen;
-    sh = nsh;
-  }
-}
 
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
 k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const unsigned short* __restrict__ wpk1,
@@ -2041,7 +2068,7 @@ int resnet_layer3_head(ResNet* r, float* const out[3], long long row, hipStream_
   o.fi = r->d_fi;
   o.trash = r->d_trash;
   const ConvLaunch cl = conv_launch_3h(r->n);
-  hipLaunchKernelGGL(k_conv3h, dim3(cl.blocks), dim3(kThreads), kConv3hBytes, s, r->d_h[1], r->d_wr[2],
+  hipLaunchKernelGGL(k_conv3h, dim3(cl.blocks), dim3(kThreads3), kConv3hBytes, s, r->d_h[1], r->d_wr[2],
                      r->d_param + param_offset(2) + 27 * 16 * 16, r->d_wr[3], r->d_param + param_offset(3) + 27 * 16 * 3,
                      r->n, cl.march, o);
   COLA_HIP(hipGetLastError());

@@ -393,10 +393,10 @@ int cola_resnet_set_fusion(cola_resnet* net, int on);
    another order: results agree to float rounding, not bit for bit. */
 int cola_resnet_set_plane_order(cola_resnet* net, int by_input_planes);
 /* Launch shape of the last two layers in a context's force (no reference
-   counterpart): on = 1 (with the fused first pass and input-plane order)
-   runs layer 3 and the last layer as one launch after the force
-   transforms, h3 kept in LDS; on = 0 (default) writes h3 to memory between
-   two launches.  Both give the same forces bit for bit. */
+   counterpart): on = 1 (default; with the fused first pass and input-plane
+   order) runs layer 3 and the last layer as one launch after the force
+   transforms, h3 kept in LDS; on = 0 writes h3 to memory between two
+   launches.  Both give the same forces bit for bit. */
 int cola_resnet_set_head_fusion(cola_resnet* net, int on);
 /* Net-weights file: the trained parameters and scales, independent of the
    grid size (the net is a convolution over cells).  Little-endian layout:
