@@ -689,6 +689,56 @@ __device__ inline void head_plane(const uint4* __restrict__ row, const uint4* __
   }
 }
 
+// head_plane accumulating into d (the products of one input plane added to
+// the running sums of three output planes, one per lane group).
+template <int ROW = 2 * kBrickZ, int T = kConvTy>
+__device__ inline void head_plane_acc(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
+                                      const uint4 (&b)[kHeadChunks], floatx4 (&d)[T])
+{
+  constexpr int kRow = ROW;
+  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+  uint4 prev = zero4;
+  uint4 cen = row[0], sh = shr[0];
+#pragma unroll
+  for (int j = 0; j < T + 2; ++j) {
+    uint4 ncen = zero4, nsh = zero4;
+    if (j + 1 < T + 2) {
+      ncen = row[(j + 1) * kRow];
+      nsh = shr[(j + 1) * kRow];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (j < T) d[j] = mfma(sh, b[0], d[j]);
+    if (j >= 1 && j - 1 < T) {
+      d[j - 1] = mfma(sh, b[1], d[j - 1]);
+      d[j - 1] = mfma(sel4(lo, prev, cen), b[3], d[j - 1]);
+    }
+    if (j >= 2) {
+      d[j - 2] = mfma(sh, b[2], d[j - 2]);
+      d[j - 2] = mfma(sel4(lo, cen, zero4), b[4], d[j - 2]);
+    }
+    prev = cen;
+    cen = ncen;
+    sh = nsh;
+  }
+}
+
+// The last layer's step with rotated weights (R = step mod 3): the products
+// of an input plane accumulate in place, the contributions to output plane
+// o in lane group (o - x0) mod 3 at every step, so no partial sums move
+// between lanes; the group holding a completed plane is read out and
+// cleared.  Selects the rotation's pack at run time (wave-uniform).
+template <int ROW = 2 * kBrickZ, int T = kConvTy>
+__device__ inline void head_plane_rot(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
+                                      const uint4 (&b)[3][kHeadChunks], int R, floatx4 (&d)[T])
+{
+  if (R == 0)
+    head_plane_acc<ROW, T>(row, shr, lo, b[0], d);
+  else if (R == 1)
+    head_plane_acc<ROW, T>(row, shr, lo, b[1], d);
+  else
+    head_plane_acc<ROW, T>(row, shr, lo, b[2], d);
+}
+
 __device__ inline float lane_read(float v, int src_lane)
 {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_lane * 4, __builtin_bit_cast(int, v)));
@@ -725,9 +775,12 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
   const bool lo = g < 2;
-  uint4 b[kHeadChunks];
+  uint4 b[3][kHeadChunks];
 #pragma unroll
-  for (int c = 0; c < kHeadChunks; ++c) b[c] = *reinterpret_cast<const uint4*>(wh + ((long long)c * 64 + lane) * 8);
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int c = 0; c < kHeadChunks; ++c)
+      b[q][c] = *reinterpret_cast<const uint4*>(wh + ((long long)(q * kHeadChunks + c) * 64 + lane) * 8);
   const float bo = g < 3 ? bias[g] : 0.0f;
   extern __shared__ uint4 ring[];
   const int half = g & 1;
@@ -736,8 +789,6 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
   const int zs = lo ? zc - 1 : zc + 1;
   const int rowoff = (kConvTy * wy * kBrickZ + zc) * 2 + half;
   const int shoff = (kConvTy * wy * kBrickZ + zs) * 2 + half;
-  const int up = lane >= 16 ? lane - 16 : lane;  // one lane group down: source of the partial sums
-  const int g2 = 32 + r;                          // lane of group 2 holding this cell's finished output
   const long long nc = (long long)n * n * n;
   // this lane's output channel: its force grid (kFwdAdd) or y row (kFwdLinear),
   // selected once (indexing the argument struct by g would load it per use)
@@ -750,9 +801,9 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     const ConvUnit cu = conv_unit(u, n, march);
     const int ybase = cu.y0 + kConvTy * wy, zcell = cu.z0 + 16 * wz + r;
-    floatx4 part[kConvTy];
+    floatx4 acc[kConvTy];  // output planes in lane groups (o - x0) mod 3
 #pragma unroll
-    for (int t = 0; t < kConvTy; ++t) part[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int k = 0; k < 3; ++k) load_plane_to(ring, in, n, cu.x0 - 1 + k, cu.x0 - 1 + k, cu.y0, cu.z0);
@@ -772,21 +823,15 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
       }
       load_plane_to(ring, in, n, p + 3, min(p + 3, cu.x1), cu.y0, cu.z0);
       const uint4* slot = roll_slot(ring, p);
-      floatx4 d[kConvTy];
-      head_plane(slot + rowoff, slot + shoff, lo, b, d);
-#pragma unroll
-      for (int t = 0; t < kConvTy; ++t) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float s = lane_read(part[t][k], up);
-          if (g == 1 || g == 2) d[t][k] += s;
-        }
-        part[t] = d[t];
-      }
+      // input plane p adds to output planes p + 1, p, p - 1; p - 1 is complete
+      // in lane group (i + 1) mod 3
+      const int R = i % 3, G = (R + 1) % 3;
+      head_plane_rot(slot + rowoff, slot + shoff, lo, b, R, acc);
+      const int src = 16 * G + r;
       if (store) {
 #pragma unroll
         for (int t = 0; t < kConvTy; ++t) {
-          const float v0 = lane_read(d[t][0], g2), v1 = lane_read(d[t][1], g2), v2 = lane_read(d[t][2], g2);
+          const float v0 = lane_read(acc[t][0], src), v1 = lane_read(acc[t][1], src), v2 = lane_read(acc[t][2], src);
           const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
           if (g < 3) {
             const long long line = (long long)(p - 1) * n + ybase + t;
@@ -802,6 +847,10 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
             }
           }
         }
+      }
+      if (g == G) {
+#pragma unroll
+        for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
       }
       // plane p + 1 landed: step j issues the old-value loads F(j), the plane
       // loads and the stores S(j); count what was issued after the loads of
@@ -959,16 +1008,17 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
   }
   // ---- the last layer: output rows 8 wy .. 8 wy + 7, cells z0 + 14 wz + r
   // (the second column's first two cells are the first column's last two)
-  uint4 bh[kHeadChunks];
+  uint4 bh[3][kHeadChunks];
 #pragma unroll
-  for (int c = 0; c < kHeadChunks; ++c) bh[c] = *reinterpret_cast<const uint4*>(wh + ((long long)c * 64 + lane) * 8);
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int c = 0; c < kHeadChunks; ++c)
+      bh[q][c] = *reinterpret_cast<const uint4*>(wh + ((long long)(q * kHeadChunks + c) * 64 + lane) * 8);
   const float bo = g < 3 ? b4[g] : 0.0f;
   const int hz = 1 + 14 * wz + r;
   const int hzs = lo ? hz - 1 : hz + 1;
   const int hrowoff = (kConvTy * wy * kH3Z + hz) * 2 + half;
   const int hshoff = (kConvTy * wy * kH3Z + hzs) * 2 + half;
-  const int up = lane >= 16 ? lane - 16 : lane;
-  const int g2 = 32 + r;
   float* const gout = g == 0 ? o.add[0] : (g == 1 ? o.add[1] : o.add[2]);
   constexpr int F = kConvTy, S = kConvTy;
   static_assert(2 * kRoll3Loads + 3 * S + 2 * F < 64, "the wait's count fits the counter");
@@ -982,9 +1032,9 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
     const int zcell = z0 + 14 * wz + r;
     const bool zok = zcell < n && (wz == 0 || r >= 2);  // a cell of the grid this lane alone stores
     const int zld = zcell < n ? zcell : zcell - n;       // in range for the old-value loads
-    floatx4 part[kConvTy];
+    floatx4 acc[kConvTy];  // output planes in lane groups (o - x0) mod 3
 #pragma unroll
-    for (int t = 0; t < kConvTy; ++t) part[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();  // the previous unit is done
     for (int k = 0; k < 3; ++k) load_plane3_to(ring, in, n, x0 - 2 + k, x0 - 2 + k, y0, z0);
@@ -1008,22 +1058,16 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) fnext[t] = gout[((long long)old_plane(i + 1) * n + ybase + t) * o.row + zld];
       load_plane3_to(ring, in, n, p + 3, min(p + 3, x1 + 1), y0, z0);
+      // h3 plane q = p - 2 adds to output planes q + 1, q, q - 1; q - 1 is
+      // complete in lane group (i + 1) mod 3
       const uint4* hs = h3 + ((p - 2) & 1) * kH3SlotChunks;
-      floatx4 d[kConvTy];
-      head_plane<2 * kH3Z>(hs + hrowoff, hs + hshoff, lo, bh, d);
-#pragma unroll
-      for (int t = 0; t < kConvTy; ++t) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          const float sv = lane_read(part[t][k], up);
-          if (g == 1 || g == 2) d[t][k] += sv;
-        }
-        part[t] = d[t];
-      }
+      const int R = i % 3, G = (R + 1) % 3;
+      head_plane_rot<2 * kH3Z>(hs + hrowoff, hs + hshoff, lo, bh, R, acc);
+      const int src = 16 * G + r;
       const bool put = store && zok;
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) {
-        const float v0 = lane_read(d[t][0], g2), v1 = lane_read(d[t][1], g2), v2 = lane_read(d[t][2], g2);
+        const float v0 = lane_read(acc[t][0], src), v1 = lane_read(acc[t][1], src), v2 = lane_read(acc[t][2], src);
         const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
         if (g < 3) {
           const long long line = (long long)(put ? p - 3 : x0) * n + ybase + t;
@@ -1032,6 +1076,10 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
           *dst2 = fold[t] + dv;
           if (put) added += dv;
         }
+      }
+      if (g == G) {
+#pragma unroll
+        for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
       }
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) fold[t] = fnext[t];
@@ -1536,19 +1584,22 @@ __global__ void k_pack_roll(const float* __restrict__ w, int cout, unsigned shor
   }
 }
 
-// Last-layer weights W[t][16][3] -> k_conv_head_r's A operands: row
-// 4 dxi + co of chunk c holds the taps roll_tap(5 dxi + c, h) of output
-// channel co (co = 3 and dxi = 3 zero).
+// Last-layer weights W[t][16][3] -> the last layer's A operands, three
+// rotations R of five chunks (block 5 R + c): row 4 lg + co of chunk c
+// holds the taps roll_tap(5 dxi + c, h) of output channel co with
+// dxi = (R - lg) mod 3 (lane group lg = 3 and co = 3 zero), so the
+// contribution of x offset dxi - 1 lands in output lane group (R - dxi) mod 3.
 __global__ void k_pack_head(const float* __restrict__ w, unsigned short* __restrict__ wh)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
-  const int c = blockIdx.x;
-  const int dxi = col >> 2, co = col & 3;
-  const int tap = (dxi < 3 && co < 3) ? roll_tap(5 * dxi + c, g >> 1) : -1;
+  const int rot = blockIdx.x / kHeadChunks, c = blockIdx.x % kHeadChunks;
+  const int lg = col >> 2, co = col & 3;
+  const int dxi = (rot - lg + 3) % 3;
+  const int tap = (lg < 3 && co < 3) ? roll_tap(5 * dxi + c, g >> 1) : -1;
   for (int j = 0; j < 8; ++j) {
     const int ci = 8 * (g & 1) + j;
     const float v = tap >= 0 ? w[((long long)tap * 16 + ci) * 3 + co] : 0.0f;
-    wh[((long long)c * 64 + lane) * 8 + j] = f2bf(v);
+    wh[((long long)blockIdx.x * 64 + lane) * 8 + j] = f2bf(v);
   }
 }
 
@@ -1795,7 +1846,7 @@ int resnet_pack(ResNet* r, hipStream_t s)
       hipLaunchKernelGGL(k_pack_roll, dim3(kRollChunks), dim3(64), 0, s, r->d_param + param_offset(l), kNetCout[l],
                          r->d_wr[l]);
     if (kNetCin[l] == 16 && kNetCout[l] == 3)
-      hipLaunchKernelGGL(k_pack_head, dim3(kHeadChunks), dim3(64), 0, s, r->d_param + param_offset(l), r->d_wr[l]);
+      hipLaunchKernelGGL(k_pack_head, dim3(3 * kHeadChunks), dim3(64), 0, s, r->d_param + param_offset(l), r->d_wr[l]);
   }
   COLA_HIP(hipGetLastError());
   return kOk;
