@@ -108,17 +108,26 @@ ENSEMBLE = [(96, 288, 187.5)] * 16 + [(128, 384, 250.0)] * 16 + [(160, 480, 312.
 
 def test_c3_ensemble_batched_equals_single_runs(eh_table):
     """48 realizations of three sizes, steps batched across contexts with
-    graph capture, equal the realizations run alone (main.c:117-121: seed =
-    random_seed + i), bit for bit."""
+    graph capture, and as one ensemble (one launch per phase over all 48,
+    replayed as a graph), equal the realizations run alone (main.c:117-121:
+    seed = random_seed + i), bit for bit."""
     _say("C3 48 realizations batched")
     ctxs = [_ctx(nc, ng, box, eh_table) for nc, ng, box in ENSEMBLE]
     try:
         for c in ctxs:
             c.set_step_graph(True)
         snaps = ch.run_cola_batch(ctxs, OM, 1.0, 10, list(range(1, 49)))
+        _say("C3 48 realizations as one ensemble")
+        for c in ctxs:
+            c.set_step_graph(False)
+        with ch.Ensemble(ctxs, graph=True) as ens:
+            ens_snaps = ch.run_cola_batch(ctxs, OM, 1.0, 10, list(range(1, 49)), ensemble=ens)
+            assert ens.launches() >= 14
     finally:
         for c in ctxs:
             c.close()
+    for i in range(len(ENSEMBLE)):
+        assert np.array_equal(ens_snaps[i], snaps[i]), i
     _say("C3 single runs")
     singles = {}
     for i, (nc, ng, box) in enumerate(ENSEMBLE):
