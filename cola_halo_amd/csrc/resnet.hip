@@ -1319,8 +1319,8 @@ constexpr int kFeatPer = (kFeatCells + kThreads - 1) / kThreads;
 // loaded into registers; layer 2 adds h1 plane p's taps to the
 // accumulators of h2 planes p + 1, p, p - 1 while, row by row, the wave
 // makes one 16-cell tile of h1 plane p + 1 (from feature planes p .. p + 2,
-// into the other h1 slot; layer 1 as in k_conv1); h2 plane p - 1 is stored;
-// feature plane p + 3 takes the slot of plane p - 1.  One barrier per step.
+// into the other h1 slot; layer 1 as in k_conv1); feature plane p + 3 takes
+// the slot of plane p - 1; h2 plane p - 1 is stored.  One barrier per step.
 // Every step makes all 40 tiles (the last ones write the pad) and its next
 // h1 plane even past the march's end (never read for a stored plane), so
 // the loop has no branches inside.  The arithmetic is that of k_features,
@@ -1414,9 +1414,11 @@ k_conv12r(const float* __restrict__ delta, long long row, float in_scale, const 
       roll_plane(slot + rowoff, slot + shoff, lo, b, acc[(R + 1) % 3], acc[R], acc[(R + 2) % 3]);
 #pragma unroll 1
       for (int j = 0; j < kConvTy + 2; ++j) l1tile(p + 1, wave + 4 * j);
+      // the features before the stores: their wait for this step's loads
+      // then leaves the stores in flight through the next step
+      if (more) put(p + 3, v);
       roll_store<kFwdRelu>(acc[(R + 2) % 3], p - 1 >= cu.x0 && p - 1 < cu.x1, p - 1, n, ybase, zcell, g, bch2,
                            ConvOut{out_bf});
-      if (more) put(p + 3, v);
       __syncthreads();
     };
 #pragma unroll
