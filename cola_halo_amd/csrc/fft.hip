@@ -26,11 +26,24 @@ constexpr float kS5a = 0.951056516295153572116439333379382143f;  // sin(2pi/5)
 constexpr float kS5b = 0.587785252292473129168705954639072769f;  // sin(4pi/5)
 constexpr float kR2 = 0.707106781186547524400844362104849039f;   // sqrt(2)/2
 
+// The butterflies and twiddle products contract their multiply-adds into
+// fused multiply-adds (written out, so every kernel rounds them alike): the
+// transforms are compared with the reference's FFTW by tolerance, not bit
+// for bit, and the pass is VALU-bound.
+
 // multiply by -i (DIR < 0) or +i (DIR > 0)
 template <int DIR>
 __device__ inline float2 rot(float2 z)
 {
   return DIR < 0 ? make_float2(z.y, -z.x) : make_float2(-z.y, z.x);
+}
+
+// a * w (DIR < 0) or a * conj(w) (DIR > 0): two products, two fused adds
+template <int DIR>
+__device__ inline float2 twiddle(float2 a, float2 w)
+{
+  if (DIR < 0) return make_float2(fmaf(a.x, w.x, -(a.y * w.y)), fmaf(a.x, w.y, a.y * w.x));
+  return make_float2(fmaf(a.x, w.x, a.y * w.y), fmaf(-a.x, w.y, a.y * w.x));
 }
 
 template <int DIR>
@@ -46,11 +59,12 @@ __device__ inline void dft3(float2* v)
 {
   const float2 t = cadd(v[1], v[2]);
   const float2 d = csub(v[1], v[2]);
-  const float2 m = make_float2(v[0].x - 0.5f * t.x, v[0].y - 0.5f * t.y);
-  const float2 r = rot<DIR>(make_float2(kS3 * d.x, kS3 * d.y));
+  const float2 m = make_float2(fmaf(-0.5f, t.x, v[0].x), fmaf(-0.5f, t.y, v[0].y));
+  // m +- rot(kS3 d)
+  const float s = DIR < 0 ? kS3 : -kS3;
   v[0] = cadd(v[0], t);
-  v[1] = cadd(m, r);
-  v[2] = csub(m, r);
+  v[1] = make_float2(fmaf(s, d.y, m.x), fmaf(-s, d.x, m.y));
+  v[2] = make_float2(fmaf(-s, d.y, m.x), fmaf(s, d.x, m.y));
 }
 
 template <int DIR>
@@ -70,10 +84,10 @@ __device__ inline void dft5(float2* v)
   const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
   const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
   const float2 x0 = v[0];
-  const float2 a1 = make_float2(x0.x + kC5a * t1.x + kC5b * t2.x, x0.y + kC5a * t1.y + kC5b * t2.y);
-  const float2 a2 = make_float2(x0.x + kC5b * t1.x + kC5a * t2.x, x0.y + kC5b * t1.y + kC5a * t2.y);
-  const float2 b1 = rot<DIR>(make_float2(kS5a * t3.x + kS5b * t4.x, kS5a * t3.y + kS5b * t4.y));
-  const float2 b2 = rot<DIR>(make_float2(kS5b * t3.x - kS5a * t4.x, kS5b * t3.y - kS5a * t4.y));
+  const float2 a1 = make_float2(fmaf(kC5b, t2.x, fmaf(kC5a, t1.x, x0.x)), fmaf(kC5b, t2.y, fmaf(kC5a, t1.y, x0.y)));
+  const float2 a2 = make_float2(fmaf(kC5a, t2.x, fmaf(kC5b, t1.x, x0.x)), fmaf(kC5a, t2.y, fmaf(kC5b, t1.y, x0.y)));
+  const float2 b1 = rot<DIR>(make_float2(fmaf(kS5b, t4.x, kS5a * t3.x), fmaf(kS5b, t4.y, kS5a * t3.y)));
+  const float2 b2 = rot<DIR>(make_float2(fmaf(-kS5a, t4.x, kS5b * t3.x), fmaf(-kS5a, t4.y, kS5b * t3.y)));
   v[0] = make_float2(x0.x + t1.x + t2.x, x0.y + t1.y + t2.y);
   v[1] = cadd(a1, b1);
   v[4] = csub(a1, b1);
@@ -88,21 +102,26 @@ __device__ inline void dft8(float2* v)
   float2 o[4] = {v[1], v[3], v[5], v[7]};
   dft4<DIR>(e);
   dft4<DIR>(o);
-  // o[k] *= exp(DIR * 2 pi i k / 8)
+  // o[k] *= exp(DIR * 2 pi i k / 8): o[1] = kR2 s1, o[3] = kR2 s3, the
+  // factor kR2 taken into the final adds
   const float2 o1 = o[1], o3 = o[3];
+  float2 s1, s3;
   if (DIR < 0) {
-    o[1] = make_float2(kR2 * (o1.x + o1.y), kR2 * (o1.y - o1.x));
-    o[3] = make_float2(kR2 * (o3.y - o3.x), -kR2 * (o3.x + o3.y));
+    s1 = make_float2(o1.x + o1.y, o1.y - o1.x);
+    s3 = make_float2(o3.y - o3.x, -(o3.x + o3.y));
   } else {
-    o[1] = make_float2(kR2 * (o1.x - o1.y), kR2 * (o1.x + o1.y));
-    o[3] = make_float2(-kR2 * (o3.x + o3.y), kR2 * (o3.x - o3.y));
+    s1 = make_float2(o1.x - o1.y, o1.x + o1.y);
+    s3 = make_float2(-(o3.x + o3.y), o3.x - o3.y);
   }
-  o[2] = rot<DIR>(o[2]);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = cadd(e[k], o[k]);
-    v[k + 4] = csub(e[k], o[k]);
-  }
+  const float2 o2 = rot<DIR>(o[2]);
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[1] = make_float2(fmaf(kR2, s1.x, e[1].x), fmaf(kR2, s1.y, e[1].y));
+  v[5] = make_float2(fmaf(-kR2, s1.x, e[1].x), fmaf(-kR2, s1.y, e[1].y));
+  v[3] = make_float2(fmaf(kR2, s3.x, e[3].x), fmaf(kR2, s3.y, e[3].y));
+  v[7] = make_float2(fmaf(-kR2, s3.x, e[3].x), fmaf(-kR2, s3.y, e[3].y));
 }
 
 template <int R, int DIR>
@@ -343,9 +362,7 @@ __device__ __forceinline__ void tile_run(const void* __restrict__ in_, void* __r
           const int j = b - i * p.L;
 #pragma unroll
           for (int s = 1; s < R; ++s) {
-            float2 w = tws[s * j * p.twstep];
-            if (DIR > 0) w.y = -w.y;
-            reg[q * R + s] = cmul(reg[q * R + s], w);
+            reg[q * R + s] = twiddle<DIR>(reg[q * R + s], tws[s * j * p.twstep]);
           }
         }
         dft<R, DIR>(&reg[q * R]);
@@ -500,9 +517,7 @@ __device__ inline void col_compute(float2 (&reg)[EMAX], const FftPass& p, const 
         const int j = b - i * p.L;
 #pragma unroll
         for (int s = 1; s < R; ++s) {
-          float2 w = x.tws[s * j * p.twstep];
-          if (DIR > 0) w.y = -w.y;
-          reg[q * R + s] = cmul(reg[q * R + s], w);
+          reg[q * R + s] = twiddle<DIR>(reg[q * R + s], x.tws[s * j * p.twstep]);
         }
       }
       dft<R, DIR>(&reg[q * R]);
