@@ -1,6 +1,7 @@
 // Shared definitions of the HIP library: error plumbing and small device
 // helpers.  All device code is compiled with -ffp-contract=off so every float
-// and double operation rounds exactly where the reference's C code rounds.
+// and double operation rounds exactly where the reference's C code rounds
+// (the transforms, which the reference leaves to FFTW, call fmaf explicitly).
 #pragma once
 
 #include <hip/hip_runtime.h>
