@@ -823,10 +823,11 @@ def step_batch(ctxs, omega_m, avel1, apos1):
 
 class Ensemble:
     """Contexts stepped together with one kernel launch per phase over the
-    workgroups of every realization (cola_ensemble_*).  Close it before its
-    contexts."""
+    workgroups of every realization (cola_ensemble_*).  Each context may be
+    one member; an ensemble whose member was closed refuses to step."""
 
     def __init__(self, ctxs, graph=False, merge=False):
+        self._h = None
         self.ctxs = list(ctxs)
         n = len(self.ctxs)
         hs = (C.c_void_p * n)(*[c._h for c in self.ctxs])

@@ -150,6 +150,7 @@ struct cola_context {
   long long phase_n[kNumPhases] = {0};
   int open_phase = -1;
   hipEvent_t open_event = nullptr;
+  std::vector<cola_ensemble*> ensembles;  // the ensembles this context is a member of
 };
 
 // A net's activation buffers serve one context at a time: `attached` is the
@@ -159,6 +160,10 @@ struct cola_resnet {
   hipStream_t stream = nullptr;
   cola_context* attached = nullptr;
 };
+
+// A member context of ensemble e is being destroyed: e drops its tables and
+// graph (they hold the context's buffers) and refuses to step from then on.
+static void ensemble_forget(cola_ensemble* e, cola_context* c);
 
 namespace {
 
@@ -504,8 +509,10 @@ int cola_context_create(int nc, int ngrid, double boxsize, cola_context** out)
 int cola_context_destroy(cola_context* c)
 {
   if (!c) return kOk;
+  // the context's stream waits for every ensemble step it was part of
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->net_handle) c->net_handle->attached = nullptr;
+  for (cola_ensemble* e : c->ensembles) ensemble_forget(e, c);
   for (const EventPair& ep : c->events) {
     (void)hipEventDestroy(ep.a);
     (void)hipEventDestroy(ep.b);
@@ -988,6 +995,21 @@ void drop_ensemble_graph(cola_ensemble* e)
   e->graph = nullptr;
 }
 
+}  // namespace
+
+static void ensemble_forget(cola_ensemble* e, cola_context* c)
+{
+  (void)hipStreamSynchronize(e->stream);
+  drop_ensemble_graph(e);
+  e->pm.destroy();
+  e->fft.destroy();
+  e->built = false;
+  for (cola_context*& m : e->ctx)
+    if (m == c) m = nullptr;
+}
+
+namespace {
+
 // (Re)builds the batched tables from the members' current buffers and plans.
 int ensemble_build(cola_ensemble* e)
 {
@@ -1073,8 +1095,14 @@ int cola_ensemble_create(cola_context* const* ctxs, int n, cola_ensemble** out)
 {
   if (!out || n <= 0 || !ctxs) return kErrArg;
   *out = nullptr;
-  for (int m = 0; m < n; ++m)
+  for (int m = 0; m < n; ++m) {
     if (!valid(ctxs[m])) return kErrArg;
+    for (int k = 0; k < m; ++k)
+      if (ctxs[k] == ctxs[m]) {
+        set_error("ensemble members " + std::to_string(k) + " and " + std::to_string(m) + " are the same context");
+        return kErrArg;
+      }
+  }
   cola_ensemble* e = new cola_ensemble;
   auto fail = [&](int st) {
     cola_ensemble_destroy(e);
@@ -1091,6 +1119,7 @@ int cola_ensemble_create(cola_context* const* ctxs, int n, cola_ensemble** out)
   std::memset(e->h_prm, 0, sizeof(StepParams) * cola_ensemble::kSlots * n);
   if (hipMalloc(&e->d_prm, sizeof(StepParams) * n) != hipSuccess) return fail(kErrAlloc);
   if (hipMemset(e->d_prm, 0, sizeof(StepParams) * n) != hipSuccess) return fail(kErrHip);
+  for (cola_context* c : e->ctx) c->ensembles.push_back(e);
   *out = e;
   return kOk;
 }
@@ -1099,6 +1128,11 @@ int cola_ensemble_step(cola_ensemble* e, float omega_m, const float* avel1, cons
 {
   if (!e || !avel1 || !apos1) return kErrArg;
   const int n = (int)e->ctx.size();
+  for (int m = 0; m < n; ++m)
+    if (!e->ctx[m]) {
+      set_error("ensemble member " + std::to_string(m) + " was destroyed");
+      return kErrState;
+    }
   bool stale = !e->built;
   for (int m = 0; m < n; ++m) stale = stale || e->ctx[m]->config_gen != e->gen[m] || e->ctx[m]->net;
   if (stale) COLA_TRY(ensemble_build(e));
@@ -1187,6 +1221,8 @@ int cola_ensemble_destroy(cola_ensemble* e)
 {
   if (!e) return kOk;
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (cola_context* c : e->ctx)
+    if (c) c->ensembles.erase(std::remove(c->ensembles.begin(), c->ensembles.end(), e), c->ensembles.end());
   drop_ensemble_graph(e);
   e->pm.destroy();
   e->fft.destroy();

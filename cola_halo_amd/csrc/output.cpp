@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <new>
 #include <vector>
 
 #include "../../include/cola_halo_amd.h"
@@ -233,7 +234,11 @@ int cola_read_gadget(const char* filename, cola_particle_min* out, long long np_
     }
     if ((st = read_header(r, &h))) return st;
     const long long np = h.np[1];  // type-1 particles only (read.c:75)
-    if (!out) {                   // counting pass: headers only
+    if (np < 0) {
+      set_error("Error: negative particle count in the header of " + name);
+      return 15005;
+    }
+    if (!out) {  // counting pass: headers only
       nread += np;
       continue;
     }
@@ -243,7 +248,13 @@ int cola_read_gadget(const char* filename, cola_particle_min* out, long long np_
     }
     const float vfac_back = std::sqrt(h.time);  // read.c:76, GADGET convention back to km/s
     const float boxsize = (float)h.boxsize;
-    std::vector<float> buf(3 * (size_t)np);
+    std::vector<float> buf;
+    try {
+      buf.resize(3 * (size_t)np);
+    } catch (const std::bad_alloc&) {
+      set_error("Error: unable to allocate the read buffer for " + name);
+      return COLA_ERR_ALLOC;
+    }
     if ((st = r.separator(12 * np))) return st;
     if (!r.get(buf.data(), sizeof(float) * buf.size())) {
       set_error("Error: short position block in " + name);

@@ -170,8 +170,9 @@ int cola_step_batch(cola_context* const* ctxs, int n, float omega_m, const float
    grid of 2^a 3^b 5^c with a last radix of 3, 4, 5 or 8 does).  An ensemble
    step orders after the work queued on the member contexts and before
    their later work; changing a member's tiles or FFT shapes rebuilds the
-   ensemble's tables at its next step.  Destroy the ensemble before its
-   contexts. */
+   ensemble's tables at its next step.  A context may be a member once
+   (9001 otherwise).  Destroying a member context drops the ensemble's
+   tables; its later steps return 9004, and it can still be destroyed. */
 typedef struct cola_ensemble cola_ensemble;
 int cola_ensemble_create(cola_context* const* ctxs, int n, cola_ensemble** out);
 /* One step of every member (cola_step's meaning, per-member avel1 / apos1). */

@@ -175,3 +175,32 @@ def test_ensemble_refuses_member_with_net(eh_table):
             ens.step(OMF, 0.15, 0.2)
             assert a.times() == (np.float32(0.2), np.float32(0.15))
         net.close()
+
+
+def test_ensemble_members_distinct_and_outlived(eh_table):
+    """A context cannot be two members of one ensemble; destroying a member
+    first leaves the ensemble refusing to step (no access to freed
+    buffers), and the ensemble still closes cleanly."""
+    print("ensemble member lifetime", flush=True)
+    a = _ctx(32, 96, 100.0, eh_table)
+    b = _ctx(32, 96, 100.0, eh_table)
+    try:
+        for c in (a, b):
+            c.lpt_set_displacement(0.1, OM, 3)
+            c.set_times(c.times()[0], 0.1)
+        with pytest.raises(ch.ColaError) as err:
+            ch.Ensemble([a, b, a])
+        assert err.value.code == 9001
+        ens = ch.Ensemble([a, b], graph=True)
+        ens.step(OMF, 0.15, 0.2)
+        ens.step(OMF, 0.25, 0.3)
+        b.close()
+        with pytest.raises(ch.ColaError) as err:
+            ens.step(OMF, 0.35, 0.4)
+        assert err.value.code == 9004
+        ens.close()
+        assert a.times() == (np.float32(0.3), np.float32(0.25))
+        a.step(OMF, 0.35, 0.4)
+    finally:
+        a.close()
+        b.close()

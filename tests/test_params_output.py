@@ -225,6 +225,14 @@ def test_gadget_reader_errors(tmp_path):
     info = ch.GadgetInfo()
     code = ch.lib().cola_read_gadget(base.encode(), out.ctypes.data, 5, ch.C.byref(info))
     assert code == 15001           # read.c:80-84: more particles than room
+    # a header with a negative type-1 count (np[1] after the 4-byte marker
+    # and np[0]) is refused as malformed, before any buffer is sized by it
+    raw = bytearray(open(f"{base}.0", "rb").read())
+    struct.pack_into("<i", raw, 8, -5)
+    open(f"{base}.0", "wb").write(bytes(raw))
+    with pytest.raises(ch.ColaError) as e:
+        ch.read_gadget(base)
+    assert e.value.code == 15005
 
 
 def test_coarse_grid_file_layout(tmp_path):
