@@ -422,10 +422,15 @@ def plain_bench(args):
 
 
 ENSEMBLE = [(96, 288, 187.5)] * 16 + [(128, 384, 250.0)] * 16 + [(160, 480, 312.5)] * 16
+# many small realizations at the same cell size, where one realization's
+# launches are too small to fill the GPU
+ENSEMBLE_SMALL = [(32, 96, 62.5)] * 32 + [(48, 144, 93.75)] * 32
 
 
-def ensemble_bench(args):
-    """48 uneven realizations (16 x 96^3/288^3, 16 x 128^3/384^3,
+def ensemble_bench(args, members=ENSEMBLE, label="48 uneven realizations",
+                   workload="16 x 96^3/288^3 L187.5 + 16 x 128^3/384^3 L250 + 16 x 160^3/480^3 L312.5, seeds 1-48, "
+                            "10 steps"):
+    """Uneven realizations (by default 16 x 96^3/288^3, 16 x 128^3/384^3,
     16 x 160^3/480^3, seeds 1-48, 10 steps each): the step loops run one
     realization after another; interleaved across all contexts with
     graph-captured steps on their own streams; and as one ensemble (one
@@ -436,7 +441,7 @@ def ensemble_bench(args):
     k, p = ch.eh98_table(**ch.EH_COSMOLOGY)
     da = 1.0 / NSTEPS
     Om = float(np.float32(OM))
-    ctxs = [ch.Context(nc, ng, box) for nc, ng, box in ENSEMBLE]
+    ctxs = [ch.Context(nc, ng, box) for nc, ng, box in members]
     for c in ctxs:
         c.power_init_table(k, p, da, 0.8, OM, 1.0 - OM)
 
@@ -459,12 +464,12 @@ def ensemble_bench(args):
     # its initial conditions), outside the timed region
     prepare(False)
     tuned = {}
-    for (nc, ng, box), c in zip(ENSEMBLE, ctxs):
+    for (nc, ng, box), c in zip(members, ctxs):
         if ng not in tuned:
             tuned[ng] = c.autotune(reps=3)
         else:
             c.apply_tuning(tuned[ng])
-    particles = sum(nc ** 3 for nc, _, _ in ENSEMBLE)
+    particles = sum(nc ** 3 for nc, _, _ in members)
     out = {}
     ens = ch.Ensemble(ctxs)
     # the ensemble builds its tables at its first step: one untimed step
@@ -509,11 +514,10 @@ def ensemble_bench(args):
     out["ensemble_graph"]["launches_per_step"] = launches
     out["ensemble_merged_graph"]["launches_per_step"] = merged.get("launches")
     best = max(out, key=lambda m: out[m]["particle_steps_per_s"])
-    print(json.dumps({"metric": "ensemble particle-steps/s, 48 uneven realizations", "value":
+    print(json.dumps({"metric": f"ensemble particle-steps/s, {label}", "value":
                       out[best]["particle_steps_per_s"], "best_mode": best, "unit": "particle-steps/s", "n_gpus": 1,
                       "dtype": "f32", "data": "synthetic",
-                      "config": {"workload": "16 x 96^3/288^3 L187.5 + 16 x 128^3/384^3 L250 + 16 x 160^3/480^3 "
-                                             "L312.5, seeds 1-48, 10 steps", "particles": particles},
+                      "config": {"workload": workload, "particles": particles},
                       "modes": out, "tuning": {str(k): v for k, v in tuned.items()}}))
 
 
@@ -535,6 +539,8 @@ def main():
     ap.add_argument("--xcd-group", type=int, default=0, help="FFT column tiles per XCD group")
     ap.add_argument("--ensemble", action="store_true",
                     help="time the 48-realization ensemble (sequential vs batched graph steps) instead")
+    ap.add_argument("--ensemble-small", action="store_true",
+                    help="time 64 small realizations (32 x 32^3/96^3, 32 x 48^3/144^3) the same way instead")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes instead of tuning them")
     ap.add_argument("--copy-probe", type=int, default=0,
@@ -552,6 +558,9 @@ def main():
         return
     if args.ensemble:
         ensemble_bench(args)
+    elif args.ensemble_small:
+        ensemble_bench(args, ENSEMBLE_SMALL, "64 small realizations",
+                       "32 x 32^3/96^3 L62.5 + 32 x 48^3/144^3 L93.75, seeds 1-64, 10 steps")
     elif args.workload == "plain":
         plain_bench(args)
     else:
