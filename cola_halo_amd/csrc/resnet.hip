@@ -457,55 +457,6 @@ __device__ inline void roll_plane(const uint4* __restrict__ row, const uint4* __
   }
 }
 
-// roll_plane with one call of extra(j) per brick row, placed with the row's
-// products (the layer-1 tile of the fused pass).
-template <int T, typename Extra>
-__device__ inline void roll_plane_with(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
-                                       const uint4 (&b)[kRollChunks], floatx4 (&am)[T], floatx4 (&a0)[T],
-                                       floatx4 (&ap)[T], Extra&& extra)
-{
-  constexpr int kRow = 2 * kBrickZ;
-  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-  uint4 prev = zero4;
-  uint4 cen = row[0], sh = shr[0];
-#pragma unroll
-  for (int j = 0; j < T + 2; ++j) {
-    uint4 ncen = zero4, nsh = zero4;
-    if (j + 1 < T + 2) {
-      ncen = row[(j + 1) * kRow];
-      nsh = shr[(j + 1) * kRow];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    extra(j);
-    if (j < T) {
-      am[j] = mfma(sh, b[0], am[j]);
-      a0[j] = mfma(sh, b[5], a0[j]);
-      ap[j] = mfma(sh, b[10], ap[j]);
-    }
-    if (j >= 1 && j - 1 < T) {
-      am[j - 1] = mfma(sh, b[1], am[j - 1]);
-      a0[j - 1] = mfma(sh, b[6], a0[j - 1]);
-      ap[j - 1] = mfma(sh, b[11], ap[j - 1]);
-      const uint4 pc = sel4(lo, prev, cen);
-      am[j - 1] = mfma(pc, b[3], am[j - 1]);
-      a0[j - 1] = mfma(pc, b[8], a0[j - 1]);
-      ap[j - 1] = mfma(pc, b[13], ap[j - 1]);
-    }
-    if (j >= 2) {
-      am[j - 2] = mfma(sh, b[2], am[j - 2]);
-      a0[j - 2] = mfma(sh, b[7], a0[j - 2]);
-      ap[j - 2] = mfma(sh, b[12], ap[j - 2]);
-      const uint4 qc = sel4(lo, cen, zero4);
-      am[j - 2] = mfma(qc, b[4], am[j - 2]);
-      a0[j - 2] = mfma(qc, b[9], a0[j - 2]);
-      ap[j - 2] = mfma(qc, b[14], ap[j - 2]);
-    }
-    prev = cen;
-    cen = ncen;
-    sh = nsh;
-  }
-}
-
 // Output plane q of a rolling convolution from its accumulators, by mode
 // (kFwdRelu: bf16 activations; kFwdLinear: fp32 y; kFwdAdd: added to the
 // force grids, old values loaded here), then the accumulators cleared.
@@ -655,42 +606,9 @@ k_conv16r(const unsigned short* __restrict__ in, const unsigned short* __restric
 // fifteen.
 constexpr int kHeadChunks = 5;
 
-// Products of one input plane into d (one tile per output row, fresh).
-// ROW: uint4 between input rows (two per cell).
-template <int ROW = 2 * kBrickZ, int T = kConvTy>
-__device__ inline void head_plane(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
-                                  const uint4 (&b)[kHeadChunks], floatx4 (&d)[T])
-{
-  constexpr int kRow = ROW;
-  const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-  const floatx4 fz = {0.0f, 0.0f, 0.0f, 0.0f};
-  uint4 prev = zero4;
-  uint4 cen = row[0], sh = shr[0];
-#pragma unroll
-  for (int j = 0; j < T + 2; ++j) {
-    uint4 ncen = zero4, nsh = zero4;
-    if (j + 1 < T + 2) {
-      ncen = row[(j + 1) * kRow];
-      nsh = shr[(j + 1) * kRow];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if (j < T) d[j] = mfma(sh, b[0], fz);  // dy = -1: the first product of output row j
-    if (j >= 1 && j - 1 < T) {
-      d[j - 1] = mfma(sh, b[1], d[j - 1]);
-      d[j - 1] = mfma(sel4(lo, prev, cen), b[3], d[j - 1]);
-    }
-    if (j >= 2) {
-      d[j - 2] = mfma(sh, b[2], d[j - 2]);
-      d[j - 2] = mfma(sel4(lo, cen, zero4), b[4], d[j - 2]);
-    }
-    prev = cen;
-    cen = ncen;
-    sh = nsh;
-  }
-}
-
-// head_plane accumulating into d (the products of one input plane added to
-// the running sums of three output planes, one per lane group).
+// Products of one input plane added into d (one tile per output row: the
+// running sums of three output planes, one per lane group).  ROW: uint4
+// between input rows (two per cell).
 template <int ROW = 2 * kBrickZ, int T = kConvTy>
 __device__ inline void head_plane_acc(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
                                       const uint4 (&b)[kHeadChunks], floatx4 (&d)[T])
