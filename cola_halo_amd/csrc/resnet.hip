@@ -591,24 +591,23 @@ k_conv16r(const unsigned short* __restrict__ in, const unsigned short* __restric
 }
 
 // ---- last layer (16 -> 3) by input planes --------------------------------
-// With three output channels the three x offsets become output rows: row
-// 4 dxi + co of the A operand holds W[tap (dx = dxi - 1, dy, dz)][ci][co]
-// (rows with co = 3 and rows 12-15 zero), so one product per (dy, dz) chunk
-// of an input plane (the five chunks of roll_tap for one dx) makes all of
-// that plane's contributions: lane group g of the result holds, for its
-// cell, the contribution of input plane p to output plane p + 1 - g.  Output
-// plane q = G0(D_{q-1}) + G1(D_q) + G2(D_{q+1}): a wave keeps the last
-// plane's partial sums and adds them one lane group up (ds_bpermute) to the
-// next plane's products, after which lane group 2 holds output plane p - 1
-// complete; its three channels are then spread over lane groups 0-2, one
-// 4-byte access per lane and row (kFwdAdd loads the old force values one
+// With three output channels the three x offsets become output rows of one
+// product: row 4 co + s of the A operand holds W[tap (dx, dy, dz)][ci][co]
+// for the x offset dx whose output plane sums in accumulator element s
+// (rows with co = 3 or s = 3 zero), so one product per (dy, dz) chunk of an
+// input plane (the five chunks of roll_tap for one dx) makes all of that
+// plane's contributions.  The weights come in three rotations (step mod 3)
+// that keep output plane o in element (o - x0) mod 3 of every lane: output
+// plane q = G0(D_{q-1}) + G1(D_q) + G2(D_{q+1}) sums in place over three
+// steps, and lane group g holds channel g, one 4-byte access per lane and
+// row when the plane is complete (kFwdAdd loads the old force values one
 // step ahead).  Five products per 16-cell tile and input plane instead of
 // fifteen.
 constexpr int kHeadChunks = 5;
 
 // Products of one input plane added into d (one tile per output row: the
-// running sums of three output planes, one per lane group).  ROW: uint4
-// between input rows (two per cell).
+// running sums of three output planes, one per accumulator element).  ROW:
+// uint4 between input rows (two per cell).
 template <int ROW = 2 * kBrickZ, int T = kConvTy>
 __device__ inline void head_plane_acc(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
                                       const uint4 (&b)[kHeadChunks], floatx4 (&d)[T])
@@ -642,9 +641,11 @@ __device__ inline void head_plane_acc(const uint4* __restrict__ row, const uint4
 
 // The last layer's step with rotated weights (R = step mod 3): the products
 // of an input plane accumulate in place, the contributions to output plane
-// o in lane group (o - x0) mod 3 at every step, so no partial sums move
-// between lanes; the group holding a completed plane is read out and
-// cleared.  Selects the rotation's pack at run time (wave-uniform).
+// o in accumulator element (o - x0) mod 3 of every lane, lane group g
+// holding output channel g, so no partial sums move between lanes and a
+// completed plane is already spread one channel per lane group: its
+// element is read out and cleared.  Selects the rotation's pack at run
+// time (wave-uniform).
 template <int ROW = 2 * kBrickZ, int T = kConvTy>
 __device__ inline void head_plane_rot(const uint4* __restrict__ row, const uint4* __restrict__ shr, bool lo,
                                       const uint4 (&b)[3][kHeadChunks], int R, floatx4 (&d)[T])
@@ -657,10 +658,6 @@ __device__ inline void head_plane_rot(const uint4* __restrict__ row, const uint4
     head_plane_acc<ROW, T>(row, shr, lo, b[2], d);
 }
 
-__device__ inline float lane_read(float v, int src_lane)
-{
-  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(src_lane * 4, __builtin_bit_cast(int, v)));
-}
 
 // s_waitcnt for "at most n vector-memory operations outstanding", n rounded
 // down to the ladder 2L + 8k (L = kRollLoads) that the head's steps produce.
@@ -719,7 +716,7 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
   for (long long u = blockIdx.x; u < units; u += gridDim.x) {
     const ConvUnit cu = conv_unit(u, n, march);
     const int ybase = cu.y0 + kConvTy * wy, zcell = cu.z0 + 16 * wz + r;
-    floatx4 acc[kConvTy];  // output planes in lane groups (o - x0) mod 3
+    floatx4 acc[kConvTy];  // output plane o in element (o - x0) mod 3, channel g in lane group g
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     __builtin_amdgcn_s_waitcnt(0);
@@ -742,15 +739,13 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
       load_plane_to(ring, in, n, p + 3, min(p + 3, cu.x1), cu.y0, cu.z0);
       const uint4* slot = roll_slot(ring, p);
       // input plane p adds to output planes p + 1, p, p - 1; p - 1 is complete
-      // in lane group (i + 1) mod 3
+      // in accumulator element (i + 1) mod 3
       const int R = i % 3, G = (R + 1) % 3;
       head_plane_rot(slot + rowoff, slot + shoff, lo, b, R, acc);
-      const int src = 16 * G + r;
       if (store) {
 #pragma unroll
         for (int t = 0; t < kConvTy; ++t) {
-          const float v0 = lane_read(acc[t][0], src), v1 = lane_read(acc[t][1], src), v2 = lane_read(acc[t][2], src);
-          const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
+          const float v = (G == 0 ? acc[t][0] : (G == 1 ? acc[t][1] : acc[t][2])) + bo;
           if (g < 3) {
             const long long line = (long long)(p - 1) * n + ybase + t;
             if (MODE == kFwdAdd) {
@@ -766,9 +761,11 @@ k_conv_head_r(const unsigned short* __restrict__ in, const unsigned short* __res
           }
         }
       }
-      if (g == G) {
 #pragma unroll
-        for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int t = 0; t < kConvTy; ++t) {  // element G of every lane (G is uniform: no dynamic indexing)
+        acc[t][0] = G == 0 ? 0.0f : acc[t][0];
+        acc[t][1] = G == 1 ? 0.0f : acc[t][1];
+        acc[t][2] = G == 2 ? 0.0f : acc[t][2];
       }
       // plane p + 1 landed: step j issues the old-value loads F(j), the plane
       // loads and the stores S(j); count what was issued after the loads of
@@ -950,7 +947,7 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
     const int zcell = z0 + 14 * wz + r;
     const bool zok = zcell < n && (wz == 0 || r >= 2);  // a cell of the grid this lane alone stores
     const int zld = zcell < n ? zcell : zcell - n;       // in range for the old-value loads
-    floatx4 acc[kConvTy];  // output planes in lane groups (o - x0) mod 3
+    floatx4 acc[kConvTy];  // output plane o in element (o - x0) mod 3, channel g in lane group g
 #pragma unroll
     for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     __builtin_amdgcn_s_waitcnt(0);
@@ -977,16 +974,14 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
       for (int t = 0; t < kConvTy; ++t) fnext[t] = gout[((long long)old_plane(i + 1) * n + ybase + t) * o.row + zld];
       load_plane3_to(ring, in, n, p + 3, min(p + 3, x1 + 1), y0, z0);
       // h3 plane q = p - 2 adds to output planes q + 1, q, q - 1; q - 1 is
-      // complete in lane group (i + 1) mod 3
+      // complete in accumulator element (i + 1) mod 3
       const uint4* hs = h3 + ((p - 2) & 1) * kH3SlotChunks;
       const int R = i % 3, G = (R + 1) % 3;
       head_plane_rot<2 * kH3Z>(hs + hrowoff, hs + hshoff, lo, bh, R, acc);
-      const int src = 16 * G + r;
       const bool put = store && zok;
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) {
-        const float v0 = lane_read(acc[t][0], src), v1 = lane_read(acc[t][1], src), v2 = lane_read(acc[t][2], src);
-        const float v = (g == 0 ? v0 : (g == 1 ? v1 : v2)) + bo;
+        const float v = (G == 0 ? acc[t][0] : (G == 1 ? acc[t][1] : acc[t][2])) + bo;
         if (g < 3) {
           const long long line = (long long)(put ? p - 3 : x0) * n + ybase + t;
           const float dv = o.scale * v;
@@ -995,9 +990,11 @@ k_conv3h(const unsigned short* __restrict__ in, const unsigned short* __restrict
           if (put) added += dv;
         }
       }
-      if (g == G) {
 #pragma unroll
-        for (int t = 0; t < kConvTy; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int t = 0; t < kConvTy; ++t) {  // element G of every lane (G is uniform: no dynamic indexing)
+        acc[t][0] = G == 0 ? 0.0f : acc[t][0];
+        acc[t][1] = G == 1 ? 0.0f : acc[t][1];
+        acc[t][2] = G == 2 ? 0.0f : acc[t][2];
       }
 #pragma unroll
       for (int t = 0; t < kConvTy; ++t) fold[t] = fnext[t];
@@ -1505,17 +1502,18 @@ __global__ void k_pack_roll(const float* __restrict__ w, int cout, unsigned shor
 }
 
 // Last-layer weights W[t][16][3] -> the last layer's A operands, three
-// rotations R of five chunks (block 5 R + c): row 4 lg + co of chunk c
+// rotations R of five chunks (block 5 R + c): row 4 co + s of chunk c
 // holds the taps roll_tap(5 dxi + c, h) of output channel co with
-// dxi = (R - lg) mod 3 (lane group lg = 3 and co = 3 zero), so the
-// contribution of x offset dxi - 1 lands in output lane group (R - dxi) mod 3.
+// dxi = (R - s) mod 3 (co = 3 and s = 3 zero), so the contribution of x
+// offset dxi - 1 lands in accumulator element s = (R - dxi) mod 3 of output
+// lane group co.
 __global__ void k_pack_head(const float* __restrict__ w, unsigned short* __restrict__ wh)
 {
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
   const int rot = blockIdx.x / kHeadChunks, c = blockIdx.x % kHeadChunks;
-  const int lg = col >> 2, co = col & 3;
-  const int dxi = (rot - lg + 3) % 3;
-  const int tap = (lg < 3 && co < 3) ? roll_tap(5 * dxi + c, g >> 1) : -1;
+  const int co = col >> 2, sl = col & 3;
+  const int dxi = (rot - sl + 3) % 3;
+  const int tap = (co < 3 && sl < 3) ? roll_tap(5 * dxi + c, g >> 1) : -1;
   for (int j = 0; j < 8; ++j) {
     const int ci = 8 * (g & 1) + j;
     const float v = tap >= 0 ? w[((long long)tap * 16 + ci) * 3 + co] : 0.0f;
